@@ -1,0 +1,148 @@
+// Per-lane banded DP of BandAligner (src/BandAligner.cpp:56-239): one task per lane, band row in
+// registers, 2-bit traceback pointers stored row by row through a PtrStore.
+//
+// Band-diagonal coordinates: cell (i, j) of the (lenS+1) x (lenQ+1) matrix sits at row i,
+// column d = j - i + b, d in [0, 2b]. Neighbours: diagonal (i-1, j-1) -> previous row, d;
+// up (i-1, j) -> previous row, d+1; left (i, j-1) -> this row, d-1. Cells outside the matrix
+// (j < 0 or j > lenQ) and outside the band (d > 2b) hold kSent. A neighbour the reference treats
+// as absent (its vals[] = -MAXDUL, or IX/IY := -MAXDUL at the band edges) is exactly a kSent
+// cell here, and sentinel-derived values only ever compete against real values (which stay far
+// above -MAXDUL), so every comparison the reference makes has the same outcome.
+#pragma once
+#include "hd.h"
+
+namespace hero {
+
+constexpr int kSent = -(1 << 28);
+constexpr int kCodeN = 4;    // 'N' (BandAligner::CWILD) in the standalone aligner
+constexpr int kCodeNone = 7; // beyond the end of a sequence (never equal to a base)
+
+HD int imax_(int a, int b) { return a > b ? a : b; }
+
+// Diagonal score of the global aligner (src/BandAligner.cpp:89-97): a mismatch involving N
+// scores unobserved_match; equal characters (N==N included) score hit.
+HD int diag_score_global(const Opts &o, int a, int q) {
+	return a != q ? ((a == kCodeN || q == kCodeN) ? o.unobserved : o.miss) : o.hit;
+}
+
+// Forward pass. S.next() / Q.next() yield the aligner subject / query codes in order and
+// kCodeNone past the end. ptrs.put_row(i, words) receives row i's pointers, 16 cells per word,
+// cell d at bits 2*(d%16); codes are the reference's BM1 values: 2 diagonal, 1 left, 0 up.
+template <int BMAX, bool AFFINE, class SS, class QS, class PS>
+HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool right_al, SS &S, QS &Q, PS &ptrs) {
+	constexpr int W = 2 * BMAX + 1;
+	constexpr int NW = (W + 15) / 16;
+	int H[W + 1];
+	int X[AFFINE ? W + 1 : 1], Y[AFFINE ? W + 1 : 1];
+	int qw[W]; // qw[d] = Q[i - b - 1 + d] for the current row i
+	// Row 0 (global: src/BandAligner.cpp:63-111 at i = 0; affine: :140-171).
+	{
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+#pragma unroll
+		for (int d = 0; d <= W; d++) {
+			const int j = d - b;
+			const bool in = d < W && d <= 2 * b && j >= 0 && j <= lenQ;
+			if (AFFINE) {
+				const int v = j == 0 ? 0 : (o.dx - o.ex) + j * o.ex;
+				H[d] = in ? v : kSent;
+				X[d] = in ? (j == 0 ? o.dx - o.ex : v) : kSent;
+				Y[d] = X[d];
+			} else {
+				H[d] = in ? o.dy * j : kSent;
+			}
+			if (d < W && in && j >= 1) pw[d >> 4] |= 1u << (2 * (d & 15));
+		}
+		ptrs.put_row(0, pw);
+	}
+#pragma unroll
+	for (int d = 0; d < W; d++) qw[d] = d < b ? kCodeNone : Q.next(); // row 1: Q[d - b]
+	for (int i = 1; i <= lenS; i++) {
+		const int sc = S.next();
+		const int d0 = b - i;        // column j = 0
+		const int dQ = lenQ - i + b; // column j = lenQ
+		const int dlo = d0 > 0 ? d0 : 0;
+		const int dhi = dQ < 2 * b ? dQ : 2 * b;
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+		int hl = kSent, yl = kSent; // this row's cell at d-1
+#pragma unroll
+		for (int d = 0; d < W; d++) {
+			const bool valid = d >= dlo && d <= dhi;
+			const int qc = qw[d];
+			int h, p;
+			if (!AFFINE) {
+				int g = o.dx;
+				if (d == d0) g = right_al ? o.sx : ((lenQ == 0 && left_al) ? o.sx : o.dx);
+				else if (d == dQ) g = left_al ? o.sx : o.dx;
+				const int up = H[d + 1] + g;
+				const int lf = hl + o.dy;
+				const int dg = H[d] + diag_score_global(o, sc, qc);
+				if (dg >= lf && dg >= up) h = dg, p = 2;
+				else if (lf >= up) h = lf, p = 1;
+				else h = up, p = 0;
+				h = valid ? h : kSent;
+				H[d] = h;
+				hl = h;
+			} else {
+				int x, y;
+				if (d == d0) { // column 0 (src/BandAligner.cpp:150-163)
+					if (right_al) x = H[d + 1] + o.sx, y = o.dy;
+					else x = X[d + 1] + o.ex, y = x;
+					h = x;
+					p = 0;
+				} else { // src/BandAligner.cpp:176-222
+					y = imax_(hl + o.dy, yl + o.ex);
+					if (left_al && d == dQ) x = imax_(H[d + 1] + o.sx, X[d + 1] + o.sx);
+					else x = imax_(H[d + 1] + o.dx, X[d + 1] + o.ex);
+					const int s = sc != qc ? o.miss : o.hit;
+					h = imax_(imax_(H[d] + s, X[d] + s), Y[d] + s);
+					p = (h > x && h >= y) ? 2 : (y > x ? 1 : 0);
+				}
+				h = valid ? h : kSent;
+				x = valid ? x : kSent;
+				y = valid ? y : kSent;
+				H[d] = h;
+				X[d] = x;
+				Y[d] = y;
+				hl = h;
+				yl = y;
+			}
+			pw[d >> 4] |= (valid ? (uint32_t) p : 0u) << (2 * (d & 15));
+		}
+		ptrs.put_row(i, pw);
+		// slide the query window to row i+1: qw[d] = Q[i - b + d]
+#pragma unroll
+		for (int d = 0; d + 1 < W; d++) qw[d] = qw[d + 1];
+		qw[W - 1] = (i - b + W - 1 >= 0) ? Q.next() : kCodeNone;
+	}
+}
+
+// Traceback (src/BandAligner.cpp:114-125,227-238): from (lenS, lenQ) follow the pointers to the
+// origin. Steps come last-to-first through emit(step): kStepMatch for a diagonal (the caller
+// tells match from mismatch), kStepLeft (j-1), kStepUp (i-1). Returns the step count.
+template <class PS, class EM>
+HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
+	int i = lenS, j = lenQ, n = 0;
+	while (!(i == 0 && j == 0)) {
+		const int d = j - i + b;
+		const uint32_t word = ptrs.get(i, d >> 4);
+		const int p = (int) ((word >> (2 * (d & 15))) & 3u);
+		if (p == 2) {
+			emit(kStepMatch);
+			i--, j--;
+		} else if (p == 1) {
+			emit(kStepLeft);
+			j--;
+		} else {
+			emit(kStepUp);
+			i--;
+		}
+		n++;
+	}
+	return n;
+}
+
+} // namespace hero

@@ -1,0 +1,180 @@
+// Host/device building blocks shared by the HIP kernels and the CPU harness (hostsim.cpp).
+// Nothing here uses wave intrinsics; every function is plain per-lane code.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define HD inline
+#endif
+
+namespace hero {
+
+// Run options as the kernels see them (mirrors Config fields of the reference,
+// src/Config.cpp:17-56, and BandAligner::setCost, src/BandAligner.cpp:528-542).
+struct Opts {
+	int32_t k;
+	int32_t aligner;        // 0 affine, 1 global
+	int32_t call_consensus; // 1: vote and correct (-c); 0: alignment records only
+	int32_t band_override;  // >0 replaces setBand(2*extra)
+	int32_t max_candidates; // >0 keeps the first N query groups per subject strand
+	int32_t _pad;
+	int64_t min_overlap;
+	double mismatch_rate, indel_rate, consensus_freq;
+	int32_t dx, dy, ex, sx, miss, hit, unobserved, _pad2;
+};
+
+// 2-bit codes of the reference: (c >> 1) & 3 → A0 C1 T2 G3 (src/QueryRead.cpp:124-125).
+// Complement A<->T, C<->G is code ^ 2.
+HD int ascii_code(char c) { return ((unsigned char) c >> 1) & 3; }
+HD char code_ascii(int c) { return "ACTG"[c & 3]; }
+
+// Packed read set: 2-bit codes, MSB-first inside u64 words; read r starts at word woff[r].
+// The word array carries two zero words of tail padding so k-mer extraction may read past a
+// read's last word.
+struct Packed {
+	const uint64_t *words;
+	const uint64_t *woff;
+	const int32_t *len;
+	const int64_t *uid;
+	const uint8_t *good; // subject/query predicate of the dataset that owns the set
+	int64_t n;
+};
+
+HD int base_at(const uint64_t *w, int64_t p) { return (int) ((w[p >> 5] >> (62 - 2 * (p & 31))) & 3ull); }
+
+// Base p of read r in strand frame: forward, or of the reverse complement when rc.
+HD int strand_base(const Packed &P, int64_t r, int p, bool rc) {
+	const uint64_t *w = P.words + P.woff[r];
+	if (!rc) return base_at(w, p);
+	return base_at(w, P.len[r] - 1 - p) ^ 2;
+}
+
+// The k-mer at p (k <= 64) of a packed read, MSB-first: hi = first min(k,32) bases,
+// lo = remaining k-32 bases (0 when k <= 32).
+HD void kmer_fwd(const uint64_t *w, int p, int k, uint64_t &hi, uint64_t &lo) {
+	int wi = p >> 5, sh = 2 * (p & 31);
+	uint64_t a = w[wi], b = w[wi + 1], c = w[wi + 2];
+	uint64_t x = sh ? ((a << sh) | (b >> (64 - sh))) : a;
+	uint64_t y = sh ? ((b << sh) | (c >> (64 - sh))) : b;
+	if (k <= 32) {
+		hi = k == 32 ? x : (x >> (64 - 2 * k));
+		lo = 0;
+	} else {
+		hi = x;
+		lo = (k == 64) ? y : (y >> (64 - 2 * (k - 32)));
+	}
+}
+
+// Reverse complement of a k-mer given as (hi, lo) — returns the same (hi, lo) convention.
+HD uint64_t rev2(uint64_t x) { // reverse the 32 2-bit groups of a word
+	x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+	x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+	x = ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x & 0x00FF00FF00FF00FFull) << 8);
+	x = ((x >> 16) & 0x0000FFFF0000FFFFull) | ((x & 0x0000FFFF0000FFFFull) << 16);
+	x = (x >> 32) | (x << 32);
+	return x;
+}
+HD void kmer_revcomp(int k, uint64_t hi, uint64_t lo, uint64_t &rhi, uint64_t &rlo) {
+	// Build the full 2k-bit sequence left-aligned in a 128-bit pair, reverse, complement.
+	uint64_t L0, L1; // left-aligned: L0 holds bases 0..31, L1 bases 32..63
+	if (k <= 32) {
+		L0 = k == 32 ? hi : (hi << (64 - 2 * k));
+		L1 = 0;
+	} else {
+		L0 = hi;
+		L1 = k == 64 ? lo : (lo << (64 - 2 * (k - 32)));
+	}
+	// reversed sequence right-aligned over 128 bits: R = rev(L1):rev(L0)
+	uint64_t R0 = rev2(L1), R1 = rev2(L0); // R0 = high word, R1 = low word of the reversed 128 bits
+	// The k bases of the RC sit in the low 2k bits of (R0:R1).
+	const uint64_t A = 0xAAAAAAAAAAAAAAAAull; // complement = xor 2 per base
+	if (k <= 32) {
+		rhi = (k == 32 ? R1 : (R1 & ((1ull << (2 * k)) - 1))) ^ (k == 32 ? A : (A & ((1ull << (2 * k)) - 1)));
+		rlo = 0;
+	} else {
+		int r = k - 32; // bases in lo
+		// first 32 bases of the RC = bits [2k-1 .. 2r] of R, last r bases = bits [2r-1 .. 0]
+		uint64_t first32 = r == 32 ? R0 : ((R0 << (64 - 2 * r)) | (R1 >> (2 * r)));
+		uint64_t last = r == 32 ? R1 : (R1 & ((1ull << (2 * r)) - 1));
+		rhi = first32 ^ A;
+		rlo = last ^ (r == 32 ? A : (A & ((1ull << (2 * r)) - 1)));
+	}
+}
+
+// HashTableLongKey::hashFunction (src/HashTableLongKey.cpp:188-206) on a packed k-mer:
+// sum1 = leading 1 followed by the first 32 bases (the 1 falls off once 32 bases are in),
+// sum2 = leading 1 followed by the remaining bases; ((sum1 % P) * (sum2 % P)) % P in u64.
+HD uint64_t hash_kmer(int k, uint64_t hi, uint64_t lo, uint64_t P) {
+	uint64_t s1 = k >= 32 ? hi : ((1ull << (2 * k)) | hi);
+	uint64_t s2 = 1;
+	if (k > 32) s2 = (k - 32 >= 32) ? lo : ((1ull << (2 * (k - 32))) | lo);
+	return ((s1 % P) * (s2 % P)) % P;
+}
+
+// Key positions of a query read (src/HashTableLongKey.cpp:279-289): 0, k, 2k, ... <= L-k and,
+// when that stride leaves the tail uncovered, L-k.
+HD int num_keys(int L, int k) {
+	if (L < k) return 0;
+	int n = (L - k) / k + 1;
+	if ((L - k) % k != 0) n++;
+	return n;
+}
+HD int key_pos(int L, int k, int t) {
+	int n = (L - k) / k + 1;
+	return t < n ? t * k : L - k;
+}
+
+// Seed encoding: (innerId << 28) | (posQ << 14) | posS; ascending order is the reference's
+// (innerId, posQ, posS) order (src/SubjectRead.cpp:157-173). Reads are limited to 16383 bases.
+constexpr int kSeedPosBits = 14;
+constexpr int kMaxReadLen = (1 << kSeedPosBits) - 1;
+HD uint64_t seed_pack(uint64_t qid, int pq, int ps) {
+	return (qid << (2 * kSeedPosBits)) | ((uint64_t) pq << kSeedPosBits) | (uint64_t) ps;
+}
+HD int64_t seed_qid(uint64_t s) { return (int64_t) (s >> (2 * kSeedPosBits)); }
+HD int seed_pq(uint64_t s) { return (int) ((s >> kSeedPosBits) & kMaxReadLen); }
+HD int seed_ps(uint64_t s) { return (int) (s & kMaxReadLen); }
+
+// DP task: one BandAligner call after setSequences (src/BandAligner.cpp:559-573) has chosen the
+// aligner's subject (the longer side; the query side on ties) and setBand/widening has fixed the
+// band. Sides are described in their own frames; `rc` sides read the reverse complement.
+enum : uint8_t {
+	kTaskAlignLeft = 1,  // setAlignLeft
+	kTaskAlignRight = 2, // setAlignRight
+	kTaskSwapped = 4,    // aligner subject is the record's query side
+	kTaskSubjRC = 8,     // record subject side is a reverse complement (reverse strand under -c)
+	kTaskQryRC = 16,     // record query side is a reverse complement (reverse strand without -c)
+};
+struct DpTask {
+	uint32_t s_read, q_read; // record subject / query read indices
+	uint16_t s0, s_len;      // record subject segment
+	uint16_t q0, q_len;      // record query segment
+	uint16_t band;           // effective m_band (already widened)
+	uint8_t flags;
+	uint8_t _r;
+	uint32_t out_off;        // edit-step words at out_off; capacity ceil((s_len+q_len)/16) words
+	uint32_t _r2;
+	uint32_t _r3;
+};
+static_assert(sizeof(DpTask) == 32, "DpTask layout");
+
+HD int edit_words(int lenS, int lenQ) { return (lenS + lenQ + 15) / 16; }
+
+// Edit step codes written by the DP traceback (aligner orientation).
+enum : uint32_t { kStepMatch = 0, kStepMismatch = 1, kStepLeft = 2, kStepUp = 3 };
+
+// In-band cells of one DP call: sum over rows of the row width, minus the origin.
+HD uint64_t band_cells(int lenS, int lenQ, int b) {
+	uint64_t n = 0;
+	for (int i = 0; i <= lenS; i++) {
+		int lo = i - b > 0 ? i - b : 0, hi = lenQ < i + b ? lenQ : i + b;
+		n += (uint64_t) (hi - lo + 1);
+	}
+	return n - 1;
+}
+
+} // namespace hero

@@ -1,0 +1,328 @@
+// Per-group steps of the correction path shared by the kernels and the CPU harness:
+//   plan_group   (pass A)  chain one (subject, strand, query) seed group and emit the DP tasks of
+//                          every surviving chain (processOneSubjectRead2 + getAlignment2 geometry);
+//   score_pair   (pass B)  build the query's records over both strands from the DP results,
+//                          apply early exits and acceptance, keep the first best record
+//                          (SubjectRead::addAlignment under -c);
+//   vote_record  (pass B)  cast the kept record's votes (SubjectRead::updateQueryCoverage).
+#pragma once
+#include "dp_core.h"
+#include "record_core.h"
+
+namespace hero {
+
+// Seeds and chain state of one group: seeds[0..gn) sorted; chain arrays of group size.
+struct Group {
+	const uint64_t *seeds;
+	int32_t *cid, *cfirst, *clast, *clen;
+	int gn;
+	int nch;  // chains after chain_group
+	int best; // best covered length
+};
+
+struct SeedAt {
+	const uint64_t *s;
+	HD uint64_t operator()(int t) const { return s[t]; }
+};
+struct I32Arr {
+	int32_t *p;
+	HD int32_t &operator[](int i) const { return p[i]; }
+};
+
+HD void chain(const Opts &o, Group &g) {
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid}, cf{g.cfirst}, cl{g.clast}, ln{g.clen};
+	g.nch = chain_group(o, g.gn, sa, cid, cf, cl, ln, &g.best);
+}
+
+// Reads of one (subject, strand, query) pair and their frames.
+struct PairReads {
+	const Packed *subj, *qry;
+	int64_t s_read, q_read;
+	int Ls, Lq;
+	bool subj_rc; // record subject frame is the subject's reverse complement (-c, reverse strand)
+	bool qry_rc;  // record query frame is the query's reverse complement (no -c, reverse strand)
+	HD int sbase(int p) const { return strand_base(*subj, s_read, p, subj_rc); }
+	HD int qbase(int p) const { return strand_base(*qry, q_read, p, qry_rc); }
+};
+
+// ---- plan ---------------------------------------------------------------------------------
+struct CountDp {
+	int n = 0;
+	int words = 0;
+	HD void start(int) {}
+	HD void add(int, int) {}
+	HD void dp(const DpCall &c) { n++, words += edit_words(c.lenS, c.lenQ); }
+	HD bool check(int64_t, int64_t) { return true; }
+	HD bool finish() { return true; }
+};
+
+HD DpTask make_task(const PairReads &pr, const DpCall &c, uint32_t out_off) {
+	DpTask t;
+	t.s_read = (uint32_t) pr.s_read;
+	t.q_read = (uint32_t) pr.q_read;
+	t.s0 = (uint16_t) c.s0, t.s_len = (uint16_t) (c.s1 - c.s0);
+	t.q0 = (uint16_t) c.q0, t.q_len = (uint16_t) (c.q1 - c.q0);
+	t.band = (uint16_t) c.band;
+	t.flags = (uint8_t) ((c.flag == 1 ? kTaskAlignLeft : 0) | (c.flag == 2 ? kTaskAlignRight : 0) |
+			(c.swapped ? kTaskSwapped : 0) | (pr.subj_rc ? kTaskSubjRC : 0) | (pr.qry_rc ? kTaskQryRC : 0));
+	t._r = 0;
+	t.out_off = out_off;
+	t._r2 = 0;
+	t._r3 = 0;
+	return t;
+}
+
+struct EmitDp {
+	const PairReads &pr;
+	DpTask *tasks;
+	uint32_t next_task, next_word;
+	HD void start(int) {}
+	HD void add(int, int) {}
+	HD void dp(const DpCall &c) {
+		tasks[next_task++] = make_task(pr, c, next_word);
+		next_word += (uint32_t) edit_words(c.lenS, c.lenQ);
+	}
+	HD bool check(int64_t, int64_t) { return true; }
+	HD bool finish() { return true; }
+};
+
+// Number of DP tasks and edit words of all surviving chains of a chained group.
+HD void plan_count(const Opts &o, const Group &g, const PairReads &pr, int &ntasks, int &nwords) {
+	ntasks = 0, nwords = 0;
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid};
+	for (int c = 0; c < g.nch; c++) {
+		if (g.clen[c] != g.best) continue;
+		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
+		CountDp cv;
+		walk_record(o, pr.Ls, pr.Lq, keys, cv);
+		ntasks += cv.n, nwords += cv.words;
+	}
+}
+
+// Writes the tasks of all surviving chains (in chain order) at tasks[task0...], edit words
+// allocated from word0.
+HD void plan_emit(const Opts &o, const Group &g, const PairReads &pr, DpTask *tasks, uint32_t task0, uint32_t word0) {
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid};
+	EmitDp ev{pr, tasks, task0, word0};
+	for (int c = 0; c < g.nch; c++) {
+		if (g.clen[c] != g.best) continue;
+		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
+		walk_record(o, pr.Ls, pr.Lq, keys, ev);
+	}
+}
+
+// ---- DP results -----------------------------------------------------------------------------
+// Task t's steps sit at edit_words[t.out_off ...], capacity cap = s_len + q_len steps; the n
+// steps of the traceback occupy [cap - n, cap) in forward (first-to-last) order, 16 per word.
+struct DpResults {
+	const DpTask *tasks;
+	const uint32_t *words;
+	const uint32_t *nsteps;
+};
+
+HD uint32_t step_at(const uint32_t *w, int e) { return (w[e >> 4] >> (2 * (e & 15))) & 3u; }
+
+// Feeds one DP call's result as record ops (addAlignment, src/AlignmentRecord.cpp:50-54): left /
+// up steps are I / D (swapped when the aligner's subject is the record's query, :339-419), a
+// diagonal is M when the two bases agree, else a mismatch typed by the query base.
+template <class F>
+HD void feed_dp(const DpResults &R, uint32_t task, const PairReads &pr, const DpCall &c, F &&op) {
+	const DpTask &t = R.tasks[task];
+	const int cap = t.s_len + t.q_len;
+	const int n = (int) R.nsteps[task];
+	const uint32_t *w = R.words + t.out_off;
+	int i = 0, j = 0; // aligner coordinates
+	for (int e = cap - n; e < cap; e++) {
+		const uint32_t st = step_at(w, e);
+		if (st == kStepLeft) {
+			op(c.swapped ? kOpD : kOpI);
+			j++;
+		} else if (st == kStepUp) {
+			op(c.swapped ? kOpI : kOpD);
+			i++;
+		} else {
+			const int sp = c.swapped ? c.s0 + j : c.s0 + i;
+			const int qp = c.swapped ? c.q0 + i : c.q0 + j;
+			const int sb = pr.sbase(sp), qb = pr.qbase(qp);
+			op(sb == qb ? kOpM : kOpX + qb);
+			i++, j++;
+		}
+	}
+}
+
+// ---- score --------------------------------------------------------------------------------
+struct ScoreV {
+	const Opts &o;
+	const DpResults &R;
+	const PairReads &pr;
+	uint32_t task; // next task of this record
+	Score sc;
+	double rate_sum, len;
+	uint64_t cells_exec; // cells of the DP calls this record actually consumed
+	int calls_exec;
+	HD void start(int) {}
+	HD void add(int c, int t) { sc.add(c, t); }
+	HD void dp(const DpCall &c) {
+		feed_dp(R, task++, pr, c, [&](int t) { sc.add(1, t); });
+		cells_exec += band_cells(c.lenS, c.lenQ, c.band);
+		calls_exec++;
+	}
+	HD bool check(int64_t m, int64_t i) { return sc.good_caps(m, i); }
+	HD bool finish() { return sc.good_rates(o, rate_sum, len); }
+};
+
+// Best record of one strand group, continuing a running best (first best wins).
+struct Best {
+	bool have;
+	double rate_sum, len;
+	int strand, chain;
+	uint32_t task0; // first DP task of the winning record
+};
+
+HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, uint32_t task0, int strand,
+		Best &best, uint64_t &cells_exec, uint64_t &calls_exec, uint64_t &built) {
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid};
+	uint32_t task = task0;
+	for (int c = 0; c < g.nch; c++) {
+		if (g.clen[c] != g.best) continue;
+		int ntask, nwords;
+		{
+			ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
+			CountDp cv;
+			walk_record(o, pr.Ls, pr.Lq, keys, cv);
+			ntask = cv.n, nwords = cv.words;
+		}
+		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
+		ScoreV sv{o, R, pr, task, Score(), 0.0, 0.0, 0, 0};
+		sv.sc.reset();
+		const bool ok = walk_record(o, pr.Ls, pr.Lq, keys, sv);
+		cells_exec += sv.cells_exec;
+		calls_exec += (uint64_t) sv.calls_exec;
+		if (ok) {
+			built++;
+			if (!best.have || record_better(sv.rate_sum, sv.len, best.rate_sum, best.len)) {
+				best.have = true, best.rate_sum = sv.rate_sum, best.len = sv.len;
+				best.strand = strand, best.chain = c, best.task0 = task;
+			}
+		}
+		task += (uint32_t) ntask;
+		(void) nwords;
+	}
+}
+
+// ---- votes --------------------------------------------------------------------------------
+// Tally of one query read: 4 u32 per base at index 4*p + code. The slot of the read's own base
+// holds a difference array of M-run votes (+1 at a run's first base, -1 after its last), the
+// other three slots hold mismatch votes; decode_tally turns it into the reference's counts.
+struct VoteSink {
+	uint32_t *tally; // this query's tally
+	const PairReads &pr;
+	int Lq;
+	int *oob;        // set when a vote would leave the read (the reference throws out_of_range)
+	HD void add_u32(uint32_t *p, uint32_t v) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+		atomicAdd(p, v);
+#else
+		__atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+	}
+	HD void m_run(int p, int c) const {
+		if (c <= 0) return;
+		if (p < 0 || p + c > Lq) {
+			*oob = 1;
+			return;
+		}
+		add_u32(&tally[4 * p + pr.qbase(p)], 1u);
+		if (p + c < Lq) add_u32(&tally[4 * (p + c) + pr.qbase(p + c)], 0xFFFFFFFFu);
+	}
+	HD void mismatch(int pq, int ps) const {
+		if (pq < 0 || pq >= Lq || ps < 0 || ps >= pr.Ls) {
+			*oob = 1;
+			return;
+		}
+		add_u32(&tally[4 * pq + pr.sbase(ps)], 1u);
+	}
+};
+
+struct VoteV {
+	const DpResults &R;
+	const PairReads &pr;
+	const VoteSink &sink;
+	uint32_t task;
+	int ps, pq;       // record frames (iPosOnSubjectRead, iPosOnQueryRead)
+	int run_p, run_c; // pending M run
+	HD void flush() {
+		if (run_c) sink.m_run(run_p, run_c);
+		run_c = 0;
+	}
+	HD void op(int c, int t) {
+		if (t == kOpM) {
+			if (run_c && run_p + run_c == pq) run_c += c;
+			else flush(), run_p = pq, run_c = c;
+			pq += c, ps += c;
+		} else if (t == kOpI) {
+			pq += c;
+		} else if (t == kOpD) {
+			ps += c;
+		} else {
+			for (int x = 0; x < c; x++) sink.mismatch(pq + x, ps + x);
+			pq += c, ps += c;
+		}
+	}
+	HD void start(int pos) { ps = pos, pq = 0; }
+	HD void add(int c, int t) { op(c, t); }
+	HD void dp(const DpCall &c) { feed_dp(R, task++, pr, c, [&](int t) { op(1, t); }); }
+	HD bool check(int64_t, int64_t) { return true; }
+	HD bool finish() {
+		flush();
+		return true;
+	}
+};
+
+HD void vote_record(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, int chain, uint32_t task0,
+		const VoteSink &sink) {
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid};
+	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
+	VoteV vv{R, pr, sink, task0, 0, 0, 0, 0};
+	walk_record(o, pr.Ls, pr.Lq, keys, vv);
+}
+
+// ---- consensus ----------------------------------------------------------------------------
+// QueryRead::callConsensus (src/QueryRead.cpp:148-180) over a difference-coded tally: the own
+// base's count is the running sum of its slot plus the self vote; aDNA = {A,C,T,G} (:11) is
+// exactly the code order. Writes the corrected read as codes; returns changed bases.
+HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed &Q, int64_t r, uint8_t *out_codes,
+		uint32_t *decoded /* optional 4*L, pre-self-vote counts */) {
+	uint32_t run = 0;
+	int changed = 0;
+	for (int p = 0; p < L; p++) {
+		const int own = strand_base(Q, r, p, false);
+		uint32_t c[4];
+		for (int j = 0; j < 4; j++) c[j] = tally[4 * p + j];
+		run += c[own];
+		c[own] = run;
+		if (decoded)
+			for (int j = 0; j < 4; j++) decoded[4 * p + j] = c[j];
+		c[own] += 1; // self vote (:149)
+		double sum = 0;
+		for (int j = 0; j < 4; j++) sum += c[j];
+		int id = -1;
+		double largest = 0, larger = 0;
+		for (int j = 0; j < 4; j++) {
+			const double f = ((double) c[j]) / sum;
+			if (largest < f) larger = largest, largest = f, id = j;
+		}
+		int outc = own;
+		if (largest - larger >= o.consensus_freq) outc = id;
+		if (outc != own) changed++;
+		out_codes[p] = (uint8_t) outc;
+	}
+	return changed;
+}
+
+} // namespace hero

@@ -1,0 +1,310 @@
+// CPU harness for the device logic: runs hero_amd/csrc/{dp_core,record_core,pipeline_core}.h
+// through the same dataflow as the kernels (pass A plan -> DP -> pass B score/dedup/votes ->
+// consensus) with plain loops, so tests can compare that logic with the oracle without a GPU.
+// Test infrastructure only; the library never calls it.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../hero_amd/csrc/pack.h"
+#include "../../hero_amd/csrc/pipeline_core.h"
+
+using namespace hero;
+
+extern "C" {
+typedef struct {
+	int32_t k, aligner, call_consensus, hash_factor;
+	int64_t min_overlap;
+	double mismatch_rate, indel_rate, consensus_freq;
+	int32_t band_override, max_candidates;
+	int32_t dx, dy, ex, sx, miss, hit, unobserved;
+} hs_cfg;
+typedef struct {
+	int64_t n;
+	const char *seq;
+	const int64_t *offset;
+	const int32_t *length;
+	const int64_t *unique_id;
+} hs_reads;
+}
+
+static Opts to_opts(const hs_cfg *c) {
+	Opts o;
+	std::memset(&o, 0, sizeof o);
+	o.k = c->k, o.aligner = c->aligner, o.call_consensus = c->call_consensus;
+	o.band_override = c->band_override, o.max_candidates = c->max_candidates;
+	o.min_overlap = c->min_overlap;
+	o.mismatch_rate = c->mismatch_rate, o.indel_rate = c->indel_rate, o.consensus_freq = c->consensus_freq;
+	o.dx = c->dx, o.dy = c->dy, o.ex = c->ex, o.sx = c->sx, o.miss = c->miss, o.hit = c->hit, o.unobserved = c->unobserved;
+	return o;
+}
+
+struct PackedSet {
+	std::vector<uint64_t> words, woff;
+	std::vector<int32_t> len;
+	std::vector<int64_t> uid;
+	std::vector<uint8_t> good;
+	Packed P;
+	void build(const hs_reads *r, const std::vector<uint8_t> &g) {
+		pack_reads(r->seq, r->offset, r->length, r->n, words, woff);
+		len.assign(r->length, r->length + r->n);
+		uid.assign(r->unique_id, r->unique_id + r->n);
+		good = g;
+		P.words = words.data(), P.woff = woff.data(), P.len = len.data(), P.uid = uid.data(), P.good = good.data();
+		P.n = r->n;
+	}
+};
+
+struct Stream {
+	const Packed *P;
+	int64_t r;
+	int start, len, i;
+	bool rc;
+	int next() { return i < len ? strand_base(*P, r, start + i++, rc) : kCodeNone; }
+};
+struct RowStore {
+	std::vector<uint32_t> v;
+	int nw;
+	void put_row(int i, const uint32_t *w) {
+		if ((size_t) (i + 1) * nw > v.size()) v.resize((size_t) (i + 1) * nw);
+		for (int t = 0; t < nw; t++) v[(size_t) i * nw + t] = w[t];
+	}
+	uint32_t get(int i, int w) const { return v[(size_t) i * nw + w]; }
+};
+
+template <int BMAX, bool AFF, class SS, class QS>
+static int run_dp(const Opts &o, int lenS, int lenQ, int b, bool la, bool ra, SS &S, QS &Q, std::vector<uint32_t> &steps) {
+	RowStore ps;
+	ps.nw = (2 * BMAX + 1 + 15) / 16;
+	dp_forward<BMAX, AFF>(o, lenS, lenQ, b, la, ra, S, Q, ps);
+	steps.clear();
+	auto em = [&](uint32_t s) { steps.push_back(s); };
+	dp_traceback(lenS, lenQ, b, ps, em);
+	std::reverse(steps.begin(), steps.end());
+	return 0;
+}
+
+template <class SS, class QS>
+static int dispatch_dp(const Opts &o, int lenS, int lenQ, int b, bool la, bool ra, SS &S, QS &Q, std::vector<uint32_t> &steps) {
+	const bool aff = o.aligner == 0;
+#define HS_CASE(BM)                                                                                   \
+	if (b <= BM) return aff ? run_dp<BM, true>(o, lenS, lenQ, b, la, ra, S, Q, steps)                  \
+	                        : run_dp<BM, false>(o, lenS, lenQ, b, la, ra, S, Q, steps);
+	HS_CASE(8) HS_CASE(16) HS_CASE(32) HS_CASE(64) HS_CASE(128) HS_CASE(256)
+#undef HS_CASE
+	return -1;
+}
+
+extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads *sr, char *corrected, uint32_t *tally_out,
+		uint64_t *stats) {
+	const Opts o = to_opts(cfg);
+	const int k = o.k;
+	if (k < 1 || k > 64) return -1;
+	std::vector<uint8_t> qg(qr->n), sg(sr->n);
+	for (int64_t i = 0; i < qr->n; i++) qg[i] = query_good(qr->seq + qr->offset[i], qr->length[i], o.min_overlap);
+	for (int64_t i = 0; i < sr->n; i++) sg[i] = subject_active(sr->seq + sr->offset[i], sr->length[i], o.min_overlap, k);
+	PackedSet Q, S;
+	Q.build(qr, qg);
+	S.build(sr, sg);
+	// index: (hi, lo) -> entries (innerId, pos) in insertion order
+	std::map<std::pair<uint64_t, uint64_t>, std::vector<std::pair<int64_t, int>>> index;
+	for (int64_t q = 0; q < qr->n; q++) {
+		if (!qg[q]) continue;
+		const int L = qr->length[q];
+		for (int t = 0; t < num_keys(L, k); t++) {
+			uint64_t hi, lo;
+			const int p = key_pos(L, k, t);
+			kmer_fwd(Q.P.words + Q.P.woff[q], p, k, hi, lo);
+			index[{hi, lo}].push_back({q, p});
+		}
+	}
+	// tally per query (4 u32 per base at the query's base offset)
+	int64_t qbases = 0;
+	for (int64_t q = 0; q < qr->n; q++) qbases = std::max<int64_t>(qbases, qr->offset[q] + qr->length[q]);
+	std::vector<uint32_t> tally((size_t) 4 * qbases, 0);
+	uint64_t st_seeds = 0, st_tasks = 0, st_cells_exec = 0, st_calls_exec = 0, st_built = 0, st_kept = 0, st_cells_all = 0;
+	int oob = 0;
+	std::vector<uint32_t> steps;
+	for (int64_t s = 0; s < sr->n; s++) {
+		if (!sg[s]) continue;
+		const int Ls = sr->length[s];
+		struct StrandData {
+			std::vector<uint64_t> seeds;
+			std::vector<int> gstart; // group starts (+ end)
+			std::vector<int32_t> cid, cf, cl, clen;
+			std::vector<Group> groups;
+			std::vector<uint32_t> gtask0;
+		} sd[2];
+		std::vector<DpTask> tasks;
+		std::vector<uint32_t> words_nsteps;
+		uint32_t words_total = 0;
+		for (int strand = 0; strand < 2; strand++) {
+			StrandData &D = sd[strand];
+			for (int p = 0; p + k <= Ls; p++) {
+				uint64_t hi, lo;
+				if (strand == 0) kmer_fwd(S.P.words + S.P.woff[s], p, k, hi, lo);
+				else {
+					uint64_t fh, fl;
+					kmer_fwd(S.P.words + S.P.woff[s], Ls - p - k, k, fh, fl);
+					kmer_revcomp(k, fh, fl, hi, lo);
+				}
+				auto it = index.find({hi, lo});
+				if (it == index.end()) continue;
+				for (auto &e : it->second) {
+					if (Q.P.uid[e.first] == S.P.uid[s]) continue;
+					int pq = e.second, ps = p;
+					if (!o.call_consensus && strand == 1) ps = Ls - (ps + k), pq = qr->length[e.first] - (pq + k);
+					D.seeds.push_back(seed_pack((uint64_t) e.first, pq, ps));
+				}
+			}
+			std::sort(D.seeds.begin(), D.seeds.end());
+			st_seeds += D.seeds.size();
+			const int n = (int) D.seeds.size();
+			D.cid.assign(n, 0), D.cf.assign(n, 0), D.cl.assign(n, 0), D.clen.assign(n, 0);
+			for (int t = 0; t < n; t++)
+				if (t == 0 || seed_qid(D.seeds[t]) != seed_qid(D.seeds[t - 1])) D.gstart.push_back(t);
+			if (o.max_candidates > 0 && (int) D.gstart.size() > o.max_candidates) D.gstart.resize(o.max_candidates);
+			const int ng = (int) D.gstart.size();
+			for (int gi = 0; gi < ng; gi++) {
+				const int g0 = D.gstart[gi];
+				int g1 = g0 + 1;
+				while (g1 < n && seed_qid(D.seeds[g1]) == seed_qid(D.seeds[g0])) g1++;
+				Group g;
+				g.seeds = D.seeds.data() + g0;
+				g.cid = D.cid.data() + g0, g.cfirst = D.cf.data() + g0, g.clast = D.cl.data() + g0, g.clen = D.clen.data() + g0;
+				g.gn = g1 - g0;
+				chain(o, g);
+				const int64_t q = seed_qid(D.seeds[g0]);
+				PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], o.call_consensus && strand == 1, !o.call_consensus && strand == 1};
+				int nt, nw;
+				plan_count(o, g, pr, nt, nw);
+				const uint32_t t0 = (uint32_t) tasks.size();
+				tasks.resize(tasks.size() + nt);
+				plan_emit(o, g, pr, tasks.data(), t0, words_total);
+				words_total += (uint32_t) nw;
+				D.groups.push_back(g);
+				D.gtask0.push_back(t0);
+			}
+		}
+		// DP
+		std::vector<uint32_t> words(words_total + 1, 0), nsteps(tasks.size(), 0);
+		st_tasks += tasks.size();
+		for (size_t t = 0; t < tasks.size(); t++) {
+			const DpTask &T = tasks[t];
+			const bool sw = T.flags & kTaskSwapped;
+			Stream subj{&S.P, (int64_t) T.s_read, T.s0, T.s_len, 0, (T.flags & kTaskSubjRC) != 0};
+			Stream qry{&Q.P, (int64_t) T.q_read, T.q0, T.q_len, 0, (T.flags & kTaskQryRC) != 0};
+			Stream &AS = sw ? qry : subj;
+			Stream &AQ = sw ? subj : qry;
+			const int lenS = sw ? T.q_len : T.s_len, lenQ = sw ? T.s_len : T.q_len;
+			st_cells_all += band_cells(lenS, lenQ, T.band);
+			if (dispatch_dp(o, lenS, lenQ, T.band, T.flags & kTaskAlignLeft, T.flags & kTaskAlignRight, AS, AQ, steps) != 0)
+				return -5;
+			const int cap = T.s_len + T.q_len;
+			const int ns = (int) steps.size();
+			for (int e = 0; e < ns; e++) {
+				const int pos = cap - ns + e;
+				words[T.out_off + (pos >> 4)] |= steps[e] << (2 * (pos & 15));
+			}
+			nsteps[t] = (uint32_t) ns;
+		}
+		DpResults R{tasks.data(), words.data(), nsteps.data()};
+		// pass B: pair groups of both strands by query
+		std::map<int64_t, std::pair<int, int>> pairs;
+		for (int strand = 0; strand < 2; strand++)
+			for (size_t gi = 0; gi < sd[strand].groups.size(); gi++) {
+				const int64_t q = seed_qid(sd[strand].groups[gi].seeds[0]);
+				auto &pp = pairs.emplace(q, std::make_pair(-1, -1)).first->second;
+				(strand == 0 ? pp.first : pp.second) = (int) gi;
+			}
+		for (auto &kv : pairs) {
+			const int64_t q = kv.first;
+			Best best;
+			best.have = false;
+			for (int strand = 0; strand < 2; strand++) {
+				const int gi = strand == 0 ? kv.second.first : kv.second.second;
+				if (gi < 0) continue;
+				PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], o.call_consensus && strand == 1, !o.call_consensus && strand == 1};
+				score_group(o, sd[strand].groups[gi], pr, R, sd[strand].gtask0[gi], strand, best, st_cells_exec, st_calls_exec, st_built);
+			}
+			if (!best.have) continue;
+			st_kept++;
+			if (!o.call_consensus) continue;
+			PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], best.strand == 1, false};
+			const int gi = best.strand == 0 ? kv.second.first : kv.second.second;
+			VoteSink sink{tally.data() + 4 * qr->offset[q], pr, qr->length[q], &oob};
+			vote_record(o, sd[best.strand].groups[gi], pr, R, best.chain, best.task0, sink);
+		}
+	}
+	// consensus
+	std::vector<uint32_t> decoded((size_t) 4 * qbases, 0);
+	std::vector<uint8_t> codes;
+	if (corrected) std::memcpy(corrected, qr->seq, (size_t) qbases);
+	for (int64_t q = 0; q < qr->n; q++) {
+		const int L = qr->length[q];
+		codes.resize(L);
+		consensus_read(o, tally.data() + 4 * qr->offset[q], L, Q.P, q, codes.data(), decoded.data() + 4 * qr->offset[q]);
+		if (qg[q] && corrected)
+			for (int p = 0; p < L; p++) corrected[qr->offset[q] + p] = code_ascii(codes[p]);
+	}
+	if (tally_out) std::memcpy(tally_out, decoded.data(), decoded.size() * 4);
+	if (stats) {
+		stats[0] = st_seeds, stats[1] = st_tasks, stats[2] = st_calls_exec, stats[3] = st_cells_exec;
+		stats[4] = st_built, stats[5] = st_kept, stats[6] = st_cells_all, stats[7] = (uint64_t) oob;
+	}
+	return oob ? 1 : 0;
+}
+
+// One BandAligner call on ASCII strings (A/C/G/T/N), BandAligner CIGAR as text.
+struct ByteStream {
+	const char *s;
+	int len, i;
+	int next() {
+		if (i >= len) return kCodeNone;
+		const char c = s[i++];
+		return c == 'N' ? kCodeN : ascii_code(c);
+	}
+};
+extern "C" int hostsim_band_align(const hs_cfg *cfg, const char *subj, int ls, const char *qry, int lq, int band, int flag,
+		char *cigar_out, int cap, int *band_out) {
+	const Opts o = to_opts(cfg);
+	const bool sw = !(lq <= ls);
+	const char *A = sw ? qry : subj, *B = sw ? subj : qry;
+	const int lenS = sw ? lq : ls, lenQ = sw ? ls : lq;
+	if (lenS - lenQ > band) band = lenS - lenQ;
+	*band_out = band;
+	ByteStream SS{A, lenS, 0}, QS{B, lenQ, 0};
+	std::vector<uint32_t> steps;
+	if (dispatch_dp(o, lenS, lenQ, band, flag == 1, flag == 2, SS, QS, steps) != 0) return -1;
+	std::string out;
+	int i = 0, j = 0;
+	char cur = 0;
+	int cnt = 0;
+	auto push = [&](char t, bool mismatch) {
+		if (cnt > 0 && t == cur && !mismatch) {
+			cnt++;
+			return;
+		}
+		if (cnt > 0) out += std::to_string(cnt) + cur;
+		cur = t, cnt = 1;
+	};
+	for (uint32_t st : steps) {
+		if (st == kStepLeft) push(sw ? 'D' : 'I', false), j++;
+		else if (st == kStepUp) push(sw ? 'I' : 'D', false), i++;
+		else {
+			const char a = A[i], q = B[j];
+			if (a == q) push('M', false);
+			else push(sw ? a : q, true);
+			i++, j++;
+		}
+	}
+	if (cnt > 0) out += std::to_string(cnt) + cur;
+	if ((int) out.size() + 1 > cap) return -2;
+	std::memcpy(cigar_out, out.c_str(), out.size() + 1);
+	return 0;
+}

@@ -1,0 +1,49 @@
+"""ctypes binding of tests/native/libhero_hostsim.so (CPU run of the device logic)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhero_hostsim.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        from oracle.oracle import Cfg, Reads  # same layouts
+        L = ctypes.CDLL(LIB_PATH)
+        L.hostsim_run.restype = ctypes.c_int
+        L.hostsim_run.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(Reads), ctypes.POINTER(Reads), ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p]
+        L.hostsim_band_align.restype = ctypes.c_int
+        L.hostsim_band_align.argtypes = [ctypes.POINTER(Cfg), ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def run(cfg, queries, subjects=None):
+    from oracle.oracle import _reads
+    subjects = queries if subjects is None else subjects
+    corrected = np.empty_like(queries.seq)
+    tally = np.zeros(4 * len(queries.seq), dtype=np.uint32)
+    stats = np.zeros(8, dtype=np.uint64)
+    rc = lib().hostsim_run(ctypes.byref(cfg), _reads(queries), _reads(subjects), corrected.ctypes.data,
+                           tally.ctypes.data, stats.ctypes.data)
+    if rc < 0:
+        raise RuntimeError(f"hostsim_run failed: {rc}")
+    keys = ["seeds", "tasks", "dp_calls", "dp_cells", "records_built", "records_kept", "cells_all", "oob"]
+    return dict(corrected=corrected, tally=tally, stats=dict(zip(keys, map(int, stats))))
+
+
+def band_align(cfg, S, Q, band, flag=0):
+    buf = ctypes.create_string_buffer(4 * (len(S) + len(Q)) + 16)
+    b = ctypes.c_int(0)
+    rc = lib().hostsim_band_align(ctypes.byref(cfg), S.encode(), len(S), Q.encode(), len(Q), band, flag, buf,
+                                  len(buf), ctypes.byref(b))
+    if rc != 0:
+        raise RuntimeError(f"hostsim_band_align failed: {rc}")
+    return buf.value.decode(), b.value
