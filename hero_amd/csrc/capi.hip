@@ -1,0 +1,832 @@
+// C ABI of libhero_gpu: context, device residency of reads / index / tallies, and the per-chunk
+// orchestration of the kernels in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hero_gpu.h"
+#include "kernels.h"
+#include "pack.h"
+#include "pipeline_core.h"
+
+using namespace hero;
+
+namespace {
+
+// Smallest prime strictly above n (deterministic Miller-Rabin for 64-bit n). Role of
+// HashTableLongKey::getPrimeLargerThanNumber (src/HashTableLongKey.cpp:101-166): the table size
+// only places slots, which never reaches the output.
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t) (((__uint128_t) a * b) % m); }
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t m) {
+	uint64_t r = 1;
+	a %= m;
+	while (e) {
+		if (e & 1) r = mulmod(r, a, m);
+		a = mulmod(a, a, m);
+		e >>= 1;
+	}
+	return r;
+}
+bool is_prime(uint64_t n) {
+	static const uint64_t B[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+	if (n < 2) return false;
+	for (uint64_t b : B) {
+		if (n == b) return true;
+		if (n % b == 0) return false;
+	}
+	uint64_t d = n - 1;
+	int r = 0;
+	while (!(d & 1)) d >>= 1, r++;
+	for (uint64_t b : B) {
+		uint64_t x = powmod(b, d, n);
+		if (x == 1 || x == n - 1) continue;
+		bool comp = true;
+		for (int j = 1; j < r; j++) {
+			x = mulmod(x, x, n);
+			if (x == n - 1) {
+				comp = false;
+				break;
+			}
+		}
+		if (comp) return false;
+	}
+	return true;
+}
+uint64_t next_prime_above(uint64_t n) {
+	uint64_t p = n + 1;
+	while (!is_prime(p)) p++;
+	return p;
+}
+
+template <class T>
+struct DevBuf {
+	T *p = nullptr;
+	size_t n = 0;
+	void release() {
+		if (p) (void) hipFree(p);
+		p = nullptr, n = 0;
+	}
+	hipError_t ensure(size_t want) {
+		if (want <= n && p) return hipSuccess;
+		release();
+		size_t alloc = want + want / 4 + 64;
+		hipError_t e = hipMalloc((void **) &p, alloc * sizeof(T));
+		if (e != hipSuccess) {
+			p = nullptr;
+			return e;
+		}
+		n = alloc;
+		return hipSuccess;
+	}
+	hipError_t exact(size_t want) {
+		release();
+		hipError_t e = hipMalloc((void **) &p, (want ? want : 1) * sizeof(T));
+		if (e != hipSuccess) {
+			p = nullptr;
+			return e;
+		}
+		n = want;
+		return hipSuccess;
+	}
+};
+
+struct ReadsDev {
+	DevBuf<uint64_t> words, woff, boff;
+	DevBuf<int32_t> len;
+	DevBuf<int64_t> uid;
+	DevBuf<uint8_t> good;
+	int64_t n = 0;
+	void release() {
+		words.release(), woff.release(), boff.release(), len.release(), uid.release(), good.release();
+		n = 0;
+	}
+};
+
+struct Timer {
+	hipEvent_t a = nullptr, b = nullptr;
+};
+
+} // namespace
+
+struct hero_ctx {
+	hero_options opt;
+	Opts o;
+	hipStream_t st = nullptr;
+	std::string err;
+	int cu_count = 0;
+	// queries (host copy kept for the uncorrected reads of the output)
+	std::vector<char> q_seq;
+	std::vector<int64_t> q_off;
+	std::vector<int32_t> q_len;
+	std::vector<uint8_t> q_good, q_active_as_subject;
+	std::vector<int64_t> q_not_good;
+	bool q_contiguous = true; // caller offsets are the back-to-back prefix sums of the lengths
+	int64_t q_bases = 0;
+	DevBuf<uint8_t> cons_out;
+	ReadsDev Qd;
+	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
+	// subjects (separate stream)
+	ReadsDev Sd;
+	// index
+	DevBuf<Slot> slots;
+	DevBuf<uint64_t> key_lo, entries;
+	uint64_t P = 0;
+	uint32_t max_probe = 0;
+	bool index_built = false;
+	uint64_t distinct_keys = 0, total_keys = 0;
+	// tallies: 4 u32 per query base (own-base slot difference coded, see VoteSink)
+	DevBuf<uint32_t> tally;
+	// chunk work buffers
+	DevBuf<uint32_t> seg_n, seg_off, scan_tmp;
+	DevBuf<uint64_t> seeds;
+	DevBuf<int32_t> cid, cfirst, clast, clen;
+	DevBuf<uint8_t> gflag;
+	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
+	DevBuf<DpTask> tasks;
+	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
+	DevBuf<unsigned long long> dstats, dtotal;
+	DevBuf<uint32_t> counter;
+	// stats
+	hero_stats st_host;
+	Timer ev[8];
+};
+
+namespace {
+
+int fail(hero_ctx *c, int code, const char *fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	if (c) c->err = buf;
+	return code;
+}
+#define HIPCHK(c, expr)                                                                                     \
+	do {                                                                                                    \
+		hipError_t e_ = (expr);                                                                             \
+		if (e_ != hipSuccess) return fail((c), e_ == hipErrorOutOfMemory ? HERO_ERR_NOMEM : HERO_ERR_HIP, "%s: %s (%s:%d)", \
+				#expr, hipGetErrorString(e_), __FILE__, __LINE__);                                          \
+	} while (0)
+
+Opts make_opts(const hero_options &p) {
+	Opts o;
+	std::memset(&o, 0, sizeof o);
+	o.k = p.k, o.aligner = p.aligner, o.call_consensus = p.call_consensus;
+	o.band_override = p.band_override, o.max_candidates = p.max_candidates;
+	o.min_overlap = p.min_overlap;
+	o.mismatch_rate = p.mismatch_rate, o.indel_rate = p.indel_rate, o.consensus_freq = p.consensus_freq;
+	o.dx = p.dx, o.dy = p.dy, o.ex = p.ex, o.sx = p.sx, o.miss = p.miss, o.hit = p.hit, o.unobserved = p.unobserved;
+	return o;
+}
+
+Packed view(const ReadsDev &r, const uint8_t *good) {
+	Packed P;
+	P.words = r.words.p, P.woff = r.woff.p, P.len = r.len.p, P.uid = r.uid.p, P.good = good, P.n = r.n;
+	return P;
+}
+
+int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
+		const int64_t *uid, const std::vector<uint8_t> &good) {
+	std::vector<uint64_t> words, woff;
+	pack_reads(seq, off, len, n, words, woff);
+	std::vector<uint64_t> boff((size_t) n);
+	uint64_t acc = 0;
+	for (int64_t i = 0; i < n; i++) boff[(size_t) i] = acc, acc += (uint64_t) len[i];
+	d.release();
+	HIPCHK(c, d.words.exact(words.size()));
+	HIPCHK(c, d.woff.exact((size_t) n));
+	HIPCHK(c, d.boff.exact((size_t) n));
+	HIPCHK(c, d.len.exact((size_t) n));
+	HIPCHK(c, d.uid.exact((size_t) n));
+	HIPCHK(c, d.good.exact((size_t) n));
+	HIPCHK(c, hipMemcpyAsync(d.words.p, words.data(), words.size() * 8, hipMemcpyHostToDevice, c->st));
+	if (n) {
+		HIPCHK(c, hipMemcpyAsync(d.woff.p, woff.data(), (size_t) n * 8, hipMemcpyHostToDevice, c->st));
+		HIPCHK(c, hipMemcpyAsync(d.boff.p, boff.data(), (size_t) n * 8, hipMemcpyHostToDevice, c->st));
+		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, c->st));
+		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, c->st));
+		HIPCHK(c, hipMemcpyAsync(d.good.p, good.data(), (size_t) n, hipMemcpyHostToDevice, c->st));
+	}
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	d.n = n;
+	return HERO_OK;
+}
+
+int check_reads(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len, const int64_t *uid) {
+	if (n < 0 || (n > 0 && (!seq || !off || !len || !uid))) return fail(c, HERO_ERR_ARG, "null read arrays");
+	if (n >= (int64_t) 1 << 32) return fail(c, HERO_ERR_LIMIT, "at most 2^32-1 reads per set");
+	for (int64_t i = 0; i < n; i++)
+		if (len[i] < 0 || len[i] > kMaxReadLen) return fail(c, HERO_ERR_LIMIT, "read %lld has length %d (limit %d)", (long long) i, len[i], kMaxReadLen);
+	return HERO_OK;
+}
+
+double elapsed(hero_ctx *c, int a, int b) {
+	float ms = 0;
+	(void) hipEventElapsedTime(&ms, c->ev[a].a, c->ev[b].a);
+	return ms;
+}
+
+// One subject chunk [s0, s0 + nsub) of the packed subject set S.
+int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_needed) {
+	const Packed Q = view(c->Qd, c->Qd.good.p);
+	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, c->o.k};
+	ChunkDev cd;
+	std::memset(&cd, 0, sizeof cd);
+	cd.s_begin = s0, cd.nsub = nsub;
+	HIPCHK(c, c->seg_n.ensure((size_t) 2 * nsub));
+	HIPCHK(c, c->seg_off.ensure((size_t) 2 * nsub));
+	cd.seg_n = c->seg_n.p, cd.seg_off = c->seg_off.p;
+	unsigned long long *stats = c->dstats.p;
+	auto scan = [&](const uint32_t *in, uint32_t *out, int64_t n, uint64_t &total) -> int {
+		HIPCHK(c, c->scan_tmp.ensure(scan_tmp_words(n)));
+		HIPCHK(c, scan_u32(in, out, n, c->dtotal.p, c->scan_tmp.p, c->scan_tmp.n, c->st));
+		unsigned long long t = 0;
+		HIPCHK(c, hipMemcpyAsync(&t, c->dtotal.p, 8, hipMemcpyDeviceToHost, c->st));
+		HIPCHK(c, hipStreamSynchronize(c->st));
+		total = t;
+		return HERO_OK;
+	};
+	HIPCHK(c, hipEventRecord(c->ev[0].a, c->st));
+	HIPCHK(c, launch_seed_count(S, Q, ix, c->o, cd, stats, c->st));
+	uint64_t nseeds = 0;
+	if (int r = scan(cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
+	if (nseeds >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu seeds; lower chunk_subjects", nsub, (unsigned long long) nseeds);
+	const size_t ns = (size_t) nseeds;
+	HIPCHK(c, c->seeds.ensure(ns));
+	HIPCHK(c, c->cid.ensure(ns));
+	HIPCHK(c, c->cfirst.ensure(ns));
+	HIPCHK(c, c->clast.ensure(ns));
+	HIPCHK(c, c->clen.ensure(ns));
+	HIPCHK(c, c->gflag.ensure(ns));
+	HIPCHK(c, c->g_tasks.ensure(ns));
+	HIPCHK(c, c->g_task_off.ensure(ns));
+	HIPCHK(c, c->g_words.ensure(ns));
+	HIPCHK(c, c->g_word_off.ensure(ns));
+	cd.seeds = c->seeds.p, cd.cid = c->cid.p, cd.cfirst = c->cfirst.p, cd.clast = c->clast.p, cd.clen = c->clen.p;
+	cd.gflag = c->gflag.p, cd.g_tasks = c->g_tasks.p, cd.g_task_off = c->g_task_off.p;
+	cd.g_words = c->g_words.p, cd.g_word_off = c->g_word_off.p;
+	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, c->st));
+	HIPCHK(c, hipEventRecord(c->ev[1].a, c->st));
+	uint64_t ntasks = 0, nwords = 0;
+	if (int r = scan(cd.g_tasks, cd.g_task_off, (int64_t) ns, ntasks)) return r;
+	if (int r = scan(cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
+	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
+		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
+	HIPCHK(c, c->tasks.ensure((size_t) ntasks));
+	HIPCHK(c, c->words.ensure((size_t) nwords + 1));
+	HIPCHK(c, c->nsteps.ensure((size_t) ntasks));
+	cd.tasks = c->tasks.p, cd.words = c->words.p, cd.nsteps = c->nsteps.p;
+	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, c->st));
+	HIPCHK(c, hipEventRecord(c->ev[2].a, c->st));
+	// DP
+	DpLaunch L;
+	std::memset(&L, 0, sizeof L);
+	L.tasks = cd.tasks, L.ntasks = (uint32_t) ntasks, L.order = nullptr, L.S = S, L.Q = Q, L.o = c->o;
+	L.words = cd.words, L.nsteps = cd.nsteps, L.stats = stats, L.batch_counter = c->counter.p;
+	const int nwaves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 16;
+	const int bmax = 16, nw = (2 * bmax + 1 + 15) / 16;
+	const int rows = std::min(rows_needed, 512);
+	L.route_bmax = -1, L.route_rows = 0;
+	if (c->opt.dp_form == 0) {
+		HIPCHK(c, c->dp_scratch.ensure((size_t) nwaves * rows * nw * 64));
+		L.scratch = c->dp_scratch.p, L.rows_max = rows;
+		HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
+		HIPCHK(c, launch_dp_batched(L, bmax, nwaves, c->st));
+		L.route_bmax = bmax, L.route_rows = rows;
+	}
+	{
+		// plain form: all tasks, or (batched form) the tasks with band > 16 or rows beyond the scratch
+		const int bmax_plain = 64;
+		HIPCHK(c, c->plain_scratch.ensure((size_t) nwaves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+		L.scratch = c->plain_scratch.p, L.rows_max = 512;
+		HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
+		HIPCHK(c, launch_dp_plain(L, bmax_plain, c->opt.dp_form == 0 ? std::max(1, nwaves / 16) : nwaves, c->st));
+	}
+	HIPCHK(c, hipEventRecord(c->ev[3].a, c->st));
+	{
+		unsigned long long skipped = 0;
+		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, c->st));
+		HIPCHK(c, hipStreamSynchronize(c->st));
+		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
+	}
+	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, c->Qd.boff.p, c->tally.p, stats, c->st));
+	HIPCHK(c, hipEventRecord(c->ev[4].a, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	c->st_host.ms_seed += elapsed(c, 0, 1);
+	c->st_host.ms_plan += elapsed(c, 1, 2);
+	c->st_host.ms_dp += elapsed(c, 2, 3);
+	c->st_host.ms_score_vote += elapsed(c, 3, 4);
+	c->st_host.dp_tasks += ntasks;
+	c->st_host.subjects += (uint64_t) nsub;
+	return HERO_OK;
+}
+
+int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len) {
+	if (!c->index_built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
+	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
+	for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+		const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
+		if (int r = run_chunk(c, S, s0, m, max_len + 1)) return r;
+	}
+	return HERO_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void hero_default_options(hero_options *p) {
+	std::memset(p, 0, sizeof *p);
+	p->k = 39, p->aligner = 1, p->call_consensus = 0, p->hash_factor = 32, p->min_overlap = 40;
+	p->mismatch_rate = 1.0, p->indel_rate = 1.0, p->consensus_freq = 0.6;
+	p->dx = -4, p->dy = -4, p->ex = -1, p->sx = 0, p->miss = -4, p->hit = 1, p->unobserved = -1;
+	p->device = 0, p->dp_form = 0, p->chunk_subjects = 0, p->dp_waves = 0;
+}
+
+int hero_create(const hero_options *opt, hero_ctx **out) {
+	if (!opt || !out) return HERO_ERR_ARG;
+	*out = nullptr;
+	if (opt->k < 1 || opt->k > 64) return HERO_ERR_ARG;
+	if (opt->aligner != 0 && opt->aligner != 1) return HERO_ERR_ARG;
+	if (opt->min_overlap < 0 || opt->min_overlap > 65535) return HERO_ERR_ARG;
+	if (opt->hash_factor < 1) return HERO_ERR_ARG;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt->device || opt->device < 0) return HERO_ERR_HIP;
+	if (hipSetDevice(opt->device) != hipSuccess) return HERO_ERR_HIP;
+	hero_ctx *c = new hero_ctx();
+	c->opt = *opt;
+	c->o = make_opts(*opt);
+	std::memset(&c->st_host, 0, sizeof c->st_host);
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, opt->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+		delete c;
+		return HERO_ERR_HIP;
+	}
+	c->cu_count = prop.multiProcessorCount;
+	for (auto &t : c->ev)
+		if (hipEventCreate(&t.a) != hipSuccess) {
+			delete c;
+			return HERO_ERR_HIP;
+		}
+	if (c->dstats.exact(kStCount) != hipSuccess || c->dtotal.exact(1) != hipSuccess || c->counter.exact(1) != hipSuccess ||
+			hipMemset(c->dstats.p, 0, kStCount * 8) != hipSuccess) {
+		delete c;
+		return HERO_ERR_NOMEM;
+	}
+	*out = c;
+	return HERO_OK;
+}
+
+void hero_destroy(hero_ctx *c) {
+	if (!c) return;
+	(void) hipSetDevice(c->opt.device);
+	(void) hipStreamSynchronize(c->st);
+	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
+	c->slots.release(), c->key_lo.release(), c->entries.release(), c->tally.release();
+	c->seg_n.release(), c->seg_off.release(), c->scan_tmp.release(), c->seeds.release();
+	c->cid.release(), c->cfirst.release(), c->clast.release(), c->clen.release(), c->gflag.release();
+	c->g_tasks.release(), c->g_task_off.release(), c->g_words.release(), c->g_word_off.release();
+	c->tasks.release(), c->words.release(), c->nsteps.release(), c->dp_scratch.release(), c->plain_scratch.release();
+	c->dstats.release(), c->dtotal.release(), c->counter.release(), c->cons_out.release();
+	for (auto &t : c->ev)
+		if (t.a) (void) hipEventDestroy(t.a);
+	if (c->st) (void) hipStreamDestroy(c->st);
+	delete c;
+}
+
+const char *hero_last_error(const hero_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len, const int64_t *uid) {
+	if (!c) return HERO_ERR_ARG;
+	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	c->q_off.assign(off, off + n);
+	c->q_len.assign(len, len + n);
+	int64_t bases = 0;
+	for (int64_t i = 0; i < n; i++) bases = std::max<int64_t>(bases, off[i] + len[i]);
+	c->q_seq.assign(seq, seq + bases);
+	c->q_good.resize((size_t) n);
+	c->q_active_as_subject.resize((size_t) n);
+	c->q_not_good.clear();
+	c->q_contiguous = true;
+	int64_t expect = 0;
+	for (int64_t i = 0; i < n; i++) {
+		c->q_good[(size_t) i] = query_good(seq + off[i], len[i], c->o.min_overlap);
+		c->q_active_as_subject[(size_t) i] = subject_active(seq + off[i], len[i], c->o.min_overlap, c->o.k);
+		if (!c->q_good[(size_t) i]) c->q_not_good.push_back(i);
+		if (off[i] != expect) c->q_contiguous = false;
+		expect = off[i] + len[i];
+	}
+	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good)) return r;
+	HIPCHK(c, c->q_subject_good.exact((size_t) n));
+	if (n) HIPCHK(c, hipMemcpy(c->q_subject_good.p, c->q_active_as_subject.data(), (size_t) n, hipMemcpyHostToDevice));
+	int64_t qb = 0;
+	for (int64_t i = 0; i < n; i++) qb += len[i];
+	c->q_bases = qb;
+	HIPCHK(c, c->tally.exact((size_t) 4 * qb));
+	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 16 * qb));
+	c->index_built = false;
+	c->st_host.queries = (uint64_t) n;
+	return HERO_OK;
+}
+
+int hero_build_index(hero_ctx *c) {
+	if (!c) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	const int k = c->o.k;
+	const int64_t n = c->Qd.n;
+	std::vector<uint64_t> koff((size_t) n + 1, 0);
+	uint64_t nocc = 0;
+	for (int64_t i = 0; i < n; i++) {
+		koff[(size_t) i] = nocc;
+		if (c->q_good[(size_t) i]) nocc += (uint64_t) num_keys(c->q_len[(size_t) i], k);
+	}
+	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu query keys exceed 2^32-1", (unsigned long long) nocc);
+	c->P = next_prime_above((uint64_t) n * (uint64_t) c->opt.hash_factor);
+	if (c->P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) c->P);
+	if (nocc >= c->P) return fail(c, HERO_ERR_REF_ABORT, "hash table too small (%llu keys, %llu slots); the reference exits here",
+			(unsigned long long) nocc, (unsigned long long) c->P);
+	hipEvent_t e0 = c->ev[5].a, e1 = c->ev[6].a;
+	HIPCHK(c, hipEventRecord(e0, c->st));
+	// a new index starts a new correction run: clear the vote tallies
+	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 16 * c->q_bases, c->st));
+	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
+	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
+	DevBuf<int> errf;
+	DevBuf<unsigned long long> cursor;
+	auto cleanup = [&]() {
+		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
+		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
+	};
+	int rc = HERO_OK;
+	do {
+#define IXCHK(expr)                                                                                                 \
+	{                                                                                                               \
+		hipError_t e_ = (expr);                                                                                     \
+		if (e_ != hipSuccess) {                                                                                     \
+			rc = fail(c, e_ == hipErrorOutOfMemory ? HERO_ERR_NOMEM : HERO_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+			break;                                                                                                  \
+		}                                                                                                           \
+	}
+		IXCHK(dkoff.exact((size_t) n + 1));
+		IXCHK(occ_hi.exact(nocc));
+		IXCHK(occ_lo.exact(nocc));
+		IXCHK(occ_entry.exact(nocc));
+		IXCHK(occ_slot.exact(nocc));
+		IXCHK(occ_rank.exact(nocc));
+		IXCHK(owner.exact(c->P));
+		IXCHK(maxp.exact(1));
+		IXCHK(errf.exact(1));
+		IXCHK(cursor.exact(1));
+		IXCHK(c->slots.exact(c->P));
+		IXCHK(c->key_lo.exact(k > 32 ? c->P : 1));
+		IXCHK(c->entries.exact(nocc));
+		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->st));
+		IXCHK(hipMemsetAsync(owner.p, 0, c->P * 4, c->st));
+		IXCHK(hipMemsetAsync(c->slots.p, 0, c->P * sizeof(Slot), c->st));
+		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->st));
+		IXCHK(hipMemsetAsync(errf.p, 0, 4, c->st));
+		IXCHK(hipMemsetAsync(cursor.p, 0, 8, c->st));
+		const Packed Q = view(c->Qd, c->Qd.good.p);
+		IXCHK(launch_index_gen(Q, dkoff.p, k, occ_hi.p, occ_lo.p, occ_entry.p, c->st));
+		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, c->P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->st));
+		IXCHK(launch_index_count(occ_slot.p, nocc, c->slots.p, occ_rank.p, c->st));
+		IXCHK(launch_index_alloc(owner.p, c->P, occ_hi.p, occ_lo.p, k, c->slots.p, c->key_lo.p, cursor.p, c->st));
+		IXCHK(launch_index_fill(occ_slot.p, occ_rank.p, occ_entry.p, nocc, c->slots.p, c->entries.p, c->st));
+		IXCHK(launch_index_sort_buckets(c->slots.p, c->P, c->entries.p, c->st));
+		IXCHK(hipEventRecord(e1, c->st));
+		int herr = 0;
+		unsigned long long used = 0;
+		IXCHK(hipMemcpyAsync(&c->max_probe, maxp.p, 4, hipMemcpyDeviceToHost, c->st));
+		IXCHK(hipMemcpyAsync(&herr, errf.p, 4, hipMemcpyDeviceToHost, c->st));
+		IXCHK(hipMemcpyAsync(&used, cursor.p, 8, hipMemcpyDeviceToHost, c->st));
+		IXCHK(hipStreamSynchronize(c->st));
+		if (herr) {
+			rc = fail(c, HERO_ERR_REF_ABORT, "hash table full");
+			break;
+		}
+		if (used != nocc) {
+			rc = fail(c, HERO_ERR_HIP, "index build placed %llu of %llu keys", used, (unsigned long long) nocc);
+			break;
+		}
+		// distinct keys = claimed slots
+		std::vector<uint32_t> dummy;
+		c->total_keys = nocc;
+		float ms = 0;
+		(void) hipEventElapsedTime(&ms, e0, e1);
+		c->st_host.ms_index += ms;
+#undef IXCHK
+	} while (0);
+	cleanup();
+	if (rc != HERO_OK) return rc;
+	c->index_built = true;
+	c->st_host.table_size = c->P;
+	c->st_host.total_keys = c->total_keys;
+	c->st_host.max_probe = c->max_probe;
+	return HERO_OK;
+}
+
+int hero_align_subjects(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len, const int64_t *uid) {
+	if (!c) return HERO_ERR_ARG;
+	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	std::vector<uint8_t> act((size_t) n);
+	for (int64_t i = 0; i < n; i++) act[(size_t) i] = subject_active(seq + off[i], len[i], c->o.min_overlap, c->o.k);
+	if (int r = upload_reads(c, c->Sd, n, seq, off, len, uid, act)) return r;
+	int ml = 0;
+	for (int64_t i = 0; i < n; i++) ml = std::max(ml, len[i]);
+	for (int32_t l : c->q_len) ml = std::max(ml, l);
+	return align_packed(c, view(c->Sd, c->Sd.good.p), n, ml);
+}
+
+int hero_align_self(hero_ctx *c) {
+	if (!c) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	int ml = 0;
+	for (int32_t l : c->q_len) ml = std::max(ml, l);
+	return align_packed(c, view(c->Qd, c->q_subject_good.p), c->Qd.n, ml);
+}
+
+int hero_call_consensus(hero_ctx *c, char *corrected) {
+	if (!c || !corrected) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	const int64_t n = c->Qd.n;
+	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
+	HIPCHK(c, hipEventRecord(c->ev[5].a, c->st));
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, n, c->cons_out.p, nullptr,
+			c->dstats.p + kStReadsChanged, c->st));
+	HIPCHK(c, hipEventRecord(c->ev[6].a, c->st));
+	// The device writes corrected bases at packed base offsets; the caller's layout usually is that
+	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
+	if (c->q_contiguous) {
+		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->st));
+		HIPCHK(c, hipStreamSynchronize(c->st));
+		for (int64_t i : c->q_not_good)
+			std::memcpy(corrected + c->q_off[(size_t) i], c->q_seq.data() + c->q_off[(size_t) i], (size_t) c->q_len[(size_t) i]);
+	} else {
+		std::vector<char> h((size_t) c->q_bases);
+		HIPCHK(c, hipMemcpyAsync(h.data(), c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->st));
+		HIPCHK(c, hipStreamSynchronize(c->st));
+		std::memcpy(corrected, c->q_seq.data(), c->q_seq.size());
+		uint64_t b = 0;
+		for (int64_t i = 0; i < n; i++) {
+			const int L = c->q_len[(size_t) i];
+			if (c->q_good[(size_t) i]) std::memcpy(corrected + c->q_off[(size_t) i], h.data() + b, (size_t) L);
+			b += (uint64_t) L;
+		}
+	}
+	float ms = 0;
+	(void) hipEventElapsedTime(&ms, c->ev[5].a, c->ev[6].a);
+	c->st_host.ms_consensus += ms;
+	return HERO_OK;
+}
+
+int hero_get_tally(hero_ctx *c, uint32_t *tally) {
+	if (!c || !tally) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	DevBuf<uint8_t> codes;
+	DevBuf<uint32_t> dec;
+	HIPCHK(c, codes.exact((size_t) c->q_bases));
+	HIPCHK(c, dec.exact((size_t) 4 * c->q_bases));
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, c->Qd.n, codes.p, dec.p,
+			nullptr, c->st));
+	std::vector<uint32_t> h((size_t) 4 * c->q_bases);
+	HIPCHK(c, hipMemcpyAsync(h.data(), dec.p, h.size() * 4, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	// device layout is by base offset (reads packed back to back); map to the caller's offsets
+	uint64_t b = 0;
+	for (int64_t i = 0; i < c->Qd.n; i++) {
+		const int L = c->q_len[(size_t) i];
+		std::memcpy(tally + 4 * c->q_off[(size_t) i], h.data() + 4 * b, (size_t) L * 16);
+		b += (uint64_t) L;
+	}
+	return HERO_OK;
+}
+
+int hero_get_stats(hero_ctx *c, hero_stats *out) {
+	if (!c || !out) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	unsigned long long s[kStCount];
+	HIPCHK(c, hipMemcpy(s, c->dstats.p, sizeof s, hipMemcpyDeviceToHost));
+	*out = c->st_host;
+	out->lookups = s[kStLookups];
+	out->seeds = s[kStSeeds];
+	out->dp_calls_exec = s[kStCallsExec];
+	out->dp_cells_exec = s[kStCellsExec];
+	out->records_built = s[kStBuilt];
+	out->records_kept = s[kStKept];
+	out->vote_oob = s[kStOob];
+	out->dp_skipped = s[kStDpSkipped];
+	out->reads_changed = s[kStReadsChanged];
+	out->ms_total = out->ms_index + out->ms_seed + out->ms_plan + out->ms_dp + out->ms_score_vote + out->ms_consensus;
+	return HERO_OK;
+}
+
+int hero_reset_stats(hero_ctx *c) {
+	if (!c) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	HIPCHK(c, hipMemset(c->dstats.p, 0, kStCount * 8));
+	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
+	std::memset(&c->st_host, 0, sizeof c->st_host);
+	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;
+	return HERO_OK;
+}
+
+int hero_band_align_batch(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const char *seq, int32_t *band_out,
+		int64_t *cigar_off, int32_t *cigar_n, int32_t *cigar_counts, char *cigar_types, int64_t cigar_cap) {
+	if (!c || n < 0 || (n > 0 && (!pairs || !seq || !band_out || !cigar_off || !cigar_n))) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (n == 0) return HERO_OK;
+	// setSequences: the longer side is the aligner's subject (query side keeps it on ties)
+	std::vector<BytePair> bp((size_t) n);
+	int64_t seq_end = 0;
+	uint32_t words = 0;
+	int bmax = 0, rows = 0;
+	for (int i = 0; i < n; i++) {
+		const hero_band_pair &p = pairs[i];
+		if (p.s_len < 0 || p.q_len < 0 || p.s_off < 0 || p.q_off < 0) return fail(c, HERO_ERR_REF_ABORT, "pair %d: negative start (the reference exits)", i);
+		if (p.s_len == 0 && p.q_len == 0) return fail(c, HERO_ERR_ARG, "pair %d: both sequences empty", i);
+		seq_end = std::max<int64_t>(seq_end, std::max(p.s_off + p.s_len, p.q_off + p.q_len));
+		const bool sw = !(p.q_len <= p.s_len);
+		BytePair &b = bp[(size_t) i];
+		b.s_off = (uint64_t) (sw ? p.q_off : p.s_off), b.s_len = sw ? p.q_len : p.s_len;
+		b.q_off = (uint64_t) (sw ? p.s_off : p.q_off), b.q_len = sw ? p.s_len : p.q_len;
+		b.band = std::max(p.band, b.s_len - b.q_len);
+		b.flags = p.flag == 1 ? 1 : (p.flag == 2 ? 2 : 0);
+		b.out_off = words;
+		b._r = sw;
+		words += (uint32_t) edit_words(b.s_len, b.q_len);
+		bmax = std::max(bmax, b.band);
+		rows = std::max(rows, b.s_len + 1);
+		band_out[i] = b.band;
+	}
+	for (int64_t i = 0; i < seq_end; i++) {
+		const char ch = seq[i];
+		if (ch != 'A' && ch != 'C' && ch != 'G' && ch != 'T' && ch != 'N')
+			return fail(c, HERO_ERR_ARG, "sequence byte %lld is '%c'; only A/C/G/T/N are accepted", (long long) i, ch);
+	}
+	if (bmax > 64) return fail(c, HERO_ERR_LIMIT, "band %d above 64", bmax);
+	DevBuf<BytePair> dpairs;
+	DevBuf<char> dseq;
+	DevBuf<uint32_t> dwords, dns, dscr;
+	const int bsel = bmax <= 8 ? 8 : bmax <= 16 ? 16 : bmax <= 32 ? 32 : 64;
+	const int nw = (2 * bsel + 1 + 15) / 16;
+	HIPCHK(c, dpairs.exact((size_t) n));
+	HIPCHK(c, dseq.exact((size_t) seq_end + 1));
+	HIPCHK(c, dwords.exact((size_t) words + 1));
+	HIPCHK(c, dns.exact((size_t) n));
+	HIPCHK(c, dscr.exact((size_t) ((n + 63) / 64) * rows * nw * 64));
+	HIPCHK(c, hipMemcpyAsync(dpairs.p, bp.data(), bp.size() * sizeof(BytePair), hipMemcpyHostToDevice, c->st));
+	HIPCHK(c, hipMemcpyAsync(dseq.p, seq, (size_t) seq_end, hipMemcpyHostToDevice, c->st));
+	HIPCHK(c, launch_band_align_bytes(dpairs.p, n, dseq.p, c->o, dwords.p, dns.p, dscr.p, rows, bsel, c->st));
+	std::vector<uint32_t> hw((size_t) words + 1), hn((size_t) n);
+	HIPCHK(c, hipMemcpyAsync(hw.data(), dwords.p, hw.size() * 4, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipMemcpyAsync(hn.data(), dns.p, hn.size() * 4, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	// BandAligner::calculateCigarAndPosition[Reversed] (src/BandAligner.cpp:258-419)
+	int64_t pos = 0;
+	for (int i = 0; i < n; i++) {
+		const BytePair &b = bp[(size_t) i];
+		const bool sw = b._r != 0;
+		const char *A = seq + b.s_off, *B = seq + b.q_off;
+		const int cap = b.s_len + b.q_len, ns = (int) hn[(size_t) i];
+		cigar_off[i] = pos;
+		int nops = 0;
+		char cur = 0;
+		int cnt = 0;
+		int ii = 0, jj = 0;
+		auto push = [&](char t, bool mismatch) -> bool {
+			if (cnt > 0 && t == cur && !mismatch) {
+				cnt++;
+				return true;
+			}
+			if (cnt > 0) {
+				if (pos + nops >= cigar_cap) return false;
+				if (cigar_counts) cigar_counts[pos + nops] = cnt;
+				if (cigar_types) cigar_types[pos + nops] = cur;
+				nops++;
+			}
+			cur = t, cnt = 1;
+			return true;
+		};
+		bool ok = true;
+		for (int e = cap - ns; e < cap && ok; e++) {
+			const uint32_t st = (hw[b.out_off + (e >> 4)] >> (2 * (e & 15))) & 3u;
+			if (st == kStepLeft) ok = push(sw ? 'D' : 'I', false), jj++;
+			else if (st == kStepUp) ok = push(sw ? 'I' : 'D', false), ii++;
+			else {
+				const char a = A[ii], q = B[jj];
+				ok = a == q ? push('M', false) : push(sw ? a : q, true);
+				ii++, jj++;
+			}
+		}
+		if (ok && cnt > 0) {
+			if (pos + nops >= cigar_cap) ok = false;
+			else {
+				if (cigar_counts) cigar_counts[pos + nops] = cnt;
+				if (cigar_types) cigar_types[pos + nops] = cur;
+				nops++;
+			}
+		}
+		if (!ok) return fail(c, HERO_ERR_LIMIT, "cigar_cap %lld too small", (long long) cigar_cap);
+		cigar_n[i] = nops;
+		pos += nops;
+	}
+	return HERO_OK;
+}
+
+} // extern "C"
+
+// ---- small probes over the index ----------------------------------------------------------------
+namespace {
+__global__ void k_hash_kmers(const char *kmers, int n, int k, uint64_t P, uint64_t *out) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	uint64_t hi = 0, lo = 0;
+	for (int t = 0; t < k; t++) {
+		const uint64_t c = (uint64_t) ascii_code(kmers[(size_t) i * k + t]);
+		if (t < 32) hi = (hi << 2) | c;
+		else lo = (lo << 2) | c;
+	}
+	out[i] = hash_kmer(k, hi, lo, P);
+}
+__global__ void k_lookup(DevIndex ix, const char *win, int n, int64_t *count, uint64_t *first_entry_idx) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int k = ix.k;
+	uint64_t hi = 0, lo = 0;
+	for (int t = 0; t < k; t++) {
+		const uint64_t c = (uint64_t) ascii_code(win[(size_t) i * k + t]);
+		if (t < 32) hi = (hi << 2) | c;
+		else lo = (lo << 2) | c;
+	}
+	uint64_t h = hash_kmer(k, hi, lo, ix.P);
+	count[i] = 0;
+	first_entry_idx[i] = 0;
+	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
+		const Slot s = ix.slots[h];
+		if (s.count == 0) return;
+		if (s.key == hi && (k <= 32 || ix.key_lo[h] == lo)) {
+			count[i] = s.count;
+			first_entry_idx[i] = s.start;
+			return;
+		}
+		h = (h + 1 == ix.P) ? 0 : h + 1;
+	}
+}
+} // namespace
+
+extern "C" int hero_hash_kmers(hero_ctx *c, int32_t n, int32_t k, const char *kmers, uint64_t P, uint64_t *out) {
+	if (!c || n < 0 || k < 1 || k > 64 || P == 0 || (n > 0 && (!kmers || !out))) return HERO_ERR_ARG;
+	if (n == 0) return HERO_OK;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	DevBuf<char> dk;
+	DevBuf<uint64_t> dout;
+	HIPCHK(c, dk.exact((size_t) n * k));
+	HIPCHK(c, dout.exact((size_t) n));
+	HIPCHK(c, hipMemcpyAsync(dk.p, kmers, (size_t) n * k, hipMemcpyHostToDevice, c->st));
+	k_hash_kmers<<<(n + 255) / 256, 256, 0, c->st>>>(dk.p, n, k, P, dout.p);
+	HIPCHK(c, hipGetLastError());
+	HIPCHK(c, hipMemcpyAsync(out, dout.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	return HERO_OK;
+}
+
+extern "C" int hero_lookup(hero_ctx *c, int32_t n, const char *windows, int64_t *count, int64_t *entry_off, uint64_t *entries,
+		int64_t entry_cap) {
+	if (!c || n < 0 || (n > 0 && (!windows || !count || !entry_off))) return HERO_ERR_ARG;
+	if (!c->index_built) return fail(c, HERO_ERR_STATE, "index not built");
+	if (n == 0) return HERO_OK;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	const int k = c->o.k;
+	DevBuf<char> dw;
+	DevBuf<int64_t> dcount;
+	DevBuf<uint64_t> dfirst;
+	HIPCHK(c, dw.exact((size_t) n * k));
+	HIPCHK(c, dcount.exact((size_t) n));
+	HIPCHK(c, dfirst.exact((size_t) n));
+	HIPCHK(c, hipMemcpyAsync(dw.p, windows, (size_t) n * k, hipMemcpyHostToDevice, c->st));
+	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, k};
+	k_lookup<<<(n + 255) / 256, 256, 0, c->st>>>(ix, dw.p, n, dcount.p, dfirst.p);
+	HIPCHK(c, hipGetLastError());
+	std::vector<uint64_t> first((size_t) n);
+	HIPCHK(c, hipMemcpyAsync(count, dcount.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipMemcpyAsync(first.data(), dfirst.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	int64_t pos = 0;
+	for (int i = 0; i < n; i++) {
+		entry_off[i] = pos;
+		if (count[i] > 0 && entries) {
+			if (pos + count[i] > entry_cap) return fail(c, HERO_ERR_LIMIT, "entry_cap too small");
+			HIPCHK(c, hipMemcpy(entries + pos, c->entries.p + first[(size_t) i], (size_t) count[i] * 8, hipMemcpyDeviceToHost));
+		}
+		pos += count[i];
+	}
+	return HERO_OK;
+}

@@ -25,6 +25,55 @@ HD int diag_score_global(const Opts &o, int a, int q) {
 	return a != q ? ((a == kCodeN || q == kCodeN) ? o.unobserved : o.miss) : o.hit;
 }
 
+// One cell of the recurrence. d0 / dQ flag the cell as column 0 / column lenQ; *_diag, *_up,
+// *_left are the neighbours (kSent when absent). Writes the cell's values and pointer code
+// (the reference's BM1: 2 diagonal, 1 left, 0 up).
+template <bool AFFINE>
+HD void cell_update(const Opts &o, int sc, int qc, bool col0, bool colQ, bool left_al, bool right_al, int lenQ,
+		int h_diag, int x_diag, int y_diag, int h_up, int x_up, int h_left, int y_left, int &h, int &x, int &y, int &p) {
+	if (!AFFINE) { // src/BandAligner.cpp:63-111
+		int g = o.dx;
+		if (col0) g = right_al ? o.sx : ((lenQ == 0 && left_al) ? o.sx : o.dx);
+		else if (colQ) g = left_al ? o.sx : o.dx;
+		const int up = h_up + g;
+		const int lf = h_left + o.dy;
+		const int dg = h_diag + diag_score_global(o, sc, qc);
+		if (dg >= lf && dg >= up) h = dg, p = 2;
+		else if (lf >= up) h = lf, p = 1;
+		else h = up, p = 0;
+		x = kSent, y = kSent;
+	} else if (col0) { // column 0 (src/BandAligner.cpp:150-163)
+		if (right_al) x = h_up + o.sx, y = o.dy;
+		else x = x_up + o.ex, y = x;
+		h = x;
+		p = 0;
+	} else { // src/BandAligner.cpp:176-222
+		y = imax_(h_left + o.dy, y_left + o.ex);
+		if (left_al && colQ) x = imax_(h_up + o.sx, x_up + o.sx);
+		else x = imax_(h_up + o.dx, x_up + o.ex);
+		const int s = sc != qc ? o.miss : o.hit;
+		h = imax_(imax_(h_diag + s, x_diag + s), y_diag + s);
+		p = (h > x && h >= y) ? 2 : (y > x ? 1 : 0);
+	}
+}
+
+// Row 0 of the matrix (global: src/BandAligner.cpp:63-111 at i = 0; affine: :140-171).
+// Returns whether cell j of row 0 exists and its values; its pointer is 1 (left) for j >= 1.
+template <bool AFFINE>
+HD bool row0_cell(const Opts &o, int j, int b, int lenQ, int &h, int &x, int &y) {
+	const bool in = j >= 0 && j <= b && j <= lenQ;
+	if (AFFINE) {
+		const int v = j == 0 ? 0 : (o.dx - o.ex) + j * o.ex;
+		h = in ? v : kSent;
+		x = in ? (j == 0 ? o.dx - o.ex : v) : kSent;
+		y = x;
+	} else {
+		h = in ? o.dy * j : kSent;
+		x = kSent, y = kSent;
+	}
+	return in;
+}
+
 // Forward pass. S.next() / Q.next() yield the aligner subject / query codes in order and
 // kCodeNone past the end. ptrs.put_row(i, words) receives row i's pointers, 16 cells per word,
 // cell d at bits 2*(d%16); codes are the reference's BM1 values: 2 diagonal, 1 left, 0 up.
@@ -42,17 +91,11 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 		for (int w = 0; w < NW; w++) pw[w] = 0;
 #pragma unroll
 		for (int d = 0; d <= W; d++) {
-			const int j = d - b;
-			const bool in = d < W && d <= 2 * b && j >= 0 && j <= lenQ;
-			if (AFFINE) {
-				const int v = j == 0 ? 0 : (o.dx - o.ex) + j * o.ex;
-				H[d] = in ? v : kSent;
-				X[d] = in ? (j == 0 ? o.dx - o.ex : v) : kSent;
-				Y[d] = X[d];
-			} else {
-				H[d] = in ? o.dy * j : kSent;
-			}
-			if (d < W && in && j >= 1) pw[d >> 4] |= 1u << (2 * (d & 15));
+			int h, x, y;
+			const bool in = row0_cell<AFFINE>(o, d - b, b, lenQ, h, x, y) && d < W;
+			H[d] = in ? h : kSent;
+			if (AFFINE) X[d] = in ? x : kSent, Y[d] = in ? y : kSent;
+			if (d < W && in && d - b >= 1) pw[d >> 4] |= 1u << (2 * (d & 15));
 		}
 		ptrs.put_row(0, pw);
 	}
@@ -71,44 +114,16 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 #pragma unroll
 		for (int d = 0; d < W; d++) {
 			const bool valid = d >= dlo && d <= dhi;
-			const int qc = qw[d];
-			int h, p;
-			if (!AFFINE) {
-				int g = o.dx;
-				if (d == d0) g = right_al ? o.sx : ((lenQ == 0 && left_al) ? o.sx : o.dx);
-				else if (d == dQ) g = left_al ? o.sx : o.dx;
-				const int up = H[d + 1] + g;
-				const int lf = hl + o.dy;
-				const int dg = H[d] + diag_score_global(o, sc, qc);
-				if (dg >= lf && dg >= up) h = dg, p = 2;
-				else if (lf >= up) h = lf, p = 1;
-				else h = up, p = 0;
-				h = valid ? h : kSent;
-				H[d] = h;
-				hl = h;
-			} else {
-				int x, y;
-				if (d == d0) { // column 0 (src/BandAligner.cpp:150-163)
-					if (right_al) x = H[d + 1] + o.sx, y = o.dy;
-					else x = X[d + 1] + o.ex, y = x;
-					h = x;
-					p = 0;
-				} else { // src/BandAligner.cpp:176-222
-					y = imax_(hl + o.dy, yl + o.ex);
-					if (left_al && d == dQ) x = imax_(H[d + 1] + o.sx, X[d + 1] + o.sx);
-					else x = imax_(H[d + 1] + o.dx, X[d + 1] + o.ex);
-					const int s = sc != qc ? o.miss : o.hit;
-					h = imax_(imax_(H[d] + s, X[d] + s), Y[d] + s);
-					p = (h > x && h >= y) ? 2 : (y > x ? 1 : 0);
-				}
-				h = valid ? h : kSent;
-				x = valid ? x : kSent;
-				y = valid ? y : kSent;
-				H[d] = h;
-				X[d] = x;
-				Y[d] = y;
-				hl = h;
-				yl = y;
+			int h, x, y, p;
+			cell_update<AFFINE>(o, sc, qw[d], d == d0, d == dQ, left_al, right_al, lenQ, H[d], AFFINE ? X[d] : kSent,
+					AFFINE ? Y[d] : kSent, H[d + 1], AFFINE ? X[d + 1] : kSent, hl, yl, h, x, y, p);
+			h = valid ? h : kSent;
+			H[d] = h;
+			hl = h;
+			if (AFFINE) {
+				X[d] = valid ? x : kSent;
+				Y[d] = valid ? y : kSent;
+				yl = Y[d];
 			}
 			pw[d >> 4] |= (valid ? (uint32_t) p : 0u) << (2 * (d & 15));
 		}
@@ -122,12 +137,14 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 
 // Traceback (src/BandAligner.cpp:114-125,227-238): from (lenS, lenQ) follow the pointers to the
 // origin. Steps come last-to-first through emit(step): kStepMatch for a diagonal (the caller
-// tells match from mismatch), kStepLeft (j-1), kStepUp (i-1). Returns the step count.
+// tells match from mismatch), kStepLeft (j-1), kStepUp (i-1). Returns the step count, or -1 if
+// the path leaves the band (never for pointers written by dp_forward).
 template <class PS, class EM>
 HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 	int i = lenS, j = lenQ, n = 0;
 	while (!(i == 0 && j == 0)) {
 		const int d = j - i + b;
+		if (i < 0 || j < 0 || d < 0 || d > 2 * b) return -1; // left the band: corrupt pointers
 		const uint32_t word = ptrs.get(i, d >> 4);
 		const int p = (int) ((word >> (2 * (d & 15))) & 3u);
 		if (p == 2) {

@@ -1,0 +1,107 @@
+// Device data structures and kernel launchers of the correction path (internal to libhero_gpu).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hd.h"
+
+namespace hero {
+
+// Open-addressing slot of the seed index: the k-mer (first 32 bases), its bucket in `entries`.
+// count == 0 marks an empty slot. k-mers longer than 32 keep their tail in key_lo[slot].
+struct Slot {
+	uint64_t key;
+	uint32_t start;
+	uint32_t count;
+};
+
+struct DevIndex {
+	const Slot *slots;
+	const uint64_t *key_lo;
+	const uint64_t *entries; // (pos << 40) | innerId (src/HashTableLongKey.cpp:334-336)
+	uint64_t P;
+	uint32_t max_probe;
+	int32_t k;
+};
+
+// Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
+// task and edit-word counts and their exclusive scans.
+struct ChunkDev {
+	int64_t s_begin;
+	int32_t nsub;
+	uint32_t *seg_n, *seg_off; // 2 per subject (forward, reverse)
+	uint64_t *seeds;
+	int32_t *cid, *cfirst, *clast, *clen;
+	uint8_t *gflag; // bit0 group start, bit1 group kept (within max_candidates)
+	uint32_t *g_tasks, *g_task_off, *g_words, *g_word_off;
+	DpTask *tasks;
+	uint32_t *words, *nsteps;
+};
+
+// Stats slots (u64 atomics on the device).
+enum StatSlot : int {
+	kStLookups = 0,
+	kStSeeds,
+	kStCallsExec,
+	kStCellsExec,
+	kStBuilt,
+	kStKept,
+	kStOob,
+	kStDpSkipped,
+	kStReadsChanged,
+	kStCount
+};
+
+struct DpLaunch {
+	const DpTask *tasks;
+	uint32_t ntasks;
+	const uint32_t *order; // optional permutation (length buckets)
+	Packed S, Q;
+	Opts o;
+	uint32_t *words, *nsteps;
+	uint32_t *scratch;
+	int rows_max;
+	int route_bmax, route_rows; // plain form: >= 0 restricts it to tasks the batched form cannot take
+	uint32_t *batch_counter;
+	unsigned long long *stats;
+};
+
+// launchers (kernels.hip)
+hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo,
+		uint64_t *occ_entry, hipStream_t st);
+hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
+		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st);
+hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st);
+hipError_t launch_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
+		Slot *slots, uint64_t *key_lo, unsigned long long *cursor, hipStream_t st);
+hipError_t launch_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank, const uint64_t *occ_entry, uint64_t nocc,
+		const Slot *slots, uint64_t *entries, hipStream_t st);
+hipError_t launch_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries, hipStream_t st);
+
+hipError_t launch_seed_count(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
+		unsigned long long *stats, hipStream_t st);
+hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
+		unsigned long long *stats, hipStream_t st);
+hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, hipStream_t st);
+hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
+		uint32_t *tally, unsigned long long *stats, hipStream_t st);
+hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st);
+hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
+		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st);
+
+// Exclusive scan of n u32 values (n < 2^31); *total_dev receives the u64 sum.
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long long *total_dev, uint32_t *tmp,
+		size_t tmp_words, hipStream_t st);
+size_t scan_tmp_words(int64_t n);
+
+// Standalone BandAligner batch over byte sequences (A/C/G/T/N).
+struct BytePair {
+	uint64_t s_off, q_off; // aligner subject / query bytes in the sequence buffer
+	int32_t s_len, q_len, band, flags; // flags: 1 left, 2 right
+	uint32_t out_off;      // edit words
+	int32_t _r;
+};
+hipError_t launch_band_align_bytes(const BytePair *pairs, int n, const char *seq, const Opts &o, uint32_t *words,
+		uint32_t *nsteps, uint32_t *scratch, int rows_max, int bmax, hipStream_t st);
+
+} // namespace hero

@@ -1,0 +1,878 @@
+// HIP kernels of the correction path for gfx950 (CDNA4): seed index build, seed search and
+// sort, chaining and planning (pass A), banded DP (batched lane-per-pair and plain
+// wave-per-pair forms), record scoring / dedup / votes (pass B) and consensus.
+// Every kernel that works per subject runs one 64-lane wavefront per subject read.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "pipeline_core.h"
+
+namespace hero {
+
+constexpr uint64_t kIdMask = (1ull << 40) - 1; // iMask for iBitOfReadId = 40
+
+__device__ __forceinline__ int lane_id() { return (int) threadIdx.x & 63; }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	return v;
+}
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+	const int lane = lane_id();
+	uint32_t x = v;
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1) {
+		const uint32_t t = __shfl_up(x, off, 64);
+		if (lane >= off) x += t;
+	}
+	return x - v;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+	uint32_t lo = (uint32_t) v, hi = (uint32_t) (v >> 32);
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		const uint32_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+		const uint32_t s = lo + l2;
+		hi = hi + h2 + (s < lo ? 1u : 0u);
+		lo = s;
+	}
+	return ((uint64_t) hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t lanes_below_mask() {
+	const int lane = lane_id();
+	return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Seed index (roles of HashTableLongKey::createHashTables / insertQueryDataset / getListOfReads,
+// src/HashTableLongKey.cpp:41-99,250-342). The home slot is the reference hash; linear probing.
+// Buckets hold exactly the occurrences of one k-mer, so lookups return what getListOfReads
+// returns, whatever the slot placement.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo, uint64_t *occ_entry) {
+	const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= Q.n || !Q.good[q]) return;
+	const int L = Q.len[q];
+	const uint64_t *w = Q.words + Q.woff[q];
+	uint64_t o = koff[q];
+	const int nk = num_keys(L, k);
+	for (int t = 0; t < nk; t++) {
+		const int p = key_pos(L, k, t);
+		uint64_t hi, lo;
+		kmer_fwd(w, p, k, hi, lo);
+		occ_hi[o] = hi;
+		occ_lo[o] = lo;
+		occ_entry[o] = ((uint64_t) p << 40) | (uint64_t) q;
+		o++;
+	}
+}
+hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo,
+		uint64_t *occ_entry, hipStream_t st) {
+	if (Q.n == 0) return hipSuccess;
+	k_index_gen<<<(unsigned) ((Q.n + 255) / 256), 256, 0, st>>>(Q, koff, k, occ_hi, occ_lo, occ_entry);
+	return hipGetLastError();
+}
+
+// One occurrence per thread: claim the first free slot from the home slot with a CAS of the
+// occurrence id, or stop at the slot already claimed by an occurrence of the same k-mer (whose
+// key is readable in occ_* since they were written by the previous kernel).
+__global__ void k_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
+		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err) {
+	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= nocc) return;
+	const uint64_t hi = occ_hi[o], lo = occ_lo[o];
+	uint64_t h = hash_kmer(k, hi, lo, P);
+	for (uint64_t probe = 0; probe < P; probe++) {
+		const uint32_t cur = atomicCAS(&owner[h], 0u, (uint32_t) (o + 1));
+		bool done = cur == 0;
+		if (!done) {
+			const uint64_t other = cur - 1;
+			done = occ_hi[other] == hi && (k <= 32 || occ_lo[other] == lo);
+		}
+		if (done) {
+			occ_slot[o] = (uint32_t) h;
+			atomicMax(max_probe, (uint32_t) probe);
+			return;
+		}
+		h = (h + 1 == P) ? 0 : h + 1;
+	}
+	*err = 1; // table full ("The size of hash table is too small.")
+}
+hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
+		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st) {
+	if (nocc == 0) return hipSuccess;
+	k_index_claim<<<(unsigned) ((nocc + 255) / 256), 256, 0, st>>>(occ_hi, occ_lo, nocc, P, k, owner, occ_slot, max_probe, err);
+	return hipGetLastError();
+}
+
+__global__ void k_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank) {
+	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= nocc) return;
+	occ_rank[o] = atomicAdd(&slots[occ_slot[o]].count, 1u);
+}
+hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st) {
+	if (nocc == 0) return hipSuccess;
+	k_index_count<<<(unsigned) ((nocc + 255) / 256), 256, 0, st>>>(occ_slot, nocc, slots, occ_rank);
+	return hipGetLastError();
+}
+
+__global__ void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
+		Slot *slots, uint64_t *key_lo, unsigned long long *cursor) {
+	const uint64_t s = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= P) return;
+	const uint32_t own = owner[s];
+	if (own == 0) return;
+	const uint64_t o = own - 1;
+	Slot sl = slots[s];
+	sl.key = occ_hi[o];
+	sl.start = (uint32_t) atomicAdd(cursor, (unsigned long long) sl.count);
+	slots[s] = sl;
+	if (k > 32) key_lo[s] = occ_lo[o];
+}
+hipError_t launch_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
+		Slot *slots, uint64_t *key_lo, unsigned long long *cursor, hipStream_t st) {
+	k_index_alloc<<<(unsigned) ((P + 255) / 256), 256, 0, st>>>(owner, P, occ_hi, occ_lo, k, slots, key_lo, cursor);
+	return hipGetLastError();
+}
+
+__global__ void k_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank, const uint64_t *occ_entry, uint64_t nocc,
+		const Slot *slots, uint64_t *entries) {
+	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= nocc) return;
+	entries[(uint64_t) slots[occ_slot[o]].start + occ_rank[o]] = occ_entry[o];
+}
+hipError_t launch_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank, const uint64_t *occ_entry, uint64_t nocc,
+		const Slot *slots, uint64_t *entries, hipStream_t st) {
+	if (nocc == 0) return hipSuccess;
+	k_index_fill<<<(unsigned) ((nocc + 255) / 256), 256, 0, st>>>(occ_slot, occ_rank, occ_entry, nocc, slots, entries);
+	return hipGetLastError();
+}
+
+// Puts each bucket in the reference's insertion order (innerId, then position). Seeds are
+// re-sorted by (innerId, posQ, posS) downstream, so this only makes the index deterministic.
+__global__ void k_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries) {
+	const uint64_t s = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= P) return;
+	const Slot sl = slots[s];
+	if (sl.count < 2 || sl.count > 512) return;
+	uint64_t *e = entries + sl.start;
+	auto key = [](uint64_t v) { return ((v & kIdMask) << 24) | (v >> 40); };
+	for (uint32_t i = 1; i < sl.count; i++) {
+		const uint64_t v = e[i], kv = key(v);
+		int j = (int) i - 1;
+		while (j >= 0 && key(e[j]) > kv) {
+			e[j + 1] = e[j];
+			j--;
+		}
+		e[j + 1] = v;
+	}
+}
+hipError_t launch_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries, hipStream_t st) {
+	k_index_sort_buckets<<<(unsigned) ((P + 255) / 256), 256, 0, st>>>(slots, P, entries);
+	return hipGetLastError();
+}
+
+__device__ __forceinline__ bool ix_lookup(const DevIndex &ix, uint64_t hi, uint64_t lo, uint32_t &start, uint32_t &count) {
+	uint64_t h = hash_kmer(ix.k, hi, lo, ix.P);
+	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
+		const Slot s = ix.slots[h];
+		if (s.count == 0) return false;
+		if (s.key == hi && (ix.k <= 32 || ix.key_lo[h] == lo)) {
+			start = s.start;
+			count = s.count;
+			return true;
+		}
+		h = (h + 1 == ix.P) ? 0 : h + 1;
+	}
+	return false;
+}
+
+// The k-mer of subject window w in strand frame (reverse strand: window of the RC).
+__device__ __forceinline__ void window_key(const Packed &S, int64_t s, int Ls, int w, int strand, int k, uint64_t &hi,
+		uint64_t &lo) {
+	const uint64_t *ws = S.words + S.woff[s];
+	if (strand == 0) {
+		kmer_fwd(ws, w, k, hi, lo);
+	} else {
+		uint64_t fh, fl;
+		kmer_fwd(ws, Ls - w - k, k, fh, fl);
+		kmer_revcomp(k, fh, fl, hi, lo);
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// Seed search (SubjectReadAligner::searchHashTable2, src/SubjectReadAligner.cpp:444-491)
+// ---------------------------------------------------------------------------------------------
+// Seeds of one window as they are emitted: skip hits on the query with the subject's uniqueId;
+// without -c, reverse-strand coordinates map back to forward frames (:474-481).
+template <class F>
+__device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o,
+		int64_t s, int Ls, int64_t uid, int w, int strand, F &&emit) {
+	uint64_t hi, lo;
+	window_key(S, s, Ls, w, strand, o.k, hi, lo);
+	uint32_t start = 0, count = 0;
+	if (!ix_lookup(ix, hi, lo, start, count)) return 0;
+	uint32_t n = 0;
+	for (uint32_t e = start; e < start + count; e++) {
+		const uint64_t data = ix.entries[e];
+		const uint64_t qid = data & kIdMask;
+		if (Q.uid[qid] == uid) continue;
+		int pq = (int) (data >> 40), ps = w;
+		if (!o.call_consensus && strand == 1) ps = Ls - (ps + o.k), pq = Q.len[qid] - (pq + o.k);
+		emit(seed_pack(qid, pq, ps));
+		n++;
+	}
+	return n;
+}
+
+__global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
+		unsigned long long *stats) {
+	const int sl = blockIdx.x;
+	const int64_t s = c.s_begin + sl;
+	const int lane = lane_id();
+	if (!S.good[s]) {
+		if (lane == 0) c.seg_n[2 * sl] = 0, c.seg_n[2 * sl + 1] = 0;
+		return;
+	}
+	const int Ls = S.len[s];
+	const int64_t uid = S.uid[s];
+	const int nwin = Ls - o.k + 1;
+	for (int strand = 0; strand < 2; strand++) {
+		uint32_t cnt = 0;
+		for (int w = lane; w < nwin; w += 64) cnt += window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [](uint64_t) {});
+		cnt = wave_sum(cnt);
+		if (lane == 0) c.seg_n[2 * sl + strand] = cnt;
+	}
+	if (lane == 0) atomicAdd(&stats[kStLookups], (unsigned long long) (2 * nwin));
+}
+hipError_t launch_seed_count(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
+		unsigned long long *stats, hipStream_t st) {
+	if (c.nsub == 0) return hipSuccess;
+	k_seed_count<<<c.nsub, 64, 0, st>>>(S, Q, ix, o, c, stats);
+	return hipGetLastError();
+}
+
+// Ascending bitonic network over n keys padded (virtually) with +inf to a power of two: every
+// compare-exchange touching a pad element is a no-op, so skipping it sorts any n.
+template <class A>
+__device__ void bitonic_sort(A &a, int n) {
+	int N = 1;
+	while (N < n) N <<= 1;
+	const int lane = lane_id();
+	for (int kk = 2; kk <= N; kk <<= 1) {
+		for (int i = lane; i < N / 2; i += 64) { // flip stage
+			const int blk = i / (kk / 2), off = i % (kk / 2);
+			const int x = blk * kk + off, y = blk * kk + kk - 1 - off;
+			if (y < n) {
+				const uint64_t u = a[x], v = a[y];
+				if (u > v) a[x] = v, a[y] = u;
+			}
+		}
+		__syncthreads();
+		for (int j = kk / 4; j > 0; j >>= 1) {
+			for (int i = lane; i < N / 2; i += 64) {
+				const int blk = i / j, off = i % j;
+				const int x = blk * 2 * j + off, y = x + j;
+				if (y < n) {
+					const uint64_t u = a[x], v = a[y];
+					if (u > v) a[x] = v, a[y] = u;
+				}
+			}
+			__syncthreads();
+		}
+	}
+}
+struct LdsArr {
+	uint64_t *p;
+	__device__ uint64_t &operator[](int i) const { return p[i]; }
+};
+struct GlobArr {
+	uint64_t *p;
+	__device__ uint64_t &operator[](int i) const { return p[i]; }
+};
+
+constexpr int kLdsSeeds = 2048;
+
+__device__ __forceinline__ PairReads make_pair(const Packed &S, const Packed &Q, const Opts &o, int64_t s, int Ls,
+		int64_t q, int strand) {
+	PairReads pr;
+	pr.subj = &S, pr.qry = &Q, pr.s_read = s, pr.q_read = q, pr.Ls = Ls, pr.Lq = Q.len[q];
+	pr.subj_rc = o.call_consensus && strand == 1;
+	pr.qry_rc = !o.call_consensus && strand == 1;
+	return pr;
+}
+
+__device__ __forceinline__ Group make_group(const ChunkDev &c, uint32_t base, int gn) {
+	Group g;
+	g.seeds = c.seeds + base;
+	g.cid = c.cid + base, g.cfirst = c.cfirst + base, g.clast = c.clast + base, g.clen = c.clen + base;
+	g.gn = gn, g.nch = 0, g.best = 0;
+	return g;
+}
+
+// Pass A1: per subject strand, emit the seeds, sort them, chain every kept query group and count
+// the DP tasks / edit words its surviving chains need.
+__global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
+		unsigned long long *stats) {
+	__shared__ uint64_t lds[kLdsSeeds];
+	const int sl = blockIdx.x;
+	const int64_t s = c.s_begin + sl;
+	const int lane = lane_id();
+	if (!S.good[s]) return;
+	const int Ls = S.len[s];
+	const int64_t uid = S.uid[s];
+	const int nwin = Ls - o.k + 1;
+	for (int strand = 0; strand < 2; strand++) {
+		const uint32_t n = c.seg_n[2 * sl + strand], off = c.seg_off[2 * sl + strand];
+		if (n == 0) continue;
+		const bool in_lds = n <= (uint32_t) kLdsSeeds;
+		uint64_t *dst = in_lds ? lds : c.seeds + off;
+		uint32_t cnt = 0;
+		for (int w = lane; w < nwin; w += 64) cnt += window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [](uint64_t) {});
+		uint32_t pos = wave_excl_scan(cnt);
+		for (int w = lane; w < nwin; w += 64)
+			window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [&](uint64_t v) { dst[pos++] = v; });
+		__syncthreads();
+		if (in_lds) {
+			LdsArr a{lds};
+			bitonic_sort(a, (int) n);
+			for (uint32_t t = lane; t < n; t += 64) c.seeds[off + t] = lds[t];
+		} else {
+			GlobArr a{c.seeds + off};
+			bitonic_sort(a, (int) n);
+		}
+		__syncthreads();
+		// groups: a seed starts a group when its query differs from the previous seed's
+		uint32_t groups_before = 0;
+		for (uint32_t t0 = 0; t0 < n; t0 += 64) {
+			const uint32_t t = t0 + lane;
+			bool start = false;
+			if (t < n) start = t == 0 || seed_qid(c.seeds[off + t]) != seed_qid(c.seeds[off + t - 1]);
+			const uint64_t m = __ballot(start);
+			const uint32_t rank = groups_before + (uint32_t) __popcll(m & lanes_below_mask());
+			groups_before += (uint32_t) __popcll(m);
+			uint32_t ntask = 0, nword = 0;
+			uint8_t flag = 0;
+			if (start) {
+				const bool kept = o.max_candidates <= 0 || rank < (uint32_t) o.max_candidates;
+				flag = kept ? 3 : 1;
+				if (kept) {
+					const int64_t q = seed_qid(c.seeds[off + t]);
+					uint32_t e = t + 1;
+					while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
+					Group g = make_group(c, off + t, (int) (e - t));
+					chain(o, g);
+					const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
+					int nt, nw;
+					plan_count(o, g, pr, nt, nw);
+					ntask = (uint32_t) nt, nword = (uint32_t) nw;
+				}
+			}
+			if (t < n) {
+				c.gflag[off + t] = flag;
+				c.g_tasks[off + t] = ntask;
+				c.g_words[off + t] = nword;
+			}
+		}
+	}
+	if (lane == 0) atomicAdd(&stats[kStSeeds], (unsigned long long) (c.seg_n[2 * sl] + c.seg_n[2 * sl + 1]));
+}
+hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
+		unsigned long long *stats, hipStream_t st) {
+	if (c.nsub == 0) return hipSuccess;
+	k_pass_a1<<<c.nsub, 64, 0, st>>>(S, Q, ix, o, c, stats);
+	return hipGetLastError();
+}
+
+// Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
+__global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
+	const int sl = blockIdx.x;
+	const int64_t s = c.s_begin + sl;
+	const int lane = lane_id();
+	if (!S.good[s]) return;
+	const int Ls = S.len[s];
+	for (int strand = 0; strand < 2; strand++) {
+		const uint32_t n = c.seg_n[2 * sl + strand], off = c.seg_off[2 * sl + strand];
+		for (uint32_t t = lane; t < n; t += 64) {
+			if (c.gflag[off + t] != 3 || c.g_tasks[off + t] == 0) continue;
+			const int64_t q = seed_qid(c.seeds[off + t]);
+			uint32_t e = t + 1;
+			while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
+			Group g = make_group(c, off + t, (int) (e - t));
+			chain(o, g);
+			const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
+			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t]);
+		}
+	}
+}
+hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, hipStream_t st) {
+	if (c.nsub == 0) return hipSuccess;
+	k_pass_a2<<<c.nsub, 64, 0, st>>>(S, Q, o, c);
+	return hipGetLastError();
+}
+
+// First seed of query q's group in a sorted segment, or -1.
+__device__ __forceinline__ int find_group(const uint64_t *seeds, uint32_t n, int64_t q) {
+	uint32_t lo = 0, hi = n;
+	while (lo < hi) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (seed_qid(seeds[mid]) < q) lo = mid + 1;
+		else hi = mid;
+	}
+	return (lo < n && seed_qid(seeds[lo]) == q) ? (int) lo : -1;
+}
+__device__ __forceinline__ uint32_t group_end(const uint64_t *seeds, uint32_t n, uint32_t t) {
+	const int64_t q = seed_qid(seeds[t]);
+	uint32_t e = t + 1;
+	while (e < n && seed_qid(seeds[e]) == q) e++;
+	return e;
+}
+
+// Pass B: per query of a subject (forward group and/or reverse group), build its records from the
+// DP results in the reference's order (forward chains, then reverse chains), keep the first best
+// (SubjectRead::addAlignment), then cast that record's votes.
+__global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, const uint64_t *q_boff,
+		uint32_t *tally, unsigned long long *stats) {
+	const int sl = blockIdx.x;
+	const int64_t s = c.s_begin + sl;
+	const int lane = lane_id();
+	if (!S.good[s]) return;
+	const int Ls = S.len[s];
+	const uint32_t n0 = c.seg_n[2 * sl], off0 = c.seg_off[2 * sl];
+	const uint32_t n1 = c.seg_n[2 * sl + 1], off1 = c.seg_off[2 * sl + 1];
+	DpResults R{c.tasks, c.words, c.nsteps};
+	uint64_t cells = 0, calls = 0, built = 0, kept = 0;
+	int oob = 0;
+	for (int pass = 0; pass < 2; pass++) {
+		const uint32_t n = pass == 0 ? n0 : n1, off = pass == 0 ? off0 : off1;
+		for (uint32_t t = lane; t < n; t += 64) {
+			if (c.gflag[off + t] != 3) continue;
+			const int64_t q = seed_qid(c.seeds[off + t]);
+			int gf = -1, gr = -1;
+			if (pass == 0) {
+				gf = (int) t;
+				const int r = find_group(c.seeds + off1, n1, q);
+				if (r >= 0 && c.gflag[off1 + r] == 3) gr = r;
+			} else {
+				const int f = find_group(c.seeds + off0, n0, q);
+				if (f >= 0 && c.gflag[off0 + f] == 3) continue; // handled with the forward group
+				gr = (int) t;
+			}
+			Best best;
+			best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
+			Group groups[2];
+			for (int strand = 0; strand < 2; strand++) {
+				const int gi = strand == 0 ? gf : gr;
+				if (gi < 0) continue;
+				const uint32_t so = strand == 0 ? off0 : off1, sn = strand == 0 ? n0 : n1;
+				groups[strand] = make_group(c, so + gi, (int) (group_end(c.seeds + so, sn, gi) - gi));
+				chain(o, groups[strand]);
+				const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
+				score_group(o, groups[strand], pr, R, c.g_task_off[so + gi], strand, best, cells, calls, built);
+			}
+			if (!best.have) continue;
+			kept++;
+			if (!o.call_consensus) continue;
+			const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
+			VoteSink sink{tally + 4 * q_boff[q], pr, pr.Lq, &oob};
+			vote_record(o, groups[best.strand], pr, R, best.chain, best.task0, sink);
+		}
+	}
+	cells = wave_sum64(cells);
+	calls = wave_sum64(calls);
+	built = wave_sum64(built);
+	kept = wave_sum64(kept);
+	const uint32_t anyoob = wave_sum((uint32_t) oob);
+	if (lane == 0) {
+		atomicAdd(&stats[kStCellsExec], (unsigned long long) cells);
+		atomicAdd(&stats[kStCallsExec], (unsigned long long) calls);
+		atomicAdd(&stats[kStBuilt], (unsigned long long) built);
+		atomicAdd(&stats[kStKept], (unsigned long long) kept);
+		if (anyoob) atomicAdd(&stats[kStOob], (unsigned long long) anyoob);
+	}
+}
+hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
+		uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+	if (c.nsub == 0) return hipSuccess;
+	k_pass_b<<<c.nsub, 64, 0, st>>>(S, Q, o, c, q_boff, tally, stats);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Banded DP (BandAligner::bandedAlign / bandedAlignAffineGap)
+// ---------------------------------------------------------------------------------------------
+// Sequential reader of one side of a task in its frame (reverse complement when rc).
+struct PackedStream {
+	const uint64_t *w;
+	int L, start, len, i;
+	bool rc;
+	__device__ __forceinline__ int next() {
+		if (i >= len) return kCodeNone;
+		const int p = start + i++;
+		return rc ? (base_at(w, L - 1 - p) ^ 2) : base_at(w, p);
+	}
+};
+
+__device__ __forceinline__ void task_streams(const DpTask &t, const Packed &S, const Packed &Q, PackedStream &as,
+		PackedStream &aq, int &lenS, int &lenQ) {
+	PackedStream subj{S.words + S.woff[t.s_read], S.len[t.s_read], t.s0, t.s_len, 0, (t.flags & kTaskSubjRC) != 0};
+	PackedStream qry{Q.words + Q.woff[t.q_read], Q.len[t.q_read], t.q0, t.q_len, 0, (t.flags & kTaskQryRC) != 0};
+	const bool sw = (t.flags & kTaskSwapped) != 0;
+	as = sw ? qry : subj;
+	aq = sw ? subj : qry;
+	lenS = sw ? t.q_len : t.s_len;
+	lenQ = sw ? t.s_len : t.q_len;
+}
+
+// Writes traceback steps last-to-first into a task's word range: step m goes to slot cap-1-m.
+struct StepWriter {
+	uint32_t *w;
+	int e;
+	int cur;
+	uint32_t acc;
+	__device__ __forceinline__ void operator()(uint32_t st) {
+		e--;
+		const int wi = e >> 4;
+		if (wi != cur) {
+			if (cur >= 0) w[cur] = acc;
+			acc = 0;
+			cur = wi;
+		}
+		acc |= st << (2 * (e & 15));
+	}
+	__device__ __forceinline__ void flush() {
+		if (cur >= 0) w[cur] = acc;
+	}
+};
+
+// Pointer rows of the lanes of one wavefront, interleaved so a row store is one coalesced
+// 256-byte write: word w of row i of lane l at ((i * NW) + w) * 64 + l.
+template <int NW>
+struct LanePtrs {
+	uint32_t *base;
+	int lane;
+	__device__ __forceinline__ void put_row(int i, const uint32_t *pw) {
+#pragma unroll
+		for (int w = 0; w < NW; w++) base[((size_t) i * NW + w) * 64 + lane] = pw[w];
+	}
+	__device__ __forceinline__ uint32_t get(int i, int w) const { return base[((size_t) i * NW + w) * 64 + lane]; }
+};
+
+// Batched form: one task per lane, persistent waves pulling 64 tasks at a time.
+template <int BMAX, bool AFF>
+__global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	const int lane = lane_id();
+	LanePtrs<NW> ptrs{L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 64, lane};
+	for (;;) {
+		uint32_t batch = 0;
+		if (lane == 0) batch = atomicAdd(L.batch_counter, 1u);
+		batch = __shfl(batch, 0, 64);
+		const uint32_t first = batch * 64;
+		if (first >= L.ntasks) break;
+		const uint32_t ti = first + lane;
+		if (ti < L.ntasks) {
+			const uint32_t t = L.order ? L.order[ti] : ti;
+			const DpTask task = L.tasks[t];
+			PackedStream as, aq;
+			int lenS, lenQ;
+			task_streams(task, L.S, L.Q, as, aq, lenS, lenQ);
+			if (task.band <= BMAX && lenS < L.rows_max) {
+				dp_forward<BMAX, AFF>(L.o, lenS, lenQ, task.band, (task.flags & kTaskAlignLeft) != 0,
+						(task.flags & kTaskAlignRight) != 0, as, aq, ptrs);
+				const int cap = task.s_len + task.q_len;
+				StepWriter wr{L.words + task.out_off, cap, -1, 0};
+				const int ns = dp_traceback(lenS, lenQ, task.band, ptrs, wr);
+				wr.flush();
+				L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
+				if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+			}
+		}
+	}
+}
+
+hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st) {
+	if (L.ntasks == 0) return hipSuccess;
+	const bool aff = L.o.aligner == 0;
+#define HERO_DPB(BM)                                                                               \
+	if (bmax == BM) {                                                                              \
+		if (aff) k_dp_batched<BM, true><<<nwaves, 64, 0, st>>>(L);                                   \
+		else k_dp_batched<BM, false><<<nwaves, 64, 0, st>>>(L);                                      \
+		return hipGetLastError();                                                                  \
+	}
+	HERO_DPB(4) HERO_DPB(8) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
+#undef HERO_DPB
+	return hipErrorInvalidValue;
+}
+
+// Plain form: one wavefront per task, one anti-diagonal per step. Lane l owns the cell of row
+// i_lo(t) + l on anti-diagonal t = i + j; the two previous anti-diagonals live in LDS indexed by
+// row, pointers as bytes in per-wave global scratch, traceback by lane 0.
+constexpr int kPlainRows = 512;
+template <bool AFF>
+__global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
+	__shared__ int hb[3][kPlainRows + 1];
+	__shared__ int xb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
+	__shared__ int yb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
+	__shared__ uint8_t sc[kPlainRows], qc[kPlainRows];
+	const int lane = lane_id();
+	uint8_t *ptr = (uint8_t *) (L.scratch) + (size_t) blockIdx.x * (size_t) kPlainRows * (2 * bmax_plain + 1);
+	for (;;) {
+		uint32_t ti = 0;
+		if (lane == 0) ti = atomicAdd(L.batch_counter, 1u);
+		ti = __shfl(ti, 0, 64);
+		if (ti >= L.ntasks) break;
+		const uint32_t t = L.order ? L.order[ti] : ti;
+		const DpTask task = L.tasks[t];
+		PackedStream as, aq;
+		int lenS, lenQ;
+		task_streams(task, L.S, L.Q, as, aq, lenS, lenQ);
+		const int b = task.band;
+		if (L.route_bmax >= 0 && b <= L.route_bmax && lenS < L.route_rows) continue; // done by the batched form
+		if (b > bmax_plain || lenS >= kPlainRows || lenQ >= kPlainRows) {
+			if (lane == 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+			continue;
+		}
+		const int W = 2 * b + 1;
+		// stage the codes in LDS (each lane reads a strided slice of the packed sides)
+		for (int i = lane; i < lenS; i += 64) {
+			PackedStream x = as;
+			x.i = i;
+			sc[i] = (uint8_t) x.next();
+		}
+		for (int j = lane; j < lenQ; j += 64) {
+			PackedStream x = aq;
+			x.i = j;
+			qc[j] = (uint8_t) x.next();
+		}
+		__syncthreads();
+		const bool la = (task.flags & kTaskAlignLeft) != 0, ra = (task.flags & kTaskAlignRight) != 0;
+		auto valid = [&](int i, int j) { return i >= 0 && i <= lenS && j >= 0 && j <= lenQ && j - i <= b && i - j <= b; };
+		for (int ad = 0; ad <= lenS + lenQ; ad++) {
+			const int cur = ad % 3, prv = (ad + 2) % 3, pp = (ad + 1) % 3;
+			int ilo = ad - lenQ > 0 ? ad - lenQ : 0;
+			int ihi = ad < lenS ? ad : lenS;
+			const int blo = (ad - b + 1) >> 1; // ceil((ad-b)/2) for ad-b >= -1; clamp below
+			const int bhi = (ad + b) >> 1;
+			if (blo > ilo) ilo = blo;
+			if (bhi < ihi) ihi = bhi;
+			for (int i = ilo + lane; i <= ihi; i += 64) {
+				const int j = ad - i;
+				int h, x, y, p;
+				if (i == 0) {
+					row0_cell<AFF>(L.o, j, b, lenQ, h, x, y);
+					p = 1;
+				} else {
+					const int hd = valid(i - 1, j - 1) ? hb[pp][i - 1] : kSent;
+					const int hu = valid(i - 1, j) ? hb[prv][i - 1] : kSent;
+					const int hl = valid(i, j - 1) ? hb[prv][i] : kSent;
+					int xd = kSent, yd = kSent, xu = kSent, yl = kSent;
+					if (AFF) {
+						xd = valid(i - 1, j - 1) ? xb[pp][i - 1] : kSent;
+						yd = valid(i - 1, j - 1) ? yb[pp][i - 1] : kSent;
+						xu = valid(i - 1, j) ? xb[prv][i - 1] : kSent;
+						yl = valid(i, j - 1) ? yb[prv][i] : kSent;
+					}
+					cell_update<AFF>(L.o, sc[i - 1], j >= 1 ? qc[j - 1] : kCodeNone, j == 0, j == lenQ, la, ra, lenQ, hd, xd, yd,
+							hu, xu, hl, yl, h, x, y, p);
+				}
+				hb[cur][i] = h;
+				if (AFF) xb[cur][i] = x, yb[cur][i] = y;
+				ptr[(size_t) i * W + (j - i + b)] = (uint8_t) p;
+			}
+			__syncthreads();
+		}
+		if (lane == 0) {
+			struct BytePtrs {
+				const uint8_t *p;
+				int W, b;
+				__device__ uint32_t get(int i, int w) const {
+					// repack the 16 cells of word w of row i into the 2-bit layout the traceback reads
+					uint32_t v = 0;
+					for (int d = 16 * w; d < 16 * w + 16 && d < W; d++) v |= (uint32_t) p[(size_t) i * W + d] << (2 * (d & 15));
+					return v;
+				}
+			} bp{ptr, W, b};
+			const int cap = task.s_len + task.q_len;
+			StepWriter wr{L.words + task.out_off, cap, -1, 0};
+			const int ns = dp_traceback(lenS, lenQ, b, bp, wr);
+			wr.flush();
+			L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
+				if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+		}
+		__syncthreads();
+	}
+}
+
+hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st) {
+	if (L.ntasks == 0) return hipSuccess;
+	if (L.o.aligner == 0) k_dp_plain<true><<<nwaves, 64, 0, st>>>(L, bmax_plain);
+	else k_dp_plain<false><<<nwaves, 64, 0, st>>>(L, bmax_plain);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Consensus (QueryDataset::callConsensus, src/QueryDataset.cpp:393-412)
+// ---------------------------------------------------------------------------------------------
+// Writes the corrected bases of good reads as ASCII at their base offsets.
+__global__ void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0, int64_t nq,
+		uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed) {
+	const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nq) return;
+	const int64_t q = q0 + i;
+	const uint64_t b = q_boff[q];
+	const int L = Q.len[q];
+	if (!Q.good[q]) { // not corrected (src/QueryDataset.cpp:404-406); the host copies its bytes
+		if (decoded_out) {
+			uint32_t run = 0;
+			for (int p = 0; p < L; p++) {
+				const int own = strand_base(Q, q, p, false);
+				for (int j = 0; j < 4; j++) decoded_out[4 * (b + p) + j] = tally[4 * (b + p) + j];
+				run += tally[4 * (b + p) + own];
+				decoded_out[4 * (b + p) + own] = run;
+			}
+		}
+		return;
+	}
+	const int changed = consensus_read(o, tally + 4 * b, L, Q, q, codes_out + b, decoded_out ? decoded_out + 4 * b : nullptr);
+	for (int p = 0; p < L; p++) codes_out[b + p] = (uint8_t) code_ascii(codes_out[b + p]);
+	if (changed && reads_changed) atomicAdd(reads_changed, 1ull);
+}
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
+		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st) {
+	if (nq == 0) return hipSuccess;
+	k_consensus<<<(unsigned) ((nq + 255) / 256), 256, 0, st>>>(Q, o, q_boff, tally, q0, nq, codes_out, decoded_out,
+			reads_changed);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exclusive scan of u32 (block-local scan, recursive scan of block sums, add-back)
+// ---------------------------------------------------------------------------------------------
+constexpr int kScanThreads = 256, kScanItems = 16, kScanBlock = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(256) void k_scan_block(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *block_sums) {
+	__shared__ uint32_t sh[kScanThreads];
+	const int64_t base = (int64_t) blockIdx.x * kScanBlock + (int64_t) threadIdx.x * kScanItems;
+	uint32_t v[kScanItems];
+	uint32_t sum = 0;
+#pragma unroll
+	for (int i = 0; i < kScanItems; i++) {
+		v[i] = base + i < n ? in[base + i] : 0u;
+		sum += v[i];
+	}
+	sh[threadIdx.x] = sum;
+	__syncthreads();
+	for (int off = 1; off < kScanThreads; off <<= 1) {
+		const uint32_t t = threadIdx.x >= (unsigned) off ? sh[threadIdx.x - off] : 0u;
+		__syncthreads();
+		sh[threadIdx.x] += t;
+		__syncthreads();
+	}
+	uint32_t run = sh[threadIdx.x] - sum;
+#pragma unroll
+	for (int i = 0; i < kScanItems; i++) {
+		if (base + i < n) out[base + i] = run;
+		run += v[i];
+	}
+	if (threadIdx.x == kScanThreads - 1) block_sums[blockIdx.x] = sh[threadIdx.x];
+}
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *out, int64_t n, const uint32_t *block_offsets) {
+	const int64_t i = (int64_t) blockIdx.x * kScanThreads + threadIdx.x;
+	if (i < n) out[i] += block_offsets[i / kScanBlock];
+}
+__global__ void k_sum64(const uint32_t *in, int64_t n, unsigned long long *total) {
+	uint64_t s = 0;
+	for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) s += in[i];
+	s = wave_sum64(s);
+	if (lane_id() == 0 && s) atomicAdd(total, (unsigned long long) s);
+}
+
+size_t scan_tmp_words(int64_t n) {
+	size_t words = 0;
+	int64_t m = n;
+	while (m > 1) {
+		m = (m + kScanBlock - 1) / kScanBlock;
+		words += (size_t) m + 1;
+	}
+	return words + 1;
+}
+
+static hipError_t scan_rec(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+	uint32_t *sums = tmp;
+	k_scan_block<<<(unsigned) nb, kScanThreads, 0, st>>>(in, out, n, sums);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess || nb == 1) return e;
+	e = scan_rec(sums, sums, nb, tmp + nb + 1, st);
+	if (e != hipSuccess) return e;
+	k_scan_add<<<(unsigned) ((n + kScanThreads - 1) / kScanThreads), kScanThreads, 0, st>>>(out, n, sums);
+	return hipGetLastError();
+}
+
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long long *total_dev, uint32_t *tmp,
+		size_t tmp_words, hipStream_t st) {
+	if (tmp_words < scan_tmp_words(n)) return hipErrorInvalidValue;
+	hipError_t e = hipMemsetAsync(total_dev, 0, sizeof(unsigned long long), st);
+	if (e != hipSuccess) return e;
+	if (n > 0) {
+		k_sum64<<<(unsigned) (n / 4096 + 1 < 1024 ? n / 4096 + 1 : 1024), 256, 0, st>>>(in, n, total_dev);
+		e = hipGetLastError();
+		if (e != hipSuccess) return e;
+	}
+	return scan_rec(in, out, n, tmp, st);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Standalone BandAligner batch over bytes (A/C/G/T/N)
+// ---------------------------------------------------------------------------------------------
+struct ByteStreamDev {
+	const char *s;
+	int len, i;
+	__device__ __forceinline__ int next() {
+		if (i >= len) return kCodeNone;
+		const char c = s[i++];
+		return c == 'N' ? kCodeN : ascii_code(c);
+	}
+};
+
+template <int BMAX, bool AFF>
+__global__ __launch_bounds__(64) void k_band_align_bytes(const BytePair *pairs, int n, const char *seq, Opts o,
+		uint32_t *words, uint32_t *nsteps, uint32_t *scratch, int rows_max) {
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	const int lane = lane_id();
+	const int t = blockIdx.x * 64 + lane;
+	LanePtrs<NW> ptrs{scratch + (size_t) blockIdx.x * rows_max * NW * 64, lane};
+	if (t >= n) return;
+	const BytePair p = pairs[t];
+	if (p.band > BMAX || p.s_len >= rows_max) {
+		nsteps[t] = 0xFFFFFFFFu;
+		return;
+	}
+	ByteStreamDev as{seq + p.s_off, p.s_len, 0}, aq{seq + p.q_off, p.q_len, 0};
+	dp_forward<BMAX, AFF>(o, p.s_len, p.q_len, p.band, (p.flags & 1) != 0, (p.flags & 2) != 0, as, aq, ptrs);
+	StepWriter wr{words + p.out_off, p.s_len + p.q_len, -1, 0};
+	const int ns = dp_traceback(p.s_len, p.q_len, p.band, ptrs, wr);
+	wr.flush();
+	nsteps[t] = ns < 0 ? 0xFFFFFFFFu : (uint32_t) ns;
+}
+
+hipError_t launch_band_align_bytes(const BytePair *pairs, int n, const char *seq, const Opts &o, uint32_t *words,
+		uint32_t *nsteps, uint32_t *scratch, int rows_max, int bmax, hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	const unsigned blocks = (unsigned) ((n + 63) / 64);
+	const bool aff = o.aligner == 0;
+#define HERO_BAB(BM)                                                                                                  \
+	if (bmax <= BM) {                                                                                                 \
+		if (aff) k_band_align_bytes<BM, true><<<blocks, 64, 0, st>>>(pairs, n, seq, o, words, nsteps, scratch, rows_max);  \
+		else k_band_align_bytes<BM, false><<<blocks, 64, 0, st>>>(pairs, n, seq, o, words, nsteps, scratch, rows_max);     \
+		return hipGetLastError();                                                                                     \
+	}
+	HERO_BAB(8) HERO_BAB(16) HERO_BAB(32) HERO_BAB(64)
+#undef HERO_BAB
+	return hipErrorInvalidValue;
+}
+
+} // namespace hero

@@ -132,7 +132,7 @@ template <class F>
 HD void feed_dp(const DpResults &R, uint32_t task, const PairReads &pr, const DpCall &c, F &&op) {
 	const DpTask &t = R.tasks[task];
 	const int cap = t.s_len + t.q_len;
-	const int n = (int) R.nsteps[task];
+	const int n = (int) R.nsteps[task] <= cap ? (int) R.nsteps[task] : 0;
 	const uint32_t *w = R.words + t.out_off;
 	int i = 0, j = 0; // aligner coordinates
 	for (int e = cap - n; e < cap; e++) {

@@ -1,0 +1,215 @@
+"""ctypes binding of libhero_gpu.so (include/hero_gpu.h).
+
+This is the product path: it raises if the HIP library is missing and never falls back to a CPU
+implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libhero_gpu.so")
+
+HERO_OK = 0
+ERRORS = {-1: "HERO_ERR_ARG", -2: "HERO_ERR_HIP", -3: "HERO_ERR_NOMEM", -4: "HERO_ERR_STATE", -5: "HERO_ERR_LIMIT",
+          -6: "HERO_ERR_REF_ABORT"}
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_int32), ("aligner", ctypes.c_int32), ("call_consensus", ctypes.c_int32),
+        ("hash_factor", ctypes.c_int32), ("min_overlap", ctypes.c_int64), ("mismatch_rate", ctypes.c_double),
+        ("indel_rate", ctypes.c_double), ("consensus_freq", ctypes.c_double),
+        ("dx", ctypes.c_int32), ("dy", ctypes.c_int32), ("ex", ctypes.c_int32), ("sx", ctypes.c_int32),
+        ("miss", ctypes.c_int32), ("hit", ctypes.c_int32), ("unobserved", ctypes.c_int32),
+        ("band_override", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
+        ("dp_waves", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "table_size", "distinct_keys", "total_keys", "max_probe", "lookups", "seeds", "dp_tasks", "dp_calls_exec",
+        "dp_cells_exec", "dp_cells_all", "records_built", "records_kept", "dp_skipped", "vote_oob", "subjects",
+        "queries", "reads_changed")] + [(n, ctypes.c_double) for n in (
+            "ms_index", "ms_seed", "ms_plan", "ms_dp", "ms_score_vote", "ms_consensus", "ms_total")]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class BandPair(ctypes.Structure):
+    _fields_ = [("s_off", ctypes.c_int64), ("q_off", ctypes.c_int64), ("s_len", ctypes.c_int32),
+                ("q_len", ctypes.c_int32), ("band", ctypes.c_int32), ("flag", ctypes.c_int32)]
+
+
+# exported symbols and their signatures (kept in step with include/hero_gpu.h)
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "hero_default_options": (None, [ctypes.POINTER(Options)]),
+    "hero_create": (ctypes.c_int, [ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p)]),
+    "hero_destroy": (None, [_P]),
+    "hero_last_error": (ctypes.c_char_p, [_P]),
+    "hero_set_queries": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
+    "hero_build_index": (ctypes.c_int, [_P]),
+    "hero_align_subjects": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
+    "hero_align_self": (ctypes.c_int, [_P]),
+    "hero_call_consensus": (ctypes.c_int, [_P, _P]),
+    "hero_get_tally": (ctypes.c_int, [_P, _P]),
+    "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "hero_reset_stats": (ctypes.c_int, [_P]),
+    "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
+    "hero_hash_kmers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_uint64, _P]),
+    "hero_lookup": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_int64]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"{_LIB_PATH} is missing: the HIP library must be built (__graft_entry__.build())")
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_options(**kw):
+    o = Options()
+    lib().hero_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class HeroError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Context:
+    """One device context (hero_ctx). Reads are ReadSet-like objects (seq/offset/length/unique_id)."""
+
+    def __init__(self, **opts):
+        self.opts = default_options(**opts)
+        h = ctypes.c_void_p()
+        rc = lib().hero_create(ctypes.byref(self.opts), ctypes.byref(h))
+        if rc != HERO_OK:
+            raise HeroError(rc, "hero_create failed (no usable GPU or invalid options)")
+        self.h = h
+        self._keep = []
+
+    def _check(self, rc):
+        if rc != HERO_OK:
+            raise HeroError(rc, lib().hero_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hero_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def set_queries(self, rs):
+        self._queries = rs
+        self._check(lib().hero_set_queries(self.h, rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data,
+                                           rs.length.ctypes.data, rs.unique_id.ctypes.data))
+
+    def build_index(self):
+        self._check(lib().hero_build_index(self.h))
+
+    def align_subjects(self, rs):
+        self._check(lib().hero_align_subjects(self.h, rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data,
+                                              rs.length.ctypes.data, rs.unique_id.ctypes.data))
+
+    def align_self(self):
+        self._check(lib().hero_align_self(self.h))
+
+    def consensus(self):
+        out = np.empty_like(self._queries.seq)
+        self._check(lib().hero_call_consensus(self.h, out.ctypes.data))
+        return out
+
+    def tally(self):
+        t = np.zeros(4 * len(self._queries.seq), dtype=np.uint32)
+        self._check(lib().hero_get_tally(self.h, t.ctypes.data))
+        return t
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().hero_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(lib().hero_reset_stats(self.h))
+
+    def band_align(self, pairs):
+        """pairs: list of (S, Q, band, flag) strings. Returns list of (cigar, band)."""
+        seq = "".join(s + q for s, q, _, _ in pairs).encode()
+        arr = (BandPair * len(pairs))()
+        off = 0
+        for i, (s, q, b, f) in enumerate(pairs):
+            arr[i] = BandPair(off, off + len(s), len(s), len(q), b, f)
+            off += len(s) + len(q)
+        n = len(pairs)
+        cap = sum(len(s) + len(q) for s, q, _, _ in pairs) + 4
+        band = np.zeros(n, dtype=np.int32)
+        coff = np.zeros(n, dtype=np.int64)
+        cn = np.zeros(n, dtype=np.int32)
+        cc = np.zeros(cap, dtype=np.int32)
+        ct = np.zeros(cap, dtype=np.uint8)
+        buf = ctypes.create_string_buffer(seq, len(seq) + 1)
+        self._check(lib().hero_band_align_batch(self.h, n, ctypes.addressof(arr), buf, band.ctypes.data,
+                                                coff.ctypes.data, cn.ctypes.data, cc.ctypes.data, ct.ctypes.data, cap))
+        out = []
+        for i in range(n):
+            o = int(coff[i])
+            out.append(("".join(f"{cc[o + t]}{chr(ct[o + t])}" for t in range(cn[i])), int(band[i])))
+        return out
+
+    def hash_kmers(self, kmers, table_size):
+        k = len(kmers[0])
+        buf = "".join(kmers).encode()
+        out = np.zeros(len(kmers), dtype=np.uint64)
+        self._check(lib().hero_hash_kmers(self.h, len(kmers), k, buf, table_size, out.ctypes.data))
+        return out
+
+    def lookup(self, windows):
+        n = len(windows)
+        buf = "".join(windows).encode()
+        cnt = np.zeros(n, dtype=np.int64)
+        off = np.zeros(n, dtype=np.int64)
+        cap = 1 << 20
+        ent = np.zeros(cap, dtype=np.uint64)
+        self._check(lib().hero_lookup(self.h, n, buf, cnt.ctypes.data, off.ctypes.data, ent.ctypes.data, cap))
+        return [ent[off[i]:off[i] + cnt[i]].copy() for i in range(n)]
+
+
+def correct(rs, subjects=None, **opts):
+    """Self-correction (or queries vs a subject set) in one call; returns (corrected bytes, stats)."""
+    with Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        if subjects is None:
+            ctx.align_self()
+        else:
+            ctx.align_subjects(subjects)
+        out = ctx.consensus() if ctx.opts.call_consensus else None
+        return out, ctx.stats()

@@ -1,0 +1,129 @@
+/*
+ * hero_gpu.h — C ABI of the MI355X (gfx950) implementation of HERO's read-correction hot path:
+ * k-mer seed index and lookup, banded global / affine DP, alignment records, votes, consensus.
+ *
+ * Plain pointers and sizes only; all device memory is owned by the context. Every function
+ * returns HERO_OK (0) or a negative status; hero_last_error() describes the last failure.
+ * Calls on one context must not overlap (one context per host thread).
+ *
+ * Each entry point names the reference interface it replaces (guo-xuan/Hero, src/).
+ */
+#ifndef HERO_GPU_H
+#define HERO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HERO_OK 0
+#define HERO_ERR_ARG (-1)        /* invalid argument / unsupported input */
+#define HERO_ERR_HIP (-2)        /* HIP runtime or kernel launch failure */
+#define HERO_ERR_NOMEM (-3)      /* device allocation failed / admission refused */
+#define HERO_ERR_STATE (-4)      /* call out of order (e.g. align before the index exists) */
+#define HERO_ERR_LIMIT (-5)      /* input beyond a documented limit */
+#define HERO_ERR_REF_ABORT (-6)  /* input on which the reference aborts (see hero_last_error) */
+
+typedef struct hero_ctx hero_ctx;
+
+/* Run options. The first block mirrors the reference's Config fields read on the hot path
+ * (src/Config.cpp:17-56 defaults, flags parsed at :118-243) and BandAligner::setCost
+ * (src/BandAligner.cpp:528-542); hero_default_options() fills the reference defaults. */
+typedef struct {
+	int32_t k;              /* Config::iKmer (-k), 1..64 */
+	int32_t aligner;        /* Config::iAlignerType (-a): 0 affine, 1 global */
+	int32_t call_consensus; /* Config::bCallConsensus (-c) */
+	int32_t hash_factor;    /* Config::iHashTableFactor (-f) */
+	int64_t min_overlap;    /* Config::minimumOverlapLength (-l) */
+	double mismatch_rate;   /* Config::dMaxMismatchRate (-m / 100) */
+	double indel_rate;      /* Config::dMaxIndelRate (-i / 100) */
+	double consensus_freq;  /* Config::dMinFrequencyCallConsensus (-c <f>) */
+	int32_t dx, dy, ex, sx, miss, hit, unobserved; /* BandAligner::setCost */
+	/* knobs the reference does not have (0 = reference behaviour) */
+	int32_t band_override;  /* >0: setBand(band_override) instead of setBand(2*extra) */
+	int32_t max_candidates; /* >0: align only the first N query reads per subject strand */
+	/* execution */
+	int32_t device;         /* HIP device ordinal */
+	int32_t dp_form;        /* 0 batched (lane per pair, length buckets), 1 plain (wavefront per pair) */
+	int64_t chunk_subjects; /* subject reads per device pass; 0 = automatic */
+	int32_t dp_waves;       /* persistent DP wavefronts; 0 = automatic */
+	int32_t reserved[7];
+} hero_options;
+
+/* Counters and device timings (milliseconds, HIP events on the context's stream). */
+typedef struct {
+	uint64_t table_size, distinct_keys, total_keys, max_probe;
+	uint64_t lookups, seeds, dp_tasks, dp_calls_exec, dp_cells_exec, dp_cells_all;
+	uint64_t records_built, records_kept, dp_skipped, vote_oob, subjects, queries, reads_changed;
+	double ms_index, ms_seed, ms_plan, ms_dp, ms_score_vote, ms_consensus, ms_total;
+} hero_stats;
+
+void hero_default_options(hero_options *opt);
+
+/* Context; replaces the process-wide Config + per-thread BandAligner scratch. */
+int hero_create(const hero_options *opt, hero_ctx **out);
+void hero_destroy(hero_ctx *ctx);
+const char *hero_last_error(const hero_ctx *ctx);
+
+/* Query reads (QueryDataset::loadDatasetParallel result, src/QueryDataset.cpp:103-244): upper-case
+ * ASCII `seq` with per-read offset/length and uniqueId = atoi(name). Good reads
+ * (len > minOverlap && qualityFilter, src/QueryDataset.cpp:60-75,226) are indexed and corrected.
+ * Resets the vote tallies. */
+int hero_set_queries(hero_ctx *ctx, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id);
+
+/* HashTableMethod::createHashTables + insertQueryDataset (src/HashTableLongKey.cpp:41-68,250-342). */
+int hero_build_index(hero_ctx *ctx);
+
+/* SubjectReadAligner::start (src/SubjectReadAligner.cpp:493-645) over one batch of subject reads
+ * (SubjectDataset::loadNextChunkParallel chunks, src/SubjectDataset.cpp:45-179): seeds, chains,
+ * banded DP, records, dedup, and (under -c) votes into the query tallies. May be called
+ * repeatedly; votes accumulate. */
+int hero_align_subjects(hero_ctx *ctx, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id);
+
+/* Same, with the query set itself as the subject stream (self-correction: -q f -s f); no second
+ * upload. */
+int hero_align_self(hero_ctx *ctx);
+
+/* QueryDataset::callConsensus (src/QueryDataset.cpp:393-412, QueryRead.cpp:148-180): writes the
+ * corrected reads in the layout of hero_set_queries' `seq` (reads that are not good are copied
+ * unchanged, as the reference writes them). */
+int hero_call_consensus(hero_ctx *ctx, char *corrected);
+
+/* The vote matrices (QueryRead::aCoverageMatrix, 4 u32 per base in A,C,T,G order) as they stand
+ * after the subject passes, before consensus adds the self vote. */
+int hero_get_tally(hero_ctx *ctx, uint32_t *tally);
+
+int hero_get_stats(hero_ctx *ctx, hero_stats *out);
+int hero_reset_stats(hero_ctx *ctx);
+
+/* BandAligner drop-in (src/BandAligner.h:20-92): n independent calls of
+ *   setBand(band); setSequences(S[s_off, s_off+s_len), Q[q_off, q_off+q_len)); flag 1 setAlignLeft /
+ *   2 setAlignRight; calculateAlignment();
+ * over bytes A/C/G/T/N in `seq`. Outputs per call the widened m_band and the CIGAR of
+ * getCigarType()/getCigarCount() (mismatch ops typed by the query base, unmerged) at
+ * cigar_types/cigar_counts[cigar_off[i] .. cigar_off[i] + cigar_n[i]); cigar_cap bounds the total. */
+typedef struct {
+	int64_t s_off, q_off;
+	int32_t s_len, q_len;
+	int32_t band, flag;
+} hero_band_pair;
+int hero_band_align_batch(hero_ctx *ctx, int32_t n, const hero_band_pair *pairs, const char *seq, int32_t *band_out,
+		int64_t *cigar_off, int32_t *cigar_n, int32_t *cigar_counts, char *cigar_types, int64_t cigar_cap);
+
+/* HashTableLongKey::hashFunction (src/HashTableLongKey.cpp:188-206) on the device for n k-mers
+ * of length k packed back to back in `kmers` (A/C/G/T). */
+int hero_hash_kmers(hero_ctx *ctx, int32_t n, int32_t k, const char *kmers, uint64_t table_size, uint64_t *out);
+
+/* HashTableLongKey::getListOfReads (src/HashTableLongKey.cpp:79-99) on the device: for n windows
+ * of length k, the bucket size and its entries ((pos << 40) | innerId, insertion order) packed at
+ * entries[entry_off[i] ..]; entry_cap bounds the total. */
+int hero_lookup(hero_ctx *ctx, int32_t n, const char *windows, int64_t *count, int64_t *entry_off, uint64_t *entries,
+		int64_t entry_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,216 @@
+"""Writes the golden fixtures of tests/golden/ from the reference compiled in this container.
+
+Inputs are generated here (seeded); outputs come from the reference's own code:
+  oracle/_ref/ref_probe  (tests/golden/ref_probe.cpp over the reference's BandAligner,
+                          HashTableLongKey, AlignmentRecord and QueryRead objects)
+  oracle/_ref/hero_ref   (the reference program, run on seeded synthetic FASTA)
+Run `make -C oracle ref` first. The fixtures are data only (inputs and expected outputs).
+
+  dp_cases.jsonl       BandAligner calls: S, Q, band, flag, aligner -> widened band, CIGAR
+  hash_cases.jsonl     hashFunction(kmer) for table sizes P
+  record_cases.jsonl   AlignmentRecord scoring / acceptance over CIGAR op lists
+  consensus_cases.jsonl QueryRead::callConsensus on given tallies
+  e2e.json             reference program runs: synthetic read-set parameters, flags, and sha256 /
+                       per-read digests of the corrected FASTA and the .aln output
+"""
+import hashlib
+import json
+import os
+import random
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+PROBE = os.path.join(ROOT, "oracle", "_ref", "ref_probe")
+HERO_REF = os.path.join(ROOT, "oracle", "_ref", "hero_ref")
+
+
+def probe(lines):
+    p = subprocess.run([PROBE], input="\n".join(lines) + "\n", capture_output=True, text=True, check=True)
+    return p.stdout.strip("\n").split("\n")
+
+
+def rand_seq(rng, n, alphabet="ACGT"):
+    return "".join(rng.choice(alphabet) for _ in range(n))
+
+
+def mutate(rng, s, rate, alphabet="ACGT"):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < rate / 3:
+            continue  # deletion
+        if r < 2 * rate / 3:
+            out.append(rng.choice(alphabet))  # insertion
+        out.append(rng.choice(alphabet) if rng.random() < rate else c)
+    return "".join(out)
+
+
+def dp_cases(rng, n_per_aligner=1500):
+    cases = []
+    for aligner in (1, 0):
+        for _ in range(n_per_aligner):
+            L = rng.choice([1, 2, 3, 5, 8, 13, 21, 34, 55, 80, 120])
+            base = rand_seq(rng, L)
+            other = mutate(rng, base, rng.choice([0.0, 0.02, 0.05, 0.15, 0.4]))
+            if not other:
+                other = rand_seq(rng, 1)
+            if aligner == 1 and rng.random() < 0.15:  # N in the global aligner (unobserved_match)
+                pos = rng.randrange(len(base))
+                base = base[:pos] + "N" + base[pos + 1:]
+            S, Q = (base, other) if rng.random() < 0.5 else (other, base)
+            band = rng.choice([0, 1, 2, 3, 4, 5, 6, 8, 10, 16, 24])
+            flag = rng.choice([0, 1, 2])
+            cases.append(dict(S=S, Q=Q, band=band, flag=flag, aligner=aligner))
+    # edges: equal strings, length-1 sides, large bands
+    for aligner in (1, 0):
+        for S, Q in [("A", "A"), ("A", "C"), ("ACGT", "ACGT"), ("ACGTACGT", "A"), ("A", "ACGTACGT"),
+                     ("AAAAAAAAAA", "TTTTTTTTTT"), ("ACGTTGCA", "ACGTGCA")]:
+            for band in (0, 1, 4, 16):
+                for flag in (0, 1, 2):
+                    cases.append(dict(S=S, Q=Q, band=band, flag=flag, aligner=aligner))
+    lines = []
+    cur = None
+    for c in cases:
+        if c["aligner"] != cur:
+            lines.append(f"T {c['aligner']}")
+            cur = c["aligner"]
+        lines.append(f"B {c['band']}")
+        lines.append(f"D {c['S'] or '-'} {c['Q'] or '-'} {c['flag']}")
+    out = probe(lines)
+    res = [o for o in out if o != "ok"]
+    assert len(res) == len(cases), (len(res), len(cases))
+    for c, r in zip(cases, res):
+        b, cig = r.split(" ")
+        c["out_band"] = int(b)
+        c["cigar"] = cig
+    return cases
+
+
+def hash_cases(rng, n=600):
+    cases = []
+    primes = [3359, 498857, 950497, 1114523, 54526019, 1140852767, 2281701827, 4294967311, 12884902223]
+    for _ in range(n):
+        k = rng.choice([1, 2, 5, 11, 16, 21, 25, 31, 32, 33, 39, 41, 48, 63, 64])
+        cases.append(dict(kmer=rand_seq(rng, k), P=rng.choice(primes)))
+    out = probe([f"H {c['P']} {c['kmer']}" for c in cases])
+    for c, r in zip(cases, out):
+        c["hash"] = int(r)
+    return cases
+
+
+def record_cases(rng, n=600):
+    cases = []
+    for _ in range(n):
+        ops = []
+        for _ in range(rng.randint(1, 9)):
+            t = rng.choice("MMMMIDACGT")
+            ops.append((rng.randint(1, 30), t))
+        cases.append(dict(ops=ops, mis_rate=rng.choice([0.02, 0.04, 0.1]), indel_rate=rng.choice([0.0, 0.02, 0.05]),
+                          cap_mis=rng.randint(0, 6), cap_indel=rng.randint(0, 6)))
+    lines = []
+    for c in cases:
+        lines.append("R %r %r %d %d %s" % (c["mis_rate"], c["indel_rate"], c["cap_mis"], c["cap_indel"],
+                                          " ".join(f"{a}{t}" for a, t in c["ops"])))
+    out = probe(lines)
+    for c, r in zip(cases, out):
+        f = r.split(" ")
+        c.update(nmis=float(f[0]), nindel=float(f[1]), length=float(f[2]), good_caps=int(f[3]), good_rates=int(f[4]),
+                 rate_sum=float(f[5]), merged=f[6])
+    return cases
+
+
+def consensus_cases(rng, n=400):
+    cases = []
+    for _ in range(n):
+        L = rng.randint(1, 12)
+        seq = rand_seq(rng, L)
+        t = [rng.choice([0, 0, 0, 1, 2, 3, 5, 9, 20]) for _ in range(4 * L)]
+        thr = rng.choice([0.0, 0.3, 0.5, 0.6, 0.75, 0.9, 1.0])
+        cases.append(dict(seq=seq, tally=t, thr=thr))
+    out = probe([f"C {c['thr']!r} {c['seq']} " + " ".join(map(str, c["tally"])) for c in cases])
+    for c, r in zip(cases, out):
+        c["out"] = r
+    return cases
+
+
+E2E = [
+    dict(name="c1_global_i2", synth=dict(seed=101, genome_len=20000, len_min=101, coverage=30.0, sub_5p=0.01),
+         flags=["-k", "31", "-m", "4", "-i", "2", "-l", "40", "-a", "g"]),
+    dict(name="c1_affine_i3", synth=dict(seed=101, genome_len=20000, len_min=101, coverage=30.0, sub_5p=0.01),
+         flags=["-k", "31", "-m", "4", "-i", "3", "-l", "40", "-a", "a"]),
+    dict(name="c2_affine", synth=dict(seed=202, genome_len=20100, len_min=150, coverage=50.0, sub_5p=0.005,
+                                      sub_3p=0.02, indel_rate=0.001),
+         flags=["-k", "39", "-m", "4", "-i", "2.67", "-l", "40", "-a", "a"]),
+    dict(name="c3_mixed_k25", synth=dict(seed=303, genome_len=15000, len_min=75, len_max=250, coverage=40.0,
+                                         sub_5p=0.01, indel_rate=0.002),
+         flags=["-k", "25", "-m", "4", "-i", "2.5", "-l", "40", "-a", "g"]),
+    dict(name="c3_mixed_k41", synth=dict(seed=303, genome_len=15000, len_min=75, len_max=250, coverage=40.0,
+                                         sub_5p=0.01, indel_rate=0.002),
+         flags=["-k", "41", "-m", "4", "-i", "2.5", "-l", "40", "-a", "a"]),
+    dict(name="c5_repeats", synth=dict(seed=505, genome_len=30100, len_min=150, coverage=30.0, sub_5p=0.02,
+                                       indel_rate=0.002, repeat_frac=0.2, repeat_seg_len=2000,
+                                       repeat_copies_per_family=5, repeat_div=0.01),
+         flags=["-k", "39", "-m", "4", "-i", "3", "-l", "40", "-a", "a"]),
+]
+
+
+def read_digests(seqs):
+    return [hashlib.blake2b(s.encode(), digest_size=8).hexdigest() for s in seqs]
+
+
+def e2e_cases():
+    from hero_amd import synth
+    out = []
+    for case in E2E:
+        rs = synth.generate(**case["synth"])
+        assert rs.n % 400 != 0, "the reference's subject streamer aborts on multiples of 400 reads"
+        d = tempfile.mkdtemp(prefix="golden_", dir="/tmp")
+        fa = os.path.join(d, "reads.fasta")
+        rs.write_fasta(fa)
+        res = dict(name=case["name"], synth=case["synth"], flags=case["flags"], n_reads=rs.n)
+        for mode in ("c", "aln"):
+            prefix = os.path.join(d, "out_" + mode)
+            cmd = [HERO_REF, "-q", fa, "-s", fa, "-o", prefix, "-t", "8"] + case["flags"]
+            if mode == "c":
+                cmd += ["-c", "--singleReads"]
+            subprocess.run(cmd, capture_output=True, check=True)
+            if mode == "c":
+                text = open(prefix + ".fasta").read()
+                lines = text.split("\n")
+                seqs = [lines[i + 1] for i in range(0, len(lines) - 1, 2)]
+                res["fasta_sha256"] = hashlib.sha256(text.encode()).hexdigest()
+                res["read_digests"] = read_digests(seqs)
+                res["reads_changed"] = sum(1 for i, s in enumerate(seqs) if s != rs.read(i))
+            else:
+                text = open(prefix + ".aln").read()
+                res["aln_sha256"] = hashlib.sha256(text.encode()).hexdigest()
+                res["aln_bytes"] = len(text)
+        out.append(res)
+        print(case["name"], rs.n, res["reads_changed"], flush=True)
+    return out
+
+
+def main():
+    rng = random.Random(20261203)
+    with open(os.path.join(HERE, "dp_cases.jsonl"), "w") as f:
+        for c in dp_cases(rng):
+            f.write(json.dumps(c, separators=(",", ":")) + "\n")
+    with open(os.path.join(HERE, "hash_cases.jsonl"), "w") as f:
+        for c in hash_cases(rng):
+            f.write(json.dumps(c, separators=(",", ":")) + "\n")
+    with open(os.path.join(HERE, "record_cases.jsonl"), "w") as f:
+        for c in record_cases(rng):
+            f.write(json.dumps(c, separators=(",", ":")) + "\n")
+    with open(os.path.join(HERE, "consensus_cases.jsonl"), "w") as f:
+        for c in consensus_cases(rng):
+            f.write(json.dumps(c, separators=(",", ":")) + "\n")
+    with open(os.path.join(HERE, "e2e.json"), "w") as f:
+        json.dump(e2e_cases(), f, separators=(",", ":"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+"""End-to-end correction on the GPU against the reference's digests and the oracle's tallies."""
+import numpy as np
+import pytest
+
+from helpers import fasta_text, flags_to_opts, load_e2e, read_digests, sha256
+from hero_amd import gpu, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", load_e2e(), ids=lambda c: c["name"])
+def test_corrected_reads_match_reference(case):
+    rs = synth.generate(**case["synth"])
+    opts = flags_to_opts(case["flags"])
+    with gpu.Context(call_consensus=1, **opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        tally = ctx.tally()
+        out = ctx.consensus()
+        st = ctx.stats()
+    assert st["vote_oob"] == 0 and st["dp_skipped"] == 0
+    assert read_digests(out, rs) == case["read_digests"]
+    assert sha256(fasta_text(out, rs)) == case["fasta_sha256"]
+    ro = O.run(O.make_cfg(call_consensus=1, **opts), rs, threads=8, want_tally=True)
+    assert np.array_equal(tally, ro["tally"])
+    assert st["dp_calls_exec"] == ro["stats"]["dp_calls"]
+    assert st["dp_cells_exec"] == ro["stats"]["dp_cells"]
+    assert st["records_kept"] == ro["stats"]["records_kept"]
+
+
+KNOBS = [
+    dict(k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02, band_override=5),
+    dict(k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267, dx=-7, dy=-7, max_candidates=16),
+    dict(k=25, aligner=0, mismatch_rate=0.05, indel_rate=0.025),
+]
+
+
+@pytest.mark.parametrize("opts", KNOBS, ids=["band5", "affine_costs_cap", "k25_affine"])
+def test_build_knobs_match_oracle(opts):
+    rs = synth.generate(seed=77, genome_len=12000, len_min=75, len_max=150, coverage=30.0, sub_5p=0.01,
+                        indel_rate=0.002)
+    with gpu.Context(call_consensus=1, min_overlap=40, **opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        tally = ctx.tally()
+        out = ctx.consensus()
+    ro = O.run(O.make_cfg(call_consensus=1, min_overlap=40, **opts), rs, threads=8, want_tally=True)
+    assert np.array_equal(tally, ro["tally"])
+    assert np.array_equal(out, ro["corrected"])
+
+
+def test_plain_form_equals_batched_and_chunking_is_invisible():
+    rs = synth.generate(seed=88, genome_len=10000, len_min=100, len_max=200, coverage=30.0, sub_5p=0.01)
+    base = dict(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40)
+    outs = []
+    for extra in (dict(dp_form=0), dict(dp_form=1), dict(dp_form=0, chunk_subjects=333)):
+        with gpu.Context(**base, **extra) as ctx:
+            ctx.set_queries(rs)
+            ctx.build_index()
+            ctx.align_self()
+            outs.append((ctx.tally(), ctx.consensus()))
+    for t, c in outs[1:]:
+        assert np.array_equal(t, outs[0][0]) and np.array_equal(c, outs[0][1])
+
+
+def test_separate_subject_set_matches_oracle():
+    rs = synth.generate(seed=99, genome_len=10000, len_min=120, coverage=40.0, sub_5p=0.01)
+    q = rs.subset(np.arange(0, rs.n, 3))
+    s = rs.subset(np.arange(1, rs.n, 2))
+    opts = dict(call_consensus=1, k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02, min_overlap=40)
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(q)
+        ctx.build_index()
+        ctx.align_subjects(s)
+        tally = ctx.tally()
+        out = ctx.consensus()
+    ro = O.run(O.make_cfg(**opts), q, s, threads=8, want_tally=True)
+    assert np.array_equal(tally, ro["tally"])
+    assert np.array_equal(out, ro["corrected"])
+
+
+def test_edge_inputs():
+    # N-containing, low-complexity, shorter-than-overlap and shorter-than-k reads are filtered exactly
+    # as the reference's predicates do; uncorrected reads come back unchanged.
+    rng = np.random.default_rng(5)
+    base = "".join(rng.choice(list("ACGT"), 3000))
+    reads = [base[i:i + 100] for i in range(0, 2900, 7)]
+    reads += ["N" + base[10:109], "A" * 100, base[50:80], base[60:70], "ACGTN" * 20, base[200:300].lower().upper()]
+    rs = synth.ReadSet.from_strings(reads)
+    opts = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.02, min_overlap=40)
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        tally = ctx.tally()
+        out = ctx.consensus()
+    ro = O.run(O.make_cfg(**opts), rs, threads=4, want_tally=True)
+    assert np.array_equal(tally, ro["tally"])
+    assert np.array_equal(out, ro["corrected"])
+
+
+def test_empty_and_tiny_sets():
+    rs = synth.ReadSet.from_strings([])
+    with gpu.Context(call_consensus=1, k=21, min_overlap=40) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        assert len(ctx.consensus()) == 0
+    rs = synth.ReadSet.from_strings(["ACGT" * 20])
+    with gpu.Context(call_consensus=1, k=21, min_overlap=40) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        assert ctx.consensus().tobytes() == b"ACGT" * 20

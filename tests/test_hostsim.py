@@ -1,0 +1,45 @@
+"""The device logic (hero_amd/csrc/*_core.h) run on the CPU against the oracle."""
+import numpy as np
+import pytest
+
+from helpers import load_jsonl
+from hero_amd import synth
+from oracle import oracle as O
+import hostsim as H
+
+
+def test_dp_core_known_answers():
+    bad = []
+    for c in load_jsonl("dp_cases.jsonl"):
+        cfg = O.make_cfg(aligner=c["aligner"])
+        cig, band = H.band_align(cfg, c["S"], c["Q"], c["band"], c["flag"])
+        if (cig, band) != (c["cigar"], c["out_band"]):
+            bad.append((c, cig, band))
+    assert not bad, bad[:3]
+
+
+CASES = [
+    dict(rs=dict(seed=11, genome_len=8000, len_min=101, coverage=25.0, sub_5p=0.01),
+         cfg=dict(k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02)),
+    dict(rs=dict(seed=12, genome_len=8000, len_min=150, coverage=30.0, sub_5p=0.005, sub_3p=0.02, indel_rate=0.002),
+         cfg=dict(k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267)),
+    dict(rs=dict(seed=13, genome_len=8000, len_min=75, len_max=250, coverage=30.0, sub_5p=0.01, indel_rate=0.002),
+         cfg=dict(k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025)),
+    dict(rs=dict(seed=14, genome_len=8000, len_min=75, len_max=250, coverage=30.0, sub_5p=0.01, indel_rate=0.002),
+         cfg=dict(k=41, aligner=0, mismatch_rate=0.05, indel_rate=0.03, dx=-7, dy=-7)),
+    dict(rs=dict(seed=15, genome_len=8000, len_min=101, coverage=25.0, sub_5p=0.01),
+         cfg=dict(k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02, band_override=5, max_candidates=8)),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
+def test_pipeline_core_matches_oracle(case):
+    rs = synth.generate(**case["rs"])
+    cfg = O.make_cfg(call_consensus=1, min_overlap=40, **case["cfg"])
+    ro = O.run(cfg, rs, threads=4, want_tally=True)
+    rh = H.run(cfg, rs)
+    assert np.array_equal(ro["tally"], rh["tally"])
+    assert np.array_equal(ro["corrected"], rh["corrected"])
+    for k in ("dp_calls", "dp_cells", "records_built", "records_kept", "seeds"):
+        assert ro["stats"][k] == rh["stats"][k], k
+    assert rh["stats"]["oob"] == 0
