@@ -1,0 +1,242 @@
+"""Benchmark of the read-correction hot path on one MI355X.
+
+Workload (step = one pass of the hot path over the whole read set, self-correction):
+  5 Mb uniform-random genome, 150 bp reads at 50x (1,666,667 reads), substitution rate rising
+  linearly from 0.5% (5' end) to 2% (3' end), 0.1% 1-bp indels; k = 39 seeds; affine banded DP
+  (match +1, mismatch -4, gap open -6, extend -1), band half-width 8, at most 64 candidate query
+  reads per subject strand; consensus threshold 0.6.
+A step = index build + seed search / chaining / DP / records / votes over all subjects +
+consensus, with the packed reads already resident in HBM; the corrected bases end in host memory.
+
+Prints one JSON line (see the keys below). Extra measurements:
+  gcups            executed in-band DP cells / DP kernel time, batched form vs plain form, on this
+                   workload and on a mixed-length workload (75-250 bp, band 5% of length)
+  roofline         DP kernel: integer VALU ops (18 per affine cell) per second vs the VALU peak
+  roofline_probe   seed-probe kernel: algorithmic HBM bytes per second vs 8 TB/s
+  cpu_baseline     the reference program (oracle/_ref/hero_ref) or the CPU oracle on host cores
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from hero_amd import gpu, synth  # noqa: E402
+
+METRIC = "banded-DP GCUPS + corrected reads/s at 1 GPU vs plain; p50/p99 batch latency"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes/clk x 2.4 GHz int32 ops
+HBM_PEAK_GBS = 8000.0
+AFFINE_OPS_PER_CELL = 18
+GLOBAL_OPS_PER_CELL = 9
+
+C2_READS = synth.WORKLOADS["c2"]
+C2_OPTS = dict(call_consensus=1, k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267, min_overlap=40,
+               dx=-7, dy=-7, ex=-1, miss=-4, hit=1, band_override=8, max_candidates=64, consensus_freq=0.6)
+C3_READS = dict(seed=303, genome_len=1_000_000, len_min=75, len_max=250, coverage=40.0, sub_5p=0.01)
+C3_OPTS = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_steps(ctx, steps):
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.build_index()
+        ctx.align_self()
+        out = ctx.consensus()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1000.0 / steps, ctx.stats(), out
+
+
+def gcups_of(st):
+    return st["dp_cells_exec"] / (st["ms_dp"] / 1000.0) / 1e9 if st["ms_dp"] > 0 else 0.0
+
+
+def one_form(rs, opts, dp_form):
+    with gpu.Context(dp_form=dp_form, **opts) as ctx:
+        ctx.set_queries(rs)
+        ms, st, _ = run_steps(ctx, 1)
+    return st
+
+
+def cpu_threads():
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() else n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def reference_baseline(rs, sample, threads):
+    """The reference program on host cores: all reads indexed, `sample` subjects streamed."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "hero_ref")
+    if not os.path.exists(exe):
+        return None
+    if sample % 400 == 0:  # the reference's subject streamer aborts on multiples of its 400-read chunk
+        sample -= 1
+    d = tempfile.mkdtemp(prefix="hero_cpu_")
+    q = os.path.join(d, "reads.fasta")
+    s = os.path.join(d, "sample.fasta")
+    rs.write_fasta(q)
+    rs.write_fasta(s, 0, sample)
+    cmd = [exe, "-q", q, "-s", s, "-o", os.path.join(d, "out"), "-t", str(threads), "-c", "-k", "39", "-m", "4",
+           "-i", "2.67", "-l", "40", "--singleReads", "-a", "a"]
+    log("cpu baseline (reference):", " ".join(cmd[1:]))
+    t0 = time.perf_counter()
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    wall = time.perf_counter() - t0
+    if p.returncode != 0:
+        log(p.stdout[-2000:], p.stderr[-2000:])
+        return None
+    out = p.stdout
+
+    def grab(pat):
+        m = re.search(pat, out)
+        return float(m.group(1)) if m else float("nan")
+
+    t_index = grab(r"Function insertQueryDataset\(\) finished in ([0-9.e+-]+) Seconds")
+    t_start = grab(r"Function start\(\) finished in ([0-9.e+-]+) Seconds")
+    t_load = grab(r"([0-9.e+-]+) seconds \(Time for loading\)")
+    t_cons = grab(r"([0-9.e+-]+) seconds \(Time for calling consensus\)")
+    t_pass = t_start - t_load
+    n = rs.n
+    est = t_index + t_pass * n / sample + t_cons
+    return dict(value=round(n / est, 1), unit="corrected reads/s", cores=threads, kind="reference",
+                sample=(f"reference -c -k 39 -m 4 -i 2.67 -l 40 -a a (its own costs: open -3/extend -1, band "
+                        f"2*round(0.0267*overlap), no candidate cap) on {cpu_model()}: all {n} reads indexed, first "
+                        f"{sample} subjects streamed; reads/s = N / (index {t_index:.2f} s + subject pass "
+                        f"{t_pass:.2f} s x N/{sample} + consensus {t_cons:.2f} s), from the program's own timers; "
+                        f"wall {wall:.1f} s"))
+
+
+def port_baseline(rs, sample, threads):
+    """The CPU oracle at the exact workload configuration on the same sample."""
+    try:
+        from oracle import oracle as O
+    except Exception:
+        return None
+    sub = rs.subset(np.arange(sample))
+    cfg = O.make_cfg(**{k: v for k, v in C2_OPTS.items()})
+    t0 = time.perf_counter()
+    r = O.run(cfg, rs, sub, threads=threads)
+    wall = time.perf_counter() - t0
+    st = r["stats"]
+    n = rs.n
+    est = st["t_index"] + st["t_align"] * n / sample + st["t_consensus"]
+    cells_rate = st["dp_cells"] / st["t_align"] / 1e9 if st["t_align"] > 0 else 0
+    return dict(value=round(n / est, 1), unit="corrected reads/s", cores=threads, kind="port",
+                sample=(f"CPU oracle (C, OpenMP) at the exact workload options on {cpu_model()}: all {n} reads "
+                        f"indexed, first {sample} subjects aligned; reads/s = N / (index {st['t_index']:.2f} s + "
+                        f"align {st['t_align']:.2f} s x N/{sample} + consensus {st['t_consensus']:.2f} s); "
+                        f"wall {wall:.1f} s"),
+                gcups_incl_seeding=round(cells_rate, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=50_000, help="subjects streamed by the CPU baselines")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain-form and mixed-length runs")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    rs = synth.generate(**C2_READS)
+    log(f"workload: {rs.n} reads x 150 bp generated in {time.perf_counter() - t0:.1f} s")
+    ctx = gpu.Context(**C2_OPTS)
+    ctx.set_queries(rs)  # packed reads resident in HBM before any timed step
+    for _ in range(args.warmup):
+        run_steps(ctx, 1)
+    ms, st, out = run_steps(ctx, args.steps)
+    ctx.close()
+    value = rs.n / (ms / 1000.0)
+    steps = args.steps
+    dp_ms = st["ms_dp"] / steps
+    cells_all = st["dp_cells_all"] / steps
+    gcups_batched = gcups_of(st)
+    log(f"step {ms:.1f} ms: index {st['ms_index'] / steps:.1f}, seeds {st['ms_seed'] / steps:.1f}, plan "
+        f"{st['ms_plan'] / steps:.1f}, dp {dp_ms:.1f}, score+votes {st['ms_score_vote'] / steps:.1f}, consensus "
+        f"{st['ms_consensus'] / steps:.1f} ms; {value:.0f} reads/s; {gcups_batched:.1f} GCUPS")
+
+    achieved = cells_all * AFFINE_OPS_PER_CELL / (dp_ms / 1000.0) / 1e12
+    roofline = dict(bound="valu", kernel="k_dp_batched (all band classes)", achieved=round(achieved, 3),
+                    peak=round(VALU_PEAK_TOPS, 2), unit="Tops/s (int32 VALU)", frac=round(achieved / VALU_PEAK_TOPS, 4),
+                    traffic=None, ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all),
+                    dp_ms_per_step=round(dp_ms, 3))
+    # seed probes: packed subject bases + 24 B per window probe (k > 32) + 8 B per bucket entry read
+    probe_ms = st["ms_seed_probe"] / steps
+    bases = int(rs.length.astype(np.int64).sum())
+    probe_bytes = bases / 4 + (st["lookups"] / steps) * 24 + (st["seeds"] / steps) * 8
+    probe_gbs = probe_bytes / (probe_ms / 1000.0) / 1e9 if probe_ms > 0 else 0.0
+    roofline_probe = dict(bound="hbm", kernel="k_seed_count", achieved=round(probe_gbs, 1), peak=HBM_PEAK_GBS,
+                          unit="GB/s", frac=round(probe_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                          bytes_per_step=int(probe_bytes), ms_per_step=round(probe_ms, 3))
+
+    gcups = dict(batched=round(gcups_batched, 2))
+    if not args.no_compare:
+        stp = one_form(rs, C2_OPTS, 1)
+        gcups["plain"] = round(gcups_of(stp), 2)
+        gcups["batched_over_plain"] = round(gcups_batched / gcups["plain"], 2) if gcups["plain"] else None
+        rs3 = synth.generate(**C3_READS)
+        s3b, s3p = one_form(rs3, C3_OPTS, 0), one_form(rs3, C3_OPTS, 1)
+        gcups["mixed_length"] = dict(workload=f"1 Mb genome, 75-250 bp reads at 40x ({rs3.n} reads), k=25, global, "
+                                              "band 2*round(0.025*overlap)",
+                                     batched=round(gcups_of(s3b), 2), plain=round(gcups_of(s3p), 2),
+                                     batched_over_plain=round(gcups_of(s3b) / max(gcups_of(s3p), 1e-9), 2))
+        log("gcups:", json.dumps(gcups))
+
+    cpu = None
+    port = None
+    if not args.no_cpu:
+        threads = cpu_threads()
+        sample = min(args.cpu_sample, rs.n - 1)
+        cpu = reference_baseline(rs, sample, threads)
+        port = port_baseline(rs, sample, threads)
+        if cpu is None:
+            cpu, port = port, None
+
+    line = dict(metric=METRIC, value=round(value, 1), unit="corrected reads/s", n_gpus=1, steps=steps,
+                warmup=args.warmup, ms_per_step=round(ms, 3), higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="int32", data="synthetic",
+                config=dict(workload=("c2: 5 Mb genome, 150 bp, 50x, 0.5->2% substitutions, 0.1% indels, k=39, affine "
+                                      "+1/-4/-6/-1, band 8, cap 64 candidates, self-correction"),
+                            reads=rs.n, bases=bases, parallelism="1 GPU"),
+                gcups=gcups, latency=None, roofline=roofline, roofline_probe=roofline_probe, cpu_baseline=cpu,
+                cpu_baseline_port=port,
+                stages_ms=dict(index=round(st["ms_index"] / steps, 3), seed=round(st["ms_seed"] / steps, 3),
+                               plan=round(st["ms_plan"] / steps, 3), dp=round(dp_ms, 3),
+                               score_vote=round(st["ms_score_vote"] / steps, 3),
+                               consensus=round(st["ms_consensus"] / steps, 3)),
+                counts=dict(dp_tasks=st["dp_tasks"] // steps, dp_calls_exec=st["dp_calls_exec"] // steps,
+                            dp_cells_exec=st["dp_cells_exec"] // steps, records_kept=st["records_kept"] // steps,
+                            reads_changed=st["reads_changed"] // steps))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

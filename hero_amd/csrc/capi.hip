@@ -149,11 +149,14 @@ struct hero_ctx {
 	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
 	DevBuf<DpTask> tasks;
 	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
+	DevBuf<uint32_t> bucket_hist, bucket_off, order;
+	DevBuf<int32_t> g_nch, g_best;
+	DevBuf<StepSummary> sums;
 	DevBuf<unsigned long long> dstats, dtotal;
 	DevBuf<uint32_t> counter;
 	// stats
 	hero_stats st_host;
-	Timer ev[8];
+	Timer ev[10];
 };
 
 namespace {
@@ -254,6 +257,7 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	};
 	HIPCHK(c, hipEventRecord(c->ev[0].a, c->st));
 	HIPCHK(c, launch_seed_count(S, Q, ix, c->o, cd, stats, c->st));
+	HIPCHK(c, hipEventRecord(c->ev[8].a, c->st));
 	uint64_t nseeds = 0;
 	if (int r = scan(cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
 	if (nseeds >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu seeds; lower chunk_subjects", nsub, (unsigned long long) nseeds);
@@ -264,12 +268,15 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	HIPCHK(c, c->clast.ensure(ns));
 	HIPCHK(c, c->clen.ensure(ns));
 	HIPCHK(c, c->gflag.ensure(ns));
+	HIPCHK(c, c->g_nch.ensure(ns));
+	HIPCHK(c, c->g_best.ensure(ns));
 	HIPCHK(c, c->g_tasks.ensure(ns));
 	HIPCHK(c, c->g_task_off.ensure(ns));
 	HIPCHK(c, c->g_words.ensure(ns));
 	HIPCHK(c, c->g_word_off.ensure(ns));
 	cd.seeds = c->seeds.p, cd.cid = c->cid.p, cd.cfirst = c->cfirst.p, cd.clast = c->clast.p, cd.clen = c->clen.p;
 	cd.gflag = c->gflag.p, cd.g_tasks = c->g_tasks.p, cd.g_task_off = c->g_task_off.p;
+	cd.g_nch = c->g_nch.p, cd.g_best = c->g_best.p;
 	cd.g_words = c->g_words.p, cd.g_word_off = c->g_word_off.p;
 	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, c->st));
 	HIPCHK(c, hipEventRecord(c->ev[1].a, c->st));
@@ -281,32 +288,65 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	HIPCHK(c, c->tasks.ensure((size_t) ntasks));
 	HIPCHK(c, c->words.ensure((size_t) nwords + 1));
 	HIPCHK(c, c->nsteps.ensure((size_t) ntasks));
-	cd.tasks = c->tasks.p, cd.words = c->words.p, cd.nsteps = c->nsteps.p;
+	HIPCHK(c, c->sums.ensure((size_t) ntasks));
+	cd.tasks = c->tasks.p, cd.words = c->words.p, cd.nsteps = c->nsteps.p, cd.sums = c->sums.p;
 	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, c->st));
 	HIPCHK(c, hipEventRecord(c->ev[2].a, c->st));
 	// DP
 	DpLaunch L;
 	std::memset(&L, 0, sizeof L);
 	L.tasks = cd.tasks, L.ntasks = (uint32_t) ntasks, L.order = nullptr, L.S = S, L.Q = Q, L.o = c->o;
-	L.words = cd.words, L.nsteps = cd.nsteps, L.stats = stats, L.batch_counter = c->counter.p;
-	const int nwaves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 16;
-	const int bmax = 16, nw = (2 * bmax + 1 + 15) / 16;
+	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = c->counter.p;
+	const bool aff = c->o.aligner == 0;
 	const int rows = std::min(rows_needed, 512);
+	const int bmax_plain = 64;
+	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
 	L.route_bmax = -1, L.route_rows = 0;
-	if (c->opt.dp_form == 0) {
-		HIPCHK(c, c->dp_scratch.ensure((size_t) nwaves * rows * nw * 64));
-		L.scratch = c->dp_scratch.p, L.rows_max = rows;
-		HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
-		HIPCHK(c, launch_dp_batched(L, bmax, nwaves, c->st));
-		L.route_bmax = bmax, L.route_rows = rows;
+	uint32_t plain_begin = 0, plain_n = (uint32_t) ntasks;
+	if (c->opt.dp_form == 0 && ntasks > 0) {
+		// batched form: bucket tasks by (band class, descending length), then one persistent launch
+		// per class at the smallest band width that holds it
+		HIPCHK(c, c->bucket_hist.ensure(kNumBuckets));
+		HIPCHK(c, c->bucket_off.ensure(kNumBuckets));
+		HIPCHK(c, c->order.ensure((size_t) ntasks));
+		HIPCHK(c, hipMemsetAsync(c->bucket_hist.p, 0, kNumBuckets * 4, c->st));
+		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, c->bucket_hist.p, c->st));
+		uint64_t total = 0;
+		if (int r = scan(c->bucket_hist.p, c->bucket_off.p, kNumBuckets, total)) return r;
+		std::vector<uint32_t> off(kNumBuckets);
+		HIPCHK(c, hipMemcpy(off.data(), c->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, c->bucket_off.p, c->order.p, c->st));
+		HIPCHK(c, hipEventRecord(c->ev[7].a, c->st));
+		for (int cls = 0; cls <= kNumBandClasses; cls++) {
+			const uint32_t b0 = off[(size_t) cls * kLenBins];
+			const uint32_t b1 = cls + 1 <= kNumBandClasses ? off[(size_t) (cls + 1) * kLenBins] : (uint32_t) ntasks;
+			if (cls == kNumBandClasses) {
+				plain_begin = b0, plain_n = b1 - b0;
+				break;
+			}
+			if (b1 == b0) continue;
+			const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
+			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * dp_batched_waves_per_cu(bm, aff);
+			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 63) / 64);
+			HIPCHK(c, c->dp_scratch.ensure((size_t) waves * rows * nw * 64));
+			DpLaunch Lc = L;
+			Lc.order = c->order.p + b0, Lc.ntasks = b1 - b0;
+			Lc.scratch = c->dp_scratch.p, Lc.rows_max = rows;
+			HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
+			HIPCHK(c, launch_dp_batched(Lc, bm, waves, c->st));
+		}
+	} else {
+		HIPCHK(c, hipEventRecord(c->ev[7].a, c->st));
 	}
-	{
-		// plain form: all tasks, or (batched form) the tasks with band > 16 or rows beyond the scratch
-		const int bmax_plain = 64;
-		HIPCHK(c, c->plain_scratch.ensure((size_t) nwaves * 512 * (2 * bmax_plain + 1) / 4 + 64));
-		L.scratch = c->plain_scratch.p, L.rows_max = 512;
+	if (plain_n > 0) {
+		// plain form: every task (dp_form 1), or the bucket beyond the batched band classes
+		HIPCHK(c, c->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+		DpLaunch Lp = L;
+		Lp.order = c->opt.dp_form == 0 ? c->order.p + plain_begin : nullptr;
+		Lp.ntasks = plain_n;
+		Lp.scratch = c->plain_scratch.p, Lp.rows_max = 512;
 		HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
-		HIPCHK(c, launch_dp_plain(L, bmax_plain, c->opt.dp_form == 0 ? std::max(1, nwaves / 16) : nwaves, c->st));
+		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), c->st));
 	}
 	HIPCHK(c, hipEventRecord(c->ev[3].a, c->st));
 	{
@@ -319,8 +359,9 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	HIPCHK(c, hipEventRecord(c->ev[4].a, c->st));
 	HIPCHK(c, hipStreamSynchronize(c->st));
 	c->st_host.ms_seed += elapsed(c, 0, 1);
-	c->st_host.ms_plan += elapsed(c, 1, 2);
-	c->st_host.ms_dp += elapsed(c, 2, 3);
+	c->st_host.ms_seed_probe += elapsed(c, 0, 8);
+	c->st_host.ms_plan += elapsed(c, 1, 7); // pass A2 + task bucketing
+	c->st_host.ms_dp += elapsed(c, 7, 3);   // DP kernels only
 	c->st_host.ms_score_vote += elapsed(c, 3, 4);
 	c->st_host.dp_tasks += ntasks;
 	c->st_host.subjects += (uint64_t) nsub;
@@ -394,6 +435,8 @@ void hero_destroy(hero_ctx *c) {
 	c->g_tasks.release(), c->g_task_off.release(), c->g_words.release(), c->g_word_off.release();
 	c->tasks.release(), c->words.release(), c->nsteps.release(), c->dp_scratch.release(), c->plain_scratch.release();
 	c->dstats.release(), c->dtotal.release(), c->counter.release(), c->cons_out.release();
+	c->bucket_hist.release(), c->bucket_off.release(), c->order.release();
+	c->g_nch.release(), c->g_best.release(), c->sums.release();
 	for (auto &t : c->ev)
 		if (t.a) (void) hipEventDestroy(t.a);
 	if (c->st) (void) hipStreamDestroy(c->st);
@@ -624,6 +667,7 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	out->vote_oob = s[kStOob];
 	out->dp_skipped = s[kStDpSkipped];
 	out->reads_changed = s[kStReadsChanged];
+	out->dp_cells_all = s[kStCellsAll];
 	out->ms_total = out->ms_index + out->ms_seed + out->ms_plan + out->ms_dp + out->ms_score_vote + out->ms_consensus;
 	return HERO_OK;
 }

@@ -26,8 +26,10 @@ HD int diag_score_global(const Opts &o, int a, int q) {
 }
 
 // One cell of the recurrence. d0 / dQ flag the cell as column 0 / column lenQ; *_diag, *_up,
-// *_left are the neighbours (kSent when absent). Writes the cell's values and pointer code
-// (the reference's BM1: 2 diagonal, 1 left, 0 up).
+// *_left are the neighbours (kSent when absent). Writes the cell's values and pointer code:
+// the reference's BM1 (0 up, 1 left, 2 diagonal) with the diagonal split into 2 (the two
+// characters are equal, a CIGAR 'M') and 3 (they differ, a mismatch op), so the traceback needs
+// no sequence access.
 template <bool AFFINE>
 HD void cell_update(const Opts &o, int sc, int qc, bool col0, bool colQ, bool left_al, bool right_al, int lenQ,
 		int h_diag, int x_diag, int y_diag, int h_up, int x_up, int h_left, int y_left, int &h, int &x, int &y, int &p) {
@@ -38,7 +40,7 @@ HD void cell_update(const Opts &o, int sc, int qc, bool col0, bool colQ, bool le
 		const int up = h_up + g;
 		const int lf = h_left + o.dy;
 		const int dg = h_diag + diag_score_global(o, sc, qc);
-		if (dg >= lf && dg >= up) h = dg, p = 2;
+		if (dg >= lf && dg >= up) h = dg, p = 2 + (sc != qc);
 		else if (lf >= up) h = lf, p = 1;
 		else h = up, p = 0;
 		x = kSent, y = kSent;
@@ -53,7 +55,7 @@ HD void cell_update(const Opts &o, int sc, int qc, bool col0, bool colQ, bool le
 		else x = imax_(h_up + o.dx, x_up + o.ex);
 		const int s = sc != qc ? o.miss : o.hit;
 		h = imax_(imax_(h_diag + s, x_diag + s), y_diag + s);
-		p = (h > x && h >= y) ? 2 : (y > x ? 1 : 0);
+		p = (h > x && h >= y) ? 2 + (sc != qc) : (y > x ? 1 : 0);
 	}
 }
 
@@ -136,9 +138,9 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 }
 
 // Traceback (src/BandAligner.cpp:114-125,227-238): from (lenS, lenQ) follow the pointers to the
-// origin. Steps come last-to-first through emit(step): kStepMatch for a diagonal (the caller
-// tells match from mismatch), kStepLeft (j-1), kStepUp (i-1). Returns the step count, or -1 if
-// the path leaves the band (never for pointers written by dp_forward).
+// origin. Steps come last-to-first through emit(step): kStepMatch / kStepMismatch for a diagonal,
+// kStepLeft (j-1), kStepUp (i-1). Returns the step count, or -1 if the path leaves the band
+// (never for pointers written by dp_forward).
 template <class PS, class EM>
 HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 	int i = lenS, j = lenQ, n = 0;
@@ -147,8 +149,8 @@ HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 		if (i < 0 || j < 0 || d < 0 || d > 2 * b) return -1; // left the band: corrupt pointers
 		const uint32_t word = ptrs.get(i, d >> 4);
 		const int p = (int) ((word >> (2 * (d & 15))) & 3u);
-		if (p == 2) {
-			emit(kStepMatch);
+		if (p >= 2) {
+			emit(p == 2 ? kStepMatch : kStepMismatch);
 			i--, j--;
 		} else if (p == 1) {
 			emit(kStepLeft);

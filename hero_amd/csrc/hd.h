@@ -167,6 +167,43 @@ HD int edit_words(int lenS, int lenQ) { return (lenS + lenQ + 15) / 16; }
 // Edit step codes written by the DP traceback (aligner orientation).
 enum : uint32_t { kStepMatch = 0, kStepMismatch = 1, kStepLeft = 2, kStepUp = 3 };
 
+// Run summary of one DP call's steps in forward order: the count of each step code and the
+// code / length of the first and last maximal runs. Record scoring needs nothing else (a record's
+// score depends on op totals and on its first / last op when those are I or D).
+struct StepSummary {
+	uint16_t n[4];
+	uint16_t first_len, last_len;
+	uint8_t first_code, last_code;
+	uint16_t _r;
+};
+static_assert(sizeof(StepSummary) == 16, "StepSummary layout");
+
+// Builds a StepSummary from steps arriving last-to-first (the traceback's order).
+struct SummaryBuilder {
+	uint16_t n[4] = {0, 0, 0, 0};
+	int cur = -1, cur_len = 0;
+	int last_code = -1, last_len = 0; // the run emitted first = last in forward order
+	HD void add(uint32_t code) {
+		n[code]++;
+		if ((int) code == cur) {
+			cur_len++;
+			return;
+		}
+		if (cur >= 0 && last_code < 0) last_code = cur, last_len = cur_len;
+		cur = (int) code, cur_len = 1;
+	}
+	HD StepSummary finish() const {
+		StepSummary s;
+		for (int i = 0; i < 4; i++) s.n[i] = n[i];
+		s.first_code = (uint8_t) (cur < 0 ? 0 : cur);
+		s.first_len = (uint16_t) cur_len;
+		s.last_code = (uint8_t) (last_code < 0 ? s.first_code : last_code);
+		s.last_len = (uint16_t) (last_code < 0 ? cur_len : last_len);
+		s._r = 0;
+		return s;
+	}
+};
+
 // In-band cells of one DP call: sum over rows of the row width, minus the origin.
 HD uint64_t band_cells(int lenS, int lenQ, int b) {
 	uint64_t n = 0;

@@ -27,6 +27,8 @@ int64_t hero_synth_num_bases(const void *h);
 int hero_synth_copy(const void *h, char *seq, int64_t *offset, int32_t *length);
 int hero_synth_genome(const void *h, char *out, uint64_t cap);
 void hero_synth_destroy(void *h);
+int hero_write_fasta(const char *path, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id, int64_t r0, int64_t r1);
 
 #ifdef __cplusplus
 }

@@ -32,9 +32,11 @@ struct ChunkDev {
 	uint64_t *seeds;
 	int32_t *cid, *cfirst, *clast, *clen;
 	uint8_t *gflag; // bit0 group start, bit1 group kept (within max_candidates)
+	int32_t *g_nch, *g_best; // chain count and best covered length of a kept group
 	uint32_t *g_tasks, *g_task_off, *g_words, *g_word_off;
 	DpTask *tasks;
 	uint32_t *words, *nsteps;
+	StepSummary *sums;
 };
 
 // Stats slots (u64 atomics on the device).
@@ -48,6 +50,7 @@ enum StatSlot : int {
 	kStOob,
 	kStDpSkipped,
 	kStReadsChanged,
+	kStCellsAll,
 	kStCount
 };
 
@@ -58,6 +61,7 @@ struct DpLaunch {
 	Packed S, Q;
 	Opts o;
 	uint32_t *words, *nsteps;
+	StepSummary *sums;
 	uint32_t *scratch;
 	int rows_max;
 	int route_bmax, route_rows; // plain form: >= 0 restricts it to tasks the batched form cannot take
@@ -85,6 +89,18 @@ hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const
 hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
 		uint32_t *tally, unsigned long long *stats, hipStream_t st);
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st);
+int dp_batched_waves_per_cu(int bmax, bool affine); // occupancy of one batched instantiation
+
+// Band classes of the batched form; tasks above the last class (or with rows beyond the pointer
+// scratch) go to the plain form. Buckets are (class, 1023 - min(lenS, 1023)): one class at a time,
+// longest rows first, so the 64 lanes of a wavefront run tasks of similar length.
+constexpr int kNumBandClasses = 6;
+constexpr int kBandClass[kNumBandClasses] = {4, 8, 12, 16, 24, 32};
+constexpr int kLenBins = 1024;
+constexpr int kNumBuckets = (kNumBandClasses + 1) * kLenBins;
+hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *hist, hipStream_t st);
+hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *cursor, uint32_t *order,
+		hipStream_t st);
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
 		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st);

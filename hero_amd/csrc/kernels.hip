@@ -120,13 +120,20 @@ hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slo
 __global__ void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
 		Slot *slots, uint64_t *key_lo, unsigned long long *cursor) {
 	const uint64_t s = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= P) return;
-	const uint32_t own = owner[s];
-	if (own == 0) return;
+	const uint32_t own = s < P ? owner[s] : 0u;
+	Slot sl;
+	uint32_t cnt = 0;
+	if (own) sl = slots[s], cnt = sl.count;
+	// one cursor atomic per wavefront (the slots of a wave get consecutive bucket ranges)
+	const uint32_t pre = wave_excl_scan(cnt);
+	const uint32_t tot = __shfl(pre + cnt, 63, 64);
+	uint32_t b = 0;
+	if (lane_id() == 0 && tot) b = (uint32_t) atomicAdd(cursor, (unsigned long long) tot);
+	b = __shfl(b, 0, 64);
+	if (!own) return;
 	const uint64_t o = own - 1;
-	Slot sl = slots[s];
 	sl.key = occ_hi[o];
-	sl.start = (uint32_t) atomicAdd(cursor, (unsigned long long) sl.count);
+	sl.start = b + pre;
 	slots[s] = sl;
 	if (k > 32) key_lo[s] = occ_lo[o];
 }
@@ -296,18 +303,22 @@ constexpr int kLdsSeeds = 2048;
 
 __device__ __forceinline__ PairReads make_pair(const Packed &S, const Packed &Q, const Opts &o, int64_t s, int Ls,
 		int64_t q, int strand) {
-	PairReads pr;
-	pr.subj = &S, pr.qry = &Q, pr.s_read = s, pr.q_read = q, pr.Ls = Ls, pr.Lq = Q.len[q];
-	pr.subj_rc = o.call_consensus && strand == 1;
-	pr.qry_rc = !o.call_consensus && strand == 1;
-	return pr;
+	return make_pair_reads(S, Q, o.call_consensus != 0, s, Ls, q, strand);
 }
 
+// A group's seeds and chain arrays start at its first seed. Pass A1 chains it once and keeps the
+// chain count and best covered length there; later passes reload them.
 __device__ __forceinline__ Group make_group(const ChunkDev &c, uint32_t base, int gn) {
 	Group g;
 	g.seeds = c.seeds + base;
 	g.cid = c.cid + base, g.cfirst = c.cfirst + base, g.clast = c.clast + base, g.clen = c.clen + base;
 	g.gn = gn, g.nch = 0, g.best = 0;
+	return g;
+}
+__device__ __forceinline__ Group chained_group(const ChunkDev &c, uint32_t base, int gn) {
+	Group g = make_group(c, base, gn);
+	g.nch = c.g_nch[base];
+	g.best = c.g_best[base];
 	return g;
 }
 
@@ -363,6 +374,8 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 					while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
 					Group g = make_group(c, off + t, (int) (e - t));
 					chain(o, g);
+					c.g_nch[off + t] = g.nch;
+					c.g_best[off + t] = g.best;
 					const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
 					int nt, nw;
 					plan_count(o, g, pr, nt, nw);
@@ -399,8 +412,7 @@ __global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, Chun
 			const int64_t q = seed_qid(c.seeds[off + t]);
 			uint32_t e = t + 1;
 			while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
-			Group g = make_group(c, off + t, (int) (e - t));
-			chain(o, g);
+			const Group g = chained_group(c, off + t, (int) (e - t));
 			const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
 			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t]);
 		}
@@ -441,9 +453,9 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	const int Ls = S.len[s];
 	const uint32_t n0 = c.seg_n[2 * sl], off0 = c.seg_off[2 * sl];
 	const uint32_t n1 = c.seg_n[2 * sl + 1], off1 = c.seg_off[2 * sl + 1];
-	DpResults R{c.tasks, c.words, c.nsteps};
+	DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	uint64_t cells = 0, calls = 0, built = 0, kept = 0;
-	int oob = 0;
+
 	for (int pass = 0; pass < 2; pass++) {
 		const uint32_t n = pass == 0 ? n0 : n1, off = pass == 0 ? off0 : off1;
 		for (uint32_t t = lane; t < n; t += 64) {
@@ -461,35 +473,41 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 			}
 			Best best;
 			best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
-			Group groups[2];
+			uint32_t gbase[2] = {0, 0};
+			int gsize[2] = {0, 0};
 			for (int strand = 0; strand < 2; strand++) {
 				const int gi = strand == 0 ? gf : gr;
 				if (gi < 0) continue;
 				const uint32_t so = strand == 0 ? off0 : off1, sn = strand == 0 ? n0 : n1;
-				groups[strand] = make_group(c, so + gi, (int) (group_end(c.seeds + so, sn, gi) - gi));
-				chain(o, groups[strand]);
+				const uint32_t base = so + gi;
+				const int gn = (int) (group_end(c.seeds + so, sn, gi) - gi);
+				if (strand == 0) gbase[0] = base, gsize[0] = gn;
+				else gbase[1] = base, gsize[1] = gn;
+				const Group g = chained_group(c, base, gn);
 				const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-				score_group(o, groups[strand], pr, R, c.g_task_off[so + gi], strand, best, cells, calls, built);
+				score_group(o, g, pr, R, c.g_task_off[base], strand, best, cells, calls, built);
 			}
 			if (!best.have) continue;
 			kept++;
 			if (!o.call_consensus) continue;
+			const bool wr = best.strand == 1;
+			const Group g = chained_group(c, wr ? gbase[1] : gbase[0], wr ? gsize[1] : gsize[0]);
 			const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
-			VoteSink sink{tally + 4 * q_boff[q], pr, pr.Lq, &oob};
-			vote_record(o, groups[best.strand], pr, R, best.chain, best.task0, sink);
+			VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
+			vote_record(o, g, pr, R, best.chain, best.task0, sink);
 		}
 	}
 	cells = wave_sum64(cells);
 	calls = wave_sum64(calls);
 	built = wave_sum64(built);
 	kept = wave_sum64(kept);
-	const uint32_t anyoob = wave_sum((uint32_t) oob);
+
 	if (lane == 0) {
 		atomicAdd(&stats[kStCellsExec], (unsigned long long) cells);
 		atomicAdd(&stats[kStCallsExec], (unsigned long long) calls);
 		atomicAdd(&stats[kStBuilt], (unsigned long long) built);
 		atomicAdd(&stats[kStKept], (unsigned long long) kept);
-		if (anyoob) atomicAdd(&stats[kStOob], (unsigned long long) anyoob);
+
 	}
 }
 hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
@@ -531,7 +549,9 @@ struct StepWriter {
 	int e;
 	int cur;
 	uint32_t acc;
+	SummaryBuilder sb;
 	__device__ __forceinline__ void operator()(uint32_t st) {
+		sb.add(st);
 		e--;
 		const int wi = e >> 4;
 		if (wi != cur) {
@@ -565,6 +585,7 @@ __global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
 	const int lane = lane_id();
 	LanePtrs<NW> ptrs{L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 64, lane};
+	uint64_t cells = 0;
 	for (;;) {
 		uint32_t batch = 0;
 		if (lane == 0) batch = atomicAdd(L.batch_counter, 1u);
@@ -586,10 +607,90 @@ __global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
 				const int ns = dp_traceback(lenS, lenQ, task.band, ptrs, wr);
 				wr.flush();
 				L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
+				L.sums[t] = wr.sb.finish();
 				if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+				cells += band_cells(lenS, lenQ, task.band);
 			}
 		}
 	}
+	cells = wave_sum64(cells);
+	if (lane == 0) atomicAdd(&L.stats[kStCellsAll], (unsigned long long) cells);
+}
+
+int dp_batched_waves_per_cu(int bmax, bool aff) {
+	int blocks = 0;
+	hipError_t e = hipErrorInvalidValue;
+#define HERO_OCC(BM)                                                                                            \
+	if (bmax == BM)                                                                                             \
+		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, true>, 64, 0)          \
+		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false>, 64, 0);
+	HERO_OCC(4) HERO_OCC(8) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
+#undef HERO_OCC
+	return (e == hipSuccess && blocks > 0) ? blocks : 4;
+}
+
+__device__ __forceinline__ int task_bucket(const DpTask &t, int rows_max) {
+	const int lenS = t.s_len > t.q_len ? t.s_len : t.q_len; // the aligner's subject is the longer side
+	int cls = kNumBandClasses;
+	if (lenS < rows_max)
+		for (int c = kNumBandClasses - 1; c >= 0; c--)
+			if (t.band <= kBandClass[c]) cls = c;
+	const int bin = kLenBins - 1 - (lenS < kLenBins ? lenS : kLenBins - 1);
+	return cls * kLenBins + bin;
+}
+
+__global__ __launch_bounds__(1024) void k_task_histogram(const DpTask *tasks, uint32_t n, int rows_max, uint32_t *hist) {
+	__shared__ uint32_t h[kNumBuckets];
+	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) h[i] = 0;
+	__syncthreads();
+	for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+		atomicAdd(&h[task_bucket(tasks[t], rows_max)], 1u);
+	__syncthreads();
+	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x)
+		if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *hist, hipStream_t st) {
+	if (ntasks == 0) return hipSuccess;
+	const unsigned blocks = ntasks / (1024 * 16) + 1 < 512 ? ntasks / (1024 * 16) + 1 : 512;
+	k_task_histogram<<<blocks, 1024, 0, st>>>(tasks, ntasks, rows_max, hist);
+	return hipGetLastError();
+}
+
+// Scatter into the bucket order. A block ranks its tasks per bucket in LDS, reserves each
+// bucket's range with one global atomic, then writes: few global atomics even when most tasks
+// share a handful of buckets.
+constexpr int kScatterThreads = 1024, kScatterPerThread = 8;
+__global__ __launch_bounds__(1024) void k_task_scatter(const DpTask *tasks, uint32_t n, int rows_max, uint32_t *cursor,
+		uint32_t *order) {
+	__shared__ uint32_t cnt[kNumBuckets];
+	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) cnt[i] = 0;
+	__syncthreads();
+	const uint32_t base = blockIdx.x * (kScatterThreads * kScatterPerThread);
+	int bk[kScatterPerThread];
+	uint32_t rk[kScatterPerThread];
+#pragma unroll
+	for (int i = 0; i < kScatterPerThread; i++) {
+		const uint32_t t = base + i * kScatterThreads + threadIdx.x;
+		bk[i] = -1;
+		if (t < n) {
+			bk[i] = task_bucket(tasks[t], rows_max);
+			rk[i] = atomicAdd(&cnt[bk[i]], 1u);
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x)
+		if (cnt[i]) cnt[i] = atomicAdd(&cursor[i], cnt[i]);
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < kScatterPerThread; i++)
+		if (bk[i] >= 0) order[cnt[bk[i]] + rk[i]] = base + i * kScatterThreads + threadIdx.x;
+}
+hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *cursor, uint32_t *order,
+		hipStream_t st) {
+	if (ntasks == 0) return hipSuccess;
+	const uint32_t per_block = kScatterThreads * kScatterPerThread;
+	k_task_scatter<<<(ntasks + per_block - 1) / per_block, kScatterThreads, 0, st>>>(tasks, ntasks, rows_max, cursor, order);
+	return hipGetLastError();
 }
 
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st) {
@@ -699,7 +800,9 @@ __global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
 			const int ns = dp_traceback(lenS, lenQ, b, bp, wr);
 			wr.flush();
 			L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
-				if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+				L.sums[t] = wr.sb.finish();
+			if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+			atomicAdd(&L.stats[kStCellsAll], (unsigned long long) band_cells(lenS, lenQ, b));
 		}
 		__syncthreads();
 	}

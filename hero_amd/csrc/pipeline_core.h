@@ -1,10 +1,10 @@
 // Per-group steps of the correction path shared by the kernels and the CPU harness:
-//   plan_group   (pass A)  chain one (subject, strand, query) seed group and emit the DP tasks of
-//                          every surviving chain (processOneSubjectRead2 + getAlignment2 geometry);
-//   score_pair   (pass B)  build the query's records over both strands from the DP results,
-//                          apply early exits and acceptance, keep the first best record
-//                          (SubjectRead::addAlignment under -c);
-//   vote_record  (pass B)  cast the kept record's votes (SubjectRead::updateQueryCoverage).
+//   plan    (pass A)  chain one (subject, strand, query) seed group and emit the DP tasks of
+//                     every surviving chain (processOneSubjectRead2 + getAlignment2 geometry);
+//   score   (pass B)  build the query's records over both strands from the DP results, apply
+//                     early exits and acceptance, keep the first best record
+//                     (SubjectRead::addAlignment under -c);
+//   vote    (pass B)  cast the kept record's votes (SubjectRead::updateQueryCoverage).
 #pragma once
 #include "dp_core.h"
 #include "record_core.h"
@@ -35,16 +35,27 @@ HD void chain(const Opts &o, Group &g) {
 	g.nch = chain_group(o, g.gn, sa, cid, cf, cl, ln, &g.best);
 }
 
-// Reads of one (subject, strand, query) pair and their frames.
+// Reads of one (subject, strand, query) pair and their frames (word pointers cached).
 struct PairReads {
-	const Packed *subj, *qry;
 	int64_t s_read, q_read;
 	int Ls, Lq;
 	bool subj_rc; // record subject frame is the subject's reverse complement (-c, reverse strand)
 	bool qry_rc;  // record query frame is the query's reverse complement (no -c, reverse strand)
-	HD int sbase(int p) const { return strand_base(*subj, s_read, p, subj_rc); }
-	HD int qbase(int p) const { return strand_base(*qry, q_read, p, qry_rc); }
+	const uint64_t *sw, *qw;
+	HD int sbase(int p) const { return subj_rc ? (base_at(sw, Ls - 1 - p) ^ 2) : base_at(sw, p); }
+	HD int qbase(int p) const { return qry_rc ? (base_at(qw, Lq - 1 - p) ^ 2) : base_at(qw, p); }
 };
+
+HD PairReads make_pair_reads(const Packed &S, const Packed &Q, bool call_consensus, int64_t s, int Ls, int64_t q,
+		int strand) {
+	PairReads pr;
+	pr.s_read = s, pr.q_read = q, pr.Ls = Ls, pr.Lq = Q.len[q];
+	pr.subj_rc = call_consensus && strand == 1;
+	pr.qry_rc = !call_consensus && strand == 1;
+	pr.sw = S.words + S.woff[s];
+	pr.qw = Q.words + Q.woff[q];
+	return pr;
+}
 
 // ---- plan ---------------------------------------------------------------------------------
 struct CountDp {
@@ -87,7 +98,7 @@ struct EmitDp {
 	HD bool finish() { return true; }
 };
 
-// Number of DP tasks and edit words of all surviving chains of a chained group.
+// DP tasks and edit words of all surviving chains of a chained group.
 HD void plan_count(const Opts &o, const Group &g, const PairReads &pr, int &ntasks, int &nwords) {
 	ntasks = 0, nwords = 0;
 	SeedAt sa{g.seeds};
@@ -114,42 +125,37 @@ HD void plan_emit(const Opts &o, const Group &g, const PairReads &pr, DpTask *ta
 	}
 }
 
-// ---- DP results -----------------------------------------------------------------------------
-// Task t's steps sit at edit_words[t.out_off ...], capacity cap = s_len + q_len steps; the n
-// steps of the traceback occupy [cap - n, cap) in forward (first-to-last) order, 16 per word.
+// ---- DP results ---------------------------------------------------------------------------
+// Task t's steps sit at words[t.out_off ...], capacity cap = s_len + q_len steps; the n steps of
+// the traceback occupy [cap - n, cap) in forward (first-to-last) order, 16 per word. sums[t] is
+// their run summary.
 struct DpResults {
 	const DpTask *tasks;
 	const uint32_t *words;
 	const uint32_t *nsteps;
+	const StepSummary *sums;
 };
 
-HD uint32_t step_at(const uint32_t *w, int e) { return (w[e >> 4] >> (2 * (e & 15))) & 3u; }
+// Record op type of each step code for one call: a left step consumes the aligner's query, an
+// up step its subject, which are the record's subject / query when the pair was swapped
+// (calculateCigarAndPositionReversed, src/BandAligner.cpp:339-419).
+HD int step_type(uint32_t code, bool swapped) {
+	if (code < 2) return code == kStepMatch ? kOpM : kOpX;
+	return ((code == kStepLeft) != swapped) ? kOpI : kOpD;
+}
 
-// Feeds one DP call's result as record ops (addAlignment, src/AlignmentRecord.cpp:50-54): left /
-// up steps are I / D (swapped when the aligner's subject is the record's query, :339-419), a
-// diagonal is M when the two bases agree, else a mismatch typed by the query base.
+// Feeds one DP call's steps in order as record ops (addAlignment, src/AlignmentRecord.cpp:50-54).
 template <class F>
-HD void feed_dp(const DpResults &R, uint32_t task, const PairReads &pr, const DpCall &c, F &&op) {
+HD void feed_steps(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 	const DpTask &t = R.tasks[task];
 	const int cap = t.s_len + t.q_len;
 	const int n = (int) R.nsteps[task] <= cap ? (int) R.nsteps[task] : 0;
 	const uint32_t *w = R.words + t.out_off;
-	int i = 0, j = 0; // aligner coordinates
-	for (int e = cap - n; e < cap; e++) {
-		const uint32_t st = step_at(w, e);
-		if (st == kStepLeft) {
-			op(c.swapped ? kOpD : kOpI);
-			j++;
-		} else if (st == kStepUp) {
-			op(c.swapped ? kOpI : kOpD);
-			i++;
-		} else {
-			const int sp = c.swapped ? c.s0 + j : c.s0 + i;
-			const int qp = c.swapped ? c.q0 + i : c.q0 + j;
-			const int sb = pr.sbase(sp), qb = pr.qbase(qp);
-			op(sb == qb ? kOpM : kOpX + qb);
-			i++, j++;
-		}
+	int e = cap - n;
+	while (e < cap) {
+		uint32_t word = w[e >> 4] >> (2 * (e & 15));
+		const int stop = (e | 15) + 1 < cap ? (e | 15) + 1 : cap;
+		for (; e < stop; e++, word >>= 2) op(step_type(word & 3u, swapped));
 	}
 }
 
@@ -157,24 +163,28 @@ HD void feed_dp(const DpResults &R, uint32_t task, const PairReads &pr, const Dp
 struct ScoreV {
 	const Opts &o;
 	const DpResults &R;
-	const PairReads &pr;
 	uint32_t task; // next task of this record
 	Score sc;
 	double rate_sum, len;
 	uint64_t cells_exec; // cells of the DP calls this record actually consumed
 	int calls_exec;
+	bool finished; // reached the final test (every DP call of the record was consumed)
 	HD void start(int) {}
 	HD void add(int c, int t) { sc.add(c, t); }
 	HD void dp(const DpCall &c) {
-		feed_dp(R, task++, pr, c, [&](int t) { sc.add(1, t); });
+		sc.add_piece(R.sums[task++], c.swapped);
 		cells_exec += band_cells(c.lenS, c.lenQ, c.band);
 		calls_exec++;
 	}
 	HD bool check(int64_t m, int64_t i) { return sc.good_caps(m, i); }
-	HD bool finish() { return sc.good_rates(o, rate_sum, len); }
+	HD bool finish() {
+		finished = true;
+		return sc.good_rates(o, rate_sum, len);
+	}
 };
 
-// Best record of one strand group, continuing a running best (first best wins).
+// Best record of one (subject, query) pair so far: the first record with the lowest rate sum,
+// then the longest alignment (SubjectRead::addAlignment, src/SubjectRead.cpp:25-79).
 struct Best {
 	bool have;
 	double rate_sum, len;
@@ -189,15 +199,8 @@ HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const Dp
 	uint32_t task = task0;
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
-		int ntask, nwords;
-		{
-			ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-			CountDp cv;
-			walk_record(o, pr.Ls, pr.Lq, keys, cv);
-			ntask = cv.n, nwords = cv.words;
-		}
 		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-		ScoreV sv{o, R, pr, task, Score(), 0.0, 0.0, 0, 0};
+		ScoreV sv{o, R, task, Score(), 0.0, 0.0, 0, 0, false};
 		sv.sc.reset();
 		const bool ok = walk_record(o, pr.Ls, pr.Lq, keys, sv);
 		cells_exec += sv.cells_exec;
@@ -209,20 +212,26 @@ HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const Dp
 				best.strand = strand, best.chain = c, best.task0 = task;
 			}
 		}
-		task += (uint32_t) ntask;
-		(void) nwords;
+		// an early exit leaves this record's later DP tasks unread: count all of them to move on
+		uint32_t ntask = sv.task - task;
+		if (!sv.finished) {
+			ChainKeys<SeedAt, I32Arr> k2(sa, cid, g.gn, c, g.clast[c]);
+			CountDp cv;
+			walk_record(o, pr.Ls, pr.Lq, k2, cv);
+			ntask = (uint32_t) cv.n;
+		}
+		task += ntask;
 	}
 }
 
 // ---- votes --------------------------------------------------------------------------------
 // Tally of one query read: 4 u32 per base at index 4*p + code. The slot of the read's own base
 // holds a difference array of M-run votes (+1 at a run's first base, -1 after its last), the
-// other three slots hold mismatch votes; decode_tally turns it into the reference's counts.
+// other three slots hold mismatch votes; consensus_read decodes it.
 struct VoteSink {
 	uint32_t *tally; // this query's tally
-	const PairReads &pr;
-	int Lq;
-	int *oob;        // set when a vote would leave the read (the reference throws out_of_range)
+	PairReads pr;
+	unsigned long long *oob; // counts votes that would leave a read (the reference throws out_of_range)
 	HD void add_u32(uint32_t *p, uint32_t v) const {
 #if defined(__HIP_DEVICE_COMPILE__)
 		atomicAdd(p, v);
@@ -230,18 +239,25 @@ struct VoteSink {
 		__atomic_fetch_add(p, v, __ATOMIC_RELAXED);
 #endif
 	}
+	HD void flag_oob() const {
+#if defined(__HIP_DEVICE_COMPILE__)
+		atomicAdd(oob, 1ull);
+#else
+		__atomic_fetch_add(oob, 1ull, __ATOMIC_RELAXED);
+#endif
+	}
 	HD void m_run(int p, int c) const {
 		if (c <= 0) return;
-		if (p < 0 || p + c > Lq) {
-			*oob = 1;
+		if (p < 0 || p + c > pr.Lq) {
+			flag_oob();
 			return;
 		}
 		add_u32(&tally[4 * p + pr.qbase(p)], 1u);
-		if (p + c < Lq) add_u32(&tally[4 * (p + c) + pr.qbase(p + c)], 0xFFFFFFFFu);
+		if (p + c < pr.Lq) add_u32(&tally[4 * (p + c) + pr.qbase(p + c)], 0xFFFFFFFFu);
 	}
 	HD void mismatch(int pq, int ps) const {
-		if (pq < 0 || pq >= Lq || ps < 0 || ps >= pr.Ls) {
-			*oob = 1;
+		if (pq < 0 || pq >= pr.Lq || ps < 0 || ps >= pr.Ls) {
+			flag_oob();
 			return;
 		}
 		add_u32(&tally[4 * pq + pr.sbase(ps)], 1u);
@@ -250,7 +266,6 @@ struct VoteSink {
 
 struct VoteV {
 	const DpResults &R;
-	const PairReads &pr;
 	const VoteSink &sink;
 	uint32_t task;
 	int ps, pq;       // record frames (iPosOnSubjectRead, iPosOnQueryRead)
@@ -275,7 +290,7 @@ struct VoteV {
 	}
 	HD void start(int pos) { ps = pos, pq = 0; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void dp(const DpCall &c) { feed_dp(R, task++, pr, c, [&](int t) { op(1, t); }); }
+	HD void dp(const DpCall &c) { feed_steps(R, task++, c.swapped, [&](int t) { op(1, t); }); }
 	HD bool check(int64_t, int64_t) { return true; }
 	HD bool finish() {
 		flush();
@@ -288,7 +303,7 @@ HD void vote_record(const Opts &o, const Group &g, const PairReads &pr, const Dp
 	SeedAt sa{g.seeds};
 	I32Arr cid{g.cid};
 	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
-	VoteV vv{R, pr, sink, task0, 0, 0, 0, 0};
+	VoteV vv{R, sink, task0, 0, 0, 0, 0};
 	walk_record(o, pr.Ls, pr.Lq, keys, vv);
 }
 
@@ -300,8 +315,9 @@ HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed 
 		uint32_t *decoded /* optional 4*L, pre-self-vote counts */) {
 	uint32_t run = 0;
 	int changed = 0;
+	const uint64_t *w = Q.words + Q.woff[r];
 	for (int p = 0; p < L; p++) {
-		const int own = strand_base(Q, r, p, false);
+		const int own = base_at(w, p);
 		uint32_t c[4];
 		for (int j = 0; j < 4; j++) c[j] = tally[4 * p + j];
 		run += c[own];

@@ -239,6 +239,39 @@ struct Score {
 		if (t == kOpI || t == kOpD) sum_id += count;
 		else if (t >= kOpX) sum_mis += count;
 	}
+	// One DP call's ops at once, from its run summary (steps: 0 match, 1 mismatch, 2 left, 3 up;
+	// left / up are I / D, or D / I when the call's aligner subject is the record's query).
+	// Equivalent to add(1, t) per step: the first run merges with the current last op, the
+	// inner runs only add to the totals, the last run becomes the last op. Mismatches count as
+	// one op type here: merging equal-letter mismatch ops never changes a score (they are never
+	// I/D, and every mismatch op counts whole wherever it sits).
+	HD static int piece_type(int code, bool swapped) {
+		if (code < 2) return code == 0 ? kOpM : kOpX;
+		return ((code == 2) != swapped) ? kOpI : kOpD;
+	}
+	HD void add_piece(const StepSummary &s, bool swapped) {
+		const int steps = s.n[0] + s.n[1] + s.n[2] + s.n[3];
+		if (steps == 0) return;
+		const int ft = piece_type(s.first_code, swapped), lt = piece_type(s.last_code, swapped);
+		if (s.first_len == steps) {
+			add(steps, ft);
+			return;
+		}
+		add(s.first_len, ft);
+		int64_t mid_tot = steps - s.first_len - s.last_len, mid_id = 0, mid_mis = 0;
+		for (int c = 0; c < 4; c++) {
+			const int cnt = s.n[c] - (c == s.first_code ? s.first_len : 0) - (c == s.last_code ? s.last_len : 0);
+			if (c >= 2) mid_id += cnt;
+			else if (c == 1) mid_mis += cnt;
+		}
+		if (mid_tot > 0) {
+			total += mid_tot, sum_id += mid_id, sum_mis += mid_mis;
+			n++;
+			last_t = -2; // inner runs: a type no run can have, so the last run starts a new op
+			last_c = 0;
+		}
+		add(s.last_len, lt);
+	}
 	HD static bool is_id(int t) { return t == kOpI || t == kOpD; }
 	// calculateScore: (nMismatch, nIndel, alignment length)
 	HD void counts(double &nmis, double &nindel, double &len) const {

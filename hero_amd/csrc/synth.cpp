@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -159,5 +160,28 @@ int hero_synth_genome(const void *h, char *out, uint64_t cap) {
 }
 
 void hero_synth_destroy(void *h) { delete (Synth *) h; }
+
+// Two-line FASTA of reads [r0, r1) named by their unique ids (the layout the reference parses,
+// src/QueryDataset.cpp:103-244). Returns 0 or -1 when the file cannot be written.
+int hero_write_fasta(const char *path, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id, int64_t r0, int64_t r1) {
+	FILE *f = std::fopen(path, "wb");
+	if (!f) return -1;
+	std::vector<char> buf;
+	buf.reserve(1 << 22);
+	char name[32];
+	for (int64_t r = r0; r < r1; r++) {
+		const int m = std::snprintf(name, sizeof name, ">%lld\n", (long long) unique_id[r]);
+		buf.insert(buf.end(), name, name + m);
+		buf.insert(buf.end(), seq + offset[r], seq + offset[r] + length[r]);
+		buf.push_back('\n');
+		if (buf.size() > (1u << 22)) {
+			std::fwrite(buf.data(), 1, buf.size(), f);
+			buf.clear();
+		}
+	}
+	std::fwrite(buf.data(), 1, buf.size(), f);
+	return std::fclose(f) == 0 ? 0 : -1;
+}
 
 } // extern "C"

@@ -33,7 +33,8 @@ class Stats(ctypes.Structure):
         "table_size", "distinct_keys", "total_keys", "max_probe", "lookups", "seeds", "dp_tasks", "dp_calls_exec",
         "dp_cells_exec", "dp_cells_all", "records_built", "records_kept", "dp_skipped", "vote_oob", "subjects",
         "queries", "reads_changed")] + [(n, ctypes.c_double) for n in (
-            "ms_index", "ms_seed", "ms_plan", "ms_dp", "ms_score_vote", "ms_consensus", "ms_total")]
+            "ms_index", "ms_seed", "ms_plan", "ms_dp", "ms_score_vote", "ms_consensus", "ms_total",
+            "ms_seed_probe")]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

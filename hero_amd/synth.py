@@ -50,6 +50,9 @@ def _load():
         lib.hero_synth_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.hero_synth_genome.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
         lib.hero_synth_destroy.argtypes = [ctypes.c_void_p]
+        lib.hero_write_fasta.restype = ctypes.c_int
+        lib.hero_write_fasta.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
         _lib = lib
     return _lib
 
@@ -94,12 +97,13 @@ class ReadSet:
         uid = np.arange(len(seqs), dtype=np.int64) if unique_ids is None else np.asarray(unique_ids, dtype=np.int64)
         return ReadSet(seq, offs, lens, uid)
 
-    def write_fasta(self, path, names=None):
-        with open(path, "wb") as f:
-            for i in range(self.n):
-                o = int(self.offset[i])
-                name = str(int(self.unique_id[i])) if names is None else names[i]
-                f.write(b">" + name.encode() + b"\n" + self.seq[o:o + int(self.length[i])].tobytes() + b"\n")
+    def write_fasta(self, path, r0=0, r1=None):
+        """Two-line FASTA of reads [r0, r1), named by unique id."""
+        r1 = self.n if r1 is None else r1
+        rc = _load().hero_write_fasta(path.encode(), self.seq.ctypes.data, self.offset.ctypes.data,
+                                      self.length.ctypes.data, self.unique_id.ctypes.data, r0, r1)
+        if rc != 0:
+            raise OSError(f"cannot write {path}")
 
 
 def generate(seed, genome_len, len_min, len_max=None, coverage=0.0, n_reads=0, sub_5p=0.0, sub_3p=None,

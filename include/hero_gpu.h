@@ -57,6 +57,7 @@ typedef struct {
 	uint64_t lookups, seeds, dp_tasks, dp_calls_exec, dp_cells_exec, dp_cells_all;
 	uint64_t records_built, records_kept, dp_skipped, vote_oob, subjects, queries, reads_changed;
 	double ms_index, ms_seed, ms_plan, ms_dp, ms_score_vote, ms_consensus, ms_total;
+	double ms_seed_probe; /* the seed-count probe kernel alone (part of ms_seed) */
 } hero_stats;
 
 void hero_default_options(hero_options *opt);

@@ -128,7 +128,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 	for (int64_t q = 0; q < qr->n; q++) qbases = std::max<int64_t>(qbases, qr->offset[q] + qr->length[q]);
 	std::vector<uint32_t> tally((size_t) 4 * qbases, 0);
 	uint64_t st_seeds = 0, st_tasks = 0, st_cells_exec = 0, st_calls_exec = 0, st_built = 0, st_kept = 0, st_cells_all = 0;
-	int oob = 0;
+	unsigned long long oob = 0;
 	std::vector<uint32_t> steps;
 	for (int64_t s = 0; s < sr->n; s++) {
 		if (!sg[s]) continue;
@@ -180,7 +180,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 				g.gn = g1 - g0;
 				chain(o, g);
 				const int64_t q = seed_qid(D.seeds[g0]);
-				PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], o.call_consensus && strand == 1, !o.call_consensus && strand == 1};
+				PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, strand);
 				int nt, nw;
 				plan_count(o, g, pr, nt, nw);
 				const uint32_t t0 = (uint32_t) tasks.size();
@@ -193,6 +193,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 		}
 		// DP
 		std::vector<uint32_t> words(words_total + 1, 0), nsteps(tasks.size(), 0);
+		std::vector<StepSummary> sums(tasks.size());
 		st_tasks += tasks.size();
 		for (size_t t = 0; t < tasks.size(); t++) {
 			const DpTask &T = tasks[t];
@@ -212,8 +213,11 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 				words[T.out_off + (pos >> 4)] |= steps[e] << (2 * (pos & 15));
 			}
 			nsteps[t] = (uint32_t) ns;
+			SummaryBuilder sb;
+			for (int e = ns - 1; e >= 0; e--) sb.add(steps[e]);
+			sums[t] = sb.finish();
 		}
-		DpResults R{tasks.data(), words.data(), nsteps.data()};
+		DpResults R{tasks.data(), words.data(), nsteps.data(), sums.data()};
 		// pass B: pair groups of both strands by query
 		std::map<int64_t, std::pair<int, int>> pairs;
 		for (int strand = 0; strand < 2; strand++)
@@ -229,15 +233,15 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 			for (int strand = 0; strand < 2; strand++) {
 				const int gi = strand == 0 ? kv.second.first : kv.second.second;
 				if (gi < 0) continue;
-				PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], o.call_consensus && strand == 1, !o.call_consensus && strand == 1};
+				PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, strand);
 				score_group(o, sd[strand].groups[gi], pr, R, sd[strand].gtask0[gi], strand, best, st_cells_exec, st_calls_exec, st_built);
 			}
 			if (!best.have) continue;
 			st_kept++;
 			if (!o.call_consensus) continue;
-			PairReads pr{&S.P, &Q.P, s, q, Ls, qr->length[q], best.strand == 1, false};
+			PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, best.strand);
 			const int gi = best.strand == 0 ? kv.second.first : kv.second.second;
-			VoteSink sink{tally.data() + 4 * qr->offset[q], pr, qr->length[q], &oob};
+			VoteSink sink{tally.data() + 4 * qr->offset[q], pr, &oob};
 			vote_record(o, sd[best.strand].groups[gi], pr, R, best.chain, best.task0, sink);
 		}
 	}
