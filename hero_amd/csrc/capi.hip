@@ -142,8 +142,8 @@ struct hero_ctx {
 	// tallies: 4 u32 per query base (own-base slot difference coded, see VoteSink)
 	DevBuf<uint32_t> tally;
 	// chunk work buffers
-	DevBuf<uint32_t> seg_n, seg_off, scan_tmp;
-	DevBuf<uint64_t> seeds;
+	DevBuf<uint32_t> seg_n, seg_off, scan_tmp, lane_n;
+	DevBuf<uint64_t> seeds, win_info;
 	DevBuf<int32_t> cid, cfirst, clast, clen;
 	DevBuf<uint8_t> gflag;
 	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
@@ -160,6 +160,10 @@ struct hero_ctx {
 };
 
 namespace {
+
+// Largest per-chunk window-bucket buffer (8 B per subject window and strand); a chunk beyond it
+// probes the index a second time in pass A1 instead.
+constexpr uint64_t kWinInfoBytes = 1ull << 30;
 
 int fail(hero_ctx *c, int code, const char *fmt, ...) {
 	char buf[512];
@@ -238,13 +242,20 @@ double elapsed(hero_ctx *c, int a, int b) {
 // One subject chunk [s0, s0 + nsub) of the packed subject set S.
 int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_needed) {
 	const Packed Q = view(c->Qd, c->Qd.good.p);
-	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, c->o.k};
+	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, c->o.k, make_fastmod(c->P)};
 	ChunkDev cd;
 	std::memset(&cd, 0, sizeof cd);
 	cd.s_begin = s0, cd.nsub = nsub;
 	HIPCHK(c, c->seg_n.ensure((size_t) 2 * nsub));
 	HIPCHK(c, c->seg_off.ensure((size_t) 2 * nsub));
 	cd.seg_n = c->seg_n.p, cd.seg_off = c->seg_off.p;
+	// window buckets kept between the seed count and pass A1 when they fit kWinInfoBytes
+	const int64_t win_stride = std::max(0, rows_needed - c->o.k);
+	if (win_stride > 0 && (uint64_t) nsub * 2 * win_stride * 8 <= kWinInfoBytes) {
+		HIPCHK(c, c->win_info.ensure((size_t) nsub * 2 * win_stride));
+		HIPCHK(c, c->lane_n.ensure((size_t) nsub * 2 * 64));
+		cd.win_stride = (int32_t) win_stride, cd.win_info = c->win_info.p, cd.lane_n = c->lane_n.p;
+	}
 	unsigned long long *stats = c->dstats.p;
 	auto scan = [&](const uint32_t *in, uint32_t *out, int64_t n, uint64_t &total) -> int {
 		HIPCHK(c, c->scan_tmp.ensure(scan_tmp_words(n)));
@@ -431,6 +442,7 @@ void hero_destroy(hero_ctx *c) {
 	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
 	c->slots.release(), c->key_lo.release(), c->entries.release(), c->tally.release();
 	c->seg_n.release(), c->seg_off.release(), c->scan_tmp.release(), c->seeds.release();
+	c->win_info.release(), c->lane_n.release();
 	c->cid.release(), c->cfirst.release(), c->clast.release(), c->clen.release(), c->gflag.release();
 	c->g_tasks.release(), c->g_task_off.release(), c->g_words.release(), c->g_word_off.release();
 	c->tasks.release(), c->words.release(), c->nsteps.release(), c->dp_scratch.release(), c->plain_scratch.release();
@@ -798,7 +810,7 @@ __global__ void k_hash_kmers(const char *kmers, int n, int k, uint64_t P, uint64
 		if (t < 32) hi = (hi << 2) | c;
 		else lo = (lo << 2) | c;
 	}
-	out[i] = hash_kmer(k, hi, lo, P);
+	out[i] = hash_kmer(k, hi, lo, make_fastmod(P));
 }
 __global__ void k_lookup(DevIndex ix, const char *win, int n, int64_t *count, uint64_t *first_entry_idx) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -810,7 +822,7 @@ __global__ void k_lookup(DevIndex ix, const char *win, int n, int64_t *count, ui
 		if (t < 32) hi = (hi << 2) | c;
 		else lo = (lo << 2) | c;
 	}
-	uint64_t h = hash_kmer(k, hi, lo, ix.P);
+	uint64_t h = hash_kmer(k, hi, lo, ix.fm);
 	count[i] = 0;
 	first_entry_idx[i] = 0;
 	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
@@ -856,7 +868,7 @@ extern "C" int hero_lookup(hero_ctx *c, int32_t n, const char *windows, int64_t 
 	HIPCHK(c, dcount.exact((size_t) n));
 	HIPCHK(c, dfirst.exact((size_t) n));
 	HIPCHK(c, hipMemcpyAsync(dw.p, windows, (size_t) n * k, hipMemcpyHostToDevice, c->st));
-	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, k};
+	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, k, make_fastmod(c->P)};
 	k_lookup<<<(n + 255) / 256, 256, 0, c->st>>>(ix, dw.p, n, dcount.p, dfirst.p);
 	HIPCHK(c, hipGetLastError());
 	std::vector<uint64_t> first((size_t) n);

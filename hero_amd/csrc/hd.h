@@ -115,6 +115,34 @@ HD uint64_t hash_kmer(int k, uint64_t hi, uint64_t lo, uint64_t P) {
 	return ((s1 % P) * (s2 % P)) % P;
 }
 
+HD uint64_t umulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __umul64hi(a, b);
+#else
+	return (uint64_t) (((unsigned __int128) a * b) >> 64);
+#endif
+}
+// x mod P without a 64-bit division: with m = floor((2^64-1)/P) the quotient estimate
+// floor(x*m / 2^64) is at most 2 below floor(x/P) for every 64-bit x, so the remainder
+// needs at most two corrections.
+struct FastMod {
+	uint64_t P, m;
+	HD uint64_t mod(uint64_t x) const {
+		uint64_t r = x - umulhi64(x, m) * P;
+		if (r >= P) r -= P;
+		if (r >= P) r -= P;
+		return r;
+	}
+};
+HD FastMod make_fastmod(uint64_t P) { return FastMod{P, ~0ull / P}; }
+// hash_kmer with FastMod (identical values).
+HD uint64_t hash_kmer(int k, uint64_t hi, uint64_t lo, const FastMod &fm) {
+	uint64_t s1 = k >= 32 ? hi : ((1ull << (2 * k)) | hi);
+	uint64_t s2 = 1;
+	if (k > 32) s2 = (k - 32 >= 32) ? lo : ((1ull << (2 * (k - 32))) | lo);
+	return fm.mod(fm.mod(s1) * fm.mod(s2));
+}
+
 // Key positions of a query read (src/HashTableLongKey.cpp:279-289): 0, k, 2k, ... <= L-k and,
 // when that stride leaves the tail uncovered, L-k.
 HD int num_keys(int L, int k) {

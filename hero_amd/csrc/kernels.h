@@ -21,6 +21,7 @@ struct DevIndex {
 	uint64_t P;
 	uint32_t max_probe;
 	int32_t k;
+	FastMod fm; // P with its reciprocal
 };
 
 // Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
@@ -28,6 +29,9 @@ struct DevIndex {
 struct ChunkDev {
 	int64_t s_begin;
 	int32_t nsub;
+	int32_t win_stride;        // windows per (subject, strand) slot of win_info
+	uint64_t *win_info;        // per window: bucket (start << 32 | count), 0 = no bucket; null = probe again
+	uint32_t *lane_n;          // per (subject, strand) slot and lane: seeds of that lane's windows
 	uint32_t *seg_n, *seg_off; // 2 per subject (forward, reverse)
 	uint64_t *seeds;
 	int32_t *cid, *cfirst, *clast, *clen;

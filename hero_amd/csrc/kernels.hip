@@ -82,7 +82,7 @@ __global__ void k_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, ui
 	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= nocc) return;
 	const uint64_t hi = occ_hi[o], lo = occ_lo[o];
-	uint64_t h = hash_kmer(k, hi, lo, P);
+	uint64_t h = hash_kmer(k, hi, lo, make_fastmod(P));
 	for (uint64_t probe = 0; probe < P; probe++) {
 		const uint32_t cur = atomicCAS(&owner[h], 0u, (uint32_t) (o + 1));
 		bool done = cur == 0;
@@ -117,29 +117,53 @@ hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slo
 	return hipGetLastError();
 }
 
-__global__ void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
-		Slot *slots, uint64_t *key_lo, unsigned long long *cursor) {
-	const uint64_t s = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	const uint32_t own = s < P ? owner[s] : 0u;
-	Slot sl;
+// Bucket ranges: one cursor atomic per block of kAllocPerThread * 256 slots (the slots of a block
+// get consecutive ranges; where buckets lie in `entries` is invisible to lookups).
+constexpr int kAllocPerThread = 16;
+__global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi,
+		const uint64_t *occ_lo, int k, Slot *slots, uint64_t *key_lo, unsigned long long *cursor) {
+	__shared__ uint32_t wave_tot[4];
+	__shared__ uint32_t block_base;
+	const uint64_t base = (uint64_t) blockIdx.x * 256 * kAllocPerThread + threadIdx.x;
 	uint32_t cnt = 0;
-	if (own) sl = slots[s], cnt = sl.count;
-	// one cursor atomic per wavefront (the slots of a wave get consecutive bucket ranges)
+	for (int i = 0; i < kAllocPerThread; i++) {
+		const uint64_t s = base + (uint64_t) i * 256;
+		if (s < P && owner[s]) cnt += slots[s].count;
+	}
 	const uint32_t pre = wave_excl_scan(cnt);
-	const uint32_t tot = __shfl(pre + cnt, 63, 64);
-	uint32_t b = 0;
-	if (lane_id() == 0 && tot) b = (uint32_t) atomicAdd(cursor, (unsigned long long) tot);
-	b = __shfl(b, 0, 64);
-	if (!own) return;
-	const uint64_t o = own - 1;
-	sl.key = occ_hi[o];
-	sl.start = b + pre;
-	slots[s] = sl;
-	if (k > 32) key_lo[s] = occ_lo[o];
+	const int wv = threadIdx.x >> 6;
+	if (lane_id() == 63) wave_tot[wv] = pre + cnt;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t t = 0;
+		for (int i = 0; i < 4; i++) {
+			const uint32_t v = wave_tot[i];
+			wave_tot[i] = t;
+			t += v;
+		}
+		block_base = t ? (uint32_t) atomicAdd(cursor, (unsigned long long) t) : 0u;
+	}
+	__syncthreads();
+	uint32_t b = block_base + wave_tot[wv] + pre;
+	for (int i = 0; i < kAllocPerThread; i++) {
+		const uint64_t s = base + (uint64_t) i * 256;
+		if (s >= P) break;
+		const uint32_t own = owner[s];
+		if (!own) continue;
+		const uint64_t o = own - 1;
+		Slot sl = slots[s];
+		sl.key = occ_hi[o];
+		sl.start = b;
+		b += sl.count;
+		slots[s] = sl;
+		if (k > 32) key_lo[s] = occ_lo[o];
+	}
 }
 hipError_t launch_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
 		Slot *slots, uint64_t *key_lo, unsigned long long *cursor, hipStream_t st) {
-	k_index_alloc<<<(unsigned) ((P + 255) / 256), 256, 0, st>>>(owner, P, occ_hi, occ_lo, k, slots, key_lo, cursor);
+	const uint64_t per_block = 256ull * kAllocPerThread;
+	k_index_alloc<<<(unsigned) ((P + per_block - 1) / per_block), 256, 0, st>>>(owner, P, occ_hi, occ_lo, k, slots,
+			key_lo, cursor);
 	return hipGetLastError();
 }
 
@@ -181,7 +205,7 @@ hipError_t launch_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *en
 }
 
 __device__ __forceinline__ bool ix_lookup(const DevIndex &ix, uint64_t hi, uint64_t lo, uint32_t &start, uint32_t &count) {
-	uint64_t h = hash_kmer(ix.k, hi, lo, ix.P);
+	uint64_t h = hash_kmer(ix.k, hi, lo, ix.fm);
 	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
 		const Slot s = ix.slots[h];
 		if (s.count == 0) return false;
@@ -211,15 +235,12 @@ __device__ __forceinline__ void window_key(const Packed &S, int64_t s, int Ls, i
 // ---------------------------------------------------------------------------------------------
 // Seed search (SubjectReadAligner::searchHashTable2, src/SubjectReadAligner.cpp:444-491)
 // ---------------------------------------------------------------------------------------------
-// Seeds of one window as they are emitted: skip hits on the query with the subject's uniqueId;
-// without -c, reverse-strand coordinates map back to forward frames (:474-481).
+// Seeds of one window's bucket as they are emitted: skip hits on the query with the subject's
+// uniqueId; without -c, reverse-strand coordinates map back to forward frames (:474-481).
 template <class F>
-__device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o,
-		int64_t s, int Ls, int64_t uid, int w, int strand, F &&emit) {
-	uint64_t hi, lo;
-	window_key(S, s, Ls, w, strand, o.k, hi, lo);
-	uint32_t start = 0, count = 0;
-	if (!ix_lookup(ix, hi, lo, start, count)) return 0;
+__device__ __forceinline__ uint32_t bucket_seeds(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o,
+		int Ls, int64_t uid, int w, int strand, uint32_t start, uint32_t count, F &&emit) {
+	(void) S;
 	uint32_t n = 0;
 	for (uint32_t e = start; e < start + count; e++) {
 		const uint64_t data = ix.entries[e];
@@ -233,6 +254,18 @@ __device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &
 	return n;
 }
 
+template <class F>
+__device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o,
+		int64_t s, int Ls, int64_t uid, int w, int strand, F &&emit) {
+	uint64_t hi, lo;
+	window_key(S, s, Ls, w, strand, o.k, hi, lo);
+	uint32_t start = 0, count = 0;
+	if (!ix_lookup(ix, hi, lo, start, count)) return 0;
+	return bucket_seeds(S, Q, ix, o, Ls, uid, w, strand, start, count, emit);
+}
+
+// Seed count per subject strand. With c.win_info set it also keeps every window's bucket and each
+// lane's seed count, so pass A1 emits the seeds without hashing or probing again.
 __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
 	const int sl = blockIdx.x;
@@ -246,10 +279,19 @@ __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex 
 	const int64_t uid = S.uid[s];
 	const int nwin = Ls - o.k + 1;
 	for (int strand = 0; strand < 2; strand++) {
+		const int slot = 2 * sl + strand;
 		uint32_t cnt = 0;
-		for (int w = lane; w < nwin; w += 64) cnt += window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [](uint64_t) {});
+		for (int w = lane; w < nwin; w += 64) {
+			uint64_t hi, lo;
+			window_key(S, s, Ls, w, strand, o.k, hi, lo);
+			uint32_t start = 0, count = 0;
+			const bool hit = ix_lookup(ix, hi, lo, start, count);
+			if (c.win_info) c.win_info[(int64_t) slot * c.win_stride + w] = hit ? ((uint64_t) start << 32) | count : 0;
+			if (hit) cnt += bucket_seeds(S, Q, ix, o, Ls, uid, w, strand, start, count, [](uint64_t) {});
+		}
+		if (c.win_info) c.lane_n[(int64_t) slot * 64 + lane] = cnt;
 		cnt = wave_sum(cnt);
-		if (lane == 0) c.seg_n[2 * sl + strand] = cnt;
+		if (lane == 0) c.seg_n[slot] = cnt;
 	}
 	if (lane == 0) atomicAdd(&stats[kStLookups], (unsigned long long) (2 * nwin));
 }
@@ -299,7 +341,7 @@ struct GlobArr {
 	__device__ uint64_t &operator[](int i) const { return p[i]; }
 };
 
-constexpr int kLdsSeeds = 2048;
+constexpr int kLdsSeeds = 1024;
 
 __device__ __forceinline__ PairReads make_pair(const Packed &S, const Packed &Q, const Opts &o, int64_t s, int Ls,
 		int64_t q, int strand) {
@@ -339,11 +381,22 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 		if (n == 0) continue;
 		const bool in_lds = n <= (uint32_t) kLdsSeeds;
 		uint64_t *dst = in_lds ? lds : c.seeds + off;
-		uint32_t cnt = 0;
-		for (int w = lane; w < nwin; w += 64) cnt += window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [](uint64_t) {});
-		uint32_t pos = wave_excl_scan(cnt);
-		for (int w = lane; w < nwin; w += 64)
-			window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [&](uint64_t v) { dst[pos++] = v; });
+		if (c.win_info) {
+			const int slot = 2 * sl + strand;
+			uint32_t pos = wave_excl_scan(c.lane_n[(int64_t) slot * 64 + lane]);
+			const uint64_t *wi = c.win_info + (int64_t) slot * c.win_stride;
+			for (int w = lane; w < nwin; w += 64) {
+				const uint64_t b = wi[w];
+				if (b) bucket_seeds(S, Q, ix, o, Ls, uid, w, strand, (uint32_t) (b >> 32), (uint32_t) b,
+						[&](uint64_t v) { dst[pos++] = v; });
+			}
+		} else {
+			uint32_t cnt = 0;
+			for (int w = lane; w < nwin; w += 64) cnt += window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [](uint64_t) {});
+			uint32_t pos = wave_excl_scan(cnt);
+			for (int w = lane; w < nwin; w += 64)
+				window_seeds(S, Q, ix, o, s, Ls, uid, w, strand, [&](uint64_t v) { dst[pos++] = v; });
+		}
 		__syncthreads();
 		if (in_lds) {
 			LdsArr a{lds};
@@ -441,11 +494,67 @@ __device__ __forceinline__ uint32_t group_end(const uint64_t *seeds, uint32_t n,
 	return e;
 }
 
-// Pass B: per query of a subject (forward group and/or reverse group), build its records from the
-// DP results in the reference's order (forward chains, then reverse chains), keep the first best
-// (SubjectRead::addAlignment), then cast that record's votes.
+struct PassBCounters {
+	uint64_t cells = 0, calls = 0, built = 0, kept = 0;
+};
+
+// One query of a subject: its forward group (seed index gf of segment 0, or -1) and reverse group
+// (gr of segment 1, or -1). Records are built from the DP results in the reference's order
+// (forward chains, then reverse chains), the first best is kept (SubjectRead::addAlignment), and
+// that record casts its votes.
+__device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c,
+		const DpResults &R, int64_t s, int Ls, uint32_t off0, uint32_t n0, uint32_t off1, uint32_t n1, int gf, int gr,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, PassBCounters &k) {
+	const int64_t q = seed_qid(gf >= 0 ? c.seeds[off0 + gf] : c.seeds[off1 + gr]);
+	Best best;
+	best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
+	uint32_t gbase0 = 0, gbase1 = 0;
+	int gsize0 = 0, gsize1 = 0;
+	for (int strand = 0; strand < 2; strand++) {
+		const int gi = strand == 0 ? gf : gr;
+		if (gi < 0) continue;
+		const uint32_t so = strand == 0 ? off0 : off1, sn = strand == 0 ? n0 : n1;
+		const uint32_t base = so + gi;
+		const int gn = (int) (group_end(c.seeds + so, sn, gi) - gi);
+		if (strand == 0) gbase0 = base, gsize0 = gn;
+		else gbase1 = base, gsize1 = gn;
+		const Group g = chained_group(c, base, gn);
+		const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
+		score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built);
+	}
+	if (!best.have) return;
+	k.kept++;
+	if (!o.call_consensus) return;
+	const bool wr = best.strand == 1;
+	const Group g = chained_group(c, wr ? gbase1 : gbase0, wr ? gsize1 : gsize0);
+	const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
+	VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
+	vote_record(o, g, pr, R, best.chain, best.task0, sink);
+}
+
+// Kept group starts of one segment into an LDS list (wave-wide ballot compaction).
+__device__ __forceinline__ uint32_t collect_groups(const uint8_t *gflag, uint32_t n, uint32_t *list, uint32_t cap) {
+	uint32_t cnt = 0;
+	for (uint32_t t0 = 0; t0 < n; t0 += 64) {
+		const uint32_t t = t0 + lane_id();
+		const bool k = t < n && gflag[t] == 3;
+		const uint64_t m = __ballot(k);
+		const uint32_t pos = cnt + (uint32_t) __popcll(m & lanes_below_mask());
+		if (k && pos < cap) list[pos] = t;
+		cnt += (uint32_t) __popcll(m);
+	}
+	return cnt;
+}
+
+// Pass B: one wavefront per subject. The kept groups of both strands are listed in LDS and paired
+// by query (binary search over the reverse list); lanes then take the pairs densely.
+constexpr uint32_t kPassBGroups = 256;
 __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, const uint64_t *q_boff,
 		uint32_t *tally, unsigned long long *stats) {
+	__shared__ uint32_t list0[kPassBGroups], list1[kPassBGroups];
+	__shared__ int32_t match0[kPassBGroups];
+	__shared__ uint32_t rev_only[kPassBGroups];
+	__shared__ uint8_t used1[kPassBGroups];
 	const int sl = blockIdx.x;
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
@@ -454,49 +563,62 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	const uint32_t n0 = c.seg_n[2 * sl], off0 = c.seg_off[2 * sl];
 	const uint32_t n1 = c.seg_n[2 * sl + 1], off1 = c.seg_off[2 * sl + 1];
 	DpResults R{c.tasks, c.words, c.nsteps, c.sums};
-	uint64_t cells = 0, calls = 0, built = 0, kept = 0;
-
-	for (int pass = 0; pass < 2; pass++) {
-		const uint32_t n = pass == 0 ? n0 : n1, off = pass == 0 ? off0 : off1;
-		for (uint32_t t = lane; t < n; t += 64) {
-			if (c.gflag[off + t] != 3) continue;
-			const int64_t q = seed_qid(c.seeds[off + t]);
+	PassBCounters k;
+	const uint32_t c0 = collect_groups(c.gflag + off0, n0, list0, kPassBGroups);
+	const uint32_t c1 = collect_groups(c.gflag + off1, n1, list1, kPassBGroups);
+	if (c0 <= kPassBGroups && c1 <= kPassBGroups) {
+		for (uint32_t j = lane; j < c1; j += 64) used1[j] = 0;
+		__syncthreads();
+		for (uint32_t i = lane; i < c0; i += 64) {
+			const int64_t q = seed_qid(c.seeds[off0 + list0[i]]);
+			uint32_t lo = 0, hi = c1;
+			while (lo < hi) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (seed_qid(c.seeds[off1 + list1[mid]]) < q) lo = mid + 1;
+				else hi = mid;
+			}
+			const bool found = lo < c1 && seed_qid(c.seeds[off1 + list1[lo]]) == q;
+			match0[i] = found ? (int32_t) lo : -1;
+			if (found) used1[lo] = 1;
+		}
+		__syncthreads();
+		uint32_t nrev = 0;
+		for (uint32_t j0 = 0; j0 < c1; j0 += 64) {
+			const uint32_t j = j0 + lane;
+			const bool r = j < c1 && !used1[j];
+			const uint64_t m = __ballot(r);
+			if (r) rev_only[nrev + (uint32_t) __popcll(m & lanes_below_mask())] = j;
+			nrev += (uint32_t) __popcll(m);
+		}
+		__syncthreads();
+		for (uint32_t p = lane; p < c0 + nrev; p += 64) {
 			int gf = -1, gr = -1;
-			if (pass == 0) {
-				gf = (int) t;
-				const int r = find_group(c.seeds + off1, n1, q);
-				if (r >= 0 && c.gflag[off1 + r] == 3) gr = r;
-			} else {
-				const int f = find_group(c.seeds + off0, n0, q);
-				if (f >= 0 && c.gflag[off0 + f] == 3) continue; // handled with the forward group
-				gr = (int) t;
+			if (p < c0) gf = (int) list0[p], gr = match0[p] >= 0 ? (int) list1[match0[p]] : -1;
+			else gr = (int) list1[rev_only[p - c0]];
+			pass_b_pair(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, q_boff, tally, stats, k);
+		}
+	} else {
+		// more kept groups than the LDS lists hold: walk the segments directly
+		for (int pass = 0; pass < 2; pass++) {
+			const uint32_t n = pass == 0 ? n0 : n1, off = pass == 0 ? off0 : off1;
+			for (uint32_t t = lane; t < n; t += 64) {
+				if (c.gflag[off + t] != 3) continue;
+				const int64_t q = seed_qid(c.seeds[off + t]);
+				int gf = -1, gr = -1;
+				if (pass == 0) {
+					gf = (int) t;
+					const int r = find_group(c.seeds + off1, n1, q);
+					if (r >= 0 && c.gflag[off1 + r] == 3) gr = r;
+				} else {
+					const int f = find_group(c.seeds + off0, n0, q);
+					if (f >= 0 && c.gflag[off0 + f] == 3) continue; // handled with the forward group
+					gr = (int) t;
+				}
+				pass_b_pair(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, q_boff, tally, stats, k);
 			}
-			Best best;
-			best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
-			uint32_t gbase[2] = {0, 0};
-			int gsize[2] = {0, 0};
-			for (int strand = 0; strand < 2; strand++) {
-				const int gi = strand == 0 ? gf : gr;
-				if (gi < 0) continue;
-				const uint32_t so = strand == 0 ? off0 : off1, sn = strand == 0 ? n0 : n1;
-				const uint32_t base = so + gi;
-				const int gn = (int) (group_end(c.seeds + so, sn, gi) - gi);
-				if (strand == 0) gbase[0] = base, gsize[0] = gn;
-				else gbase[1] = base, gsize[1] = gn;
-				const Group g = chained_group(c, base, gn);
-				const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-				score_group(o, g, pr, R, c.g_task_off[base], strand, best, cells, calls, built);
-			}
-			if (!best.have) continue;
-			kept++;
-			if (!o.call_consensus) continue;
-			const bool wr = best.strand == 1;
-			const Group g = chained_group(c, wr ? gbase[1] : gbase[0], wr ? gsize[1] : gsize[0]);
-			const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
-			VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
-			vote_record(o, g, pr, R, best.chain, best.task0, sink);
 		}
 	}
+	uint64_t cells = k.cells, calls = k.calls, built = k.built, kept = k.kept;
 	cells = wave_sum64(cells);
 	calls = wave_sum64(calls);
 	built = wave_sum64(built);

@@ -161,8 +161,8 @@ HD void feed_steps(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 
 // ---- score --------------------------------------------------------------------------------
 struct ScoreV {
-	const Opts &o;
-	const DpResults &R;
+	double mismatch_rate, indel_rate; // held by value: the visitor then lives in registers
+	DpResults R;
 	uint32_t task; // next task of this record
 	Score sc;
 	double rate_sum, len;
@@ -179,7 +179,7 @@ struct ScoreV {
 	HD bool check(int64_t m, int64_t i) { return sc.good_caps(m, i); }
 	HD bool finish() {
 		finished = true;
-		return sc.good_rates(o, rate_sum, len);
+		return sc.good_rates(mismatch_rate, indel_rate, rate_sum, len);
 	}
 };
 
@@ -200,7 +200,7 @@ HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const Dp
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
 		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-		ScoreV sv{o, R, task, Score(), 0.0, 0.0, 0, 0, false};
+		ScoreV sv{o.mismatch_rate, o.indel_rate, R, task, Score(), 0.0, 0.0, 0, 0, false};
 		sv.sc.reset();
 		const bool ok = walk_record(o, pr.Ls, pr.Lq, keys, sv);
 		cells_exec += sv.cells_exec;

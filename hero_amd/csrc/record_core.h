@@ -235,9 +235,11 @@ struct Score {
 			last_t = t, last_c = count;
 			if (n == 1) first_t = t, first_c = count;
 		}
+		// selects on values, not on which field: a branch per field lets the compiler merge the
+		// two updates into one store at a computed offset, which moves the struct to scratch
 		total += count;
-		if (t == kOpI || t == kOpD) sum_id += count;
-		else if (t >= kOpX) sum_mis += count;
+		sum_id += (t == kOpI || t == kOpD) ? count : 0;
+		sum_mis += (t >= kOpX) ? count : 0;
 	}
 	// One DP call's ops at once, from its run summary (steps: 0 match, 1 mismatch, 2 left, 3 up;
 	// left / up are I / D, or D / I when the call's aligner subject is the record's query).
@@ -289,13 +291,13 @@ struct Score {
 		return !(a > (double) cap_mis || b > (double) cap_indel);
 	}
 	// isGoodAlignment() (:302-308) -> rates for dedup
-	HD bool good_rates(const Opts &o, double &rate_sum, double &len) const {
+	HD bool good_rates(double mismatch_rate, double indel_rate, double &rate_sum, double &len) const {
 		double nmis, nindel;
 		counts(nmis, nindel, len);
 		double irate = 100, mrate = 100;
 		if (len != 0) irate = nindel / len, mrate = nmis / len;
 		rate_sum = irate + mrate;
-		return mrate <= o.mismatch_rate && irate <= o.indel_rate;
+		return mrate <= mismatch_rate && irate <= indel_rate;
 	}
 };
 
