@@ -124,7 +124,7 @@ struct hero_ctx {
 	std::vector<int64_t> q_off;
 	std::vector<int32_t> q_len;
 	std::vector<uint8_t> q_good, q_active_as_subject;
-	std::vector<int64_t> q_not_good;
+	std::vector<int64_t> q_not_good, q_uid;
 	bool q_contiguous = true; // caller offsets are the back-to-back prefix sums of the lengths
 	int64_t q_bases = 0;
 	DevBuf<uint8_t> cons_out;
@@ -151,6 +151,11 @@ struct hero_ctx {
 	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
 	DevBuf<uint32_t> bucket_hist, bucket_off, order;
 	DevBuf<int32_t> g_nch, g_best;
+	// record mode: .aln text of the subjects aligned since the last hero_take_alignments
+	std::string aln;
+	DevBuf<uint32_t> g_recs, g_rec_off, rec_opn, rec_opoff, rec_ops;
+	DevBuf<RecHead> recs;
+	DevBuf<int32_t> rec_pos;
 	DevBuf<StepSummary> sums;
 	DevBuf<unsigned long long> dstats, dtotal;
 	DevBuf<uint32_t> counter;
@@ -240,7 +245,10 @@ double elapsed(hero_ctx *c, int a, int b) {
 }
 
 // One subject chunk [s0, s0 + nsub) of the packed subject set S.
-int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_needed) {
+int format_records(hero_ctx *c, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots);
+
+int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_needed, const int64_t *s_uid) {
+	const bool rec_mode = c->opt.records != 0;
 	const Packed Q = view(c->Qd, c->Qd.good.p);
 	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, c->o.k, make_fastmod(c->P)};
 	ChunkDev cd;
@@ -289,6 +297,11 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	cd.gflag = c->gflag.p, cd.g_tasks = c->g_tasks.p, cd.g_task_off = c->g_task_off.p;
 	cd.g_nch = c->g_nch.p, cd.g_best = c->g_best.p;
 	cd.g_words = c->g_words.p, cd.g_word_off = c->g_word_off.p;
+	if (rec_mode) {
+		HIPCHK(c, c->g_recs.ensure(ns));
+		HIPCHK(c, c->g_rec_off.ensure(ns));
+		cd.g_recs = c->g_recs.p;
+	}
 	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, c->st));
 	HIPCHK(c, hipEventRecord(c->ev[1].a, c->st));
 	uint64_t ntasks = 0, nwords = 0;
@@ -296,6 +309,14 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	if (int r = scan(cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
+	uint64_t nslots = 0;
+	if (rec_mode) {
+		if (int r = scan(cd.g_recs, c->g_rec_off.p, (int64_t) ns, nslots)) return r;
+		if (nslots >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu record slots; lower chunk_subjects", nsub, (unsigned long long) nslots);
+		HIPCHK(c, c->recs.ensure((size_t) nslots));
+		HIPCHK(c, hipMemsetAsync(c->recs.p, 0xFF, (size_t) nslots * sizeof(RecHead), c->st));
+		cd.recs = c->recs.p, cd.g_rec_off = c->g_rec_off.p;
+	}
 	HIPCHK(c, c->tasks.ensure((size_t) ntasks));
 	HIPCHK(c, c->words.ensure((size_t) nwords + 1));
 	HIPCHK(c, c->nsteps.ensure((size_t) ntasks));
@@ -376,15 +397,136 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	c->st_host.ms_score_vote += elapsed(c, 3, 4);
 	c->st_host.dp_tasks += ntasks;
 	c->st_host.subjects += (uint64_t) nsub;
+	if (!rec_mode) return HERO_OK;
+	// records: op counts, their scan, ops; then the host applies SubjectRead::addAlignment and
+	// writes the .aln text
+	const uint32_t nsl = (uint32_t) nslots;
+	HIPCHK(c, c->rec_opn.ensure((size_t) nsl + 1));
+	HIPCHK(c, c->rec_opoff.ensure((size_t) nsl + 1));
+	HIPCHK(c, c->rec_pos.ensure((size_t) nsl + 1));
+	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, c->rec_opn.p, nullptr, nullptr, nullptr, c->st));
+	uint64_t nops = 0;
+	if (int r = scan(c->rec_opn.p, c->rec_opoff.p, (int64_t) nsl, nops)) return r;
+	if (nops >= (1ull << 32)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu CIGAR ops; lower chunk_subjects", nsub, (unsigned long long) nops);
+	HIPCHK(c, c->rec_ops.ensure((size_t) nops + 1));
+	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, c->rec_opn.p, c->rec_opoff.p, c->rec_pos.p, c->rec_ops.p, c->st));
+	HIPCHK(c, hipStreamSynchronize(c->st));
+	return format_records(c, s0, nsub, s_uid, nsl);
+}
+
+// SubjectRead::addAlignment (src/SubjectRead.cpp:25-79) over a subject's accepted records in
+// processing order, then the .aln text: ">sid\n" + "qid,strand,pos,CIGAR,interiorInsertedBases,"
+// per kept record + "\n" (AlignmentRecord.cpp:102-149, SubjectRead.cpp:179-193,
+// SubjectReadAligner.cpp:854-910). Record ids are query uniqueIds (without -c, :230-231).
+int format_records(hero_ctx *c, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots) {
+	std::vector<RecHead> h(nslots);
+	std::vector<uint32_t> opn(nslots), opoff(nslots);
+	std::vector<int32_t> pos(nslots);
+	if (nslots) {
+		HIPCHK(c, hipMemcpy(h.data(), c->recs.p, (size_t) nslots * sizeof(RecHead), hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(opn.data(), c->rec_opn.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(opoff.data(), c->rec_opoff.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(pos.data(), c->rec_pos.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+	}
+	const uint64_t nops = nslots ? (uint64_t) opoff[nslots - 1] + opn[nslots - 1] : 0;
+	std::vector<uint32_t> ops((size_t) nops);
+	if (nops) HIPCHK(c, hipMemcpy(ops.data(), c->rec_ops.p, (size_t) nops * 4, hipMemcpyDeviceToHost));
+	// accepted records per subject, in slot (= processing) order
+	std::vector<uint32_t> first((size_t) nsub + 1, 0), order;
+	for (uint32_t i = 0; i < nslots; i++)
+		if (h[i].qid != kNoRecord) first[(size_t) h[i].sub + 1]++;
+	for (int32_t j = 0; j < nsub; j++) first[(size_t) j + 1] += first[(size_t) j];
+	order.resize(first[(size_t) nsub]);
+	{
+		std::vector<uint32_t> cur(first.begin(), first.end() - 1);
+		for (uint32_t i = 0; i < nslots; i++)
+			if (h[i].qid != kNoRecord) order[cur[(size_t) h[i].sub]++] = i;
+	}
+	std::vector<std::string> text((size_t) nsub);
+#pragma omp parallel for schedule(dynamic, 256)
+	for (int32_t j = 0; j < nsub; j++) {
+		std::string &t = text[(size_t) j];
+		char buf[64];
+		t += '>';
+		t += std::to_string((long long) s_uid[s0 + j]);
+		t += '\n';
+		// addAlignment: keep one record per query id; lower rate sum wins, then the longer
+		std::vector<uint32_t> kept;
+		uint64_t largest = 0;
+		auto id_of = [&](uint32_t i) { return (uint64_t) c->q_uid[h[i].qid]; };
+		auto keep_better = [&](uint32_t &slot, uint32_t r) {
+			const double a = h[slot].rate_sum, b = h[r].rate_sum;
+			if (a < b) return;
+			if (a == b && h[slot].len >= h[r].len) return;
+			slot = r;
+		};
+		for (uint32_t x = first[(size_t) j]; x < first[(size_t) j + 1]; x++) {
+			const uint32_t r = order[x];
+			const uint64_t id = id_of(r);
+			if (kept.empty()) {
+				largest = id;
+				kept.push_back(r);
+			} else if (id_of(kept.back()) == id) {
+				keep_better(kept.back(), r);
+			} else if (largest >= id) {
+				bool found = false;
+				for (uint32_t &k : kept) {
+					if (id_of(k) > id) break;
+					if (id_of(k) == id) {
+						keep_better(k, r);
+						found = true;
+						break;
+					}
+				}
+				if (!found) kept.push_back(r);
+			} else {
+				largest = id;
+				kept.push_back(r);
+			}
+		}
+		for (uint32_t r : kept) {
+			const RecHead &x = h[r];
+			const int n = (int) opn[r];
+			const uint32_t *op = ops.data() + opoff[r];
+			snprintf(buf, sizeof buf, "%lld,%d,%d,", (long long) c->q_uid[x.qid], (int) x.strand, (int) pos[r]);
+			t += buf;
+			for (int i = 0; i < n; i++) {
+				snprintf(buf, sizeof buf, "%u", op[i] >> 8);
+				t += buf;
+				t += (char) (op[i] & 0xFF);
+			}
+			t += ',';
+			// interior inserted bases, from the query in the record's orientation
+			const char *qs = c->q_seq.data() + c->q_off[x.qid];
+			const int Lq = c->q_len[x.qid];
+			const bool rc = x.strand == 1 && !c->o.call_consensus;
+			int index = 0;
+			for (int i = 0; i < n; i++) {
+				const char ty = (char) (op[i] & 0xFF);
+				const int cnt = (int) (op[i] >> 8);
+				if (ty == 'D') continue;
+				if (ty == 'I' && i != 0 && i != n - 1) {
+					for (int m = 0; m < cnt; m++) {
+						const int p = index + m;
+						t += rc ? code_ascii(ascii_code(qs[Lq - 1 - p]) ^ 2) : qs[p];
+					}
+				}
+				index += cnt;
+			}
+			t += ',';
+		}
+		t += '\n';
+	}
+	for (const std::string &t : text) c->aln += t;
 	return HERO_OK;
 }
 
-int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len) {
+int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int64_t *s_uid) {
 	if (!c->index_built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
 	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
 	for (int64_t s0 = 0; s0 < n; s0 += chunk) {
 		const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
-		if (int r = run_chunk(c, S, s0, m, max_len + 1)) return r;
+		if (int r = run_chunk(c, S, s0, m, max_len + 1, s_uid)) return r;
 	}
 	return HERO_OK;
 }
@@ -406,6 +548,7 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 	*out = nullptr;
 	if (opt->k < 1 || opt->k > 64) return HERO_ERR_ARG;
 	if (opt->aligner != 0 && opt->aligner != 1) return HERO_ERR_ARG;
+	if (opt->records && opt->call_consensus) return HERO_ERR_ARG; // the reference writes no .aln under -c
 	if (opt->min_overlap < 0 || opt->min_overlap > 65535) return HERO_ERR_ARG;
 	if (opt->hash_factor < 1) return HERO_ERR_ARG;
 	int ndev = 0;
@@ -463,6 +606,7 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	c->q_off.assign(off, off + n);
 	c->q_len.assign(len, len + n);
+	c->q_uid.assign(uid, uid + n);
 	int64_t bases = 0;
 	for (int64_t i = 0; i < n; i++) bases = std::max<int64_t>(bases, off[i] + len[i]);
 	c->q_seq.assign(seq, seq + bases);
@@ -597,7 +741,7 @@ int hero_align_subjects(hero_ctx *c, int64_t n, const char *seq, const int64_t *
 	int ml = 0;
 	for (int64_t i = 0; i < n; i++) ml = std::max(ml, len[i]);
 	for (int32_t l : c->q_len) ml = std::max(ml, l);
-	return align_packed(c, view(c->Sd, c->Sd.good.p), n, ml);
+	return align_packed(c, view(c->Sd, c->Sd.good.p), n, ml, uid);
 }
 
 int hero_align_self(hero_ctx *c) {
@@ -605,7 +749,17 @@ int hero_align_self(hero_ctx *c) {
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	int ml = 0;
 	for (int32_t l : c->q_len) ml = std::max(ml, l);
-	return align_packed(c, view(c->Qd, c->q_subject_good.p), c->Qd.n, ml);
+	return align_packed(c, view(c->Qd, c->q_subject_good.p), c->Qd.n, ml, c->q_uid.data());
+}
+
+int hero_take_alignments(hero_ctx *c, char *out, int64_t cap, int64_t *len) {
+	if (!c || !len) return HERO_ERR_ARG;
+	*len = (int64_t) c->aln.size();
+	if (!out) return HERO_OK;
+	if (cap < *len) return fail(c, HERO_ERR_LIMIT, "alignment text is %lld bytes, buffer %lld", (long long) *len, (long long) cap);
+	std::memcpy(out, c->aln.data(), c->aln.size());
+	std::string().swap(c->aln);
+	return HERO_OK;
 }
 
 int hero_call_consensus(hero_ctx *c, char *corrected) {

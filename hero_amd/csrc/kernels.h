@@ -24,6 +24,20 @@ struct DevIndex {
 	FastMod fm; // P with its reciprocal
 };
 
+// One accepted alignment record of a chunk (record mode, without -c). Slots are laid out in the
+// reference's processing order: subject, strand, query group (innerId), chain index; a slot whose
+// chain was not accepted keeps qid = kNoRecord.
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
+struct RecHead {
+	uint32_t qid;   // innerId of the query read
+	uint32_t gbase; // first seed of the query group
+	int32_t gsize;  // seeds in the group
+	uint32_t task0; // first DP task of the record
+	int32_t sub;    // subject index within the chunk
+	int16_t strand, chain;
+	double rate_sum, len; // isGoodAlignment() rates, for SubjectRead::addAlignment
+};
+
 // Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
 // task and edit-word counts and their exclusive scans.
 struct ChunkDev {
@@ -38,6 +52,8 @@ struct ChunkDev {
 	uint8_t *gflag; // bit0 group start, bit1 group kept (within max_candidates)
 	int32_t *g_nch, *g_best; // chain count and best covered length of a kept group
 	uint32_t *g_tasks, *g_task_off, *g_words, *g_word_off;
+	uint32_t *g_recs, *g_rec_off; // record mode: chains of a kept group (record slots) and their scan
+	RecHead *recs;                // record mode: one slot per chain of every kept group
 	DpTask *tasks;
 	uint32_t *words, *nsteps;
 	StepSummary *sums;
@@ -92,6 +108,10 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, hipStream_t st);
 hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
 		uint32_t *tally, unsigned long long *stats, hipStream_t st);
+// Record mode: op count (ops == nullptr) or ops of every accepted record slot; each op is
+// (count << 8) | type, type 'M' / 'I' / 'D' or the query base letter of a mismatch run.
+hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,
+		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops, hipStream_t st);
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st);
 int dp_batched_waves_per_cu(int bmax, bool affine); // occupancy of one batched instantiation
 

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 			const uint64_t m = __ballot(start);
 			const uint32_t rank = groups_before + (uint32_t) __popcll(m & lanes_below_mask());
 			groups_before += (uint32_t) __popcll(m);
-			uint32_t ntask = 0, nword = 0;
+			uint32_t ntask = 0, nword = 0, nrec = 0;
 			uint8_t flag = 0;
 			if (start) {
 				const bool kept = o.max_candidates <= 0 || rank < (uint32_t) o.max_candidates;
@@ -432,13 +432,14 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 					const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
 					int nt, nw;
 					plan_count(o, g, pr, nt, nw);
-					ntask = (uint32_t) nt, nword = (uint32_t) nw;
+					ntask = (uint32_t) nt, nword = (uint32_t) nw, nrec = (uint32_t) g.nch;
 				}
 			}
 			if (t < n) {
 				c.gflag[off + t] = flag;
 				c.g_tasks[off + t] = ntask;
 				c.g_words[off + t] = nword;
+				if (c.g_recs) c.g_recs[off + t] = nrec;
 			}
 		}
 	}
@@ -520,7 +521,19 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 		else gbase1 = base, gsize1 = gn;
 		const Group g = chained_group(c, base, gn);
 		const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-		score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built);
+		if (c.recs) {
+			RecHead *slots = c.recs + c.g_rec_off[base];
+			const int sub = (int) (s - c.s_begin);
+			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built,
+					[&](int chain, uint32_t task, double rate_sum, double len) {
+						RecHead h;
+						h.qid = (uint32_t) q, h.gbase = base, h.gsize = gn, h.task0 = task, h.sub = sub;
+						h.strand = (int16_t) strand, h.chain = (int16_t) chain, h.rate_sum = rate_sum, h.len = len;
+						slots[chain] = h;
+					});
+		} else {
+			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built);
+		}
 	}
 	if (!best.have) return;
 	k.kept++;
@@ -530,6 +543,32 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 	const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
 	VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
 	vote_record(o, g, pr, R, best.chain, best.task0, sink);
+}
+
+// Record mode: one lane per record slot; count pass (ops == nullptr) then emit pass.
+__global__ __launch_bounds__(64) void k_record_ops(Packed S, Packed Q, Opts o, ChunkDev c, uint32_t nslots,
+		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops) {
+	const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+	if (i >= nslots) return;
+	const RecHead h = c.recs[i];
+	if (h.qid == kNoRecord) {
+		if (!ops) op_n[i] = 0;
+		return;
+	}
+	const int64_t s = c.s_begin + h.sub;
+	const Group g = chained_group(c, h.gbase, h.gsize);
+	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) h.qid, h.strand);
+	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
+	int p = 0;
+	const int n = record_ops(o, g, pr, R, h.chain, h.task0, ops ? ops + op_off[i] : nullptr, &p);
+	if (ops) pos[i] = p;
+	else op_n[i] = (uint32_t) n;
+}
+hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,
+		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops, hipStream_t st) {
+	if (nslots == 0) return hipSuccess;
+	k_record_ops<<<(nslots + 63) / 64, 64, 0, st>>>(S, Q, o, c, nslots, op_n, op_off, pos, ops);
+	return hipGetLastError();
 }
 
 // Kept group starts of one segment into an LDS list (wave-wide ballot compaction).

@@ -192,8 +192,15 @@ struct Best {
 	uint32_t task0; // first DP task of the winning record
 };
 
+// Scores every best-length chain of a group in index order (processOneSubjectRead2,
+// src/SubjectReadAligner.cpp:838-849); accept(chain, first task, rate sum, length) sees every
+// accepted record.
+struct NoAccept {
+	HD void operator()(int, uint32_t, double, double) const {}
+};
+template <class F = NoAccept>
 HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, uint32_t task0, int strand,
-		Best &best, uint64_t &cells_exec, uint64_t &calls_exec, uint64_t &built) {
+		Best &best, uint64_t &cells_exec, uint64_t &calls_exec, uint64_t &built, F &&accept = F()) {
 	SeedAt sa{g.seeds};
 	I32Arr cid{g.cid};
 	uint32_t task = task0;
@@ -207,6 +214,7 @@ HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const Dp
 		calls_exec += (uint64_t) sv.calls_exec;
 		if (ok) {
 			built++;
+			accept(c, task, sv.rate_sum, sv.len);
 			if (!best.have || record_better(sv.rate_sum, sv.len, best.rate_sum, best.len)) {
 				best.have = true, best.rate_sum = sv.rate_sum, best.len = sv.len;
 				best.strand = strand, best.chain = c, best.task0 = task;
@@ -305,6 +313,66 @@ HD void vote_record(const Opts &o, const Group &g, const PairReads &pr, const Dp
 	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
 	VoteV vv{R, sink, task0, 0, 0, 0, 0};
 	walk_record(o, pr.Ls, pr.Lq, keys, vv);
+}
+
+// ---- records (without -c) -------------------------------------------------------------------
+// The record's CIGAR as AlignmentRecord::addCigar builds it (src/AlignmentRecord.cpp:35-54):
+// adjacent ops of equal type merge (zero counts included); a mismatch step is typed by the
+// record query's base (calculateCigarAndPosition[Reversed], src/BandAligner.cpp:272-278,353-360).
+struct RecordV {
+	const DpResults &R;
+	PairReads pr;
+	uint32_t task;
+	uint32_t *out; // null: count only
+	int n;         // ops emitted
+	int ps, pq, pos;
+	bool pending;
+	int cur_t, cur_c;
+	HD void emit() {
+		if (out) out[n] = ((uint32_t) cur_c << 8) | (uint32_t) cur_t;
+		n++;
+	}
+	HD void push(int t, int c) {
+		if (pending && cur_t == t) {
+			cur_c += c;
+		} else {
+			if (pending) emit();
+			pending = true, cur_t = t, cur_c = c;
+		}
+	}
+	HD void op(int c, int t) {
+		if (t == kOpM) {
+			push('M', c), pq += c, ps += c;
+		} else if (t == kOpI) {
+			push('I', c), pq += c;
+		} else if (t == kOpD) {
+			push('D', c), ps += c;
+		} else {
+			for (int x = 0; x < c; x++, pq++, ps++) push(code_ascii(pr.qbase(pq)), 1);
+		}
+	}
+	HD void start(int p) { ps = p, pq = 0, pos = p; }
+	HD void add(int c, int t) { op(c, t); }
+	HD void dp(const DpCall &c) { feed_steps(R, task++, c.swapped, [&](int t) { op(1, t); }); }
+	HD bool check(int64_t, int64_t) { return true; }
+	HD bool finish() {
+		if (pending) emit();
+		pending = false;
+		return true;
+	}
+};
+
+// Ops of one accepted record (chain `chain` of group g, DP results from task0): returns the op
+// count and writes them to out unless it is null; *pos receives iPositionOfQueryOnSubject.
+HD int record_ops(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, int chain, uint32_t task0,
+		uint32_t *out, int *pos) {
+	SeedAt sa{g.seeds};
+	I32Arr cid{g.cid};
+	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
+	RecordV rv{R, pr, task0, out, 0, 0, 0, 0, false, 0, 0};
+	walk_record(o, pr.Ls, pr.Lq, keys, rv);
+	*pos = rv.pos;
+	return rv.n;
 }
 
 // ---- consensus ----------------------------------------------------------------------------

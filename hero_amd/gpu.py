@@ -24,7 +24,7 @@ class Options(ctypes.Structure):
         ("miss", ctypes.c_int32), ("hit", ctypes.c_int32), ("unobserved", ctypes.c_int32),
         ("band_override", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
-        ("dp_waves", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -57,6 +57,7 @@ SIGNATURES = {
     "hero_align_subjects": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
     "hero_align_self": (ctypes.c_int, [_P]),
     "hero_call_consensus": (ctypes.c_int, [_P, _P]),
+    "hero_take_alignments": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "hero_get_tally": (ctypes.c_int, [_P, _P]),
     "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "hero_reset_stats": (ctypes.c_int, [_P]),
@@ -147,6 +148,14 @@ class Context:
         out = np.empty_like(self._queries.seq)
         self._check(lib().hero_call_consensus(self.h, out.ctypes.data))
         return out
+
+    def take_alignments(self):
+        """The .aln text of the subjects aligned since the last call (record mode)."""
+        n = ctypes.c_int64(0)
+        self._check(lib().hero_take_alignments(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(1, n.value))
+        self._check(lib().hero_take_alignments(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
 
     def tally(self):
         t = np.zeros(4 * len(self._queries.seq), dtype=np.uint32)

@@ -48,7 +48,8 @@ typedef struct {
 	int32_t dp_form;        /* 0 batched (lane per pair, length buckets), 1 plain (wavefront per pair) */
 	int64_t chunk_subjects; /* subject reads per device pass; 0 = automatic */
 	int32_t dp_waves;       /* persistent DP wavefronts; 0 = automatic */
-	int32_t reserved[7];
+	int32_t records;        /* 1: keep the alignment records as .aln text (requires call_consensus = 0) */
+	int32_t reserved[6];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
@@ -87,6 +88,14 @@ int hero_align_subjects(hero_ctx *ctx, int64_t n, const char *seq, const int64_t
 /* Same, with the query set itself as the subject stream (self-correction: -q f -s f); no second
  * upload. */
 int hero_align_self(hero_ctx *ctx);
+
+/* Record mode (options.records = 1, without -c): the .aln text that SubjectReadAligner writes for
+ * the subjects aligned since the last call (src/SubjectReadAligner.cpp:854-910,
+ * AlignmentRecord.cpp:102-149, SubjectRead.cpp:25-79,179-193): ">sid\n", then
+ * "qid,strand,pos,CIGAR,interiorInsertedBases," per record, then "\n", for every subject in input
+ * order. *len receives the text's size; when out is not null and cap >= *len the text is copied
+ * (without a terminating zero) and released. */
+int hero_take_alignments(hero_ctx *ctx, char *out, int64_t cap, int64_t *len);
 
 /* QueryDataset::callConsensus (src/QueryDataset.cpp:393-412, QueryRead.cpp:148-180): writes the
  * corrected reads in the layout of hero_set_queries' `seq` (reads that are not good are copied

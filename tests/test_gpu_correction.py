@@ -115,3 +115,36 @@ def test_empty_and_tiny_sets():
         ctx.build_index()
         ctx.align_self()
         assert ctx.consensus().tobytes() == b"ACGT" * 20
+
+
+@pytest.mark.parametrize("case", load_e2e(), ids=lambda c: c["name"])
+def test_alignment_records_match_reference(case):
+    """Record mode (no -c): the .aln text is byte-identical to the reference's."""
+    rs = synth.generate(**case["synth"])
+    opts = flags_to_opts(case["flags"])
+    with gpu.Context(call_consensus=0, records=1, **opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        aln = ctx.take_alignments()
+        assert ctx.take_alignments() == b""
+        st = ctx.stats()
+    assert st["dp_skipped"] == 0
+    assert len(aln) == case["aln_bytes"]
+    assert sha256(aln) == case["aln_sha256"]
+
+
+@pytest.mark.parametrize("extra", [dict(chunk_subjects=333), dict(dp_form=1)], ids=["chunks", "plain"])
+def test_alignment_records_match_oracle(extra):
+    rs = synth.generate(seed=91, genome_len=12000, len_min=75, len_max=180, coverage=30.0, sub_5p=0.01,
+                        indel_rate=0.003)
+    q = rs.subset(np.arange(0, rs.n, 2))
+    s = rs.subset(np.arange(1, rs.n, 3))
+    opts = dict(k=27, aligner=0, mismatch_rate=0.05, indel_rate=0.03, min_overlap=40)
+    with gpu.Context(call_consensus=0, records=1, **opts, **extra) as ctx:
+        ctx.set_queries(q)
+        ctx.build_index()
+        ctx.align_subjects(s)
+        aln = ctx.take_alignments()
+    ro = O.run(O.make_cfg(call_consensus=0, **opts), q, s, threads=8, want_aln=True)
+    assert aln.decode() == ro["aln"]
