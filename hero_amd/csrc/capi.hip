@@ -3,7 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -111,12 +117,83 @@ struct Timer {
 	hipEvent_t a = nullptr, b = nullptr;
 };
 
+// A device seed index: open-addressing slots over CSR buckets of (pos << 40) | read entries.
+struct IndexBufs {
+	DevBuf<Slot> slots;
+	DevBuf<uint64_t> key_lo, entries;
+	uint64_t P = 0, total_keys = 0;
+	uint32_t max_probe = 0;
+	bool built = false;
+	DevIndex view(int k) const {
+		return DevIndex{slots.p, key_lo.p, entries.p, P, max_probe, k, make_fastmod(P ? P : 1)};
+	}
+	void release() {
+		slots.release(), key_lo.release(), entries.release();
+		P = total_keys = 0, max_probe = 0, built = false;
+	}
+};
+
+// One pipeline: a stream, its events, device counters and the per-chunk work buffers. The context
+// owns one; the serving loop owns one per step it keeps in flight.
+struct Worker {
+	hipStream_t st = nullptr;
+	Timer ev[10];
+	hero_stats *acc = nullptr; // host-side timings accumulate here
+	DevBuf<uint32_t> seg_n, seg_off, scan_tmp, lane_n;
+	DevBuf<uint64_t> seeds, win_info;
+	DevBuf<int32_t> cid, cfirst, clast, clen;
+	DevBuf<uint8_t> gflag;
+	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
+	DevBuf<DpTask> tasks;
+	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
+	DevBuf<uint32_t> bucket_hist, bucket_off, order;
+	DevBuf<int32_t> g_nch, g_best;
+	DevBuf<uint32_t> g_recs, g_rec_off, rec_opn, rec_opoff, rec_ops;
+	DevBuf<RecHead> recs;
+	DevBuf<int32_t> rec_pos;
+	DevBuf<StepSummary> sums;
+	DevBuf<unsigned long long> dstats, dtotal;
+	DevBuf<uint32_t> counter;
+	// serving: one batch of query reads and its seeds
+	ReadsDev bq;
+	DevBuf<uint8_t> bq_good, b_cons;
+	DevBuf<uint32_t> b_tally, b_cnt, b_off, b_newsub, b_subx, b_sub_ids, b_seg_end;
+	DevBuf<uint64_t> b_kref, b_keys, b_vals, b_keys2, b_vals2;
+	DevBuf<uint8_t> b_sort_tmp;
+	hipError_t init() {
+		hipError_t e = hipStreamCreate(&st);
+		for (auto &t : ev)
+			if (e == hipSuccess) e = hipEventCreate(&t.a);
+		if (e == hipSuccess) e = dstats.exact(kStCount);
+		if (e == hipSuccess) e = dtotal.exact(1);
+		if (e == hipSuccess) e = counter.exact(1);
+		if (e == hipSuccess) e = hipMemset(dstats.p, 0, kStCount * 8);
+		return e;
+	}
+	void release() {
+		if (st) (void) hipStreamSynchronize(st);
+		seg_n.release(), seg_off.release(), scan_tmp.release(), lane_n.release(), seeds.release(), win_info.release();
+		cid.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
+		g_tasks.release(), g_task_off.release(), g_words.release(), g_word_off.release();
+		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), plain_scratch.release();
+		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
+		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
+		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
+		bq.release(), bq_good.release(), b_cons.release(), b_tally.release(), b_cnt.release(), b_off.release();
+		b_newsub.release(), b_subx.release(), b_sub_ids.release(), b_seg_end.release(), b_kref.release();
+		b_keys.release(), b_vals.release(), b_keys2.release(), b_vals2.release(), b_sort_tmp.release();
+		for (auto &t : ev)
+			if (t.a) (void) hipEventDestroy(t.a), t.a = nullptr;
+		if (st) (void) hipStreamDestroy(st);
+		st = nullptr;
+	}
+};
+
 } // namespace
 
 struct hero_ctx {
 	hero_options opt;
 	Opts o;
-	hipStream_t st = nullptr;
 	std::string err;
 	int cu_count = 0;
 	// queries (host copy kept for the uncorrected reads of the output)
@@ -132,36 +209,22 @@ struct hero_ctx {
 	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
 	// subjects (separate stream)
 	ReadsDev Sd;
-	// index
-	DevBuf<Slot> slots;
-	DevBuf<uint64_t> key_lo, entries;
-	uint64_t P = 0;
-	uint32_t max_probe = 0;
-	bool index_built = false;
-	uint64_t distinct_keys = 0, total_keys = 0;
+	// seed index over the query reads (the reference's orientation)
+	IndexBufs qix;
+	// serving: resident subject reads and their all-positions index
+	ReadsDev Rd;
+	std::vector<int32_t> r_len;
+	std::vector<uint8_t> r_active;
+	int r_max_len = 0;
+	IndexBufs six;
 	// tallies: 4 u32 per query base (own-base slot difference coded, see VoteSink)
 	DevBuf<uint32_t> tally;
-	// chunk work buffers
-	DevBuf<uint32_t> seg_n, seg_off, scan_tmp, lane_n;
-	DevBuf<uint64_t> seeds, win_info;
-	DevBuf<int32_t> cid, cfirst, clast, clen;
-	DevBuf<uint8_t> gflag;
-	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
-	DevBuf<DpTask> tasks;
-	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
-	DevBuf<uint32_t> bucket_hist, bucket_off, order;
-	DevBuf<int32_t> g_nch, g_best;
 	// record mode: .aln text of the subjects aligned since the last hero_take_alignments
 	std::string aln;
-	DevBuf<uint32_t> g_recs, g_rec_off, rec_opn, rec_opoff, rec_ops;
-	DevBuf<RecHead> recs;
-	DevBuf<int32_t> rec_pos;
-	DevBuf<StepSummary> sums;
-	DevBuf<unsigned long long> dstats, dtotal;
-	DevBuf<uint32_t> counter;
+	// the context's own pipeline (stream, events, chunk buffers)
+	Worker w;
 	// stats
 	hero_stats st_host;
-	Timer ev[10];
 };
 
 namespace {
@@ -203,29 +266,35 @@ Packed view(const ReadsDev &r, const uint8_t *good) {
 	return P;
 }
 
+int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
+		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good);
 int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
 		const int64_t *uid, const std::vector<uint8_t> &good) {
+	return upload_reads_on(c, c->w.st, d, n, seq, off, len, uid, good);
+}
+int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
+		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good) {
 	std::vector<uint64_t> words, woff;
 	pack_reads(seq, off, len, n, words, woff);
 	std::vector<uint64_t> boff((size_t) n);
 	uint64_t acc = 0;
 	for (int64_t i = 0; i < n; i++) boff[(size_t) i] = acc, acc += (uint64_t) len[i];
-	d.release();
-	HIPCHK(c, d.words.exact(words.size()));
-	HIPCHK(c, d.woff.exact((size_t) n));
-	HIPCHK(c, d.boff.exact((size_t) n));
-	HIPCHK(c, d.len.exact((size_t) n));
-	HIPCHK(c, d.uid.exact((size_t) n));
-	HIPCHK(c, d.good.exact((size_t) n));
-	HIPCHK(c, hipMemcpyAsync(d.words.p, words.data(), words.size() * 8, hipMemcpyHostToDevice, c->st));
+	// grow-only buffers: a steady stream of batches allocates nothing after the largest one
+	HIPCHK(c, d.words.ensure(words.size()));
+	HIPCHK(c, d.woff.ensure((size_t) n + 1));
+	HIPCHK(c, d.boff.ensure((size_t) n + 1));
+	HIPCHK(c, d.len.ensure((size_t) n + 1));
+	HIPCHK(c, d.uid.ensure((size_t) n + 1));
+	HIPCHK(c, d.good.ensure((size_t) n + 1));
+	HIPCHK(c, hipMemcpyAsync(d.words.p, words.data(), words.size() * 8, hipMemcpyHostToDevice, st));
 	if (n) {
-		HIPCHK(c, hipMemcpyAsync(d.woff.p, woff.data(), (size_t) n * 8, hipMemcpyHostToDevice, c->st));
-		HIPCHK(c, hipMemcpyAsync(d.boff.p, boff.data(), (size_t) n * 8, hipMemcpyHostToDevice, c->st));
-		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, c->st));
-		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, c->st));
-		HIPCHK(c, hipMemcpyAsync(d.good.p, good.data(), (size_t) n, hipMemcpyHostToDevice, c->st));
+		HIPCHK(c, hipMemcpyAsync(d.woff.p, woff.data(), (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.boff.p, boff.data(), (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.good.p, good.data(), (size_t) n, hipMemcpyHostToDevice, st));
 	}
-	HIPCHK(c, hipStreamSynchronize(c->st));
+	HIPCHK(c, hipStreamSynchronize(st));
 	d.n = n;
 	return HERO_OK;
 }
@@ -238,97 +307,114 @@ int check_reads(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, con
 	return HERO_OK;
 }
 
-double elapsed(hero_ctx *c, int a, int b) {
+double elapsed(Worker *w, int a, int b) {
 	float ms = 0;
-	(void) hipEventElapsedTime(&ms, c->ev[a].a, c->ev[b].a);
+	(void) hipEventElapsedTime(&ms, w->ev[a].a, w->ev[b].a);
 	return ms;
 }
 
 // One subject chunk [s0, s0 + nsub) of the packed subject set S.
-int format_records(hero_ctx *c, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots);
+int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots);
+int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const DevIndex &ix, ChunkDev &cd,
+		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, const int64_t *s_uid, bool rec_mode);
 
-int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_needed, const int64_t *s_uid) {
+// Exclusive scan of n u32 on the worker's stream; the total comes back to the host.
+int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, uint64_t &total) {
+	HIPCHK(c, w->scan_tmp.ensure(scan_tmp_words(n)));
+	HIPCHK(c, scan_u32(in, out, n, w->dtotal.p, w->scan_tmp.p, w->scan_tmp.n, w->st));
+	unsigned long long t = 0;
+	HIPCHK(c, hipMemcpyAsync(&t, w->dtotal.p, 8, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	total = t;
+	return HERO_OK;
+}
+
+int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub, int rows_needed, const int64_t *s_uid) {
 	const bool rec_mode = c->opt.records != 0;
 	const Packed Q = view(c->Qd, c->Qd.good.p);
-	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, c->o.k, make_fastmod(c->P)};
+	const DevIndex ix = c->qix.view(c->o.k);
 	ChunkDev cd;
 	std::memset(&cd, 0, sizeof cd);
 	cd.s_begin = s0, cd.nsub = nsub;
-	HIPCHK(c, c->seg_n.ensure((size_t) 2 * nsub));
-	HIPCHK(c, c->seg_off.ensure((size_t) 2 * nsub));
-	cd.seg_n = c->seg_n.p, cd.seg_off = c->seg_off.p;
+	HIPCHK(c, w->seg_n.ensure((size_t) 2 * nsub));
+	HIPCHK(c, w->seg_off.ensure((size_t) 2 * nsub));
+	cd.seg_n = w->seg_n.p, cd.seg_off = w->seg_off.p;
 	// window buckets kept between the seed count and pass A1 when they fit kWinInfoBytes
 	const int64_t win_stride = std::max(0, rows_needed - c->o.k);
 	if (win_stride > 0 && (uint64_t) nsub * 2 * win_stride * 8 <= kWinInfoBytes) {
-		HIPCHK(c, c->win_info.ensure((size_t) nsub * 2 * win_stride));
-		HIPCHK(c, c->lane_n.ensure((size_t) nsub * 2 * 64));
-		cd.win_stride = (int32_t) win_stride, cd.win_info = c->win_info.p, cd.lane_n = c->lane_n.p;
+		HIPCHK(c, w->win_info.ensure((size_t) nsub * 2 * win_stride));
+		HIPCHK(c, w->lane_n.ensure((size_t) nsub * 2 * 64));
+		cd.win_stride = (int32_t) win_stride, cd.win_info = w->win_info.p, cd.lane_n = w->lane_n.p;
 	}
-	unsigned long long *stats = c->dstats.p;
-	auto scan = [&](const uint32_t *in, uint32_t *out, int64_t n, uint64_t &total) -> int {
-		HIPCHK(c, c->scan_tmp.ensure(scan_tmp_words(n)));
-		HIPCHK(c, scan_u32(in, out, n, c->dtotal.p, c->scan_tmp.p, c->scan_tmp.n, c->st));
-		unsigned long long t = 0;
-		HIPCHK(c, hipMemcpyAsync(&t, c->dtotal.p, 8, hipMemcpyDeviceToHost, c->st));
-		HIPCHK(c, hipStreamSynchronize(c->st));
-		total = t;
-		return HERO_OK;
-	};
-	HIPCHK(c, hipEventRecord(c->ev[0].a, c->st));
-	HIPCHK(c, launch_seed_count(S, Q, ix, c->o, cd, stats, c->st));
-	HIPCHK(c, hipEventRecord(c->ev[8].a, c->st));
+	unsigned long long *stats = w->dstats.p;
+	HIPCHK(c, hipEventRecord(w->ev[0].a, w->st));
+	HIPCHK(c, launch_seed_count(S, Q, ix, c->o, cd, stats, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
 	uint64_t nseeds = 0;
-	if (int r = scan(cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
+	if (int r = wscan(c, w, cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
+	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->Qd.boff.p, c->tally.p, s_uid, rec_mode);
+}
+
+// The rest of a chunk once its seeds are counted per (subject, strand) segment: seed emission (or
+// the in-place sort of presorted seeds), chaining, planning, DP, scoring, and votes or records.
+int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const DevIndex &ix, ChunkDev &cd,
+		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, const int64_t *s_uid, bool rec_mode) {
+	const int32_t nsub = cd.nsub;
+	const int64_t s0 = cd.s_begin;
+	unsigned long long *stats = w->dstats.p;
 	if (nseeds >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu seeds; lower chunk_subjects", nsub, (unsigned long long) nseeds);
 	const size_t ns = (size_t) nseeds;
-	HIPCHK(c, c->seeds.ensure(ns));
-	HIPCHK(c, c->cid.ensure(ns));
-	HIPCHK(c, c->cfirst.ensure(ns));
-	HIPCHK(c, c->clast.ensure(ns));
-	HIPCHK(c, c->clen.ensure(ns));
-	HIPCHK(c, c->gflag.ensure(ns));
-	HIPCHK(c, c->g_nch.ensure(ns));
-	HIPCHK(c, c->g_best.ensure(ns));
-	HIPCHK(c, c->g_tasks.ensure(ns));
-	HIPCHK(c, c->g_task_off.ensure(ns));
-	HIPCHK(c, c->g_words.ensure(ns));
-	HIPCHK(c, c->g_word_off.ensure(ns));
-	cd.seeds = c->seeds.p, cd.cid = c->cid.p, cd.cfirst = c->cfirst.p, cd.clast = c->clast.p, cd.clen = c->clen.p;
-	cd.gflag = c->gflag.p, cd.g_tasks = c->g_tasks.p, cd.g_task_off = c->g_task_off.p;
-	cd.g_nch = c->g_nch.p, cd.g_best = c->g_best.p;
-	cd.g_words = c->g_words.p, cd.g_word_off = c->g_word_off.p;
-	if (rec_mode) {
-		HIPCHK(c, c->g_recs.ensure(ns));
-		HIPCHK(c, c->g_rec_off.ensure(ns));
-		cd.g_recs = c->g_recs.p;
+	if (!cd.presorted) {
+		HIPCHK(c, w->seeds.ensure(ns));
+		cd.seeds = w->seeds.p;
 	}
-	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, c->st));
-	HIPCHK(c, hipEventRecord(c->ev[1].a, c->st));
+	HIPCHK(c, w->cid.ensure(ns));
+	HIPCHK(c, w->cfirst.ensure(ns));
+	HIPCHK(c, w->clast.ensure(ns));
+	HIPCHK(c, w->clen.ensure(ns));
+	HIPCHK(c, w->gflag.ensure(ns));
+	HIPCHK(c, w->g_nch.ensure(ns));
+	HIPCHK(c, w->g_best.ensure(ns));
+	HIPCHK(c, w->g_tasks.ensure(ns));
+	HIPCHK(c, w->g_task_off.ensure(ns));
+	HIPCHK(c, w->g_words.ensure(ns));
+	HIPCHK(c, w->g_word_off.ensure(ns));
+	cd.cid = w->cid.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
+	cd.gflag = w->gflag.p, cd.g_tasks = w->g_tasks.p, cd.g_task_off = w->g_task_off.p;
+	cd.g_nch = w->g_nch.p, cd.g_best = w->g_best.p;
+	cd.g_words = w->g_words.p, cd.g_word_off = w->g_word_off.p;
+	if (rec_mode) {
+		HIPCHK(c, w->g_recs.ensure(ns));
+		HIPCHK(c, w->g_rec_off.ensure(ns));
+		cd.g_recs = w->g_recs.p;
+	}
+	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[1].a, w->st));
 	uint64_t ntasks = 0, nwords = 0;
-	if (int r = scan(cd.g_tasks, cd.g_task_off, (int64_t) ns, ntasks)) return r;
-	if (int r = scan(cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
+	if (int r = wscan(c, w, cd.g_tasks, cd.g_task_off, (int64_t) ns, ntasks)) return r;
+	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
 	uint64_t nslots = 0;
 	if (rec_mode) {
-		if (int r = scan(cd.g_recs, c->g_rec_off.p, (int64_t) ns, nslots)) return r;
+		if (int r = wscan(c, w, cd.g_recs, w->g_rec_off.p, (int64_t) ns, nslots)) return r;
 		if (nslots >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu record slots; lower chunk_subjects", nsub, (unsigned long long) nslots);
-		HIPCHK(c, c->recs.ensure((size_t) nslots));
-		HIPCHK(c, hipMemsetAsync(c->recs.p, 0xFF, (size_t) nslots * sizeof(RecHead), c->st));
-		cd.recs = c->recs.p, cd.g_rec_off = c->g_rec_off.p;
+		HIPCHK(c, w->recs.ensure((size_t) nslots));
+		HIPCHK(c, hipMemsetAsync(w->recs.p, 0xFF, (size_t) nslots * sizeof(RecHead), w->st));
+		cd.recs = w->recs.p, cd.g_rec_off = w->g_rec_off.p;
 	}
-	HIPCHK(c, c->tasks.ensure((size_t) ntasks));
-	HIPCHK(c, c->words.ensure((size_t) nwords + 1));
-	HIPCHK(c, c->nsteps.ensure((size_t) ntasks));
-	HIPCHK(c, c->sums.ensure((size_t) ntasks));
-	cd.tasks = c->tasks.p, cd.words = c->words.p, cd.nsteps = c->nsteps.p, cd.sums = c->sums.p;
-	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, c->st));
-	HIPCHK(c, hipEventRecord(c->ev[2].a, c->st));
+	HIPCHK(c, w->tasks.ensure((size_t) ntasks));
+	HIPCHK(c, w->words.ensure((size_t) nwords + 1));
+	HIPCHK(c, w->nsteps.ensure((size_t) ntasks));
+	HIPCHK(c, w->sums.ensure((size_t) ntasks));
+	cd.tasks = w->tasks.p, cd.words = w->words.p, cd.nsteps = w->nsteps.p, cd.sums = w->sums.p;
+	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
 	// DP
 	DpLaunch L;
 	std::memset(&L, 0, sizeof L);
 	L.tasks = cd.tasks, L.ntasks = (uint32_t) ntasks, L.order = nullptr, L.S = S, L.Q = Q, L.o = c->o;
-	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = c->counter.p;
+	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = w->counter.p;
 	const bool aff = c->o.aligner == 0;
 	const int rows = std::min(rows_needed, 512);
 	const int bmax_plain = 64;
@@ -338,17 +424,17 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 	if (c->opt.dp_form == 0 && ntasks > 0) {
 		// batched form: bucket tasks by (band class, descending length), then one persistent launch
 		// per class at the smallest band width that holds it
-		HIPCHK(c, c->bucket_hist.ensure(kNumBuckets));
-		HIPCHK(c, c->bucket_off.ensure(kNumBuckets));
-		HIPCHK(c, c->order.ensure((size_t) ntasks));
-		HIPCHK(c, hipMemsetAsync(c->bucket_hist.p, 0, kNumBuckets * 4, c->st));
-		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, c->bucket_hist.p, c->st));
+		HIPCHK(c, w->bucket_hist.ensure(kNumBuckets));
+		HIPCHK(c, w->bucket_off.ensure(kNumBuckets));
+		HIPCHK(c, w->order.ensure((size_t) ntasks));
+		HIPCHK(c, hipMemsetAsync(w->bucket_hist.p, 0, kNumBuckets * 4, w->st));
+		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, w->bucket_hist.p, w->st));
 		uint64_t total = 0;
-		if (int r = scan(c->bucket_hist.p, c->bucket_off.p, kNumBuckets, total)) return r;
+		if (int r = wscan(c, w, w->bucket_hist.p, w->bucket_off.p, kNumBuckets, total)) return r;
 		std::vector<uint32_t> off(kNumBuckets);
-		HIPCHK(c, hipMemcpy(off.data(), c->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost));
-		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, c->bucket_off.p, c->order.p, c->st));
-		HIPCHK(c, hipEventRecord(c->ev[7].a, c->st));
+		HIPCHK(c, hipMemcpy(off.data(), w->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, w->bucket_off.p, w->order.p, w->st));
+		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
 		for (int cls = 0; cls <= kNumBandClasses; cls++) {
 			const uint32_t b0 = off[(size_t) cls * kLenBins];
 			const uint32_t b1 = cls + 1 <= kNumBandClasses ? off[(size_t) (cls + 1) * kLenBins] : (uint32_t) ntasks;
@@ -360,77 +446,77 @@ int run_chunk(hero_ctx *c, const Packed &S, int64_t s0, int32_t nsub, int rows_n
 			const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
 			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * dp_batched_waves_per_cu(bm, aff);
 			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 63) / 64);
-			HIPCHK(c, c->dp_scratch.ensure((size_t) waves * rows * nw * 64));
+			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64));
 			DpLaunch Lc = L;
-			Lc.order = c->order.p + b0, Lc.ntasks = b1 - b0;
-			Lc.scratch = c->dp_scratch.p, Lc.rows_max = rows;
-			HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
-			HIPCHK(c, launch_dp_batched(Lc, bm, waves, c->st));
+			Lc.order = w->order.p + b0, Lc.ntasks = b1 - b0;
+			Lc.scratch = w->dp_scratch.p, Lc.rows_max = rows;
+			HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
+			HIPCHK(c, launch_dp_batched(Lc, bm, waves, w->st));
 		}
 	} else {
-		HIPCHK(c, hipEventRecord(c->ev[7].a, c->st));
+		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
 	}
 	if (plain_n > 0) {
 		// plain form: every task (dp_form 1), or the bucket beyond the batched band classes
-		HIPCHK(c, c->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+		HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
 		DpLaunch Lp = L;
-		Lp.order = c->opt.dp_form == 0 ? c->order.p + plain_begin : nullptr;
+		Lp.order = c->opt.dp_form == 0 ? w->order.p + plain_begin : nullptr;
 		Lp.ntasks = plain_n;
-		Lp.scratch = c->plain_scratch.p, Lp.rows_max = 512;
-		HIPCHK(c, hipMemsetAsync(c->counter.p, 0, 4, c->st));
-		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), c->st));
+		Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
+		HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
+		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), w->st));
 	}
-	HIPCHK(c, hipEventRecord(c->ev[3].a, c->st));
+	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
 	{
 		unsigned long long skipped = 0;
-		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, c->st));
-		HIPCHK(c, hipStreamSynchronize(c->st));
+		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, w->st));
+		HIPCHK(c, hipStreamSynchronize(w->st));
 		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
 	}
-	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, c->Qd.boff.p, c->tally.p, stats, c->st));
-	HIPCHK(c, hipEventRecord(c->ev[4].a, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
-	c->st_host.ms_seed += elapsed(c, 0, 1);
-	c->st_host.ms_seed_probe += elapsed(c, 0, 8);
-	c->st_host.ms_plan += elapsed(c, 1, 7); // pass A2 + task bucketing
-	c->st_host.ms_dp += elapsed(c, 7, 3);   // DP kernels only
-	c->st_host.ms_score_vote += elapsed(c, 3, 4);
-	c->st_host.dp_tasks += ntasks;
-	c->st_host.subjects += (uint64_t) nsub;
+	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, q_boff, tally, stats, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	w->acc->ms_seed += elapsed(w, 0, 1);
+	w->acc->ms_seed_probe += elapsed(w, 0, 8);
+	w->acc->ms_plan += elapsed(w, 1, 7); // pass A2 + task bucketing
+	w->acc->ms_dp += elapsed(w, 7, 3);   // DP kernels only
+	w->acc->ms_score_vote += elapsed(w, 3, 4);
+	w->acc->dp_tasks += ntasks;
+	w->acc->subjects += (uint64_t) nsub;
 	if (!rec_mode) return HERO_OK;
 	// records: op counts, their scan, ops; then the host applies SubjectRead::addAlignment and
 	// writes the .aln text
 	const uint32_t nsl = (uint32_t) nslots;
-	HIPCHK(c, c->rec_opn.ensure((size_t) nsl + 1));
-	HIPCHK(c, c->rec_opoff.ensure((size_t) nsl + 1));
-	HIPCHK(c, c->rec_pos.ensure((size_t) nsl + 1));
-	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, c->rec_opn.p, nullptr, nullptr, nullptr, c->st));
+	HIPCHK(c, w->rec_opn.ensure((size_t) nsl + 1));
+	HIPCHK(c, w->rec_opoff.ensure((size_t) nsl + 1));
+	HIPCHK(c, w->rec_pos.ensure((size_t) nsl + 1));
+	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, w->rec_opn.p, nullptr, nullptr, nullptr, w->st));
 	uint64_t nops = 0;
-	if (int r = scan(c->rec_opn.p, c->rec_opoff.p, (int64_t) nsl, nops)) return r;
+	if (int r = wscan(c, w, w->rec_opn.p, w->rec_opoff.p, (int64_t) nsl, nops)) return r;
 	if (nops >= (1ull << 32)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu CIGAR ops; lower chunk_subjects", nsub, (unsigned long long) nops);
-	HIPCHK(c, c->rec_ops.ensure((size_t) nops + 1));
-	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, c->rec_opn.p, c->rec_opoff.p, c->rec_pos.p, c->rec_ops.p, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
-	return format_records(c, s0, nsub, s_uid, nsl);
+	HIPCHK(c, w->rec_ops.ensure((size_t) nops + 1));
+	HIPCHK(c, launch_record_ops(S, Q, c->o, cd, nsl, w->rec_opn.p, w->rec_opoff.p, w->rec_pos.p, w->rec_ops.p, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	return format_records(c, w, s0, nsub, s_uid, nsl);
 }
 
 // SubjectRead::addAlignment (src/SubjectRead.cpp:25-79) over a subject's accepted records in
 // processing order, then the .aln text: ">sid\n" + "qid,strand,pos,CIGAR,interiorInsertedBases,"
 // per kept record + "\n" (AlignmentRecord.cpp:102-149, SubjectRead.cpp:179-193,
 // SubjectReadAligner.cpp:854-910). Record ids are query uniqueIds (without -c, :230-231).
-int format_records(hero_ctx *c, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots) {
+int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots) {
 	std::vector<RecHead> h(nslots);
 	std::vector<uint32_t> opn(nslots), opoff(nslots);
 	std::vector<int32_t> pos(nslots);
 	if (nslots) {
-		HIPCHK(c, hipMemcpy(h.data(), c->recs.p, (size_t) nslots * sizeof(RecHead), hipMemcpyDeviceToHost));
-		HIPCHK(c, hipMemcpy(opn.data(), c->rec_opn.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
-		HIPCHK(c, hipMemcpy(opoff.data(), c->rec_opoff.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
-		HIPCHK(c, hipMemcpy(pos.data(), c->rec_pos.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(h.data(), w->recs.p, (size_t) nslots * sizeof(RecHead), hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(opn.data(), w->rec_opn.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(opoff.data(), w->rec_opoff.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
+		HIPCHK(c, hipMemcpy(pos.data(), w->rec_pos.p, (size_t) nslots * 4, hipMemcpyDeviceToHost));
 	}
 	const uint64_t nops = nslots ? (uint64_t) opoff[nslots - 1] + opn[nslots - 1] : 0;
 	std::vector<uint32_t> ops((size_t) nops);
-	if (nops) HIPCHK(c, hipMemcpy(ops.data(), c->rec_ops.p, (size_t) nops * 4, hipMemcpyDeviceToHost));
+	if (nops) HIPCHK(c, hipMemcpy(ops.data(), w->rec_ops.p, (size_t) nops * 4, hipMemcpyDeviceToHost));
 	// accepted records per subject, in slot (= processing) order
 	std::vector<uint32_t> first((size_t) nsub + 1, 0), order;
 	for (uint32_t i = 0; i < nslots; i++)
@@ -521,13 +607,204 @@ int format_records(hero_ctx *c, int64_t s0, int32_t nsub, const int64_t *s_uid, 
 	return HERO_OK;
 }
 
+// Serving: one batch of query reads corrected against the resident subjects (-c). Seeds come
+// from the subject index (every k-mer position) probed with the batch's query keys, sorted by
+// (subject, strand) and handed to the offline pipeline's per-subject passes; per-query results
+// do not depend on which other queries share the batch.
+int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
+		const int64_t *uid, char *corrected) {
+	if (!c->six.built) return fail(c, HERO_ERR_STATE, "hero_build_subject_index must run before correcting batches");
+	if (!c->o.call_consensus) return fail(c, HERO_ERR_ARG, "batch correction needs call_consensus = 1");
+	if (c->o.max_candidates > 0) return fail(c, HERO_ERR_ARG, "max_candidates depends on the batch mix; not allowed in batch correction");
+	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
+	if (n >= (1 << 22)) return fail(c, HERO_ERR_LIMIT, "at most 2^22 reads per batch");
+	const int k = c->o.k;
+	std::vector<uint8_t> good((size_t) n);
+	std::vector<uint64_t> kref;
+	int max_len = c->r_max_len;
+	for (int64_t i = 0; i < n; i++) {
+		good[(size_t) i] = query_good(seq + off[i], len[i], c->o.min_overlap);
+		max_len = std::max(max_len, len[i]);
+		if (!good[(size_t) i]) continue;
+		const int nk = num_keys(len[i], k);
+		for (int t = 0; t < nk; t++) kref.push_back(((uint64_t) i << 14) | (uint64_t) key_pos(len[i], k, t));
+	}
+	if (kref.size() >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "batch has too many keys");
+	const uint32_t nkeys = (uint32_t) kref.size();
+	if (int r = upload_reads_on(c, w->st, w->bq, n, seq, off, len, uid, good)) return r;
+	int64_t bases = 0;
+	for (int64_t i = 0; i < n; i++) bases += len[i];
+	HIPCHK(c, w->bq_good.ensure((size_t) n + 1));
+	HIPCHK(c, w->b_tally.ensure((size_t) 4 * bases + 4));
+	HIPCHK(c, w->b_cons.ensure((size_t) bases + 1));
+	HIPCHK(c, w->b_kref.ensure((size_t) nkeys + 1));
+	HIPCHK(c, w->b_cnt.ensure((size_t) nkeys + 1));
+	HIPCHK(c, w->b_off.ensure((size_t) nkeys + 1));
+	HIPCHK(c, hipMemsetAsync(w->b_tally.p, 0, (size_t) 16 * bases, w->st));
+	if (nkeys) HIPCHK(c, hipMemcpyAsync(w->b_kref.p, kref.data(), (size_t) nkeys * 8, hipMemcpyHostToDevice, w->st));
+	const Packed Q = view(w->bq, w->bq.good.p);
+	const Packed S = view(c->Rd, c->Rd.good.p);
+	const DevIndex six = c->six.view(k);
+	HIPCHK(c, hipEventRecord(w->ev[0].a, w->st));
+	HIPCHK(c, launch_query_probe(Q, S, six, c->o, w->b_kref.p, nkeys, w->b_cnt.p, nullptr, nullptr, nullptr, w->st));
+	uint64_t nseeds = 0;
+	if (int r = wscan(c, w, w->b_cnt.p, w->b_off.p, nkeys, nseeds)) return r;
+	if (nseeds >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "batch yields %llu seeds", (unsigned long long) nseeds);
+	const uint32_t ns = (uint32_t) nseeds;
+	HIPCHK(c, w->b_keys.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_vals.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_keys2.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_vals2.ensure((size_t) ns + 1));
+	HIPCHK(c, launch_query_probe(Q, S, six, c->o, w->b_kref.p, nkeys, w->b_cnt.p, w->b_off.p, w->b_keys.p, w->b_vals.p, w->st));
+	int key_bits = 1;
+	while (((uint64_t) 1 << key_bits) < (uint64_t) (2 * c->Rd.n + 2)) key_bits++;
+	size_t tmp_bytes = 0;
+	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, key_bits, nullptr, tmp_bytes, w->st));
+	HIPCHK(c, w->b_sort_tmp.ensure(tmp_bytes + 1));
+	tmp_bytes = w->b_sort_tmp.n;
+	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, key_bits, w->b_sort_tmp.p, tmp_bytes, w->st));
+	HIPCHK(c, w->b_newsub.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_subx.ensure((size_t) ns + 1));
+	HIPCHK(c, launch_seg_mark(w->b_keys2.p, ns, w->b_newsub.p, w->st));
+	uint64_t ntouched = 0;
+	if (int r = wscan(c, w, w->b_newsub.p, w->b_subx.p, ns, ntouched)) return r;
+	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
+	const uint32_t nt = (uint32_t) ntouched;
+	HIPCHK(c, w->b_sub_ids.ensure((size_t) nt + 1));
+	HIPCHK(c, w->seg_off.ensure((size_t) 2 * nt + 2));
+	HIPCHK(c, w->seg_n.ensure((size_t) 2 * nt + 2));
+	HIPCHK(c, w->b_seg_end.ensure((size_t) 2 * nt + 2));
+	HIPCHK(c, hipMemsetAsync(w->seg_off.p, 0, (size_t) 8 * nt, w->st));
+	HIPCHK(c, hipMemsetAsync(w->b_seg_end.p, 0, (size_t) 8 * nt, w->st));
+	HIPCHK(c, launch_seg_fill(w->b_keys2.p, ns, w->b_subx.p, w->b_sub_ids.p, w->seg_off.p, w->b_seg_end.p, w->st));
+	HIPCHK(c, launch_seg_finish(2 * nt, w->seg_off.p, w->b_seg_end.p, w->seg_n.p, w->st));
+	ChunkDev cd;
+	std::memset(&cd, 0, sizeof cd);
+	cd.s_begin = 0, cd.nsub = (int32_t) nt, cd.presorted = 1, cd.sub_ids = w->b_sub_ids.p;
+	cd.seg_n = w->seg_n.p, cd.seg_off = w->seg_off.p, cd.seeds = w->b_vals2.p;
+	if (nt > 0)
+		if (int r = run_from_seeds(c, w, S, Q, six, cd, nseeds, max_len + 1, w->bq.boff.p, w->b_tally.p, nullptr, false)) return r;
+	// consensus of the batch, corrected bases back to the caller's layout
+	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
+	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p + kStReadsChanged, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
+	std::vector<char> h((size_t) bases);
+	if (bases) HIPCHK(c, hipMemcpyAsync(h.data(), w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	uint64_t b = 0;
+	for (int64_t i = 0; i < n; i++) {
+		const int L = len[i];
+		std::memcpy(corrected + off[i], good[(size_t) i] ? h.data() + b : seq + off[i], (size_t) L);
+		b += (uint64_t) L;
+	}
+	if (nt == 0) w->acc->ms_seed += elapsed(w, 0, 8);
+	w->acc->ms_consensus += elapsed(w, 5, 6);
+	w->acc->queries += (uint64_t) n;
+	return HERO_OK;
+}
+
 int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int64_t *s_uid) {
-	if (!c->index_built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
+	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
 	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
 	for (int64_t s0 = 0; s0 < n; s0 += chunk) {
 		const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
-		if (int r = run_chunk(c, S, s0, m, max_len + 1, s_uid)) return r;
+		if (int r = run_chunk(c, &c->w, S, s0, m, max_len + 1, s_uid)) return r;
 	}
+	return HERO_OK;
+}
+
+// Seed index over the good reads of R: keys at 0, k, 2k, ... and L - k (the reference's query
+// index), or at every position (all_positions: the serving index over resident subjects).
+// table_size 0 = the smallest prime above 2 * keys.
+int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good, const std::vector<int32_t> &len,
+		bool all_positions, uint64_t table_size, IndexBufs &ix) {
+	const int k = c->o.k;
+	const int64_t n = R.n;
+	std::vector<uint64_t> koff((size_t) n + 1, 0);
+	uint64_t nocc = 0;
+	for (int64_t i = 0; i < n; i++) {
+		koff[(size_t) i] = nocc;
+		const int L = len[(size_t) i];
+		if (good[(size_t) i]) nocc += (uint64_t) (all_positions ? std::max(0, L - k + 1) : num_keys(L, k));
+	}
+	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu index keys exceed 2^32-1", (unsigned long long) nocc);
+	ix.release();
+	ix.P = table_size ? table_size : next_prime_above(2 * nocc + 1);
+	if (ix.P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) ix.P);
+	if (nocc >= ix.P) return fail(c, HERO_ERR_REF_ABORT, "hash table too small (%llu keys, %llu slots); the reference exits here",
+			(unsigned long long) nocc, (unsigned long long) ix.P);
+	hipEvent_t e0 = c->w.ev[5].a, e1 = c->w.ev[6].a;
+	HIPCHK(c, hipEventRecord(e0, c->w.st));
+	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
+	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
+	DevBuf<int> errf;
+	DevBuf<unsigned long long> cursor;
+	auto cleanup = [&]() {
+		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
+		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
+	};
+	int rc = HERO_OK;
+	do {
+#define IXCHK(expr)                                                                                                 \
+	{                                                                                                               \
+		hipError_t e_ = (expr);                                                                                     \
+		if (e_ != hipSuccess) {                                                                                     \
+			rc = fail(c, e_ == hipErrorOutOfMemory ? HERO_ERR_NOMEM : HERO_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+			break;                                                                                                  \
+		}                                                                                                           \
+	}
+		IXCHK(dkoff.exact((size_t) n + 1));
+		IXCHK(occ_hi.exact(nocc));
+		IXCHK(occ_lo.exact(nocc));
+		IXCHK(occ_entry.exact(nocc));
+		IXCHK(occ_slot.exact(nocc));
+		IXCHK(occ_rank.exact(nocc));
+		IXCHK(owner.exact(ix.P));
+		IXCHK(maxp.exact(1));
+		IXCHK(errf.exact(1));
+		IXCHK(cursor.exact(1));
+		IXCHK(ix.slots.exact(ix.P));
+		IXCHK(ix.key_lo.exact(k > 32 ? ix.P : 1));
+		IXCHK(ix.entries.exact(nocc));
+		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
+		IXCHK(hipMemsetAsync(owner.p, 0, ix.P * 4, c->w.st));
+		IXCHK(hipMemsetAsync(ix.slots.p, 0, ix.P * sizeof(Slot), c->w.st));
+		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->w.st));
+		IXCHK(hipMemsetAsync(errf.p, 0, 4, c->w.st));
+		IXCHK(hipMemsetAsync(cursor.p, 0, 8, c->w.st));
+		const Packed Rp = view(R, R.good.p);
+		IXCHK(launch_index_gen(Rp, dkoff.p, k, all_positions, occ_hi.p, occ_lo.p, occ_entry.p, c->w.st));
+		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, ix.P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->w.st));
+		IXCHK(launch_index_count(occ_slot.p, nocc, ix.slots.p, occ_rank.p, c->w.st));
+		IXCHK(launch_index_alloc(owner.p, ix.P, occ_hi.p, occ_lo.p, k, ix.slots.p, ix.key_lo.p, cursor.p, c->w.st));
+		IXCHK(launch_index_fill(occ_slot.p, occ_rank.p, occ_entry.p, nocc, ix.slots.p, ix.entries.p, c->w.st));
+		IXCHK(launch_index_sort_buckets(ix.slots.p, ix.P, ix.entries.p, c->w.st));
+		IXCHK(hipEventRecord(e1, c->w.st));
+		int herr = 0;
+		unsigned long long used = 0;
+		IXCHK(hipMemcpyAsync(&ix.max_probe, maxp.p, 4, hipMemcpyDeviceToHost, c->w.st));
+		IXCHK(hipMemcpyAsync(&herr, errf.p, 4, hipMemcpyDeviceToHost, c->w.st));
+		IXCHK(hipMemcpyAsync(&used, cursor.p, 8, hipMemcpyDeviceToHost, c->w.st));
+		IXCHK(hipStreamSynchronize(c->w.st));
+		if (herr) {
+			rc = fail(c, HERO_ERR_REF_ABORT, "hash table full");
+			break;
+		}
+		if (used != nocc) {
+			rc = fail(c, HERO_ERR_HIP, "index build placed %llu of %llu keys", used, (unsigned long long) nocc);
+			break;
+		}
+		// distinct keys = claimed slots
+		std::vector<uint32_t> dummy;
+		ix.total_keys = nocc;
+		float ms = 0;
+		(void) hipEventElapsedTime(&ms, e0, e1);
+		c->st_host.ms_index += ms;
+#undef IXCHK
+	} while (0);
+	cleanup();
+	if (rc != HERO_OK) return rc;
+	ix.built = true;
 	return HERO_OK;
 }
 
@@ -559,20 +836,16 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 	c->o = make_opts(*opt);
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	hipDeviceProp_t prop;
-	if (hipGetDeviceProperties(&prop, opt->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+	if (hipGetDeviceProperties(&prop, opt->device) != hipSuccess) {
 		delete c;
 		return HERO_ERR_HIP;
 	}
 	c->cu_count = prop.multiProcessorCount;
-	for (auto &t : c->ev)
-		if (hipEventCreate(&t.a) != hipSuccess) {
-			delete c;
-			return HERO_ERR_HIP;
-		}
-	if (c->dstats.exact(kStCount) != hipSuccess || c->dtotal.exact(1) != hipSuccess || c->counter.exact(1) != hipSuccess ||
-			hipMemset(c->dstats.p, 0, kStCount * 8) != hipSuccess) {
+	c->w.acc = &c->st_host;
+	if (hipError_t e = c->w.init(); e != hipSuccess) {
+		c->w.release();
 		delete c;
-		return HERO_ERR_NOMEM;
+		return e == hipErrorOutOfMemory ? HERO_ERR_NOMEM : HERO_ERR_HIP;
 	}
 	*out = c;
 	return HERO_OK;
@@ -581,20 +854,11 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 void hero_destroy(hero_ctx *c) {
 	if (!c) return;
 	(void) hipSetDevice(c->opt.device);
-	(void) hipStreamSynchronize(c->st);
+	(void) hipStreamSynchronize(c->w.st);
 	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
-	c->slots.release(), c->key_lo.release(), c->entries.release(), c->tally.release();
-	c->seg_n.release(), c->seg_off.release(), c->scan_tmp.release(), c->seeds.release();
-	c->win_info.release(), c->lane_n.release();
-	c->cid.release(), c->cfirst.release(), c->clast.release(), c->clen.release(), c->gflag.release();
-	c->g_tasks.release(), c->g_task_off.release(), c->g_words.release(), c->g_word_off.release();
-	c->tasks.release(), c->words.release(), c->nsteps.release(), c->dp_scratch.release(), c->plain_scratch.release();
-	c->dstats.release(), c->dtotal.release(), c->counter.release(), c->cons_out.release();
-	c->bucket_hist.release(), c->bucket_off.release(), c->order.release();
-	c->g_nch.release(), c->g_best.release(), c->sums.release();
-	for (auto &t : c->ev)
-		if (t.a) (void) hipEventDestroy(t.a);
-	if (c->st) (void) hipStreamDestroy(c->st);
+	c->qix.release(), c->tally.release(), c->cons_out.release();
+	c->Rd.release(), c->six.release();
+	c->w.release();
 	delete c;
 }
 
@@ -630,7 +894,7 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	c->q_bases = qb;
 	HIPCHK(c, c->tally.exact((size_t) 4 * qb));
 	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 16 * qb));
-	c->index_built = false;
+	c->qix.built = false;
 	c->st_host.queries = (uint64_t) n;
 	return HERO_OK;
 }
@@ -638,96 +902,13 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 int hero_build_index(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
-	const int k = c->o.k;
-	const int64_t n = c->Qd.n;
-	std::vector<uint64_t> koff((size_t) n + 1, 0);
-	uint64_t nocc = 0;
-	for (int64_t i = 0; i < n; i++) {
-		koff[(size_t) i] = nocc;
-		if (c->q_good[(size_t) i]) nocc += (uint64_t) num_keys(c->q_len[(size_t) i], k);
-	}
-	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu query keys exceed 2^32-1", (unsigned long long) nocc);
-	c->P = next_prime_above((uint64_t) n * (uint64_t) c->opt.hash_factor);
-	if (c->P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) c->P);
-	if (nocc >= c->P) return fail(c, HERO_ERR_REF_ABORT, "hash table too small (%llu keys, %llu slots); the reference exits here",
-			(unsigned long long) nocc, (unsigned long long) c->P);
-	hipEvent_t e0 = c->ev[5].a, e1 = c->ev[6].a;
-	HIPCHK(c, hipEventRecord(e0, c->st));
 	// a new index starts a new correction run: clear the vote tallies
-	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 16 * c->q_bases, c->st));
-	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
-	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
-	DevBuf<int> errf;
-	DevBuf<unsigned long long> cursor;
-	auto cleanup = [&]() {
-		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
-		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
-	};
-	int rc = HERO_OK;
-	do {
-#define IXCHK(expr)                                                                                                 \
-	{                                                                                                               \
-		hipError_t e_ = (expr);                                                                                     \
-		if (e_ != hipSuccess) {                                                                                     \
-			rc = fail(c, e_ == hipErrorOutOfMemory ? HERO_ERR_NOMEM : HERO_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-			break;                                                                                                  \
-		}                                                                                                           \
-	}
-		IXCHK(dkoff.exact((size_t) n + 1));
-		IXCHK(occ_hi.exact(nocc));
-		IXCHK(occ_lo.exact(nocc));
-		IXCHK(occ_entry.exact(nocc));
-		IXCHK(occ_slot.exact(nocc));
-		IXCHK(occ_rank.exact(nocc));
-		IXCHK(owner.exact(c->P));
-		IXCHK(maxp.exact(1));
-		IXCHK(errf.exact(1));
-		IXCHK(cursor.exact(1));
-		IXCHK(c->slots.exact(c->P));
-		IXCHK(c->key_lo.exact(k > 32 ? c->P : 1));
-		IXCHK(c->entries.exact(nocc));
-		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->st));
-		IXCHK(hipMemsetAsync(owner.p, 0, c->P * 4, c->st));
-		IXCHK(hipMemsetAsync(c->slots.p, 0, c->P * sizeof(Slot), c->st));
-		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->st));
-		IXCHK(hipMemsetAsync(errf.p, 0, 4, c->st));
-		IXCHK(hipMemsetAsync(cursor.p, 0, 8, c->st));
-		const Packed Q = view(c->Qd, c->Qd.good.p);
-		IXCHK(launch_index_gen(Q, dkoff.p, k, occ_hi.p, occ_lo.p, occ_entry.p, c->st));
-		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, c->P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->st));
-		IXCHK(launch_index_count(occ_slot.p, nocc, c->slots.p, occ_rank.p, c->st));
-		IXCHK(launch_index_alloc(owner.p, c->P, occ_hi.p, occ_lo.p, k, c->slots.p, c->key_lo.p, cursor.p, c->st));
-		IXCHK(launch_index_fill(occ_slot.p, occ_rank.p, occ_entry.p, nocc, c->slots.p, c->entries.p, c->st));
-		IXCHK(launch_index_sort_buckets(c->slots.p, c->P, c->entries.p, c->st));
-		IXCHK(hipEventRecord(e1, c->st));
-		int herr = 0;
-		unsigned long long used = 0;
-		IXCHK(hipMemcpyAsync(&c->max_probe, maxp.p, 4, hipMemcpyDeviceToHost, c->st));
-		IXCHK(hipMemcpyAsync(&herr, errf.p, 4, hipMemcpyDeviceToHost, c->st));
-		IXCHK(hipMemcpyAsync(&used, cursor.p, 8, hipMemcpyDeviceToHost, c->st));
-		IXCHK(hipStreamSynchronize(c->st));
-		if (herr) {
-			rc = fail(c, HERO_ERR_REF_ABORT, "hash table full");
-			break;
-		}
-		if (used != nocc) {
-			rc = fail(c, HERO_ERR_HIP, "index build placed %llu of %llu keys", used, (unsigned long long) nocc);
-			break;
-		}
-		// distinct keys = claimed slots
-		std::vector<uint32_t> dummy;
-		c->total_keys = nocc;
-		float ms = 0;
-		(void) hipEventElapsedTime(&ms, e0, e1);
-		c->st_host.ms_index += ms;
-#undef IXCHK
-	} while (0);
-	cleanup();
-	if (rc != HERO_OK) return rc;
-	c->index_built = true;
-	c->st_host.table_size = c->P;
-	c->st_host.total_keys = c->total_keys;
-	c->st_host.max_probe = c->max_probe;
+	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 16 * c->q_bases, c->w.st));
+	const uint64_t P = next_prime_above((uint64_t) c->Qd.n * (uint64_t) c->opt.hash_factor);
+	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
+	c->st_host.table_size = c->qix.P;
+	c->st_host.total_keys = c->qix.total_keys;
+	c->st_host.max_probe = c->qix.max_probe;
 	return HERO_OK;
 }
 
@@ -767,21 +948,21 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	const int64_t n = c->Qd.n;
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
-	HIPCHK(c, hipEventRecord(c->ev[5].a, c->st));
+	HIPCHK(c, hipEventRecord(c->w.ev[5].a, c->w.st));
 	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, n, c->cons_out.p, nullptr,
-			c->dstats.p + kStReadsChanged, c->st));
-	HIPCHK(c, hipEventRecord(c->ev[6].a, c->st));
+			c->w.dstats.p + kStReadsChanged, c->w.st));
+	HIPCHK(c, hipEventRecord(c->w.ev[6].a, c->w.st));
 	// The device writes corrected bases at packed base offsets; the caller's layout usually is that
 	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
 	if (c->q_contiguous) {
-		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->st));
-		HIPCHK(c, hipStreamSynchronize(c->st));
+		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
+		HIPCHK(c, hipStreamSynchronize(c->w.st));
 		for (int64_t i : c->q_not_good)
 			std::memcpy(corrected + c->q_off[(size_t) i], c->q_seq.data() + c->q_off[(size_t) i], (size_t) c->q_len[(size_t) i]);
 	} else {
 		std::vector<char> h((size_t) c->q_bases);
-		HIPCHK(c, hipMemcpyAsync(h.data(), c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->st));
-		HIPCHK(c, hipStreamSynchronize(c->st));
+		HIPCHK(c, hipMemcpyAsync(h.data(), c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
+		HIPCHK(c, hipStreamSynchronize(c->w.st));
 		std::memcpy(corrected, c->q_seq.data(), c->q_seq.size());
 		uint64_t b = 0;
 		for (int64_t i = 0; i < n; i++) {
@@ -791,7 +972,7 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 		}
 	}
 	float ms = 0;
-	(void) hipEventElapsedTime(&ms, c->ev[5].a, c->ev[6].a);
+	(void) hipEventElapsedTime(&ms, c->w.ev[5].a, c->w.ev[6].a);
 	c->st_host.ms_consensus += ms;
 	return HERO_OK;
 }
@@ -804,10 +985,10 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 	HIPCHK(c, codes.exact((size_t) c->q_bases));
 	HIPCHK(c, dec.exact((size_t) 4 * c->q_bases));
 	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, c->Qd.n, codes.p, dec.p,
-			nullptr, c->st));
+			nullptr, c->w.st));
 	std::vector<uint32_t> h((size_t) 4 * c->q_bases);
-	HIPCHK(c, hipMemcpyAsync(h.data(), dec.p, h.size() * 4, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
+	HIPCHK(c, hipMemcpyAsync(h.data(), dec.p, h.size() * 4, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	// device layout is by base offset (reads packed back to back); map to the caller's offsets
 	uint64_t b = 0;
 	for (int64_t i = 0; i < c->Qd.n; i++) {
@@ -822,7 +1003,7 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	if (!c || !out) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	unsigned long long s[kStCount];
-	HIPCHK(c, hipMemcpy(s, c->dstats.p, sizeof s, hipMemcpyDeviceToHost));
+	HIPCHK(c, hipMemcpy(s, c->w.dstats.p, sizeof s, hipMemcpyDeviceToHost));
 	*out = c->st_host;
 	out->lookups = s[kStLookups];
 	out->seeds = s[kStSeeds];
@@ -841,7 +1022,7 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 int hero_reset_stats(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
-	HIPCHK(c, hipMemset(c->dstats.p, 0, kStCount * 8));
+	HIPCHK(c, hipMemset(c->w.dstats.p, 0, kStCount * 8));
 	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;
@@ -892,13 +1073,13 @@ int hero_band_align_batch(hero_ctx *c, int32_t n, const hero_band_pair *pairs, c
 	HIPCHK(c, dwords.exact((size_t) words + 1));
 	HIPCHK(c, dns.exact((size_t) n));
 	HIPCHK(c, dscr.exact((size_t) ((n + 63) / 64) * rows * nw * 64));
-	HIPCHK(c, hipMemcpyAsync(dpairs.p, bp.data(), bp.size() * sizeof(BytePair), hipMemcpyHostToDevice, c->st));
-	HIPCHK(c, hipMemcpyAsync(dseq.p, seq, (size_t) seq_end, hipMemcpyHostToDevice, c->st));
-	HIPCHK(c, launch_band_align_bytes(dpairs.p, n, dseq.p, c->o, dwords.p, dns.p, dscr.p, rows, bsel, c->st));
+	HIPCHK(c, hipMemcpyAsync(dpairs.p, bp.data(), bp.size() * sizeof(BytePair), hipMemcpyHostToDevice, c->w.st));
+	HIPCHK(c, hipMemcpyAsync(dseq.p, seq, (size_t) seq_end, hipMemcpyHostToDevice, c->w.st));
+	HIPCHK(c, launch_band_align_bytes(dpairs.p, n, dseq.p, c->o, dwords.p, dns.p, dscr.p, rows, bsel, c->w.st));
 	std::vector<uint32_t> hw((size_t) words + 1), hn((size_t) n);
-	HIPCHK(c, hipMemcpyAsync(hw.data(), dwords.p, hw.size() * 4, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipMemcpyAsync(hn.data(), dns.p, hn.size() * 4, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
+	HIPCHK(c, hipMemcpyAsync(hw.data(), dwords.p, hw.size() * 4, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipMemcpyAsync(hn.data(), dns.p, hn.size() * 4, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	// BandAligner::calculateCigarAndPosition[Reversed] (src/BandAligner.cpp:258-419)
 	int64_t pos = 0;
 	for (int i = 0; i < n; i++) {
@@ -1000,18 +1181,18 @@ extern "C" int hero_hash_kmers(hero_ctx *c, int32_t n, int32_t k, const char *km
 	DevBuf<uint64_t> dout;
 	HIPCHK(c, dk.exact((size_t) n * k));
 	HIPCHK(c, dout.exact((size_t) n));
-	HIPCHK(c, hipMemcpyAsync(dk.p, kmers, (size_t) n * k, hipMemcpyHostToDevice, c->st));
-	k_hash_kmers<<<(n + 255) / 256, 256, 0, c->st>>>(dk.p, n, k, P, dout.p);
+	HIPCHK(c, hipMemcpyAsync(dk.p, kmers, (size_t) n * k, hipMemcpyHostToDevice, c->w.st));
+	k_hash_kmers<<<(n + 255) / 256, 256, 0, c->w.st>>>(dk.p, n, k, P, dout.p);
 	HIPCHK(c, hipGetLastError());
-	HIPCHK(c, hipMemcpyAsync(out, dout.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
+	HIPCHK(c, hipMemcpyAsync(out, dout.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	return HERO_OK;
 }
 
 extern "C" int hero_lookup(hero_ctx *c, int32_t n, const char *windows, int64_t *count, int64_t *entry_off, uint64_t *entries,
 		int64_t entry_cap) {
 	if (!c || n < 0 || (n > 0 && (!windows || !count || !entry_off))) return HERO_ERR_ARG;
-	if (!c->index_built) return fail(c, HERO_ERR_STATE, "index not built");
+	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "index not built");
 	if (n == 0) return HERO_OK;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	const int k = c->o.k;
@@ -1021,22 +1202,268 @@ extern "C" int hero_lookup(hero_ctx *c, int32_t n, const char *windows, int64_t 
 	HIPCHK(c, dw.exact((size_t) n * k));
 	HIPCHK(c, dcount.exact((size_t) n));
 	HIPCHK(c, dfirst.exact((size_t) n));
-	HIPCHK(c, hipMemcpyAsync(dw.p, windows, (size_t) n * k, hipMemcpyHostToDevice, c->st));
-	DevIndex ix{c->slots.p, c->key_lo.p, c->entries.p, c->P, c->max_probe, k, make_fastmod(c->P)};
-	k_lookup<<<(n + 255) / 256, 256, 0, c->st>>>(ix, dw.p, n, dcount.p, dfirst.p);
+	HIPCHK(c, hipMemcpyAsync(dw.p, windows, (size_t) n * k, hipMemcpyHostToDevice, c->w.st));
+	const DevIndex ix = c->qix.view(k);
+	k_lookup<<<(n + 255) / 256, 256, 0, c->w.st>>>(ix, dw.p, n, dcount.p, dfirst.p);
 	HIPCHK(c, hipGetLastError());
 	std::vector<uint64_t> first((size_t) n);
-	HIPCHK(c, hipMemcpyAsync(count, dcount.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipMemcpyAsync(first.data(), dfirst.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->st));
-	HIPCHK(c, hipStreamSynchronize(c->st));
+	HIPCHK(c, hipMemcpyAsync(count, dcount.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipMemcpyAsync(first.data(), dfirst.p, (size_t) n * 8, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	int64_t pos = 0;
 	for (int i = 0; i < n; i++) {
 		entry_off[i] = pos;
 		if (count[i] > 0 && entries) {
 			if (pos + count[i] > entry_cap) return fail(c, HERO_ERR_LIMIT, "entry_cap too small");
-			HIPCHK(c, hipMemcpy(entries + pos, c->entries.p + first[(size_t) i], (size_t) count[i] * 8, hipMemcpyDeviceToHost));
+			HIPCHK(c, hipMemcpy(entries + pos, c->qix.entries.p + first[(size_t) i], (size_t) count[i] * 8, hipMemcpyDeviceToHost));
 		}
 		pos += count[i];
 	}
 	return HERO_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Serving: resident subjects, batch correction, and the request loop (continuous batching)
+// ---------------------------------------------------------------------------------------------
+extern "C" int hero_set_subjects(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
+		const int64_t *uid) {
+	if (!c) return HERO_ERR_ARG;
+	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
+	if (n >= ((int64_t) 1 << 31)) return fail(c, HERO_ERR_LIMIT, "at most 2^31-1 resident subjects");
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	c->r_len.assign(len, len + n);
+	c->r_active.resize((size_t) n);
+	c->r_max_len = 0;
+	for (int64_t i = 0; i < n; i++) {
+		c->r_active[(size_t) i] = subject_active(seq + off[i], len[i], c->o.min_overlap, c->o.k);
+		c->r_max_len = std::max(c->r_max_len, len[i]);
+	}
+	c->six.release();
+	c->Rd.release();
+	return upload_reads(c, c->Rd, n, seq, off, len, uid, c->r_active);
+}
+
+extern "C" int hero_build_subject_index(hero_ctx *c) {
+	if (!c) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = build_index(c, c->Rd, c->r_active, c->r_len, true, 0, c->six)) return r;
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
+	return HERO_OK;
+}
+
+extern "C" int hero_correct_batch(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
+		const int64_t *uid, char *corrected) {
+	if (!c || (n > 0 && !corrected)) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	return correct_batch_on(c, &c->w, n, seq, off, len, uid, corrected);
+}
+
+namespace {
+
+double now_s() {
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+	int64_t n = 0;
+	const char *seq = nullptr;
+	const int64_t *off = nullptr;
+	const int32_t *len = nullptr;
+	const int64_t *uid = nullptr;
+	char *out = nullptr;
+	int64_t bases = 0, footprint = 0;
+	hero_request_info info{};
+	bool done = false;
+};
+
+} // namespace
+
+// The request loop: submitted batches wait in arrival order; each worker (its own stream and
+// buffers) takes the longest prefix of waiting batches that fits its share of the HBM budget and
+// the step read limit, runs them as one step, and completes them. A worker starts its next step
+// as soon as its previous one ends, so batches that arrive while others are on the device are
+// admitted as soon as a worker and memory free up, and steps of different workers overlap.
+struct hero_server {
+	hero_ctx *c = nullptr;
+	hero_server_options opt{};
+	std::vector<Worker> workers;
+	std::vector<hero_stats> acc;
+	std::vector<std::thread> threads;
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::deque<int64_t> pending;
+	std::deque<Request> reqs; // by ticket; a deque keeps references valid while others are appended
+	std::atomic<int64_t> steps{0};
+	int64_t inflight_bytes = 0;
+	int64_t bytes_per_base = 0;
+	bool stop = false;
+	std::string err;
+};
+
+namespace {
+
+// Device bytes one step needs per query base: seeds (keys and values, both sort buffers, chain
+// arrays), tallies, DP tasks and step summaries, at 60x coverage with 150 bp reads (measured in
+// the serving workload); the admission budget is charged this much per admitted base.
+constexpr int64_t kServeBytesPerBase = 2048;
+
+void serve_loop(hero_server *sv, int wi) {
+	hero_ctx *c = sv->c;
+	Worker *w = &sv->workers[(size_t) wi];
+	(void) hipSetDevice(c->opt.device);
+	std::vector<char> seq, out;
+	std::vector<int64_t> off, uid;
+	std::vector<int32_t> len;
+	const int64_t share = sv->opt.hbm_budget / std::max(1, sv->opt.workers);
+	for (;;) {
+		std::vector<int64_t> take;
+		int64_t fp = 0;
+		{
+			std::unique_lock<std::mutex> lk(sv->mu);
+			sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+			if (sv->pending.empty()) return; // stopping with nothing left
+			int64_t reads = 0;
+			while (!sv->pending.empty()) {
+				Request &r = sv->reqs[(size_t) sv->pending.front()];
+				const bool first = take.empty();
+				if (!first && (reads + r.n > sv->opt.max_step_reads || fp + r.footprint > share)) break;
+				if (!first && sv->inflight_bytes + fp + r.footprint > sv->opt.hbm_budget) break;
+				if (first && sv->inflight_bytes + r.footprint > sv->opt.hbm_budget && sv->inflight_bytes > 0) break;
+				take.push_back(sv->pending.front());
+				sv->pending.pop_front();
+				reads += r.n, fp += r.footprint;
+			}
+			if (take.empty()) {
+				// memory is held by other steps: wait for one to finish
+				sv->cv_done.wait(lk);
+				continue;
+			}
+			sv->inflight_bytes += fp;
+			const double t = now_s();
+			for (int64_t id : take) sv->reqs[(size_t) id].info.t_admit = t;
+		}
+		// one step over the merged batch
+		seq.clear(), off.clear(), uid.clear(), len.clear();
+		for (int64_t id : take) {
+			const Request &r = sv->reqs[(size_t) id];
+			for (int64_t i = 0; i < r.n; i++) {
+				off.push_back((int64_t) seq.size());
+				len.push_back(r.len[i]);
+				uid.push_back(r.uid[i]);
+				seq.insert(seq.end(), r.seq + r.off[i], r.seq + r.off[i] + r.len[i]);
+			}
+		}
+		out.resize(seq.size());
+		const int64_t step_id = sv->steps.fetch_add(1);
+		const int rc = correct_batch_on(c, w, (int64_t) len.size(), seq.data(), off.data(), len.data(), uid.data(),
+				out.data());
+		size_t i0 = 0;
+		for (int64_t id : take) {
+			Request &r = sv->reqs[(size_t) id];
+			if (rc == HERO_OK)
+				for (int64_t i = 0; i < r.n; i++) std::memcpy(r.out + r.off[i], out.data() + off[i0 + (size_t) i], (size_t) r.len[i]);
+			i0 += (size_t) r.n;
+		}
+		{
+			std::lock_guard<std::mutex> lk(sv->mu);
+			if (rc != HERO_OK && sv->err.empty()) sv->err = c->err;
+			const double t = now_s();
+			for (int64_t id : take) {
+				Request &r = sv->reqs[(size_t) id];
+				r.info.status = rc, r.info.t_done = t, r.info.step_id = step_id;
+				r.info.step_reads = (int32_t) len.size(), r.info.step_batches = (int32_t) take.size();
+				r.done = true;
+			}
+			sv->inflight_bytes -= fp;
+		}
+		sv->cv_done.notify_all();
+	}
+}
+
+} // namespace
+
+extern "C" int hero_server_start(hero_ctx *c, const hero_server_options *opt, hero_server **out) {
+	if (!c || !out) return HERO_ERR_ARG;
+	*out = nullptr;
+	if (!c->six.built) return fail(c, HERO_ERR_STATE, "hero_build_subject_index must run before serving");
+	if (!c->o.call_consensus) return fail(c, HERO_ERR_ARG, "serving corrects reads: call_consensus must be 1");
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	hero_server *sv = new hero_server();
+	sv->c = c;
+	if (opt) sv->opt = *opt;
+	if (sv->opt.workers <= 0) sv->opt.workers = 2;
+	if (sv->opt.max_step_reads <= 0) sv->opt.max_step_reads = 65536;
+	if (sv->opt.hbm_budget <= 0) {
+		size_t fr = 0, tot = 0;
+		(void) hipMemGetInfo(&fr, &tot);
+		sv->opt.hbm_budget = (int64_t) (fr / 2);
+	}
+	sv->bytes_per_base = kServeBytesPerBase;
+	sv->workers.resize((size_t) sv->opt.workers);
+	sv->acc.assign((size_t) sv->opt.workers, hero_stats{});
+	for (int i = 0; i < sv->opt.workers; i++) {
+		sv->workers[(size_t) i].acc = &sv->acc[(size_t) i];
+		if (hipError_t e = sv->workers[(size_t) i].init(); e != hipSuccess) {
+			for (auto &w : sv->workers) w.release();
+			delete sv;
+			return fail(c, HERO_ERR_HIP, "server worker: %s", hipGetErrorString(e));
+		}
+	}
+	for (int i = 0; i < sv->opt.workers; i++) sv->threads.emplace_back(serve_loop, sv, i);
+	*out = sv;
+	return HERO_OK;
+}
+
+extern "C" int hero_server_submit(hero_server *sv, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
+		const int64_t *uid, char *corrected, int64_t *ticket) {
+	if (!sv || !ticket || n < 0 || (n > 0 && (!seq || !off || !len || !uid || !corrected))) return HERO_ERR_ARG;
+	Request r;
+	r.n = n, r.seq = seq, r.off = off, r.len = len, r.uid = uid, r.out = corrected;
+	for (int64_t i = 0; i < n; i++) r.bases += len[i];
+	r.footprint = r.bases * sv->bytes_per_base;
+	if (r.footprint > sv->opt.hbm_budget) return HERO_ERR_NOMEM; // would never be admitted
+	r.info.t_submit = now_s();
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		if (sv->stop) return HERO_ERR_STATE;
+		*ticket = (int64_t) sv->reqs.size();
+		sv->reqs.push_back(r);
+		sv->pending.push_back(*ticket);
+	}
+	sv->cv_work.notify_one();
+	return HERO_OK;
+}
+
+extern "C" int hero_server_wait(hero_server *sv, int64_t ticket, hero_request_info *info) {
+	if (!sv) return HERO_ERR_ARG;
+	std::unique_lock<std::mutex> lk(sv->mu);
+	if (ticket < 0 || ticket >= (int64_t) sv->reqs.size()) return HERO_ERR_ARG;
+	sv->cv_done.wait(lk, [&] { return sv->reqs[(size_t) ticket].done; });
+	const Request &r = sv->reqs[(size_t) ticket];
+	if (info) *info = r.info;
+	return r.info.status;
+}
+
+extern "C" int hero_server_stop(hero_server *sv, hero_stats *merged) {
+	if (!sv) return HERO_ERR_ARG;
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		sv->stop = true;
+	}
+	sv->cv_work.notify_all();
+	for (auto &t : sv->threads) t.join();
+	if (merged) {
+		std::memset(merged, 0, sizeof *merged);
+		for (const hero_stats &a : sv->acc) {
+			merged->ms_seed += a.ms_seed, merged->ms_plan += a.ms_plan, merged->ms_dp += a.ms_dp;
+			merged->ms_score_vote += a.ms_score_vote, merged->ms_consensus += a.ms_consensus;
+			merged->queries += a.queries, merged->dp_tasks += a.dp_tasks, merged->subjects += a.subjects;
+		}
+		merged->ms_total = merged->ms_seed + merged->ms_plan + merged->ms_dp + merged->ms_score_vote + merged->ms_consensus;
+	}
+	for (auto &w : sv->workers) w.release();
+	const int rc = sv->err.empty() ? HERO_OK : HERO_ERR_HIP;
+	if (!sv->err.empty()) sv->c->err = sv->err;
+	delete sv;
+	return rc;
 }

@@ -43,6 +43,8 @@ struct RecHead {
 struct ChunkDev {
 	int64_t s_begin;
 	int32_t nsub;
+	int32_t presorted;         // seeds were written per (subject, strand) segment by the caller
+	const uint32_t *sub_ids;   // subject read of each chunk slot; null = s_begin + slot
 	int32_t win_stride;        // windows per (subject, strand) slot of win_info
 	uint64_t *win_info;        // per window: bucket (start << 32 | count), 0 = no bucket; null = probe again
 	uint32_t *lane_n;          // per (subject, strand) slot and lane: seeds of that lane's windows
@@ -89,9 +91,14 @@ struct DpLaunch {
 	unsigned long long *stats;
 };
 
+// Subject read of chunk slot sl.
+__host__ __device__ inline int64_t chunk_subject(const ChunkDev &c, int sl) {
+	return c.sub_ids ? (int64_t) c.sub_ids[sl] : c.s_begin + sl;
+}
+
 // launchers (kernels.hip)
-hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo,
-		uint64_t *occ_entry, hipStream_t st);
+hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool all_positions, uint64_t *occ_hi,
+		uint64_t *occ_lo, uint64_t *occ_entry, hipStream_t st);
 hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
 		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st);
 hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st);
@@ -128,6 +135,21 @@ hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_ma
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
 		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st);
+
+// Serving orientation: the subjects' k-mers at every position are indexed; each query key (kref =
+// query << 14 | position) and its reverse complement probe them. Count pass (keys_out == nullptr)
+// writes per-key seed counts; emit pass writes (subject << 1 | strand, seed) at off[key].
+hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &six, const Opts &o, const uint64_t *kref,
+		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st);
+// Segments of seeds sorted by (subject << 1 | strand): subject slots, per-slot strand segments.
+hipError_t launch_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub, hipStream_t st);
+hipError_t launch_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
+		uint32_t *seg_off, uint32_t *seg_end, hipStream_t st);
+hipError_t launch_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n,
+		hipStream_t st);
+// Radix sort of (key, value) u64 pairs on the low key_bits bits (serve_sort.hip).
+hipError_t sort_pairs_u64(const uint64_t *keys_in, uint64_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,
+		uint32_t n, int key_bits, void *tmp, size_t &tmp_bytes, hipStream_t st);
 
 // Exclusive scan of n u32 values (n < 2^31); *total_dev receives the u64 sum.
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long long *total_dev, uint32_t *tmp,

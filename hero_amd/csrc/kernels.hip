@@ -50,15 +50,16 @@ __device__ __forceinline__ uint64_t lanes_below_mask() {
 // Buckets hold exactly the occurrences of one k-mer, so lookups return what getListOfReads
 // returns, whatever the slot placement.
 // ---------------------------------------------------------------------------------------------
-__global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo, uint64_t *occ_entry) {
+__global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, int all_positions, uint64_t *occ_hi, uint64_t *occ_lo,
+		uint64_t *occ_entry) {
 	const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= Q.n || !Q.good[q]) return;
 	const int L = Q.len[q];
 	const uint64_t *w = Q.words + Q.woff[q];
 	uint64_t o = koff[q];
-	const int nk = num_keys(L, k);
+	const int nk = all_positions ? (L >= k ? L - k + 1 : 0) : num_keys(L, k);
 	for (int t = 0; t < nk; t++) {
-		const int p = key_pos(L, k, t);
+		const int p = all_positions ? t : key_pos(L, k, t);
 		uint64_t hi, lo;
 		kmer_fwd(w, p, k, hi, lo);
 		occ_hi[o] = hi;
@@ -67,10 +68,10 @@ __global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, uint64_t *occ
 		o++;
 	}
 }
-hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, uint64_t *occ_hi, uint64_t *occ_lo,
-		uint64_t *occ_entry, hipStream_t st) {
+hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool all_positions, uint64_t *occ_hi,
+		uint64_t *occ_lo, uint64_t *occ_entry, hipStream_t st) {
 	if (Q.n == 0) return hipSuccess;
-	k_index_gen<<<(unsigned) ((Q.n + 255) / 256), 256, 0, st>>>(Q, koff, k, occ_hi, occ_lo, occ_entry);
+	k_index_gen<<<(unsigned) ((Q.n + 255) / 256), 256, 0, st>>>(Q, koff, k, all_positions ? 1 : 0, occ_hi, occ_lo, occ_entry);
 	return hipGetLastError();
 }
 
@@ -269,7 +270,7 @@ __device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &
 __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
 	const int sl = blockIdx.x;
-	const int64_t s = c.s_begin + sl;
+	const int64_t s = chunk_subject(c, sl);
 	const int lane = lane_id();
 	if (!S.good[s]) {
 		if (lane == 0) c.seg_n[2 * sl] = 0, c.seg_n[2 * sl + 1] = 0;
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 		unsigned long long *stats) {
 	__shared__ uint64_t lds[kLdsSeeds];
 	const int sl = blockIdx.x;
-	const int64_t s = c.s_begin + sl;
+	const int64_t s = chunk_subject(c, sl);
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
@@ -381,7 +382,10 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 		if (n == 0) continue;
 		const bool in_lds = n <= (uint32_t) kLdsSeeds;
 		uint64_t *dst = in_lds ? lds : c.seeds + off;
-		if (c.win_info) {
+		if (c.presorted) {
+			if (in_lds)
+				for (uint32_t t = lane; t < n; t += 64) lds[t] = c.seeds[off + t];
+		} else if (c.win_info) {
 			const int slot = 2 * sl + strand;
 			uint32_t pos = wave_excl_scan(c.lane_n[(int64_t) slot * 64 + lane]);
 			const uint64_t *wi = c.win_info + (int64_t) slot * c.win_stride;
@@ -455,7 +459,7 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 // Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
 __global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
 	const int sl = blockIdx.x;
-	const int64_t s = c.s_begin + sl;
+	const int64_t s = chunk_subject(c, sl);
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
@@ -523,7 +527,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 		const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
 		if (c.recs) {
 			RecHead *slots = c.recs + c.g_rec_off[base];
-			const int sub = (int) (s - c.s_begin);
+			const int sub = (int) (s - c.s_begin); // record mode runs on contiguous chunks only
 			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built,
 					[&](int chain, uint32_t task, double rate_sum, double len) {
 						RecHead h;
@@ -571,6 +575,90 @@ hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, co
 	return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Serving orientation: arriving query keys probe the resident subjects' all-position index. A
+// query key K at posQ matching subject s forward at p is the reference's strand-0 window p; its
+// reverse complement matching s at p is strand-1 window Ls - p - k (searchHashTable2,
+// src/SubjectReadAligner.cpp:444-491, with the index and the probe sides exchanged).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevIndex six, Opts o, const uint64_t *kref,
+		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out) {
+	const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+	if (t >= nkeys) return;
+	const uint64_t r = kref[t];
+	const int64_t q = (int64_t) (r >> 14);
+	const int pos = (int) (r & 0x3FFF);
+	const int k = o.k;
+	uint64_t fh, fl;
+	kmer_fwd(Q.words + Q.woff[q], pos, k, fh, fl);
+	const int64_t quid = Q.uid[q];
+	const uint32_t base = keys_out ? off[t] : 0u;
+	uint32_t n = 0;
+	for (int strand = 0; strand < 2; strand++) {
+		uint64_t hi = fh, lo = fl;
+		if (strand) kmer_revcomp(k, fh, fl, hi, lo);
+		uint32_t start = 0, count = 0;
+		if (!ix_lookup(six, hi, lo, start, count)) continue;
+		for (uint32_t e = start; e < start + count; e++) {
+			const uint64_t d = six.entries[e];
+			const uint64_t sr = d & kIdMask;
+			if (S.uid[sr] == quid) continue;
+			if (keys_out) {
+				const int p = (int) (d >> 40);
+				keys_out[base + n] = (sr << 1) | (uint64_t) strand;
+				vals_out[base + n] = seed_pack(q, pos, strand ? S.len[sr] - p - k : p);
+			}
+			n++;
+		}
+	}
+	if (!keys_out) cnt[t] = n;
+}
+hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &six, const Opts &o, const uint64_t *kref,
+		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st) {
+	if (nkeys == 0) return hipSuccess;
+	k_query_probe<<<(nkeys + 255) / 256, 256, 0, st>>>(Q, S, six, o, kref, nkeys, cnt, off, keys_out, vals_out);
+	return hipGetLastError();
+}
+
+__global__ void k_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= n) return;
+	newsub[i] = (i == 0 || (keys[i] >> 1) != (keys[i - 1] >> 1)) ? 1u : 0u;
+}
+hipError_t launch_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub, hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	k_seg_mark<<<(n + 255) / 256, 256, 0, st>>>(keys, n, newsub);
+	return hipGetLastError();
+}
+__global__ void k_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
+		uint32_t *seg_off, uint32_t *seg_end) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= n) return;
+	const uint64_t key = keys[i];
+	const bool ns = i == 0 || (key >> 1) != (keys[i - 1] >> 1);
+	const uint32_t sl = sub_excl[i] + (ns ? 1u : 0u) - 1u;
+	const uint32_t seg = 2 * sl + (uint32_t) (key & 1);
+	if (ns) sub_ids[sl] = (uint32_t) (key >> 1);
+	if (i == 0 || key != keys[i - 1]) seg_off[seg] = i;
+	if (i == n - 1 || key != keys[i + 1]) seg_end[seg] = i + 1;
+}
+hipError_t launch_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
+		uint32_t *seg_off, uint32_t *seg_end, hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	k_seg_fill<<<(n + 255) / 256, 256, 0, st>>>(keys, n, sub_excl, sub_ids, seg_off, seg_end);
+	return hipGetLastError();
+}
+__global__ void k_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n) {
+	const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+	if (j < nseg) seg_n[j] = seg_end[j] - seg_off[j];
+}
+hipError_t launch_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n,
+		hipStream_t st) {
+	if (nseg == 0) return hipSuccess;
+	k_seg_finish<<<(nseg + 255) / 256, 256, 0, st>>>(nseg, seg_off, seg_end, seg_n);
+	return hipGetLastError();
+}
+
 // Kept group starts of one segment into an LDS list (wave-wide ballot compaction).
 __device__ __forceinline__ uint32_t collect_groups(const uint8_t *gflag, uint32_t n, uint32_t *list, uint32_t cap) {
 	uint32_t cnt = 0;
@@ -595,7 +683,7 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	__shared__ uint32_t rev_only[kPassBGroups];
 	__shared__ uint8_t used1[kPassBGroups];
 	const int sl = blockIdx.x;
-	const int64_t s = c.s_begin + sl;
+	const int64_t s = chunk_subject(c, sl);
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];

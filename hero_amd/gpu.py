@@ -45,6 +45,20 @@ class BandPair(ctypes.Structure):
                 ("q_len", ctypes.c_int32), ("band", ctypes.c_int32), ("flag", ctypes.c_int32)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_int32), ("max_step_reads", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("reserved", ctypes.c_int32 * 8)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("step_reads", ctypes.c_int32), ("step_batches", ctypes.c_int32),
+                ("_pad", ctypes.c_int32), ("step_id", ctypes.c_int64), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "_pad"}
+
+
 # exported symbols and their signatures (kept in step with include/hero_gpu.h)
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -64,6 +78,13 @@ SIGNATURES = {
     "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
     "hero_hash_kmers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_uint64, _P]),
     "hero_lookup": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_int64]),
+    "hero_set_subjects": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
+    "hero_build_subject_index": (ctypes.c_int, [_P]),
+    "hero_correct_batch": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P]),
+    "hero_server_start": (ctypes.c_int, [_P, ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "hero_server_submit": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
+    "hero_server_wait": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(RequestInfo)]),
+    "hero_server_stop": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
 }
 
 _lib = None
@@ -144,6 +165,22 @@ class Context:
     def align_self(self):
         self._check(lib().hero_align_self(self.h))
 
+    def set_subjects(self, rs):
+        """Resident subject set of the serving form."""
+        self._subjects = rs
+        self._check(lib().hero_set_subjects(self.h, rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data,
+                                            rs.length.ctypes.data, rs.unique_id.ctypes.data))
+
+    def build_subject_index(self):
+        self._check(lib().hero_build_subject_index(self.h))
+
+    def correct_batch(self, rs):
+        """One query batch corrected against the resident subjects (bytes in the layout of rs.seq)."""
+        out = np.empty_like(rs.seq)
+        self._check(lib().hero_correct_batch(self.h, rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data,
+                                             rs.length.ctypes.data, rs.unique_id.ctypes.data, out.ctypes.data))
+        return out
+
     def consensus(self):
         out = np.empty_like(self._queries.seq)
         self._check(lib().hero_call_consensus(self.h, out.ctypes.data))
@@ -223,3 +260,48 @@ def correct(rs, subjects=None, **opts):
             ctx.align_subjects(subjects)
         out = ctx.consensus() if ctx.opts.call_consensus else None
         return out, ctx.stats()
+
+
+class Server:
+    """The native request loop (hero_server_*) over a context with resident subjects."""
+
+    def __init__(self, ctx, workers=0, max_step_reads=0, hbm_budget=0):
+        self.ctx = ctx
+        o = ServerOptions(workers=workers, max_step_reads=max_step_reads, hbm_budget=hbm_budget)
+        h = ctypes.c_void_p()
+        ctx._check(lib().hero_server_start(ctx.h, ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, rs):
+        out = np.empty_like(rs.seq)
+        t = ctypes.c_int64(-1)
+        rc = lib().hero_server_submit(self.h, rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data, rs.length.ctypes.data,
+                                      rs.unique_id.ctypes.data, out.ctypes.data, ctypes.byref(t))
+        if rc != HERO_OK:
+            raise HeroError(rc, "hero_server_submit refused the batch")
+        self._live[t.value] = (rs, out)
+        return t.value
+
+    def wait(self, ticket):
+        info = RequestInfo()
+        rc = lib().hero_server_wait(self.h, ticket, ctypes.byref(info))
+        rs, out = self._live.pop(ticket)
+        if rc != HERO_OK:
+            raise HeroError(rc, lib().hero_last_error(self.ctx.h).decode())
+        return out, info.as_dict()
+
+    def stop(self):
+        st = Stats()
+        if self.h:
+            rc = lib().hero_server_stop(self.h, ctypes.byref(st))
+            self.h = None
+            if rc != HERO_OK:
+                raise HeroError(rc, lib().hero_last_error(self.ctx.h).decode())
+        return st.as_dict()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()

@@ -133,6 +133,49 @@ int hero_hash_kmers(hero_ctx *ctx, int32_t n, int32_t k, const char *kmers, uint
 int hero_lookup(hero_ctx *ctx, int32_t n, const char *windows, int64_t *count, int64_t *entry_off, uint64_t *entries,
 		int64_t entry_cap);
 
+/* ---- Serving -------------------------------------------------------------------------------
+ * The subject set stays resident with an index of its k-mers at every position; batches of query
+ * reads arrive and are corrected against it (-c). The index and probe sides of
+ * SubjectReadAligner::searchHashTable2 (src/SubjectReadAligner.cpp:444-491) are exchanged: each
+ * query key and its reverse complement probe the subjects, which yields the reference's seed set;
+ * everything after the seeds is the offline path. A query's result does not depend on the other
+ * queries of its batch (max_candidates must be 0). */
+int hero_set_subjects(hero_ctx *ctx, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id);
+int hero_build_subject_index(hero_ctx *ctx);
+/* One batch corrected against the resident subjects; output in the layout of `seq` (reads that are
+ * not good are copied unchanged). Uses the context's own stream. */
+int hero_correct_batch(hero_ctx *ctx, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id, char *corrected);
+
+/* Request loop with continuous batching: submitted batches wait in arrival order; each worker
+ * (stream) takes the longest prefix of waiting batches that fits its share of hbm_budget and
+ * max_step_reads, corrects them as one step and completes them, then takes the next. Steps of
+ * different workers overlap on the device. The caller's buffers must stay valid until
+ * hero_server_wait returns for that ticket. Times are steady-clock seconds. */
+typedef struct hero_server hero_server;
+typedef struct {
+	int32_t workers;        /* concurrent steps (streams); 0 = 2 */
+	int32_t max_step_reads; /* reads merged into one step at most; 0 = 65536 */
+	int64_t hbm_budget;     /* device bytes for in-flight steps; 0 = half of the free memory */
+	int32_t reserved[8];
+} hero_server_options;
+typedef struct {
+	int32_t status;        /* HERO_OK or the step's error */
+	int32_t step_reads;    /* reads of the step that carried this batch */
+	int32_t step_batches;  /* batches merged into that step */
+	int32_t _pad;
+	int64_t step_id;
+	double t_submit, t_admit, t_done;
+} hero_request_info;
+int hero_server_start(hero_ctx *ctx, const hero_server_options *opt, hero_server **out);
+/* HERO_ERR_NOMEM: the batch alone exceeds hbm_budget (never admissible). */
+int hero_server_submit(hero_server *srv, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
+		const int64_t *unique_id, char *corrected, int64_t *ticket);
+int hero_server_wait(hero_server *srv, int64_t ticket, hero_request_info *info);
+/* Drains the queue, stops the workers; `merged` (optional) receives their summed device timings. */
+int hero_server_stop(hero_server *srv, hero_stats *merged);
+
 #ifdef __cplusplus
 }
 #endif

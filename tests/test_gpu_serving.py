@@ -1,0 +1,95 @@
+"""Serving form: resident subjects, batches of query reads corrected against them, and the request
+loop. Every batch's result must equal the oracle's (the reference orientation: the batch is the
+query set, the resident reads are the subjects) and must not depend on the batch mix."""
+import numpy as np
+import pytest
+
+from hero_amd import gpu, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40)
+
+
+def _split():
+    rs = synth.generate(seed=515, genome_len=30000, len_min=90, len_max=160, coverage=40.0, sub_5p=0.005,
+                        sub_3p=0.02, indel_rate=0.002)
+    idx = np.arange(rs.n)
+    resident = rs.subset(idx[idx % 10 < 7])
+    fresh = idx[idx % 10 >= 7]
+    # a few resident reads come back as queries: their own uniqueId must be skipped
+    again = idx[idx % 10 < 7][::50]
+    return rs, resident, fresh, again
+
+
+@pytest.fixture(scope="module")
+def served():
+    rs, resident, fresh, again = _split()
+    ctx = gpu.Context(**OPTS)
+    ctx.set_subjects(resident)
+    ctx.build_subject_index()
+    yield rs, resident, fresh, again, ctx
+    ctx.close()
+
+
+def test_batch_matches_oracle(served):
+    rs, resident, fresh, again, ctx = served
+    batch = rs.subset(np.concatenate([fresh[:600], again]))
+    out = ctx.correct_batch(batch)
+    ro = O.run(O.make_cfg(**OPTS), batch, resident, threads=8)
+    assert np.array_equal(out, ro["corrected"])
+    assert ro["stats"]["records_kept"] > 0
+
+
+def test_batch_alone_equals_batch_in_a_mix(served):
+    rs, resident, fresh, again, ctx = served
+    a = rs.subset(fresh[:300])
+    b = rs.subset(np.concatenate([fresh[300:900], again]))
+    alone = ctx.correct_batch(a)
+    mixed = ctx.correct_batch(rs.subset(np.concatenate([fresh[300:600], fresh[:300], fresh[600:900], again])))
+    # a's reads sit at positions 300..599 of the mixed batch
+    m = rs.subset(np.concatenate([fresh[300:600], fresh[:300], fresh[600:900], again]))
+    got = np.concatenate([mixed[m.offset[i]:m.offset[i] + m.length[i]] for i in range(300, 600)])
+    assert np.array_equal(got, alone)
+    assert ctx.correct_batch(b) is not None
+
+
+def test_request_loop_results_equal_alone(served):
+    rs, resident, fresh, again, ctx = served
+    rng = np.random.default_rng(7)
+    pool = np.concatenate([fresh, again])
+    batches = []
+    for _ in range(24):
+        n = int(rng.integers(1, 200))
+        batches.append(rs.subset(np.sort(rng.choice(pool, size=n, replace=False))))
+    alone = [ctx.correct_batch(b) for b in batches]
+    with gpu.Server(ctx, workers=2, max_step_reads=512) as srv:
+        tickets = [srv.submit(b) for b in batches]
+        results = [srv.wait(t) for t in tickets]
+    merged_steps = 0
+    for (out, info), ref in zip(results, alone):
+        assert info["status"] == 0
+        assert info["t_submit"] <= info["t_admit"] <= info["t_done"]
+        assert np.array_equal(out, ref)
+        merged_steps += info["step_batches"] > 1
+    assert merged_steps > 0, "no step carried more than one batch"
+
+
+def test_request_loop_budget_refuses_oversized_batch(served):
+    rs, resident, fresh, again, ctx = served
+    with gpu.Server(ctx, workers=1, hbm_budget=1 << 20) as srv:
+        with pytest.raises(gpu.HeroError):
+            srv.submit(rs.subset(fresh[:400]))
+        t = srv.submit(rs.subset(fresh[:2]))
+        out, info = srv.wait(t)
+        assert info["status"] == 0
+
+
+def test_serving_requires_correction_mode():
+    rs, resident, fresh, again = _split()
+    with gpu.Context(**dict(OPTS, max_candidates=8)) as ctx:
+        ctx.set_subjects(resident)
+        ctx.build_subject_index()
+        with pytest.raises(gpu.HeroError):
+            ctx.correct_batch(rs.subset(fresh[:10]))
