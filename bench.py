@@ -40,6 +40,13 @@ GLOBAL_OPS_PER_CELL = 9
 C2_READS = synth.WORKLOADS["c2"]
 C2_OPTS = dict(call_consensus=1, k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267, min_overlap=40,
                dx=-7, dy=-7, ex=-1, miss=-4, hit=1, band_override=8, max_candidates=64, consensus_freq=0.6)
+# serving: 20 Mb genome, 8,000,000 resident 150 bp reads (60x) + a pool of fresh reads from the same genome
+C4_RESIDENT = 8_000_000
+C4_POOL = 262_144
+C4_READS = dict(seed=404, genome_len=20_000_000, len_min=150, n_reads=C4_RESIDENT + C4_POOL, sub_5p=0.005,
+                sub_3p=0.02, indel_rate=0.001)
+C4_OPTS = dict(C2_OPTS, max_candidates=0)  # a per-subject candidate cap would depend on the batch mix
+C4_RATES = (5, 20, 50, 100, 200)
 C3_READS = dict(seed=303, genome_len=1_000_000, len_min=75, len_max=250, coverage=40.0, sub_5p=0.01)
 C3_OPTS = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40)
 
@@ -151,6 +158,100 @@ def port_baseline(rs, sample, threads):
                 gcups_incl_seeding=round(cells_rate, 3))
 
 
+def _percentile(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def _drive(srv, batches, arrivals):
+    """Submits batches at the given arrival times (s) from this thread, then waits for all of them.
+    Returns per-batch latencies (s) and the reads/s between the first submit and the last completion."""
+    tickets = []
+    t0 = time.perf_counter()
+    for b, at in zip(batches, arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(b))
+    infos = [srv.wait(t)[1] for t in tickets]
+    lat = [i["t_done"] - i["t_submit"] for i in infos]
+    span = max(i["t_done"] for i in infos) - min(i["t_submit"] for i in infos)
+    reads = sum(b.n for b in batches)
+    merged = sum(1 for i in infos if i["step_batches"] > 1)
+    return lat, reads / span if span > 0 else 0.0, merged
+
+
+def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_reads=16384, sweep_rates=C4_RATES,
+                    prepared=None):
+    """Serving form on the resident-subject workload: offline throughput of the same path, a fixed
+    4096-read batch stream at 85% of that offline rate, and Poisson sweeps with log-uniform batch
+    sizes in [1024, 65536]."""
+    if prepared is None:
+        prepared = prepare_serving()
+    rs, pool_idx, ctx, index_s = prepared
+    rng = np.random.default_rng(4040)
+
+    def batch(n, start):
+        i = (start + np.arange(n)) % pool_idx.size
+        return rs.subset(pool_idx[i])
+
+    # offline: the same path over the pool in 65536-read batches, back to back
+    big = [batch(65536, i * 65536) for i in range(pool_idx.size // 65536)]
+    ctx.correct_batch(big[0])
+    t0 = time.perf_counter()
+    for b in big:
+        ctx.correct_batch(b)
+    offline = sum(b.n for b in big) / (time.perf_counter() - t0)
+    log(f"offline (65536-read batches): {offline:.0f} reads/s")
+    out = dict(workload=("c4: 20 Mb genome, 8,000,000 resident 150 bp reads (60x) indexed at every k-mer position; "
+                         "fresh query reads from the same genome; k=39, affine +1/-4/-6/-1, band 8, no candidate cap"),
+               index_build_s=round(index_s, 3), offline_reads_per_s=round(offline, 1), workers=workers,
+               max_step_reads=max_step_reads)
+    with gpu.Server(ctx, workers=workers, max_step_reads=max_step_reads) as srv:
+        # fixed 4096-read batches at 85% of the offline rate
+        rate = 0.85 * offline / 4096
+        nb = max(20, int(rate * fixed_duration))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, nb))
+        batches = [batch(4096, j * 4096) for j in range(nb)]
+        srv.wait(srv.submit(batches[0]))  # one untimed batch warms the workers' buffers
+        lat, rps, merged = _drive(srv, batches, arrivals)
+        out["fixed_4k"] = dict(rate_batches_per_s=round(rate, 2), batches=nb, p50_ms=round(1e3 * _percentile(lat, 50), 3),
+                               p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
+                               vs_offline=round(rps / offline, 4), merged_steps=merged)
+        log("serving fixed 4k:", json.dumps(out["fixed_4k"]))
+        sweep = []
+        for r in sweep_rates:
+            nb = max(10, int(r * seed_duration))
+            arrivals = np.cumsum(rng.exponential(1.0 / r, nb))
+            sizes = np.exp(rng.uniform(np.log(1024), np.log(65536), nb)).astype(int)
+            starts = np.cumsum(np.concatenate([[0], sizes[:-1]]))
+            batches = [batch(int(n), int(st)) for n, st in zip(sizes, starts)]
+            lat, rps, merged = _drive(srv, batches, arrivals)
+            sweep.append(dict(rate_batches_per_s=r, batches=nb, p50_ms=round(1e3 * _percentile(lat, 50), 3),
+                              p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
+                              merged_steps=merged))
+            log("serving poisson:", json.dumps(sweep[-1]))
+        out["poisson_log_uniform_1k_64k"] = sweep
+    return out
+
+
+def prepare_serving():
+    """The resident subject set on the device with its all-position index, and the query pool."""
+    t0 = time.perf_counter()
+    rs = synth.generate(**C4_READS)
+    resident = rs.subset(np.arange(C4_RESIDENT))
+    pool_idx = np.arange(C4_RESIDENT, rs.n)
+    log(f"serving workload: {resident.n} resident + {pool_idx.size} pool reads generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    ctx = gpu.Context(**C4_OPTS)
+    t0 = time.perf_counter()
+    ctx.set_subjects(resident)
+    t1 = time.perf_counter()
+    ctx.build_subject_index()
+    t2 = time.perf_counter()
+    log(f"resident subjects uploaded in {t1 - t0:.1f} s, all-position index built in {t2 - t1:.1f} s")
+    return rs, pool_idx, ctx, t2 - t1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +260,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=50_000, help="subjects streamed by the CPU baselines")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-form and mixed-length runs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
@@ -210,6 +312,12 @@ def main():
                                      batched_over_plain=round(gcups_of(s3b) / max(gcups_of(s3p), 1e-9), 2))
         log("gcups:", json.dumps(gcups))
 
+    latency = None
+    if not args.no_serve:
+        prepared = prepare_serving()
+        latency = serving_latency(prepared=prepared)
+        prepared[2].close()
+
     cpu = None
     port = None
     if not args.no_cpu:
@@ -226,7 +334,7 @@ def main():
                 config=dict(workload=("c2: 5 Mb genome, 150 bp, 50x, 0.5->2% substitutions, 0.1% indels, k=39, affine "
                                       "+1/-4/-6/-1, band 8, cap 64 candidates, self-correction"),
                             reads=rs.n, bases=bases, parallelism="1 GPU"),
-                gcups=gcups, latency=None, roofline=roofline, roofline_probe=roofline_probe, cpu_baseline=cpu,
+                gcups=gcups, latency=latency, roofline=roofline, roofline_probe=roofline_probe, cpu_baseline=cpu,
                 cpu_baseline_port=port,
                 stages_ms=dict(index=round(st["ms_index"] / steps, 3), seed=round(st["ms_seed"] / steps, 3),
                                plan=round(st["ms_plan"] / steps, 3), dp=round(dp_ms, 3),

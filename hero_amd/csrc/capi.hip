@@ -157,7 +157,7 @@ struct Worker {
 	// serving: one batch of query reads and its seeds
 	ReadsDev bq;
 	DevBuf<uint8_t> bq_good, b_cons;
-	DevBuf<uint32_t> b_tally, b_cnt, b_off, b_newsub, b_subx, b_sub_ids, b_seg_end;
+	DevBuf<uint32_t> b_tally, b_cnt, b_off, b_gflag, b_pflag, b_gx, b_px, b_glist, b_plist;
 	DevBuf<uint64_t> b_kref, b_keys, b_vals, b_keys2, b_vals2;
 	DevBuf<uint8_t> b_sort_tmp;
 	hipError_t init() {
@@ -180,7 +180,8 @@ struct Worker {
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
 		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
 		bq.release(), bq_good.release(), b_cons.release(), b_tally.release(), b_cnt.release(), b_off.release();
-		b_newsub.release(), b_subx.release(), b_sub_ids.release(), b_seg_end.release(), b_kref.release();
+		b_gflag.release(), b_pflag.release(), b_gx.release(), b_px.release(), b_glist.release(), b_plist.release();
+		b_kref.release();
 		b_keys.release(), b_vals.release(), b_keys2.release(), b_vals2.release(), b_sort_tmp.release();
 		for (auto &t : ev)
 			if (t.a) (void) hipEventDestroy(t.a), t.a = nullptr;
@@ -329,6 +330,77 @@ int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, 
 	return HERO_OK;
 }
 
+// The DP of a chunk's tasks (cd.tasks): batched form by band class and length bucket, plain form
+// for the rest (or for everything with dp_form 1). Records ev[7] after bucketing, ev[3] at the end.
+int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
+		int rows_needed) {
+	unsigned long long *stats = w->dstats.p;
+	DpLaunch L;
+	std::memset(&L, 0, sizeof L);
+	L.tasks = cd.tasks, L.ntasks = (uint32_t) ntasks, L.order = nullptr, L.S = S, L.Q = Q, L.o = c->o;
+	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = w->counter.p;
+	const bool aff = c->o.aligner == 0;
+	const int rows = std::min(rows_needed, 512);
+	const int bmax_plain = 64;
+	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
+	L.route_bmax = -1, L.route_rows = 0;
+	uint32_t plain_begin = 0, plain_n = (uint32_t) ntasks;
+	if (c->opt.dp_form == 0 && ntasks > 0) {
+		// batched form: bucket tasks by (band class, descending length), then one persistent launch
+		// per class at the smallest band width that holds it
+		HIPCHK(c, w->bucket_hist.ensure(kNumBuckets));
+		HIPCHK(c, w->bucket_off.ensure(kNumBuckets));
+		HIPCHK(c, w->order.ensure((size_t) ntasks));
+		HIPCHK(c, hipMemsetAsync(w->bucket_hist.p, 0, kNumBuckets * 4, w->st));
+		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, w->bucket_hist.p, w->st));
+		uint64_t total = 0;
+		if (int r = wscan(c, w, w->bucket_hist.p, w->bucket_off.p, kNumBuckets, total)) return r;
+		std::vector<uint32_t> off(kNumBuckets);
+		HIPCHK(c, hipMemcpyAsync(off.data(), w->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost, w->st));
+		HIPCHK(c, hipStreamSynchronize(w->st));
+		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, w->bucket_off.p, w->order.p, w->st));
+		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
+		for (int cls = 0; cls <= kNumBandClasses; cls++) {
+			const uint32_t b0 = off[(size_t) cls * kLenBins];
+			const uint32_t b1 = cls + 1 <= kNumBandClasses ? off[(size_t) (cls + 1) * kLenBins] : (uint32_t) ntasks;
+			if (cls == kNumBandClasses) {
+				plain_begin = b0, plain_n = b1 - b0;
+				break;
+			}
+			if (b1 == b0) continue;
+			const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
+			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * dp_batched_waves_per_cu(bm, aff);
+			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 63) / 64);
+			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64));
+			DpLaunch Lc = L;
+			Lc.order = w->order.p + b0, Lc.ntasks = b1 - b0;
+			Lc.scratch = w->dp_scratch.p, Lc.rows_max = rows;
+			HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
+			HIPCHK(c, launch_dp_batched(Lc, bm, waves, w->st));
+		}
+	} else {
+		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
+	}
+	if (plain_n > 0) {
+		// plain form: every task (dp_form 1), or the bucket beyond the batched band classes
+		HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+		DpLaunch Lp = L;
+		Lp.order = c->opt.dp_form == 0 ? w->order.p + plain_begin : nullptr;
+		Lp.ntasks = plain_n;
+		Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
+		HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
+		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), w->st));
+	}
+	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
+	{
+		unsigned long long skipped = 0;
+		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, w->st));
+		HIPCHK(c, hipStreamSynchronize(w->st));
+		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
+	}
+	return HERO_OK;
+}
+
 int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub, int rows_needed, const int64_t *s_uid) {
 	const bool rec_mode = c->opt.records != 0;
 	const Packed Q = view(c->Qd, c->Qd.good.p);
@@ -355,8 +427,8 @@ int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub,
 	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->Qd.boff.p, c->tally.p, s_uid, rec_mode);
 }
 
-// The rest of a chunk once its seeds are counted per (subject, strand) segment: seed emission (or
-// the in-place sort of presorted seeds), chaining, planning, DP, scoring, and votes or records.
+// The rest of a chunk once its seeds are counted per (subject, strand) segment: seed emission,
+// chaining, planning, DP, scoring, and votes or records.
 int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const DevIndex &ix, ChunkDev &cd,
 		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, const int64_t *s_uid, bool rec_mode) {
 	const int32_t nsub = cd.nsub;
@@ -364,10 +436,8 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	unsigned long long *stats = w->dstats.p;
 	if (nseeds >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu seeds; lower chunk_subjects", nsub, (unsigned long long) nseeds);
 	const size_t ns = (size_t) nseeds;
-	if (!cd.presorted) {
-		HIPCHK(c, w->seeds.ensure(ns));
-		cd.seeds = w->seeds.p;
-	}
+	HIPCHK(c, w->seeds.ensure(ns));
+	cd.seeds = w->seeds.p;
 	HIPCHK(c, w->cid.ensure(ns));
 	HIPCHK(c, w->cfirst.ensure(ns));
 	HIPCHK(c, w->clast.ensure(ns));
@@ -410,69 +480,7 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	cd.tasks = w->tasks.p, cd.words = w->words.p, cd.nsteps = w->nsteps.p, cd.sums = w->sums.p;
 	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
-	// DP
-	DpLaunch L;
-	std::memset(&L, 0, sizeof L);
-	L.tasks = cd.tasks, L.ntasks = (uint32_t) ntasks, L.order = nullptr, L.S = S, L.Q = Q, L.o = c->o;
-	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = w->counter.p;
-	const bool aff = c->o.aligner == 0;
-	const int rows = std::min(rows_needed, 512);
-	const int bmax_plain = 64;
-	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
-	L.route_bmax = -1, L.route_rows = 0;
-	uint32_t plain_begin = 0, plain_n = (uint32_t) ntasks;
-	if (c->opt.dp_form == 0 && ntasks > 0) {
-		// batched form: bucket tasks by (band class, descending length), then one persistent launch
-		// per class at the smallest band width that holds it
-		HIPCHK(c, w->bucket_hist.ensure(kNumBuckets));
-		HIPCHK(c, w->bucket_off.ensure(kNumBuckets));
-		HIPCHK(c, w->order.ensure((size_t) ntasks));
-		HIPCHK(c, hipMemsetAsync(w->bucket_hist.p, 0, kNumBuckets * 4, w->st));
-		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, w->bucket_hist.p, w->st));
-		uint64_t total = 0;
-		if (int r = wscan(c, w, w->bucket_hist.p, w->bucket_off.p, kNumBuckets, total)) return r;
-		std::vector<uint32_t> off(kNumBuckets);
-		HIPCHK(c, hipMemcpy(off.data(), w->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost));
-		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, w->bucket_off.p, w->order.p, w->st));
-		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
-		for (int cls = 0; cls <= kNumBandClasses; cls++) {
-			const uint32_t b0 = off[(size_t) cls * kLenBins];
-			const uint32_t b1 = cls + 1 <= kNumBandClasses ? off[(size_t) (cls + 1) * kLenBins] : (uint32_t) ntasks;
-			if (cls == kNumBandClasses) {
-				plain_begin = b0, plain_n = b1 - b0;
-				break;
-			}
-			if (b1 == b0) continue;
-			const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
-			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * dp_batched_waves_per_cu(bm, aff);
-			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 63) / 64);
-			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64));
-			DpLaunch Lc = L;
-			Lc.order = w->order.p + b0, Lc.ntasks = b1 - b0;
-			Lc.scratch = w->dp_scratch.p, Lc.rows_max = rows;
-			HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
-			HIPCHK(c, launch_dp_batched(Lc, bm, waves, w->st));
-		}
-	} else {
-		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
-	}
-	if (plain_n > 0) {
-		// plain form: every task (dp_form 1), or the bucket beyond the batched band classes
-		HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
-		DpLaunch Lp = L;
-		Lp.order = c->opt.dp_form == 0 ? w->order.p + plain_begin : nullptr;
-		Lp.ntasks = plain_n;
-		Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
-		HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
-		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), w->st));
-	}
-	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
-	{
-		unsigned long long skipped = 0;
-		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, w->st));
-		HIPCHK(c, hipStreamSynchronize(w->st));
-		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
-	}
+	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, rows_needed)) return r;
 	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, q_boff, tally, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
@@ -656,34 +664,76 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, w->b_keys2.ensure((size_t) ns + 1));
 	HIPCHK(c, w->b_vals2.ensure((size_t) ns + 1));
 	HIPCHK(c, launch_query_probe(Q, S, six, c->o, w->b_kref.p, nkeys, w->b_cnt.p, w->b_off.p, w->b_keys.p, w->b_vals.p, w->st));
-	int key_bits = 1;
-	while (((uint64_t) 1 << key_bits) < (uint64_t) (2 * c->Rd.n + 2)) key_bits++;
+	// two stable radix passes: by lo = (strand, posQ, posS), then by hi = (subject, query)
+	int hi_bits = 23;
+	while (((uint64_t) 1 << (hi_bits - 22)) < (uint64_t) c->Rd.n + 1) hi_bits++;
 	size_t tmp_bytes = 0;
-	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, key_bits, nullptr, tmp_bytes, w->st));
-	HIPCHK(c, w->b_sort_tmp.ensure(tmp_bytes + 1));
+	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, 29, nullptr, tmp_bytes, w->st));
+	size_t tmp2 = 0;
+	HIPCHK(c, sort_pairs_u64(w->b_vals2.p, w->b_vals.p, w->b_keys2.p, w->b_keys.p, ns, hi_bits, nullptr, tmp2, w->st));
+	HIPCHK(c, w->b_sort_tmp.ensure(std::max(tmp_bytes, tmp2) + 1));
 	tmp_bytes = w->b_sort_tmp.n;
-	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, key_bits, w->b_sort_tmp.p, tmp_bytes, w->st));
-	HIPCHK(c, w->b_newsub.ensure((size_t) ns + 1));
-	HIPCHK(c, w->b_subx.ensure((size_t) ns + 1));
-	HIPCHK(c, launch_seg_mark(w->b_keys2.p, ns, w->b_newsub.p, w->st));
-	uint64_t ntouched = 0;
-	if (int r = wscan(c, w, w->b_newsub.p, w->b_subx.p, ns, ntouched)) return r;
+	HIPCHK(c, sort_pairs_u64(w->b_keys.p, w->b_keys2.p, w->b_vals.p, w->b_vals2.p, ns, 29, w->b_sort_tmp.p, tmp_bytes, w->st));
+	tmp_bytes = w->b_sort_tmp.n;
+	HIPCHK(c, sort_pairs_u64(w->b_vals2.p, w->b_vals.p, w->b_keys2.p, w->b_keys.p, ns, hi_bits, w->b_sort_tmp.p, tmp_bytes, w->st));
+	// now b_vals = hi, b_keys = lo, sorted by (hi, lo): groups and pairs are contiguous runs
+	HIPCHK(c, w->b_gflag.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_pflag.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_gx.ensure((size_t) ns + 1));
+	HIPCHK(c, w->b_px.ensure((size_t) ns + 1));
+	HIPCHK(c, launch_serve_mark(w->b_vals.p, w->b_keys.p, ns, w->b_gflag.p, w->b_pflag.p, w->st));
+	uint64_t ngroups = 0, npairs = 0;
+	if (int r = wscan(c, w, w->b_gflag.p, w->b_gx.p, ns, ngroups)) return r;
+	if (int r = wscan(c, w, w->b_pflag.p, w->b_px.p, ns, npairs)) return r;
+	HIPCHK(c, w->b_glist.ensure((size_t) ngroups + 1));
+	HIPCHK(c, w->b_plist.ensure((size_t) npairs + 1));
+	HIPCHK(c, w->seeds.ensure((size_t) ns + 1));
+	HIPCHK(c, launch_serve_lists(w->b_vals.p, w->b_keys.p, ns, w->b_gflag.p, w->b_gx.p, w->b_pflag.p, w->b_px.p,
+			w->b_glist.p, w->b_plist.p, w->seeds.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
-	const uint32_t nt = (uint32_t) ntouched;
-	HIPCHK(c, w->b_sub_ids.ensure((size_t) nt + 1));
-	HIPCHK(c, w->seg_off.ensure((size_t) 2 * nt + 2));
-	HIPCHK(c, w->seg_n.ensure((size_t) 2 * nt + 2));
-	HIPCHK(c, w->b_seg_end.ensure((size_t) 2 * nt + 2));
-	HIPCHK(c, hipMemsetAsync(w->seg_off.p, 0, (size_t) 8 * nt, w->st));
-	HIPCHK(c, hipMemsetAsync(w->b_seg_end.p, 0, (size_t) 8 * nt, w->st));
-	HIPCHK(c, launch_seg_fill(w->b_keys2.p, ns, w->b_subx.p, w->b_sub_ids.p, w->seg_off.p, w->b_seg_end.p, w->st));
-	HIPCHK(c, launch_seg_finish(2 * nt, w->seg_off.p, w->b_seg_end.p, w->seg_n.p, w->st));
+	ServeDev v;
+	v.hi = w->b_vals.p, v.lo = w->b_keys.p, v.glist = w->b_glist.p, v.gx = w->b_gx.p, v.plist = w->b_plist.p;
+	v.nseeds = ns, v.ngroups = (uint32_t) ngroups, v.npairs = (uint32_t) npairs;
 	ChunkDev cd;
 	std::memset(&cd, 0, sizeof cd);
-	cd.s_begin = 0, cd.nsub = (int32_t) nt, cd.presorted = 1, cd.sub_ids = w->b_sub_ids.p;
-	cd.seg_n = w->seg_n.p, cd.seg_off = w->seg_off.p, cd.seeds = w->b_vals2.p;
-	if (nt > 0)
-		if (int r = run_from_seeds(c, w, S, Q, six, cd, nseeds, max_len + 1, w->bq.boff.p, w->b_tally.p, nullptr, false)) return r;
+	HIPCHK(c, w->cid.ensure((size_t) ns + 1));
+	HIPCHK(c, w->cfirst.ensure((size_t) ns + 1));
+	HIPCHK(c, w->clast.ensure((size_t) ns + 1));
+	HIPCHK(c, w->clen.ensure((size_t) ns + 1));
+	HIPCHK(c, w->g_nch.ensure((size_t) ns + 1));
+	HIPCHK(c, w->g_best.ensure((size_t) ns + 1));
+	HIPCHK(c, w->g_tasks.ensure((size_t) ngroups + 1));
+	HIPCHK(c, w->g_task_off.ensure((size_t) ngroups + 1));
+	HIPCHK(c, w->g_words.ensure((size_t) ngroups + 1));
+	HIPCHK(c, w->g_word_off.ensure((size_t) ngroups + 1));
+	cd.seeds = w->seeds.p, cd.cid = w->cid.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
+	cd.g_nch = w->g_nch.p, cd.g_best = w->g_best.p, cd.g_tasks = w->g_tasks.p, cd.g_task_off = w->g_task_off.p;
+	cd.g_words = w->g_words.p, cd.g_word_off = w->g_word_off.p;
+	unsigned long long *stats = w->dstats.p;
+	HIPCHK(c, launch_serve_a1(S, Q, c->o, cd, v, stats, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[1].a, w->st));
+	uint64_t ntasks = 0, nwords = 0;
+	if (int r = wscan(c, w, cd.g_tasks, cd.g_task_off, (int64_t) ngroups, ntasks)) return r;
+	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ngroups, nwords)) return r;
+	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
+		return fail(c, HERO_ERR_LIMIT, "batch yields %llu DP tasks", (unsigned long long) ntasks);
+	HIPCHK(c, w->tasks.ensure((size_t) ntasks + 1));
+	HIPCHK(c, w->words.ensure((size_t) nwords + 1));
+	HIPCHK(c, w->nsteps.ensure((size_t) ntasks + 1));
+	HIPCHK(c, w->sums.ensure((size_t) ntasks + 1));
+	cd.tasks = w->tasks.p, cd.words = w->words.p, cd.nsteps = w->nsteps.p, cd.sums = w->sums.p;
+	HIPCHK(c, launch_serve_a2(S, Q, c->o, cd, v, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
+	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, max_len + 1)) return r;
+	HIPCHK(c, launch_serve_b(S, Q, c->o, cd, v, w->bq.boff.p, w->b_tally.p, stats, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	w->acc->ms_seed += elapsed(w, 0, 1);
+	w->acc->ms_seed_probe += elapsed(w, 0, 8);
+	w->acc->ms_plan += elapsed(w, 1, 7);
+	w->acc->ms_dp += elapsed(w, 7, 3);
+	w->acc->ms_score_vote += elapsed(w, 3, 4);
+	w->acc->dp_tasks += ntasks;
 	// consensus of the batch, corrected bases back to the caller's layout
 	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
 	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p + kStReadsChanged, w->st));
@@ -697,7 +747,6 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 		std::memcpy(corrected + off[i], good[(size_t) i] ? h.data() + b : seq + off[i], (size_t) L);
 		b += (uint64_t) L;
 	}
-	if (nt == 0) w->acc->ms_seed += elapsed(w, 0, 8);
 	w->acc->ms_consensus += elapsed(w, 5, 6);
 	w->acc->queries += (uint64_t) n;
 	return HERO_OK;
@@ -1392,7 +1441,7 @@ extern "C" int hero_server_start(hero_ctx *c, const hero_server_options *opt, he
 	sv->c = c;
 	if (opt) sv->opt = *opt;
 	if (sv->opt.workers <= 0) sv->opt.workers = 2;
-	if (sv->opt.max_step_reads <= 0) sv->opt.max_step_reads = 65536;
+	if (sv->opt.max_step_reads <= 0) sv->opt.max_step_reads = 16384;
 	if (sv->opt.hbm_budget <= 0) {
 		size_t fr = 0, tot = 0;
 		(void) hipMemGetInfo(&fr, &tot);

@@ -43,8 +43,6 @@ struct RecHead {
 struct ChunkDev {
 	int64_t s_begin;
 	int32_t nsub;
-	int32_t presorted;         // seeds were written per (subject, strand) segment by the caller
-	const uint32_t *sub_ids;   // subject read of each chunk slot; null = s_begin + slot
 	int32_t win_stride;        // windows per (subject, strand) slot of win_info
 	uint64_t *win_info;        // per window: bucket (start << 32 | count), 0 = no bucket; null = probe again
 	uint32_t *lane_n;          // per (subject, strand) slot and lane: seeds of that lane's windows
@@ -91,11 +89,6 @@ struct DpLaunch {
 	unsigned long long *stats;
 };
 
-// Subject read of chunk slot sl.
-__host__ __device__ inline int64_t chunk_subject(const ChunkDev &c, int sl) {
-	return c.sub_ids ? (int64_t) c.sub_ids[sl] : c.s_begin + sl;
-}
-
 // launchers (kernels.hip)
 hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool all_positions, uint64_t *occ_hi,
 		uint64_t *occ_lo, uint64_t *occ_entry, hipStream_t st);
@@ -138,15 +131,30 @@ hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_bo
 
 // Serving orientation: the subjects' k-mers at every position are indexed; each query key (kref =
 // query << 14 | position) and its reverse complement probe them. Count pass (keys_out == nullptr)
-// writes per-key seed counts; emit pass writes (subject << 1 | strand, seed) at off[key].
+// writes per-key seed counts; emit pass writes, at off[key], keys_out = strand << 28 | posQ << 14 |
+// posS and vals_out = subject << 22 | query.
 hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &six, const Opts &o, const uint64_t *kref,
 		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st);
-// Segments of seeds sorted by (subject << 1 | strand): subject slots, per-slot strand segments.
-hipError_t launch_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub, hipStream_t st);
-hipError_t launch_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
-		uint32_t *seg_off, uint32_t *seg_end, hipStream_t st);
-hipError_t launch_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n,
+// A serving batch's sorted seeds (hi = subject << 22 | query, lo = strand << 28 | posQ << 14 |
+// posS), its groups (glist: first seed of each (subject, query, strand) run; gx: exclusive scan of
+// group starts per seed) and pairs (plist: first seed of each (subject, query) run).
+struct ServeDev {
+	const uint64_t *hi, *lo;
+	const uint32_t *glist, *gx, *plist;
+	uint32_t nseeds, ngroups, npairs;
+};
+hipError_t launch_serve_mark(const uint64_t *hi, const uint64_t *lo, uint32_t n, uint32_t *gflag, uint32_t *pflag,
 		hipStream_t st);
+hipError_t launch_serve_lists(const uint64_t *hi, const uint64_t *lo, uint32_t n, const uint32_t *gflag,
+		const uint32_t *gx, const uint32_t *pflag, const uint32_t *px, uint32_t *glist, uint32_t *plist, uint64_t *seeds,
+		hipStream_t st);
+hipError_t launch_serve_a1(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		unsigned long long *stats, hipStream_t st);
+hipError_t launch_serve_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		hipStream_t st);
+hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st);
+
 // Radix sort of (key, value) u64 pairs on the low key_bits bits (serve_sort.hip).
 hipError_t sort_pairs_u64(const uint64_t *keys_in, uint64_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,
 		uint32_t n, int key_bits, void *tmp, size_t &tmp_bytes, hipStream_t st);

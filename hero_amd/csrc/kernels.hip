@@ -270,7 +270,7 @@ __device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &
 __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
 	const int sl = blockIdx.x;
-	const int64_t s = chunk_subject(c, sl);
+	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) {
 		if (lane == 0) c.seg_n[2 * sl] = 0, c.seg_n[2 * sl + 1] = 0;
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 		unsigned long long *stats) {
 	__shared__ uint64_t lds[kLdsSeeds];
 	const int sl = blockIdx.x;
-	const int64_t s = chunk_subject(c, sl);
+	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
@@ -382,10 +382,7 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 		if (n == 0) continue;
 		const bool in_lds = n <= (uint32_t) kLdsSeeds;
 		uint64_t *dst = in_lds ? lds : c.seeds + off;
-		if (c.presorted) {
-			if (in_lds)
-				for (uint32_t t = lane; t < n; t += 64) lds[t] = c.seeds[off + t];
-		} else if (c.win_info) {
+		if (c.win_info) {
 			const int slot = 2 * sl + strand;
 			uint32_t pos = wave_excl_scan(c.lane_n[(int64_t) slot * 64 + lane]);
 			const uint64_t *wi = c.win_info + (int64_t) slot * c.win_stride;
@@ -459,7 +456,7 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 // Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
 __global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
 	const int sl = blockIdx.x;
-	const int64_t s = chunk_subject(c, sl);
+	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
@@ -605,8 +602,9 @@ __global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevInde
 			if (S.uid[sr] == quid) continue;
 			if (keys_out) {
 				const int p = (int) (d >> 40);
-				keys_out[base + n] = (sr << 1) | (uint64_t) strand;
-				vals_out[base + n] = seed_pack(q, pos, strand ? S.len[sr] - p - k : p);
+				const int ps = strand ? S.len[sr] - p - k : p;
+				keys_out[base + n] = ((uint64_t) strand << 28) | ((uint64_t) pos << 14) | (uint64_t) ps;
+				vals_out[base + n] = (sr << 22) | (uint64_t) q;
 			}
 			n++;
 		}
@@ -617,45 +615,6 @@ hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &
 		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st) {
 	if (nkeys == 0) return hipSuccess;
 	k_query_probe<<<(nkeys + 255) / 256, 256, 0, st>>>(Q, S, six, o, kref, nkeys, cnt, off, keys_out, vals_out);
-	return hipGetLastError();
-}
-
-__global__ void k_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub) {
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if (i >= n) return;
-	newsub[i] = (i == 0 || (keys[i] >> 1) != (keys[i - 1] >> 1)) ? 1u : 0u;
-}
-hipError_t launch_seg_mark(const uint64_t *keys, uint32_t n, uint32_t *newsub, hipStream_t st) {
-	if (n == 0) return hipSuccess;
-	k_seg_mark<<<(n + 255) / 256, 256, 0, st>>>(keys, n, newsub);
-	return hipGetLastError();
-}
-__global__ void k_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
-		uint32_t *seg_off, uint32_t *seg_end) {
-	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-	if (i >= n) return;
-	const uint64_t key = keys[i];
-	const bool ns = i == 0 || (key >> 1) != (keys[i - 1] >> 1);
-	const uint32_t sl = sub_excl[i] + (ns ? 1u : 0u) - 1u;
-	const uint32_t seg = 2 * sl + (uint32_t) (key & 1);
-	if (ns) sub_ids[sl] = (uint32_t) (key >> 1);
-	if (i == 0 || key != keys[i - 1]) seg_off[seg] = i;
-	if (i == n - 1 || key != keys[i + 1]) seg_end[seg] = i + 1;
-}
-hipError_t launch_seg_fill(const uint64_t *keys, uint32_t n, const uint32_t *sub_excl, uint32_t *sub_ids,
-		uint32_t *seg_off, uint32_t *seg_end, hipStream_t st) {
-	if (n == 0) return hipSuccess;
-	k_seg_fill<<<(n + 255) / 256, 256, 0, st>>>(keys, n, sub_excl, sub_ids, seg_off, seg_end);
-	return hipGetLastError();
-}
-__global__ void k_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n) {
-	const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-	if (j < nseg) seg_n[j] = seg_end[j] - seg_off[j];
-}
-hipError_t launch_seg_finish(uint32_t nseg, const uint32_t *seg_off, const uint32_t *seg_end, uint32_t *seg_n,
-		hipStream_t st) {
-	if (nseg == 0) return hipSuccess;
-	k_seg_finish<<<(nseg + 255) / 256, 256, 0, st>>>(nseg, seg_off, seg_end, seg_n);
 	return hipGetLastError();
 }
 
@@ -683,7 +642,7 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	__shared__ uint32_t rev_only[kPassBGroups];
 	__shared__ uint8_t used1[kPassBGroups];
 	const int sl = blockIdx.x;
-	const int64_t s = chunk_subject(c, sl);
+	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
@@ -763,6 +722,146 @@ hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const 
 		uint32_t *tally, unsigned long long *stats, hipStream_t st) {
 	if (c.nsub == 0) return hipSuccess;
 	k_pass_b<<<c.nsub, 64, 0, st>>>(S, Q, o, c, q_boff, tally, stats);
+	return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Serving passes. A batch's seeds are sorted by (subject, query, strand, posQ, posS) as
+// hi = subject << 22 | query and lo = strand << 28 | posQ << 14 | posS, so each (subject, query,
+// strand) group and each (subject, query) pair is one contiguous run. Pass A chains every group
+// and plans its DP (one lane per group); pass B scores both strands of every pair, keeps the
+// first best record and casts its votes (one lane per pair). Same functions as the offline
+// passes, with lanes over groups instead of wavefronts over subjects.
+// ---------------------------------------------------------------------------------------------
+constexpr uint64_t kServeQMask = (1ull << 22) - 1;
+
+__global__ void k_serve_mark(const uint64_t *hi, const uint64_t *lo, uint32_t n, uint32_t *gflag, uint32_t *pflag) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= n) return;
+	const bool p = i == 0 || hi[i] != hi[i - 1];
+	gflag[i] = (p || (lo[i] >> 28) != (lo[i - 1] >> 28)) ? 1u : 0u;
+	pflag[i] = p ? 1u : 0u;
+}
+__global__ void k_serve_lists(const uint64_t *hi, const uint64_t *lo, uint32_t n, const uint32_t *gflag,
+		const uint32_t *gx, const uint32_t *pflag, const uint32_t *px, uint32_t *glist, uint32_t *plist, uint64_t *seeds) {
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i >= n) return;
+	if (gflag[i]) glist[gx[i]] = i;
+	if (pflag[i]) plist[px[i]] = i;
+	seeds[i] = ((hi[i] & kServeQMask) << 28) | (lo[i] & ((1ull << 28) - 1));
+}
+hipError_t launch_serve_mark(const uint64_t *hi, const uint64_t *lo, uint32_t n, uint32_t *gflag, uint32_t *pflag,
+		hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	k_serve_mark<<<(n + 255) / 256, 256, 0, st>>>(hi, lo, n, gflag, pflag);
+	return hipGetLastError();
+}
+hipError_t launch_serve_lists(const uint64_t *hi, const uint64_t *lo, uint32_t n, const uint32_t *gflag,
+		const uint32_t *gx, const uint32_t *pflag, const uint32_t *px, uint32_t *glist, uint32_t *plist, uint64_t *seeds,
+		hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	k_serve_lists<<<(n + 255) / 256, 256, 0, st>>>(hi, lo, n, gflag, gx, pflag, px, glist, plist, seeds);
+	return hipGetLastError();
+}
+
+__device__ __forceinline__ PairReads serve_pair(const Packed &S, const Packed &Q, const Opts &o, const ServeDev &v,
+		uint32_t base, int64_t &s, int &strand) {
+	s = (int64_t) (v.hi[base] >> 22);
+	strand = (int) (v.lo[base] >> 28);
+	return make_pair(S, Q, o, s, S.len[s], (int64_t) (v.hi[base] & kServeQMask), strand);
+}
+
+__global__ __launch_bounds__(64) void k_serve_a1(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v,
+		unsigned long long *stats) {
+	const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+	if (g < v.ngroups) {
+		const uint32_t base = v.glist[g];
+		const uint32_t end = g + 1 < v.ngroups ? v.glist[g + 1] : v.nseeds;
+		Group gr = make_group(c, base, (int) (end - base));
+		chain(o, gr);
+		c.g_nch[base] = gr.nch;
+		c.g_best[base] = gr.best;
+		int64_t s;
+		int strand;
+		const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
+		int nt, nw;
+		plan_count(o, gr, pr, nt, nw);
+		c.g_tasks[g] = (uint32_t) nt;
+		c.g_words[g] = (uint32_t) nw;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[kStSeeds], (unsigned long long) v.nseeds);
+}
+__global__ __launch_bounds__(64) void k_serve_a2(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v) {
+	const uint32_t g = blockIdx.x * 64u + threadIdx.x;
+	if (g >= v.ngroups || c.g_tasks[g] == 0) return;
+	const uint32_t base = v.glist[g];
+	const uint32_t end = g + 1 < v.ngroups ? v.glist[g + 1] : v.nseeds;
+	const Group gr = chained_group(c, base, (int) (end - base));
+	int64_t s;
+	int strand;
+	const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
+	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g]);
+}
+__global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats) {
+	const uint32_t p = blockIdx.x * 64u + threadIdx.x;
+	PassBCounters k;
+	if (p < v.npairs) {
+		const uint32_t base = v.plist[p];
+		const uint32_t end = p + 1 < v.npairs ? v.plist[p + 1] : v.nseeds;
+		const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
+		Best best;
+		best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
+		uint32_t gstart[2] = {0, 0}, gsize[2] = {0, 0};
+		// the pair's groups: forward (strand 0) first, then reverse
+		uint32_t i = base;
+		while (i < end) {
+			const uint32_t gi = v.gx[i]; // group index of the group starting at seed i
+			const uint32_t ge = gi + 1 < v.ngroups ? v.glist[gi + 1] : v.nseeds;
+			int64_t s;
+			int strand;
+			const PairReads pr = serve_pair(S, Q, o, v, i, s, strand);
+			const Group gr = chained_group(c, i, (int) (ge - i));
+			score_group(o, gr, pr, R, c.g_task_off[gi], strand, best, k.cells, k.calls, k.built);
+			gstart[strand] = i, gsize[strand] = ge - i;
+			i = ge;
+		}
+		if (best.have) {
+			k.kept++;
+			const uint32_t b0 = gstart[best.strand];
+			int64_t s;
+			int strand;
+			const PairReads pr = serve_pair(S, Q, o, v, b0, s, strand);
+			const Group gr = chained_group(c, b0, (int) gsize[best.strand]);
+			const int64_t q = (int64_t) (v.hi[b0] & kServeQMask);
+			VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
+			vote_record(o, gr, pr, R, best.chain, best.task0, sink);
+		}
+	}
+	uint64_t cells = wave_sum64(k.cells), calls = wave_sum64(k.calls), built = wave_sum64(k.built), kept = wave_sum64(k.kept);
+	if (lane_id() == 0) {
+		atomicAdd(&stats[kStCellsExec], (unsigned long long) cells);
+		atomicAdd(&stats[kStCallsExec], (unsigned long long) calls);
+		atomicAdd(&stats[kStBuilt], (unsigned long long) built);
+		atomicAdd(&stats[kStKept], (unsigned long long) kept);
+	}
+}
+hipError_t launch_serve_a1(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		unsigned long long *stats, hipStream_t st) {
+	if (v.ngroups == 0) return hipSuccess;
+	k_serve_a1<<<(v.ngroups + 63) / 64, 64, 0, st>>>(S, Q, o, c, v, stats);
+	return hipGetLastError();
+}
+hipError_t launch_serve_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		hipStream_t st) {
+	if (v.ngroups == 0) return hipSuccess;
+	k_serve_a2<<<(v.ngroups + 63) / 64, 64, 0, st>>>(S, Q, o, c, v);
+	return hipGetLastError();
+}
+hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+	if (v.npairs == 0) return hipSuccess;
+	k_serve_b<<<(v.npairs + 63) / 64, 64, 0, st>>>(S, Q, o, c, v, q_boff, tally, stats);
 	return hipGetLastError();
 }
 
