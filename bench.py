@@ -252,6 +252,20 @@ def prepare_serving():
     return rs, pool_idx, ctx, t2 - t1
 
 
+def copy_calibration(nbytes=1 << 30):
+    """A plain device copy of nbytes (reads nbytes, writes nbytes): under a counter profiler its
+    FETCH_SIZE / WRITE_SIZE give the scale between counter units and bytes."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {nbytes} bytes copied")
+    del x, y
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,12 +275,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-form and mixed-length runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) runs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     rs = synth.generate(**C2_READS)
     log(f"workload: {rs.n} reads x 150 bp generated in {time.perf_counter() - t0:.1f} s")

@@ -151,6 +151,8 @@ struct Worker {
 	DevBuf<uint32_t> g_recs, g_rec_off, rec_opn, rec_opoff, rec_ops;
 	DevBuf<RecHead> recs;
 	DevBuf<int32_t> rec_pos;
+	DevBuf<uint32_t> sub_pairs, sub_pair_off;
+	DevBuf<VoteItem> votes;
 	DevBuf<StepSummary> sums;
 	DevBuf<unsigned long long> dstats, dtotal;
 	DevBuf<uint32_t> counter;
@@ -179,6 +181,7 @@ struct Worker {
 		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
 		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
+		sub_pairs.release(), sub_pair_off.release(), votes.release();
 		bq.release(), bq_good.release(), b_cons.release(), b_tally.release(), b_cnt.release(), b_off.release();
 		b_gflag.release(), b_pflag.release(), b_gx.release(), b_px.release(), b_glist.release(), b_plist.release();
 		b_kref.release();
@@ -458,6 +461,12 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 		HIPCHK(c, w->g_rec_off.ensure(ns));
 		cd.g_recs = w->g_recs.p;
 	}
+	const bool vote_mode = c->o.call_consensus && !rec_mode;
+	if (vote_mode) {
+		HIPCHK(c, w->sub_pairs.ensure((size_t) nsub + 1));
+		HIPCHK(c, w->sub_pair_off.ensure((size_t) nsub + 1));
+		cd.sub_pairs = w->sub_pairs.p;
+	}
 	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[1].a, w->st));
 	uint64_t ntasks = 0, nwords = 0;
@@ -465,6 +474,14 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
+	uint64_t nvotes = 0;
+	if (vote_mode) {
+		if (int r = wscan(c, w, cd.sub_pairs, w->sub_pair_off.p, nsub, nvotes)) return r;
+		if (nvotes >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu pairs; lower chunk_subjects", nsub, (unsigned long long) nvotes);
+		HIPCHK(c, w->votes.ensure((size_t) nvotes + 1));
+		HIPCHK(c, hipMemsetAsync(w->votes.p, 0xFF, (size_t) nvotes * sizeof(VoteItem), w->st));
+		cd.votes = w->votes.p, cd.sub_pair_off = w->sub_pair_off.p;
+	}
 	uint64_t nslots = 0;
 	if (rec_mode) {
 		if (int r = wscan(c, w, cd.g_recs, w->g_rec_off.p, (int64_t) ns, nslots)) return r;
@@ -481,7 +498,8 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
 	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, rows_needed)) return r;
-	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, q_boff, tally, stats, w->st));
+	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, stats, w->st));
+	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
 	w->acc->ms_seed += elapsed(w, 0, 1);

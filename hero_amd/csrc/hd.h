@@ -233,13 +233,20 @@ struct SummaryBuilder {
 };
 
 // In-band cells of one DP call: sum over rows of the row width, minus the origin.
+// Cells of one banded DP call: sum over rows i = 0..lenS of (min(lenQ, i+b) - max(0, i-b) + 1),
+// minus the origin cell. Closed form of that sum (each part summed separately, so the identity holds
+// for any arguments).
 HD uint64_t band_cells(int lenS, int lenQ, int b) {
-	uint64_t n = 0;
-	for (int i = 0; i <= lenS; i++) {
-		int lo = i - b > 0 ? i - b : 0, hi = lenQ < i + b ? lenQ : i + b;
-		n += (uint64_t) (hi - lo + 1);
-	}
-	return n - 1;
+	const int64_t S = lenS, Lq = lenQ, B = b;
+	// sum of min(lenQ, i + b): i + b for i <= lenQ - b, lenQ after
+	const int64_t A = (Lq - B < S) ? Lq - B : S;
+	int64_t hi_sum;
+	if (A < 0) hi_sum = (S + 1) * Lq;
+	else hi_sum = (A + 1) * B + A * (A + 1) / 2 + (S - A) * Lq;
+	// sum of max(0, i - b): i - b from i = max(0, b + 1)
+	const int64_t j0 = B + 1 > 0 ? B + 1 : 0;
+	const int64_t lo_sum = j0 > S ? 0 : (S - j0 + 1) * ((j0 - B) + (S - B)) / 2;
+	return (uint64_t) (hi_sum - lo_sum + (S + 1) - 1);
 }
 
 } // namespace hero

@@ -38,6 +38,16 @@ struct RecHead {
 	double rate_sum, len; // isGoodAlignment() rates, for SubjectRead::addAlignment
 };
 
+// The kept record of one (subject, query) pair, for its votes.
+struct VoteItem {
+	uint32_t q;     // innerId of the query read; kNoRecord for an unused slot
+	uint32_t gbase; // first seed of the record's query group
+	int32_t gsize;
+	uint32_t task0; // first DP task of the record
+	int32_t sub;    // subject index within the chunk
+	int16_t strand, chain;
+};
+
 // Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
 // task and edit-word counts and their exclusive scans.
 struct ChunkDev {
@@ -54,6 +64,8 @@ struct ChunkDev {
 	uint32_t *g_tasks, *g_task_off, *g_words, *g_word_off;
 	uint32_t *g_recs, *g_rec_off; // record mode: chains of a kept group (record slots) and their scan
 	RecHead *recs;                // record mode: one slot per chain of every kept group
+	uint32_t *sub_pairs, *sub_pair_off; // vote mode: kept groups of both strands per subject, scan
+	VoteItem *votes;                    // vote mode: the kept record of each pair (q = kNoRecord: none)
 	DpTask *tasks;
 	uint32_t *words, *nsteps;
 	StepSummary *sums;
@@ -106,8 +118,12 @@ hipError_t launch_seed_count(const Packed &S, const Packed &Q, const DevIndex &i
 hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
 		unsigned long long *stats, hipStream_t st);
 hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, hipStream_t st);
-hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
-		uint32_t *tally, unsigned long long *stats, hipStream_t st);
+// Pass B scores every (subject, query) pair and writes the winner of each kept pair to c.votes
+// (at c.sub_pair_off[subject] + rank); launch_votes then casts their votes.
+hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, unsigned long long *stats,
+		hipStream_t st);
+hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st);
 // Record mode: op count (ops == nullptr) or ops of every accepted record slot; each op is
 // (count << 8) | type, type 'M' / 'I' / 'D' or the query base letter of a mismatch run.
 hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,

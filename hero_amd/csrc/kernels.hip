@@ -373,10 +373,14 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 	const int sl = blockIdx.x;
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
-	if (!S.good[s]) return;
+	if (!S.good[s]) {
+		if (c.sub_pairs && lane == 0) c.sub_pairs[sl] = 0;
+		return;
+	}
 	const int Ls = S.len[s];
 	const int64_t uid = S.uid[s];
 	const int nwin = Ls - o.k + 1;
+	uint32_t kept_groups = 0; // both strands: bounds the subject's (subject, query) pairs
 	for (int strand = 0; strand < 2; strand++) {
 		const uint32_t n = c.seg_n[2 * sl + strand], off = c.seg_off[2 * sl + strand];
 		if (n == 0) continue;
@@ -443,7 +447,9 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 				if (c.g_recs) c.g_recs[off + t] = nrec;
 			}
 		}
+		kept_groups += o.max_candidates > 0 ? min(groups_before, (uint32_t) o.max_candidates) : groups_before;
 	}
+	if (c.sub_pairs && lane == 0) c.sub_pairs[sl] = kept_groups;
 	if (lane == 0) atomicAdd(&stats[kStSeeds], (unsigned long long) (c.seg_n[2 * sl] + c.seg_n[2 * sl + 1]));
 }
 hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
@@ -504,9 +510,10 @@ struct PassBCounters {
 // (gr of segment 1, or -1). Records are built from the DP results in the reference's order
 // (forward chains, then reverse chains), the first best is kept (SubjectRead::addAlignment), and
 // that record casts its votes.
+template <bool RECORDS>
 __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c,
 		const DpResults &R, int64_t s, int Ls, uint32_t off0, uint32_t n0, uint32_t off1, uint32_t n1, int gf, int gr,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, PassBCounters &k) {
+		uint32_t vbase, uint32_t *vcount, PassBCounters &k) {
 	const int64_t q = seed_qid(gf >= 0 ? c.seeds[off0 + gf] : c.seeds[off1 + gr]);
 	Best best;
 	best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
@@ -522,7 +529,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 		else gbase1 = base, gsize1 = gn;
 		const Group g = chained_group(c, base, gn);
 		const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-		if (c.recs) {
+		if constexpr (RECORDS) {
 			RecHead *slots = c.recs + c.g_rec_off[base];
 			const int sub = (int) (s - c.s_begin); // record mode runs on contiguous chunks only
 			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built,
@@ -538,12 +545,35 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 	}
 	if (!best.have) return;
 	k.kept++;
-	if (!o.call_consensus) return;
+	if (RECORDS || !o.call_consensus) return;
+	// the winner's votes are cast by k_votes
 	const bool wr = best.strand == 1;
-	const Group g = chained_group(c, wr ? gbase1 : gbase0, wr ? gsize1 : gsize0);
-	const PairReads pr = make_pair(S, Q, o, s, Ls, q, best.strand);
-	VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
-	vote_record(o, g, pr, R, best.chain, best.task0, sink);
+	VoteItem v;
+	v.q = (uint32_t) q, v.gbase = wr ? gbase1 : gbase0, v.gsize = wr ? gsize1 : gsize0, v.task0 = best.task0;
+	v.sub = (int32_t) (s - c.s_begin), v.strand = (int16_t) best.strand, v.chain = (int16_t) best.chain;
+	c.votes[vbase + atomicAdd(vcount, 1u)] = v;
+}
+
+// Votes of the kept records (SubjectRead::updateQueryCoverage, src/SubjectRead.cpp:208-330): one
+// lane per record; the record is replayed from its chain and DP results.
+__global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkDev c, uint32_t nvotes,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats) {
+	const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+	if (i >= nvotes) return;
+	const VoteItem v = c.votes[i];
+	if (v.q == kNoRecord) return;
+	const int64_t s = c.s_begin + v.sub;
+	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) v.q, v.strand);
+	const Group g = chained_group(c, v.gbase, v.gsize);
+	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
+	VoteSink sink{tally + 4 * q_boff[v.q], pr, &stats[kStOob]};
+	vote_record(o, g, pr, R, v.chain, v.task0, sink);
+}
+hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
+		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+	if (nvotes == 0) return hipSuccess;
+	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, stats);
+	return hipGetLastError();
 }
 
 // Record mode: one lane per record slot; count pass (ops == nullptr) then emit pass.
@@ -635,8 +665,9 @@ __device__ __forceinline__ uint32_t collect_groups(const uint8_t *gflag, uint32_
 // Pass B: one wavefront per subject. The kept groups of both strands are listed in LDS and paired
 // by query (binary search over the reverse list); lanes then take the pairs densely.
 constexpr uint32_t kPassBGroups = 256;
-__global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, const uint64_t *q_boff,
-		uint32_t *tally, unsigned long long *stats) {
+template <bool RECORDS>
+__global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, unsigned long long *stats) {
+	__shared__ uint32_t vcount;
 	__shared__ uint32_t list0[kPassBGroups], list1[kPassBGroups];
 	__shared__ int32_t match0[kPassBGroups];
 	__shared__ uint32_t rev_only[kPassBGroups];
@@ -650,6 +681,8 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	const uint32_t n1 = c.seg_n[2 * sl + 1], off1 = c.seg_off[2 * sl + 1];
 	DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	PassBCounters k;
+	const uint32_t vbase = c.sub_pair_off ? c.sub_pair_off[sl] : 0u;
+	if (lane == 0) vcount = 0;
 	const uint32_t c0 = collect_groups(c.gflag + off0, n0, list0, kPassBGroups);
 	const uint32_t c1 = collect_groups(c.gflag + off1, n1, list1, kPassBGroups);
 	if (c0 <= kPassBGroups && c1 <= kPassBGroups) {
@@ -681,7 +714,7 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 			int gf = -1, gr = -1;
 			if (p < c0) gf = (int) list0[p], gr = match0[p] >= 0 ? (int) list1[match0[p]] : -1;
 			else gr = (int) list1[rev_only[p - c0]];
-			pass_b_pair(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, q_boff, tally, stats, k);
+			pass_b_pair<RECORDS>(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, vbase, &vcount, k);
 		}
 	} else {
 		// more kept groups than the LDS lists hold: walk the segments directly
@@ -700,7 +733,7 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 					if (f >= 0 && c.gflag[off0 + f] == 3) continue; // handled with the forward group
 					gr = (int) t;
 				}
-				pass_b_pair(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, q_boff, tally, stats, k);
+				pass_b_pair<RECORDS>(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, vbase, &vcount, k);
 			}
 		}
 	}
@@ -718,10 +751,11 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 
 	}
 }
-hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const uint64_t *q_boff,
-		uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, unsigned long long *stats,
+		hipStream_t st) {
 	if (c.nsub == 0) return hipSuccess;
-	k_pass_b<<<c.nsub, 64, 0, st>>>(S, Q, o, c, q_boff, tally, stats);
+	if (c.recs) k_pass_b<true><<<c.nsub, 64, 0, st>>>(S, Q, o, c, stats);
+	else k_pass_b<false><<<c.nsub, 64, 0, st>>>(S, Q, o, c, stats);
 	return hipGetLastError();
 }
 
@@ -869,14 +903,24 @@ hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const
 // Banded DP (BandAligner::bandedAlign / bandedAlignAffineGap)
 // ---------------------------------------------------------------------------------------------
 // Sequential reader of one side of a task in its frame (reverse complement when rc).
+// A read segment as a stream of 2-bit codes (forward, or reverse complement when rc). The current
+// 32-base word stays in a register: one load per 32 bases instead of one per base.
 struct PackedStream {
 	const uint64_t *w;
 	int L, start, len, i;
 	bool rc;
+	int word_idx = -1;
+	uint64_t word = 0;
 	__device__ __forceinline__ int next() {
 		if (i >= len) return kCodeNone;
 		const int p = start + i++;
-		return rc ? (base_at(w, L - 1 - p) ^ 2) : base_at(w, p);
+		const int q = rc ? L - 1 - p : p;
+		if ((q >> 5) != word_idx) {
+			word_idx = q >> 5;
+			word = w[word_idx];
+		}
+		const int c = (int) ((word >> (62 - 2 * (q & 31))) & 3ull);
+		return rc ? c ^ 2 : c;
 	}
 };
 

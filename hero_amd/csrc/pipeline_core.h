@@ -144,19 +144,44 @@ HD int step_type(uint32_t code, bool swapped) {
 	return ((code == kStepLeft) != swapped) ? kOpI : kOpD;
 }
 
-// Feeds one DP call's steps in order as record ops (addAlignment, src/AlignmentRecord.cpp:50-54).
+// Feeds one DP call's steps in order as record ops (addAlignment, src/AlignmentRecord.cpp:50-54),
+// as runs: op(count, type) for each maximal run of equal op types (a mismatch run is one call
+// with its length). A 16-step word that is all matches extends the current run in one step.
 template <class F>
-HD void feed_steps(const DpResults &R, uint32_t task, bool swapped, F &&op) {
+HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 	const DpTask &t = R.tasks[task];
 	const int cap = t.s_len + t.q_len;
 	const int n = (int) R.nsteps[task] <= cap ? (int) R.nsteps[task] : 0;
 	const uint32_t *w = R.words + t.out_off;
 	int e = cap - n;
+	int run_t = -1, run_c = 0;
 	while (e < cap) {
-		uint32_t word = w[e >> 4] >> (2 * (e & 15));
 		const int stop = (e | 15) + 1 < cap ? (e | 15) + 1 : cap;
-		for (; e < stop; e++, word >>= 2) op(step_type(word & 3u, swapped));
+		const int m = stop - e;
+		uint32_t word = w[e >> 4] >> (2 * (e & 15));
+		const uint32_t mask = m == 16 ? 0xFFFFFFFFu : ((1u << (2 * m)) - 1u);
+		static_assert(kStepMatch == 0, "an all-zero word is all matches");
+		if ((word & mask) == 0u) { // m matches
+			if (run_t == kOpM) {
+				run_c += m;
+			} else {
+				if (run_c) op(run_c, run_t);
+				run_t = kOpM, run_c = m;
+			}
+			e = stop;
+			continue;
+		}
+		for (; e < stop; e++, word >>= 2) {
+			const int ty = step_type(word & 3u, swapped);
+			if (ty == run_t) {
+				run_c++;
+			} else {
+				if (run_c) op(run_c, run_t);
+				run_t = ty, run_c = 1;
+			}
+		}
 	}
+	if (run_c) op(run_c, run_t);
 }
 
 // ---- score --------------------------------------------------------------------------------
@@ -298,7 +323,7 @@ struct VoteV {
 	}
 	HD void start(int pos) { ps = pos, pq = 0; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void dp(const DpCall &c) { feed_steps(R, task++, c.swapped, [&](int t) { op(1, t); }); }
+	HD void dp(const DpCall &c) { feed_runs(R, task++, c.swapped, [&](int cnt, int t) { op(cnt, t); }); }
 	HD bool check(int64_t, int64_t) { return true; }
 	HD bool finish() {
 		flush();
@@ -353,7 +378,7 @@ struct RecordV {
 	}
 	HD void start(int p) { ps = p, pq = 0, pos = p; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void dp(const DpCall &c) { feed_steps(R, task++, c.swapped, [&](int t) { op(1, t); }); }
+	HD void dp(const DpCall &c) { feed_runs(R, task++, c.swapped, [&](int cnt, int t) { op(cnt, t); }); }
 	HD bool check(int64_t, int64_t) { return true; }
 	HD bool finish() {
 		if (pending) emit();

@@ -312,3 +312,5 @@ extern "C" int hostsim_band_align(const hs_cfg *cfg, const char *subj, int ls, c
 	std::memcpy(cigar_out, out.c_str(), out.size() + 1);
 	return 0;
 }
+
+extern "C" uint64_t hostsim_band_cells(int lenS, int lenQ, int b) { return band_cells(lenS, lenQ, b); }

@@ -43,3 +43,20 @@ def test_pipeline_core_matches_oracle(case):
     for k in ("dp_calls", "dp_cells", "records_built", "records_kept", "seeds"):
         assert ro["stats"][k] == rh["stats"][k], k
     assert rh["stats"]["oob"] == 0
+
+
+def test_band_cells_closed_form_equals_row_sum():
+    """The closed form of the per-call cell count equals the row-by-row sum it replaces."""
+    import ctypes
+
+    f = H.lib().hostsim_band_cells
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    for ls in range(0, 40):
+        for lq in range(0, 40):
+            for b in range(0, 45):
+                want = sum(min(lq, i + b) - max(0, i - b) + 1 for i in range(ls + 1)) - 1
+                assert f(ls, lq, b) == want % (1 << 64), (ls, lq, b)
+    for ls, lq, b in [(500, 480, 20), (16383, 16000, 383), (150, 149, 1), (1, 300, 299)]:
+        want = sum(min(lq, i + b) - max(0, i - b) + 1 for i in range(ls + 1)) - 1
+        assert f(ls, lq, b) == want
