@@ -141,9 +141,9 @@ struct Worker {
 	hero_stats *acc = nullptr; // host-side timings accumulate here
 	DevBuf<uint32_t> seg_n, seg_off, scan_tmp, lane_n;
 	DevBuf<uint64_t> seeds, win_info;
-	DevBuf<int32_t> cid, cfirst, clast, clen;
+	DevBuf<int32_t> cnext, cfirst, clast, clen;
 	DevBuf<uint8_t> gflag;
-	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off;
+	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off, g_prog, g_prog_off, prog;
 	DevBuf<DpTask> tasks;
 	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
 	DevBuf<uint32_t> bucket_hist, bucket_off, order;
@@ -166,17 +166,18 @@ struct Worker {
 		hipError_t e = hipStreamCreate(&st);
 		for (auto &t : ev)
 			if (e == hipSuccess) e = hipEventCreate(&t.a);
-		if (e == hipSuccess) e = dstats.exact(kStCount);
+		if (e == hipSuccess) e = dstats.exact((size_t) kStatStripes * kStatStride);
 		if (e == hipSuccess) e = dtotal.exact(1);
 		if (e == hipSuccess) e = counter.exact(1);
-		if (e == hipSuccess) e = hipMemset(dstats.p, 0, kStCount * 8);
+		if (e == hipSuccess) e = hipMemset(dstats.p, 0, (size_t) kStatStripes * kStatStride * 8);
 		return e;
 	}
 	void release() {
 		if (st) (void) hipStreamSynchronize(st);
 		seg_n.release(), seg_off.release(), scan_tmp.release(), lane_n.release(), seeds.release(), win_info.release();
-		cid.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
+		cnext.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
 		g_tasks.release(), g_task_off.release(), g_words.release(), g_word_off.release();
+		g_prog.release(), g_prog_off.release(), prog.release();
 		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), plain_scratch.release();
 		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
@@ -396,9 +397,11 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 	}
 	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
 	{
-		unsigned long long skipped = 0;
-		HIPCHK(c, hipMemcpyAsync(&skipped, stats + kStDpSkipped, 8, hipMemcpyDeviceToHost, w->st));
+		std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
+		HIPCHK(c, hipMemcpyAsync(stripes.data(), stats, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
 		HIPCHK(c, hipStreamSynchronize(w->st));
+		unsigned long long skipped = 0;
+		for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
 		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
 	}
 	return HERO_OK;
@@ -441,7 +444,7 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	const size_t ns = (size_t) nseeds;
 	HIPCHK(c, w->seeds.ensure(ns));
 	cd.seeds = w->seeds.p;
-	HIPCHK(c, w->cid.ensure(ns));
+	HIPCHK(c, w->cnext.ensure(ns));
 	HIPCHK(c, w->cfirst.ensure(ns));
 	HIPCHK(c, w->clast.ensure(ns));
 	HIPCHK(c, w->clen.ensure(ns));
@@ -452,7 +455,10 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	HIPCHK(c, w->g_task_off.ensure(ns));
 	HIPCHK(c, w->g_words.ensure(ns));
 	HIPCHK(c, w->g_word_off.ensure(ns));
-	cd.cid = w->cid.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
+	HIPCHK(c, w->g_prog.ensure(ns));
+	HIPCHK(c, w->g_prog_off.ensure(ns));
+	cd.g_prog = w->g_prog.p, cd.g_prog_off = w->g_prog_off.p;
+	cd.cnext = w->cnext.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
 	cd.gflag = w->gflag.p, cd.g_tasks = w->g_tasks.p, cd.g_task_off = w->g_task_off.p;
 	cd.g_nch = w->g_nch.p, cd.g_best = w->g_best.p;
 	cd.g_words = w->g_words.p, cd.g_word_off = w->g_word_off.p;
@@ -474,6 +480,11 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
+	uint64_t nprog = 0;
+	if (int r = wscan(c, w, cd.g_prog, cd.g_prog_off, (int64_t) ns, nprog)) return r;
+	if (nprog >= (1ull << 32) - 16) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu script words; lower chunk_subjects", nsub, (unsigned long long) nprog);
+	HIPCHK(c, w->prog.ensure((size_t) nprog + 1));
+	cd.prog = w->prog.p;
 	uint64_t nvotes = 0;
 	if (vote_mode) {
 		if (int r = wscan(c, w, cd.sub_pairs, w->sub_pair_off.p, nsub, nvotes)) return r;
@@ -714,7 +725,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	v.nseeds = ns, v.ngroups = (uint32_t) ngroups, v.npairs = (uint32_t) npairs;
 	ChunkDev cd;
 	std::memset(&cd, 0, sizeof cd);
-	HIPCHK(c, w->cid.ensure((size_t) ns + 1));
+	HIPCHK(c, w->cnext.ensure((size_t) ns + 1));
 	HIPCHK(c, w->cfirst.ensure((size_t) ns + 1));
 	HIPCHK(c, w->clast.ensure((size_t) ns + 1));
 	HIPCHK(c, w->clen.ensure((size_t) ns + 1));
@@ -724,7 +735,10 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, w->g_task_off.ensure((size_t) ngroups + 1));
 	HIPCHK(c, w->g_words.ensure((size_t) ngroups + 1));
 	HIPCHK(c, w->g_word_off.ensure((size_t) ngroups + 1));
-	cd.seeds = w->seeds.p, cd.cid = w->cid.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
+	HIPCHK(c, w->g_prog.ensure((size_t) ngroups + 1));
+	HIPCHK(c, w->g_prog_off.ensure((size_t) ngroups + 1));
+	cd.g_prog = w->g_prog.p, cd.g_prog_off = w->g_prog_off.p;
+	cd.seeds = w->seeds.p, cd.cnext = w->cnext.p, cd.cfirst = w->cfirst.p, cd.clast = w->clast.p, cd.clen = w->clen.p;
 	cd.g_nch = w->g_nch.p, cd.g_best = w->g_best.p, cd.g_tasks = w->g_tasks.p, cd.g_task_off = w->g_task_off.p;
 	cd.g_words = w->g_words.p, cd.g_word_off = w->g_word_off.p;
 	unsigned long long *stats = w->dstats.p;
@@ -735,6 +749,11 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ngroups, nwords)) return r;
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "batch yields %llu DP tasks", (unsigned long long) ntasks);
+	uint64_t nprog = 0;
+	if (int r = wscan(c, w, cd.g_prog, cd.g_prog_off, (int64_t) ngroups, nprog)) return r;
+	if (nprog >= (1ull << 32) - 16) return fail(c, HERO_ERR_LIMIT, "batch yields %llu script words", (unsigned long long) nprog);
+	HIPCHK(c, w->prog.ensure((size_t) nprog + 1));
+	cd.prog = w->prog.p;
 	HIPCHK(c, w->tasks.ensure((size_t) ntasks + 1));
 	HIPCHK(c, w->words.ensure((size_t) nwords + 1));
 	HIPCHK(c, w->nsteps.ensure((size_t) ntasks + 1));
@@ -754,7 +773,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	w->acc->dp_tasks += ntasks;
 	// consensus of the batch, corrected bases back to the caller's layout
 	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
-	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p + kStReadsChanged, w->st));
+	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
 	std::vector<char> h((size_t) bases);
 	if (bases) HIPCHK(c, hipMemcpyAsync(h.data(), w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
@@ -1017,7 +1036,7 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
 	HIPCHK(c, hipEventRecord(c->w.ev[5].a, c->w.st));
 	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, n, c->cons_out.p, nullptr,
-			c->w.dstats.p + kStReadsChanged, c->w.st));
+			c->w.dstats.p, c->w.st));
 	HIPCHK(c, hipEventRecord(c->w.ev[6].a, c->w.st));
 	// The device writes corrected bases at packed base offsets; the caller's layout usually is that
 	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
@@ -1069,8 +1088,11 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	if (!c || !out) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
-	unsigned long long s[kStCount];
-	HIPCHK(c, hipMemcpy(s, c->w.dstats.p, sizeof s, hipMemcpyDeviceToHost));
+	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
+	HIPCHK(c, hipMemcpy(stripes.data(), c->w.dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost));
+	unsigned long long s[kStCount] = {};
+	for (int i = 0; i < kStatStripes; i++)
+		for (int j = 0; j < kStCount; j++) s[j] += stripes[(size_t) i * kStatStride + j];
 	*out = c->st_host;
 	out->lookups = s[kStLookups];
 	out->seeds = s[kStSeeds];
@@ -1089,7 +1111,7 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 int hero_reset_stats(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
-	HIPCHK(c, hipMemset(c->w.dstats.p, 0, kStCount * 8));
+	HIPCHK(c, hipMemset(c->w.dstats.p, 0, (size_t) kStatStripes * kStatStride * 8));
 	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;

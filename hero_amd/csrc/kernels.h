@@ -29,23 +29,21 @@ struct DevIndex {
 // chain was not accepted keeps qid = kNoRecord.
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 struct RecHead {
-	uint32_t qid;   // innerId of the query read
-	uint32_t gbase; // first seed of the query group
-	int32_t gsize;  // seeds in the group
-	uint32_t task0; // first DP task of the record
-	int32_t sub;    // subject index within the chunk
-	int16_t strand, chain;
+	uint32_t qid;    // innerId of the query read
+	uint32_t script; // the record's script (index into the chunk's program words)
+	uint32_t task0;  // first DP task of the record
+	int32_t sub;     // subject index within the chunk
+	int32_t strand, _pad;
 	double rate_sum, len; // isGoodAlignment() rates, for SubjectRead::addAlignment
 };
 
 // The kept record of one (subject, query) pair, for its votes.
 struct VoteItem {
-	uint32_t q;     // innerId of the query read; kNoRecord for an unused slot
-	uint32_t gbase; // first seed of the record's query group
-	int32_t gsize;
-	uint32_t task0; // first DP task of the record
-	int32_t sub;    // subject index within the chunk
-	int16_t strand, chain;
+	uint32_t q;      // innerId of the query read; kNoRecord for an unused slot
+	uint32_t script; // the record's script (index into the chunk's program words)
+	uint32_t task0;  // first DP task of the record
+	int32_t sub;     // subject index within the chunk
+	int32_t strand;
 };
 
 // Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
@@ -58,11 +56,13 @@ struct ChunkDev {
 	uint32_t *lane_n;          // per (subject, strand) slot and lane: seeds of that lane's windows
 	uint32_t *seg_n, *seg_off; // 2 per subject (forward, reverse)
 	uint64_t *seeds;
-	int32_t *cid, *cfirst, *clast, *clen;
+	int32_t *cnext, *cfirst, *clast, *clen;
 	uint8_t *gflag; // bit0 group start, bit1 group kept (within max_candidates)
 	int32_t *g_nch, *g_best; // chain count and best covered length of a kept group
 	uint32_t *g_tasks, *g_task_off, *g_words, *g_word_off;
-	uint32_t *g_recs, *g_rec_off; // record mode: chains of a kept group (record slots) and their scan
+	uint32_t *g_prog, *g_prog_off; // script words of a kept group (record scripts) and their scan
+	uint32_t *prog;                // record scripts of every kept group
+	uint32_t *g_recs, *g_rec_off; // record mode: scripts of a kept group (record slots) and their scan
 	RecHead *recs;                // record mode: one slot per chain of every kept group
 	uint32_t *sub_pairs, *sub_pair_off; // vote mode: kept groups of both strands per subject, scan
 	VoteItem *votes;                    // vote mode: the kept record of each pair (q = kNoRecord: none)
@@ -85,6 +85,11 @@ enum StatSlot : int {
 	kStCellsAll,
 	kStCount
 };
+// Device counters are striped over kStatStripes cache lines (kStatStride u64 apart), chosen by
+// workgroup, so that per-wave updates do not serialize on one address; readers sum the stripes.
+constexpr int kStatStripes = 256;
+constexpr int kStatStride = 16;
+static_assert(kStCount <= kStatStride, "one stripe holds every counter");
 
 struct DpLaunch {
 	const DpTask *tasks;
@@ -143,7 +148,7 @@ hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_ma
 		hipStream_t st);
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
-		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st);
+		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st);
 
 // Serving orientation: the subjects' k-mers at every position are indexed; each query key (kref =
 // query << 14 | position) and its reverse complement probe them. Count pass (keys_out == nullptr)

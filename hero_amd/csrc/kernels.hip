@@ -39,6 +39,11 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 	}
 	return ((uint64_t) hi << 32) | lo;
 }
+// Adds v to counter `slot` in this workgroup's stripe.
+__device__ __forceinline__ void stat_add(unsigned long long *stats, int slot, unsigned long long v) {
+	atomicAdd(&stats[(blockIdx.x % kStatStripes) * kStatStride + slot], v);
+}
+
 __device__ __forceinline__ uint64_t lanes_below_mask() {
 	const int lane = lane_id();
 	return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex 
 		cnt = wave_sum(cnt);
 		if (lane == 0) c.seg_n[slot] = cnt;
 	}
-	if (lane == 0) atomicAdd(&stats[kStLookups], (unsigned long long) (2 * nwin));
+	if (lane == 0) stat_add(stats, kStLookups, (unsigned long long) (2 * nwin));
 }
 hipError_t launch_seed_count(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
 		unsigned long long *stats, hipStream_t st) {
@@ -354,7 +359,7 @@ __device__ __forceinline__ PairReads make_pair(const Packed &S, const Packed &Q,
 __device__ __forceinline__ Group make_group(const ChunkDev &c, uint32_t base, int gn) {
 	Group g;
 	g.seeds = c.seeds + base;
-	g.cid = c.cid + base, g.cfirst = c.cfirst + base, g.clast = c.clast + base, g.clen = c.clen + base;
+	g.cnext = c.cnext + base, g.cfirst = c.cfirst + base, g.clast = c.clast + base, g.clen = c.clen + base;
 	g.gn = gn, g.nch = 0, g.best = 0;
 	return g;
 }
@@ -421,7 +426,7 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 			const uint64_t m = __ballot(start);
 			const uint32_t rank = groups_before + (uint32_t) __popcll(m & lanes_below_mask());
 			groups_before += (uint32_t) __popcll(m);
-			uint32_t ntask = 0, nword = 0, nrec = 0;
+			uint32_t ntask = 0, nword = 0, nrec = 0, nprog = 0;
 			uint8_t flag = 0;
 			if (start) {
 				const bool kept = o.max_candidates <= 0 || rank < (uint32_t) o.max_candidates;
@@ -435,22 +440,23 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 					c.g_nch[off + t] = g.nch;
 					c.g_best[off + t] = g.best;
 					const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-					int nt, nw;
-					plan_count(o, g, pr, nt, nw);
-					ntask = (uint32_t) nt, nword = (uint32_t) nw, nrec = (uint32_t) g.nch;
+					int nt, nw, np, nsc;
+					plan_count(o, g, pr, nt, nw, np, nsc);
+					ntask = (uint32_t) nt, nword = (uint32_t) nw, nprog = (uint32_t) np, nrec = (uint32_t) nsc;
 				}
 			}
 			if (t < n) {
 				c.gflag[off + t] = flag;
 				c.g_tasks[off + t] = ntask;
 				c.g_words[off + t] = nword;
+				c.g_prog[off + t] = nprog;
 				if (c.g_recs) c.g_recs[off + t] = nrec;
 			}
 		}
 		kept_groups += o.max_candidates > 0 ? min(groups_before, (uint32_t) o.max_candidates) : groups_before;
 	}
 	if (c.sub_pairs && lane == 0) c.sub_pairs[sl] = kept_groups;
-	if (lane == 0) atomicAdd(&stats[kStSeeds], (unsigned long long) (c.seg_n[2 * sl] + c.seg_n[2 * sl + 1]));
+	if (lane == 0) stat_add(stats, kStSeeds, (unsigned long long) (c.seg_n[2 * sl] + c.seg_n[2 * sl + 1]));
 }
 hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, const Opts &o, const ChunkDev &c,
 		unsigned long long *stats, hipStream_t st) {
@@ -469,13 +475,13 @@ __global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, Chun
 	for (int strand = 0; strand < 2; strand++) {
 		const uint32_t n = c.seg_n[2 * sl + strand], off = c.seg_off[2 * sl + strand];
 		for (uint32_t t = lane; t < n; t += 64) {
-			if (c.gflag[off + t] != 3 || c.g_tasks[off + t] == 0) continue;
+			if (c.gflag[off + t] != 3) continue;
 			const int64_t q = seed_qid(c.seeds[off + t]);
 			uint32_t e = t + 1;
 			while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
 			const Group g = chained_group(c, off + t, (int) (e - t));
 			const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t]);
+			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t], c.prog + c.g_prog_off[off + t]);
 		}
 	}
 }
@@ -512,45 +518,37 @@ struct PassBCounters {
 // that record casts its votes.
 template <bool RECORDS>
 __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c,
-		const DpResults &R, int64_t s, int Ls, uint32_t off0, uint32_t n0, uint32_t off1, uint32_t n1, int gf, int gr,
+		const DpResults &R, int64_t s, int Ls, int64_t q, uint32_t base0, int gn0, uint32_t base1, int gn1,
 		uint32_t vbase, uint32_t *vcount, PassBCounters &k) {
-	const int64_t q = seed_qid(gf >= 0 ? c.seeds[off0 + gf] : c.seeds[off1 + gr]);
 	Best best;
-	best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
-	uint32_t gbase0 = 0, gbase1 = 0;
-	int gsize0 = 0, gsize1 = 0;
+	best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.script = 0, best.task0 = 0;
+#pragma unroll 1
 	for (int strand = 0; strand < 2; strand++) {
-		const int gi = strand == 0 ? gf : gr;
-		if (gi < 0) continue;
-		const uint32_t so = strand == 0 ? off0 : off1, sn = strand == 0 ? n0 : n1;
-		const uint32_t base = so + gi;
-		const int gn = (int) (group_end(c.seeds + so, sn, gi) - gi);
-		if (strand == 0) gbase0 = base, gsize0 = gn;
-		else gbase1 = base, gsize1 = gn;
-		const Group g = chained_group(c, base, gn);
-		const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
+		const int gn = strand == 0 ? gn0 : gn1;
+		if (gn <= 0) continue;
+		const uint32_t base = strand == 0 ? base0 : base1;
+		const uint32_t gp = c.g_prog_off[base], task0 = c.g_task_off[base];
 		if constexpr (RECORDS) {
 			RecHead *slots = c.recs + c.g_rec_off[base];
 			const int sub = (int) (s - c.s_begin); // record mode runs on contiguous chunks only
-			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built,
-					[&](int chain, uint32_t task, double rate_sum, double len) {
+			score_group(o, c.prog, gp, R, task0, strand, best, k.cells, k.calls, k.built,
+					[&](int ks, uint32_t script, uint32_t task, double rate_sum, double len) {
 						RecHead h;
-						h.qid = (uint32_t) q, h.gbase = base, h.gsize = gn, h.task0 = task, h.sub = sub;
-						h.strand = (int16_t) strand, h.chain = (int16_t) chain, h.rate_sum = rate_sum, h.len = len;
-						slots[chain] = h;
+						h.qid = (uint32_t) q, h.script = script, h.task0 = task, h.sub = sub;
+						h.strand = strand, h._pad = 0, h.rate_sum = rate_sum, h.len = len;
+						slots[ks] = h;
 					});
 		} else {
-			score_group(o, g, pr, R, c.g_task_off[base], strand, best, k.cells, k.calls, k.built);
+			score_group(o, c.prog, gp, R, task0, strand, best, k.cells, k.calls, k.built);
 		}
 	}
 	if (!best.have) return;
 	k.kept++;
 	if (RECORDS || !o.call_consensus) return;
 	// the winner's votes are cast by k_votes
-	const bool wr = best.strand == 1;
 	VoteItem v;
-	v.q = (uint32_t) q, v.gbase = wr ? gbase1 : gbase0, v.gsize = wr ? gsize1 : gsize0, v.task0 = best.task0;
-	v.sub = (int32_t) (s - c.s_begin), v.strand = (int16_t) best.strand, v.chain = (int16_t) best.chain;
+	v.q = (uint32_t) q, v.script = best.script, v.task0 = best.task0;
+	v.sub = (int32_t) (s - c.s_begin), v.strand = best.strand;
 	c.votes[vbase + atomicAdd(vcount, 1u)] = v;
 }
 
@@ -564,10 +562,9 @@ __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkD
 	if (v.q == kNoRecord) return;
 	const int64_t s = c.s_begin + v.sub;
 	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) v.q, v.strand);
-	const Group g = chained_group(c, v.gbase, v.gsize);
 	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	VoteSink sink{tally + 4 * q_boff[v.q], pr, &stats[kStOob]};
-	vote_record(o, g, pr, R, v.chain, v.task0, sink);
+	vote_record(c.prog + v.script, R, v.task0, sink);
 }
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
 		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st) {
@@ -587,11 +584,10 @@ __global__ __launch_bounds__(64) void k_record_ops(Packed S, Packed Q, Opts o, C
 		return;
 	}
 	const int64_t s = c.s_begin + h.sub;
-	const Group g = chained_group(c, h.gbase, h.gsize);
 	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) h.qid, h.strand);
 	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	int p = 0;
-	const int n = record_ops(o, g, pr, R, h.chain, h.task0, ops ? ops + op_off[i] : nullptr, &p);
+	const int n = record_ops(c.prog + h.script, pr, R, h.task0, ops ? ops + op_off[i] : nullptr, &p);
 	if (ops) pos[i] = p;
 	else op_n[i] = (uint32_t) n;
 }
@@ -648,27 +644,37 @@ hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &
 	return hipGetLastError();
 }
 
-// Kept group starts of one segment into an LDS list (wave-wide ballot compaction).
+// Kept group starts of one segment into an LDS list (wave-wide ballot compaction). Kept groups
+// are a prefix of the segment's groups, so list[cnt] receives the end of the last kept group (the
+// first dropped group's start, or n).
 __device__ __forceinline__ uint32_t collect_groups(const uint8_t *gflag, uint32_t n, uint32_t *list, uint32_t cap) {
-	uint32_t cnt = 0;
+	uint32_t cnt = 0, end = n;
 	for (uint32_t t0 = 0; t0 < n; t0 += 64) {
 		const uint32_t t = t0 + lane_id();
-		const bool k = t < n && gflag[t] == 3;
-		const uint64_t m = __ballot(k);
+		const uint8_t f = t < n ? gflag[t] : 0;
+		const uint64_t m = __ballot(f == 3);
 		const uint32_t pos = cnt + (uint32_t) __popcll(m & lanes_below_mask());
-		if (k && pos < cap) list[pos] = t;
+		if (f == 3 && pos < cap) list[pos] = t;
 		cnt += (uint32_t) __popcll(m);
+		const uint64_t dropped = __ballot(f == 1);
+		if (dropped) {
+			end = t0 + (uint32_t) __ffsll((unsigned long long) dropped) - 1;
+			break;
+		}
 	}
+	if (cnt <= cap && lane_id() == 0) list[cnt] = end; // list holds cap + 1 entries
 	return cnt;
 }
 
-// Pass B: one wavefront per subject. The kept groups of both strands are listed in LDS and paired
-// by query (binary search over the reverse list); lanes then take the pairs densely.
+// Pass B: one wavefront per subject. The kept groups of both strands are listed in LDS with their
+// query ids and paired by query (binary search over the reverse list); lanes then take the pairs
+// densely. More kept groups than the lists hold: the segments are walked directly instead.
 constexpr uint32_t kPassBGroups = 256;
 template <bool RECORDS>
 __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, unsigned long long *stats) {
 	__shared__ uint32_t vcount;
-	__shared__ uint32_t list0[kPassBGroups], list1[kPassBGroups];
+	__shared__ uint32_t list0[kPassBGroups + 1], list1[kPassBGroups + 1];
+	__shared__ uint32_t q0s[kPassBGroups], q1s[kPassBGroups];
 	__shared__ int32_t match0[kPassBGroups];
 	__shared__ uint32_t rev_only[kPassBGroups];
 	__shared__ uint8_t used1[kPassBGroups];
@@ -685,18 +691,22 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	if (lane == 0) vcount = 0;
 	const uint32_t c0 = collect_groups(c.gflag + off0, n0, list0, kPassBGroups);
 	const uint32_t c1 = collect_groups(c.gflag + off1, n1, list1, kPassBGroups);
-	if (c0 <= kPassBGroups && c1 <= kPassBGroups) {
-		for (uint32_t j = lane; j < c1; j += 64) used1[j] = 0;
+	const bool in_lds = c0 <= kPassBGroups && c1 <= kPassBGroups;
+	uint32_t total = n0 + n1; // fallback: one lane per seed of both segments
+	if (in_lds) {
+		__syncthreads();
+		for (uint32_t i = lane; i < c0; i += 64) q0s[i] = (uint32_t) seed_qid(c.seeds[off0 + list0[i]]);
+		for (uint32_t j = lane; j < c1; j += 64) q1s[j] = (uint32_t) seed_qid(c.seeds[off1 + list1[j]]), used1[j] = 0;
 		__syncthreads();
 		for (uint32_t i = lane; i < c0; i += 64) {
-			const int64_t q = seed_qid(c.seeds[off0 + list0[i]]);
+			const uint32_t q = q0s[i];
 			uint32_t lo = 0, hi = c1;
 			while (lo < hi) {
 				const uint32_t mid = (lo + hi) >> 1;
-				if (seed_qid(c.seeds[off1 + list1[mid]]) < q) lo = mid + 1;
+				if (q1s[mid] < q) lo = mid + 1;
 				else hi = mid;
 			}
-			const bool found = lo < c1 && seed_qid(c.seeds[off1 + list1[lo]]) == q;
+			const bool found = lo < c1 && q1s[lo] == q;
 			match0[i] = found ? (int32_t) lo : -1;
 			if (found) used1[lo] = 1;
 		}
@@ -710,32 +720,45 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 			nrev += (uint32_t) __popcll(m);
 		}
 		__syncthreads();
-		for (uint32_t p = lane; p < c0 + nrev; p += 64) {
-			int gf = -1, gr = -1;
-			if (p < c0) gf = (int) list0[p], gr = match0[p] >= 0 ? (int) list1[match0[p]] : -1;
-			else gr = (int) list1[rev_only[p - c0]];
-			pass_b_pair<RECORDS>(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, vbase, &vcount, k);
-		}
-	} else {
-		// more kept groups than the LDS lists hold: walk the segments directly
-		for (int pass = 0; pass < 2; pass++) {
-			const uint32_t n = pass == 0 ? n0 : n1, off = pass == 0 ? off0 : off1;
-			for (uint32_t t = lane; t < n; t += 64) {
-				if (c.gflag[off + t] != 3) continue;
-				const int64_t q = seed_qid(c.seeds[off + t]);
-				int gf = -1, gr = -1;
-				if (pass == 0) {
-					gf = (int) t;
-					const int r = find_group(c.seeds + off1, n1, q);
-					if (r >= 0 && c.gflag[off1 + r] == 3) gr = r;
-				} else {
-					const int f = find_group(c.seeds + off0, n0, q);
-					if (f >= 0 && c.gflag[off0 + f] == 3) continue; // handled with the forward group
-					gr = (int) t;
+		total = c0 + nrev;
+	}
+	for (uint32_t p = lane; p < total; p += 64) {
+		int64_t q = -1;
+		uint32_t b0 = 0, b1 = 0;
+		int g0 = 0, g1 = 0;
+		if (in_lds) {
+			if (p < c0) {
+				q = q0s[p];
+				b0 = off0 + list0[p], g0 = (int) (list0[p + 1] - list0[p]);
+				if (match0[p] >= 0) {
+					const uint32_t j = (uint32_t) match0[p];
+					b1 = off1 + list1[j], g1 = (int) (list1[j + 1] - list1[j]);
 				}
-				pass_b_pair<RECORDS>(S, Q, o, c, R, s, Ls, off0, n0, off1, n1, gf, gr, vbase, &vcount, k);
+			} else {
+				const uint32_t j = rev_only[p - c0];
+				q = q1s[j];
+				b1 = off1 + list1[j], g1 = (int) (list1[j + 1] - list1[j]);
+			}
+		} else if (p < n0) {
+			if (c.gflag[off0 + p] == 3) {
+				q = seed_qid(c.seeds[off0 + p]);
+				b0 = off0 + p, g0 = (int) (group_end(c.seeds + off0, n0, p) - p);
+				const int r = find_group(c.seeds + off1, n1, q);
+				if (r >= 0 && c.gflag[off1 + r] == 3)
+					b1 = off1 + (uint32_t) r, g1 = (int) (group_end(c.seeds + off1, n1, (uint32_t) r) - (uint32_t) r);
+			}
+		} else {
+			const uint32_t t = p - n0;
+			if (c.gflag[off1 + t] == 3) {
+				const int64_t qq = seed_qid(c.seeds[off1 + t]);
+				const int f = find_group(c.seeds + off0, n0, qq);
+				if (!(f >= 0 && c.gflag[off0 + f] == 3)) { // else handled with the forward group
+					q = qq;
+					b1 = off1 + t, g1 = (int) (group_end(c.seeds + off1, n1, t) - t);
+				}
 			}
 		}
+		if (q >= 0) pass_b_pair<RECORDS>(S, Q, o, c, R, s, Ls, q, b0, g0, b1, g1, vbase, &vcount, k);
 	}
 	uint64_t cells = k.cells, calls = k.calls, built = k.built, kept = k.kept;
 	cells = wave_sum64(cells);
@@ -744,10 +767,10 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	kept = wave_sum64(kept);
 
 	if (lane == 0) {
-		atomicAdd(&stats[kStCellsExec], (unsigned long long) cells);
-		atomicAdd(&stats[kStCallsExec], (unsigned long long) calls);
-		atomicAdd(&stats[kStBuilt], (unsigned long long) built);
-		atomicAdd(&stats[kStKept], (unsigned long long) kept);
+		stat_add(stats, kStCellsExec, (unsigned long long) cells);
+		stat_add(stats, kStCallsExec, (unsigned long long) calls);
+		stat_add(stats, kStBuilt, (unsigned long long) built);
+		stat_add(stats, kStKept, (unsigned long long) kept);
 
 	}
 }
@@ -818,23 +841,24 @@ __global__ __launch_bounds__(64) void k_serve_a1(Packed S, Packed Q, Opts o, Chu
 		int64_t s;
 		int strand;
 		const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
-		int nt, nw;
-		plan_count(o, gr, pr, nt, nw);
+		int nt, nw, np, nsc;
+		plan_count(o, gr, pr, nt, nw, np, nsc);
 		c.g_tasks[g] = (uint32_t) nt;
 		c.g_words[g] = (uint32_t) nw;
+		c.g_prog[g] = (uint32_t) np;
 	}
-	if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[kStSeeds], (unsigned long long) v.nseeds);
+	if (blockIdx.x == 0 && threadIdx.x == 0) stat_add(stats, kStSeeds, (unsigned long long) v.nseeds);
 }
 __global__ __launch_bounds__(64) void k_serve_a2(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v) {
 	const uint32_t g = blockIdx.x * 64u + threadIdx.x;
-	if (g >= v.ngroups || c.g_tasks[g] == 0) return;
+	if (g >= v.ngroups) return;
 	const uint32_t base = v.glist[g];
 	const uint32_t end = g + 1 < v.ngroups ? v.glist[g + 1] : v.nseeds;
 	const Group gr = chained_group(c, base, (int) (end - base));
 	int64_t s;
 	int strand;
 	const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
-	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g]);
+	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g], c.prog + c.g_prog_off[g]);
 }
 __global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v,
 		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats) {
@@ -845,39 +869,33 @@ __global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, Chun
 		const uint32_t end = p + 1 < v.npairs ? v.plist[p + 1] : v.nseeds;
 		const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 		Best best;
-		best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.chain = 0, best.task0 = 0;
-		uint32_t gstart[2] = {0, 0}, gsize[2] = {0, 0};
+		best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.script = 0, best.task0 = 0;
 		// the pair's groups: forward (strand 0) first, then reverse
 		uint32_t i = base;
 		while (i < end) {
 			const uint32_t gi = v.gx[i]; // group index of the group starting at seed i
 			const uint32_t ge = gi + 1 < v.ngroups ? v.glist[gi + 1] : v.nseeds;
-			int64_t s;
-			int strand;
-			const PairReads pr = serve_pair(S, Q, o, v, i, s, strand);
-			const Group gr = chained_group(c, i, (int) (ge - i));
-			score_group(o, gr, pr, R, c.g_task_off[gi], strand, best, k.cells, k.calls, k.built);
-			gstart[strand] = i, gsize[strand] = ge - i;
+			const int strand = (int) (v.lo[i] >> 28);
+			score_group(o, c.prog, c.g_prog_off[gi], R, c.g_task_off[gi], strand, best, k.cells, k.calls, k.built);
 			i = ge;
 		}
 		if (best.have) {
 			k.kept++;
-			const uint32_t b0 = gstart[best.strand];
 			int64_t s;
 			int strand;
-			const PairReads pr = serve_pair(S, Q, o, v, b0, s, strand);
-			const Group gr = chained_group(c, b0, (int) gsize[best.strand]);
-			const int64_t q = (int64_t) (v.hi[b0] & kServeQMask);
-			VoteSink sink{tally + 4 * q_boff[q], pr, &stats[kStOob]};
-			vote_record(o, gr, pr, R, best.chain, best.task0, sink);
+			const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
+			const int64_t q = (int64_t) (v.hi[base] & kServeQMask);
+			const PairReads prb = make_pair(S, Q, o, s, S.len[s], q, best.strand);
+			VoteSink sink{tally + 4 * q_boff[q], prb, &stats[kStOob]};
+			vote_record(c.prog + best.script, R, best.task0, sink);
 		}
 	}
 	uint64_t cells = wave_sum64(k.cells), calls = wave_sum64(k.calls), built = wave_sum64(k.built), kept = wave_sum64(k.kept);
 	if (lane_id() == 0) {
-		atomicAdd(&stats[kStCellsExec], (unsigned long long) cells);
-		atomicAdd(&stats[kStCallsExec], (unsigned long long) calls);
-		atomicAdd(&stats[kStBuilt], (unsigned long long) built);
-		atomicAdd(&stats[kStKept], (unsigned long long) kept);
+		stat_add(stats, kStCellsExec, (unsigned long long) cells);
+		stat_add(stats, kStCallsExec, (unsigned long long) calls);
+		stat_add(stats, kStBuilt, (unsigned long long) built);
+		stat_add(stats, kStKept, (unsigned long long) kept);
 	}
 }
 hipError_t launch_serve_a1(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
@@ -1000,13 +1018,13 @@ __global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
 				wr.flush();
 				L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
 				L.sums[t] = wr.sb.finish();
-				if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+				if (ns < 0) stat_add(L.stats, kStDpSkipped, 1ull);
 				cells += band_cells(lenS, lenQ, task.band);
 			}
 		}
 	}
 	cells = wave_sum64(cells);
-	if (lane == 0) atomicAdd(&L.stats[kStCellsAll], (unsigned long long) cells);
+	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
 int dp_batched_waves_per_cu(int bmax, bool aff) {
@@ -1124,7 +1142,7 @@ __global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
 		const int b = task.band;
 		if (L.route_bmax >= 0 && b <= L.route_bmax && lenS < L.route_rows) continue; // done by the batched form
 		if (b > bmax_plain || lenS >= kPlainRows || lenQ >= kPlainRows) {
-			if (lane == 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
+			if (lane == 0) stat_add(L.stats, kStDpSkipped, 1ull);
 			continue;
 		}
 		const int W = 2 * b + 1;
@@ -1193,8 +1211,8 @@ __global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
 			wr.flush();
 			L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
 				L.sums[t] = wr.sb.finish();
-			if (ns < 0) atomicAdd(&L.stats[kStDpSkipped], 1ull);
-			atomicAdd(&L.stats[kStCellsAll], (unsigned long long) band_cells(lenS, lenQ, b));
+			if (ns < 0) stat_add(L.stats, kStDpSkipped, 1ull);
+			stat_add(L.stats, kStCellsAll, (unsigned long long) band_cells(lenS, lenQ, b));
 		}
 		__syncthreads();
 	}
@@ -1212,7 +1230,7 @@ hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStr
 // ---------------------------------------------------------------------------------------------
 // Writes the corrected bases of good reads as ASCII at their base offsets.
 __global__ void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0, int64_t nq,
-		uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed) {
+		uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats) {
 	const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nq) return;
 	const int64_t q = q0 + i;
@@ -1232,13 +1250,15 @@ __global__ void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint
 	}
 	const int changed = consensus_read(o, tally + 4 * b, L, Q, q, codes_out + b, decoded_out ? decoded_out + 4 * b : nullptr);
 	for (int p = 0; p < L; p++) codes_out[b + p] = (uint8_t) code_ascii(codes_out[b + p]);
-	if (changed && reads_changed) atomicAdd(reads_changed, 1ull);
+	const uint64_t m = __ballot(changed != 0);
+	if (stats && m && lane_id() == (int) __ffsll((unsigned long long) m) - 1)
+		stat_add(stats, kStReadsChanged, (unsigned long long) __popcll(m));
 }
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
-		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *reads_changed, hipStream_t st) {
+		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st) {
 	if (nq == 0) return hipSuccess;
 	k_consensus<<<(unsigned) ((nq + 255) / 256), 256, 0, st>>>(Q, o, q_boff, tally, q0, nq, codes_out, decoded_out,
-			reads_changed);
+			stats);
 	return hipGetLastError();
 }
 

@@ -14,7 +14,7 @@ namespace hero {
 // Seeds and chain state of one group: seeds[0..gn) sorted; chain arrays of group size.
 struct Group {
 	const uint64_t *seeds;
-	int32_t *cid, *cfirst, *clast, *clen;
+	int32_t *cnext, *cfirst, *clast, *clen;
 	int gn;
 	int nch;  // chains after chain_group
 	int best; // best covered length
@@ -31,8 +31,8 @@ struct I32Arr {
 
 HD void chain(const Opts &o, Group &g) {
 	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid}, cf{g.cfirst}, cl{g.clast}, ln{g.clen};
-	g.nch = chain_group(o, g.gn, sa, cid, cf, cl, ln, &g.best);
+	I32Arr nx{g.cnext}, cf{g.cfirst}, cl{g.clast}, ln{g.clen};
+	g.nch = chain_group(o, g.gn, sa, nx, cf, cl, ln, &g.best);
 }
 
 // Reads of one (subject, strand, query) pair and their frames (word pointers cached).
@@ -58,16 +58,6 @@ HD PairReads make_pair_reads(const Packed &S, const Packed &Q, bool call_consens
 }
 
 // ---- plan ---------------------------------------------------------------------------------
-struct CountDp {
-	int n = 0;
-	int words = 0;
-	HD void start(int) {}
-	HD void add(int, int) {}
-	HD void dp(const DpCall &c) { n++, words += edit_words(c.lenS, c.lenQ); }
-	HD bool check(int64_t, int64_t) { return true; }
-	HD bool finish() { return true; }
-};
-
 HD DpTask make_task(const PairReads &pr, const DpCall &c, uint32_t out_off) {
 	DpTask t;
 	t.s_read = (uint32_t) pr.s_read;
@@ -84,45 +74,90 @@ HD DpTask make_task(const PairReads &pr, const DpCall &c, uint32_t out_off) {
 	return t;
 }
 
-struct EmitDp {
-	const PairReads &pr;
+// Record scripts. Planning walks each surviving chain's getAlignment2 geometry once (it never
+// depends on DP results) and keeps it as a script: a header and one word per step of the walk.
+// Scoring, votes and records then replay scripts against the DP results without revisiting
+// seeds or chains. A group's scripts follow one count word: [nscripts, script 0, script 1, ...];
+// script = [nops | ndp << 16, max_mis, max_indel, pos, op * nops]; op = kind << 30 | payload:
+//   add     type << 27 | count        a fixed CIGAR op
+//   dp      swapped << 29 | cells     the next DP call of the record (its task follows the last)
+//   check                            isGoodAlignment(max_mis, max_indel) (early exit)
+//   finish                           isGoodAlignment() on rates
+// nops == 0: the walk stopped before building anything (no keys, or overlap < minOverlap).
+enum : uint32_t { kScAdd = 0u, kScDp = 1u, kScCheck = 2u, kScFinish = 3u };
+constexpr int kScriptHead = 4;
+
+struct ScriptWriter {
+	uint32_t *out;       // script start; null: count only
+	const PairReads *pr; // with tasks: also write the DP tasks
 	DpTask *tasks;
 	uint32_t next_task, next_word;
-	HD void start(int) {}
-	HD void add(int, int) {}
-	HD void dp(const DpCall &c) {
-		tasks[next_task++] = make_task(pr, c, next_word);
-		next_word += (uint32_t) edit_words(c.lenS, c.lenQ);
+	int nops, ndp;
+	uint32_t max_mis, max_indel, pos;
+	HD void put(uint32_t w) {
+		if (out) out[kScriptHead + nops] = w;
+		nops++;
 	}
-	HD bool check(int64_t, int64_t) { return true; }
-	HD bool finish() { return true; }
+	HD void start(int p) { pos = (uint32_t) p; }
+	HD void add(int c, int t) { put((kScAdd << 30) | ((uint32_t) t << 27) | ((uint32_t) c & 0x7FFFFFFu)); }
+	HD void dp(const DpCall &c) {
+		const uint64_t cells = band_cells(c.lenS, c.lenQ, c.band);
+		put((kScDp << 30) | (c.swapped ? (1u << 29) : 0u) | (uint32_t) (cells < (1u << 29) ? cells : (1u << 29) - 1));
+		if (tasks) tasks[next_task] = make_task(*pr, c, next_word);
+		next_task++;
+		next_word += (uint32_t) edit_words(c.lenS, c.lenQ);
+		ndp++;
+	}
+	HD bool check(int64_t m, int64_t i) {
+		max_mis = (uint32_t) m, max_indel = (uint32_t) i;
+		put(kScCheck << 30);
+		return true;
+	}
+	HD bool finish() {
+		put(kScFinish << 30);
+		return true;
+	}
+	HD void head() const {
+		if (out) out[0] = (uint32_t) nops | ((uint32_t) ndp << 16), out[1] = max_mis, out[2] = max_indel, out[3] = pos;
+	}
 };
 
-// DP tasks and edit words of all surviving chains of a chained group.
-HD void plan_count(const Opts &o, const Group &g, const PairReads &pr, int &ntasks, int &nwords) {
-	ntasks = 0, nwords = 0;
+HD int script_nops(const uint32_t *sc) { return (int) (sc[0] & 0xFFFFu); }
+HD int script_ndp(const uint32_t *sc) { return (int) (sc[0] >> 16); }
+HD int script_words(const uint32_t *sc) { return kScriptHead + script_nops(sc); }
+
+// DP tasks, edit words and script words of all surviving chains of a chained group.
+HD void plan_count(const Opts &o, const Group &g, const PairReads &pr, int &ntasks, int &nwords, int &nprog,
+		int &nscripts) {
+	ntasks = 0, nwords = 0, nprog = 1, nscripts = 0;
 	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid};
+	I32Arr nx{g.cnext};
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
-		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-		CountDp cv;
-		walk_record(o, pr.Ls, pr.Lq, keys, cv);
-		ntasks += cv.n, nwords += cv.words;
+		ChainKeys<SeedAt, I32Arr> keys(sa, nx, g.cfirst[c], g.clast[c]);
+		ScriptWriter w{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0};
+		walk_record(o, pr.Ls, pr.Lq, keys, w);
+		ntasks += w.ndp, nwords += (int) w.next_word, nprog += kScriptHead + w.nops, nscripts++;
 	}
 }
 
-// Writes the tasks of all surviving chains (in chain order) at tasks[task0...], edit words
-// allocated from word0.
-HD void plan_emit(const Opts &o, const Group &g, const PairReads &pr, DpTask *tasks, uint32_t task0, uint32_t word0) {
+// Writes the tasks (at tasks[task0...], edit words from word0) and the scripts (at prog) of all
+// surviving chains, in chain order.
+HD void plan_emit(const Opts &o, const Group &g, const PairReads &pr, DpTask *tasks, uint32_t task0, uint32_t word0,
+		uint32_t *prog) {
 	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid};
-	EmitDp ev{pr, tasks, task0, word0};
+	I32Arr nx{g.cnext};
+	uint32_t next_task = task0, next_word = word0, p = 1, n = 0;
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
-		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-		walk_record(o, pr.Ls, pr.Lq, keys, ev);
+		ChainKeys<SeedAt, I32Arr> keys(sa, nx, g.cfirst[c], g.clast[c]);
+		ScriptWriter w{prog + p, &pr, tasks, next_task, next_word, 0, 0, 0, 0, 0};
+		walk_record(o, pr.Ls, pr.Lq, keys, w);
+		w.head();
+		next_task = w.next_task, next_word = w.next_word;
+		p += (uint32_t) (kScriptHead + w.nops), n++;
 	}
+	prog[0] = n;
 }
 
 // ---- DP results ---------------------------------------------------------------------------
@@ -185,76 +220,84 @@ HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 }
 
 // ---- score --------------------------------------------------------------------------------
-struct ScoreV {
-	double mismatch_rate, indel_rate; // held by value: the visitor then lives in registers
-	DpResults R;
-	uint32_t task; // next task of this record
-	Score sc;
-	double rate_sum, len;
-	uint64_t cells_exec; // cells of the DP calls this record actually consumed
-	int calls_exec;
-	bool finished; // reached the final test (every DP call of the record was consumed)
-	HD void start(int) {}
-	HD void add(int c, int t) { sc.add(c, t); }
-	HD void dp(const DpCall &c) {
-		sc.add_piece(R.sums[task++], c.swapped);
-		cells_exec += band_cells(c.lenS, c.lenQ, c.band);
-		calls_exec++;
+// One script replayed against the DP summaries (getAlignment2's checks in order, then the rates):
+// true when the record is accepted. cells / calls count the DP calls the reference executes.
+HD bool score_script(const uint32_t *sc, const DpResults &R, uint32_t task, double mismatch_rate, double indel_rate,
+		double &rate_sum, double &len, uint64_t &cells, uint64_t &calls) {
+	const int nops = script_nops(sc);
+	if (nops == 0) return false;
+	const int64_t max_mis = sc[1], max_indel = sc[2];
+	Score s;
+	s.reset();
+	for (int i = 0; i < nops; i++) {
+		const uint32_t w = sc[kScriptHead + i];
+		const uint32_t kind = w >> 30;
+		if (kind == kScAdd) {
+			s.add((int) (w & 0x7FFFFFFu), (int) ((w >> 27) & 7u));
+		} else if (kind == kScDp) {
+			s.add_piece(R.sums[task++], ((w >> 29) & 1u) != 0);
+			cells += w & 0x1FFFFFFFu;
+			calls++;
+		} else if (kind == kScCheck) {
+			if (!s.good_caps(max_mis, max_indel)) return false;
+		} else {
+			return s.good_rates(mismatch_rate, indel_rate, rate_sum, len);
+		}
 	}
-	HD bool check(int64_t m, int64_t i) { return sc.good_caps(m, i); }
-	HD bool finish() {
-		finished = true;
-		return sc.good_rates(mismatch_rate, indel_rate, rate_sum, len);
-	}
-};
+	return false;
+}
 
 // Best record of one (subject, query) pair so far: the first record with the lowest rate sum,
 // then the longest alignment (SubjectRead::addAlignment, src/SubjectRead.cpp:25-79).
 struct Best {
 	bool have;
 	double rate_sum, len;
-	int strand, chain;
-	uint32_t task0; // first DP task of the winning record
+	int strand;
+	uint32_t script; // winning record's script (index into the program words)
+	uint32_t task0;  // its first DP task
 };
 
-// Scores every best-length chain of a group in index order (processOneSubjectRead2,
-// src/SubjectReadAligner.cpp:838-849); accept(chain, first task, rate sum, length) sees every
+// Scores the scripts of one group (every best-length chain, in chain order:
+// processOneSubjectRead2, src/SubjectReadAligner.cpp:838-849) starting at prog[gp];
+// accept(script index in the group, script offset, first task, rate sum, length) sees every
 // accepted record.
 struct NoAccept {
-	HD void operator()(int, uint32_t, double, double) const {}
+	HD void operator()(int, uint32_t, uint32_t, double, double) const {}
 };
 template <class F = NoAccept>
-HD void score_group(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, uint32_t task0, int strand,
+HD void score_group(const Opts &o, const uint32_t *prog, uint32_t gp, const DpResults &R, uint32_t task0, int strand,
 		Best &best, uint64_t &cells_exec, uint64_t &calls_exec, uint64_t &built, F &&accept = F()) {
-	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid};
-	uint32_t task = task0;
-	for (int c = 0; c < g.nch; c++) {
-		if (g.clen[c] != g.best) continue;
-		ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, c, g.clast[c]);
-		ScoreV sv{o.mismatch_rate, o.indel_rate, R, task, Score(), 0.0, 0.0, 0, 0, false};
-		sv.sc.reset();
-		const bool ok = walk_record(o, pr.Ls, pr.Lq, keys, sv);
-		cells_exec += sv.cells_exec;
-		calls_exec += (uint64_t) sv.calls_exec;
-		if (ok) {
+	const uint32_t n = prog[gp];
+	uint32_t p = gp + 1, task = task0;
+	for (uint32_t k = 0; k < n; k++) {
+		const uint32_t *sc = prog + p;
+		double rate_sum = 0, len = 0;
+		if (score_script(sc, R, task, o.mismatch_rate, o.indel_rate, rate_sum, len, cells_exec, calls_exec)) {
 			built++;
-			accept(c, task, sv.rate_sum, sv.len);
-			if (!best.have || record_better(sv.rate_sum, sv.len, best.rate_sum, best.len)) {
-				best.have = true, best.rate_sum = sv.rate_sum, best.len = sv.len;
-				best.strand = strand, best.chain = c, best.task0 = task;
+			accept((int) k, p, task, rate_sum, len);
+			if (!best.have || record_better(rate_sum, len, best.rate_sum, best.len)) {
+				best.have = true, best.rate_sum = rate_sum, best.len = len;
+				best.strand = strand, best.script = p, best.task0 = task;
 			}
 		}
-		// an early exit leaves this record's later DP tasks unread: count all of them to move on
-		uint32_t ntask = sv.task - task;
-		if (!sv.finished) {
-			ChainKeys<SeedAt, I32Arr> k2(sa, cid, g.gn, c, g.clast[c]);
-			CountDp cv;
-			walk_record(o, pr.Ls, pr.Lq, k2, cv);
-			ntask = (uint32_t) cv.n;
-		}
-		task += ntask;
+		task += (uint32_t) script_ndp(sc);
+		p += (uint32_t) script_words(sc);
 	}
+}
+
+// Replays an accepted script through a visitor: start(pos), add(count, type) for fixed ops,
+// run(task, swapped) for each DP call, finish().
+template <class V>
+HD void replay_script(const uint32_t *sc, uint32_t task, V &v) {
+	const int nops = script_nops(sc);
+	v.start((int) sc[3]);
+	for (int i = 0; i < nops; i++) {
+		const uint32_t w = sc[kScriptHead + i];
+		const uint32_t kind = w >> 30;
+		if (kind == kScAdd) v.add((int) (w & 0x7FFFFFFu), (int) ((w >> 27) & 7u));
+		else if (kind == kScDp) v.run(task++, ((w >> 29) & 1u) != 0);
+	}
+	v.finish();
 }
 
 // ---- votes --------------------------------------------------------------------------------
@@ -300,7 +343,6 @@ struct VoteSink {
 struct VoteV {
 	const DpResults &R;
 	const VoteSink &sink;
-	uint32_t task;
 	int ps, pq;       // record frames (iPosOnSubjectRead, iPosOnQueryRead)
 	int run_p, run_c; // pending M run
 	HD void flush() {
@@ -323,21 +365,13 @@ struct VoteV {
 	}
 	HD void start(int pos) { ps = pos, pq = 0; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void dp(const DpCall &c) { feed_runs(R, task++, c.swapped, [&](int cnt, int t) { op(cnt, t); }); }
-	HD bool check(int64_t, int64_t) { return true; }
-	HD bool finish() {
-		flush();
-		return true;
-	}
+	HD void run(uint32_t task, bool swapped) { feed_runs(R, task, swapped, [&](int cnt, int t) { op(cnt, t); }); }
+	HD void finish() { flush(); }
 };
 
-HD void vote_record(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, int chain, uint32_t task0,
-		const VoteSink &sink) {
-	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid};
-	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
-	VoteV vv{R, sink, task0, 0, 0, 0, 0};
-	walk_record(o, pr.Ls, pr.Lq, keys, vv);
+HD void vote_record(const uint32_t *script, const DpResults &R, uint32_t task0, const VoteSink &sink) {
+	VoteV vv{R, sink, 0, 0, 0, 0};
+	replay_script(script, task0, vv);
 }
 
 // ---- records (without -c) -------------------------------------------------------------------
@@ -347,7 +381,6 @@ HD void vote_record(const Opts &o, const Group &g, const PairReads &pr, const Dp
 struct RecordV {
 	const DpResults &R;
 	PairReads pr;
-	uint32_t task;
 	uint32_t *out; // null: count only
 	int n;         // ops emitted
 	int ps, pq, pos;
@@ -378,24 +411,19 @@ struct RecordV {
 	}
 	HD void start(int p) { ps = p, pq = 0, pos = p; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void dp(const DpCall &c) { feed_runs(R, task++, c.swapped, [&](int cnt, int t) { op(cnt, t); }); }
-	HD bool check(int64_t, int64_t) { return true; }
-	HD bool finish() {
+	HD void run(uint32_t task, bool swapped) { feed_runs(R, task, swapped, [&](int cnt, int t) { op(cnt, t); }); }
+	HD void finish() {
 		if (pending) emit();
 		pending = false;
-		return true;
 	}
 };
 
-// Ops of one accepted record (chain `chain` of group g, DP results from task0): returns the op
-// count and writes them to out unless it is null; *pos receives iPositionOfQueryOnSubject.
-HD int record_ops(const Opts &o, const Group &g, const PairReads &pr, const DpResults &R, int chain, uint32_t task0,
-		uint32_t *out, int *pos) {
-	SeedAt sa{g.seeds};
-	I32Arr cid{g.cid};
-	ChainKeys<SeedAt, I32Arr> keys(sa, cid, g.gn, chain, g.clast[chain]);
-	RecordV rv{R, pr, task0, out, 0, 0, 0, 0, false, 0, 0};
-	walk_record(o, pr.Ls, pr.Lq, keys, rv);
+// Ops of one accepted record (its script, DP results from task0): returns the op count and
+// writes them to out unless it is null; *pos receives iPositionOfQueryOnSubject.
+HD int record_ops(const uint32_t *script, const PairReads &pr, const DpResults &R, uint32_t task0, uint32_t *out,
+		int *pos) {
+	RecordV rv{R, pr, out, 0, 0, 0, 0, false, 0, 0};
+	replay_script(script, task0, rv);
 	*pos = rv.pos;
 	return rv.n;
 }

@@ -37,12 +37,12 @@ HD bool same_alignment(const Opts &o, int add_pq, int add_ps, int beg_pq, int be
 // Greedy first-fit chaining of one query group (processOneSubjectRead2, :817-837) and the
 // covered-length score of each chain (calKeyLength, :711-727). `seed(t)` returns the packed
 // seed t of the group (sorted). Chain state lives in caller storage of group size:
-//   cid[t]   chain of seed t
+//   cnext[t] the next seed of t's chain (-1: t is the chain's last)
 //   cfirst[c], clast[c] first / last seed of chain c, clen[c] covered length.
 // Returns the chain count; *best receives the maximum covered length (markTheBestPossible-
 // Alignment, :729-751, keeps exactly the chains with that length, ties included).
 template <class FSeed, class FI, class FL>
-HD int chain_group(const Opts &o, int gn, const FSeed &seed, FI &cid, FI &cfirst, FI &clast, FL &clen, int *best) {
+HD int chain_group(const Opts &o, int gn, const FSeed &seed, FI &cnext, FI &cfirst, FI &clast, FL &clen, int *best) {
 	int nch = 0;
 	for (int t = 0; t < gn; t++) {
 		const uint64_t s = seed(t);
@@ -58,13 +58,14 @@ HD int chain_group(const Opts &o, int gn, const FSeed &seed, FI &cid, FI &cfirst
 		if (placed >= 0) {
 			const int eps = seed_ps(seed(clast[placed]));
 			clen[placed] += (eps + o.k <= ps) ? o.k : (ps - eps);
+			cnext[clast[placed]] = t;
+			cnext[t] = -1;
 			clast[placed] = t;
-			cid[t] = placed;
 		} else {
 			cfirst[nch] = t;
 			clast[nch] = t;
 			clen[nch] = 0;
-			cid[t] = nch;
+			cnext[t] = -1;
 			nch++;
 		}
 	}
@@ -78,18 +79,18 @@ HD int chain_group(const Opts &o, int gn, const FSeed &seed, FI &cid, FI &cfirst
 	return nch;
 }
 
-// Sequential access to the keys of one chain: seeds t of the group with cid[t] == c, in order.
+// Sequential access to the keys of one chain, following the chain links from its first seed.
 template <class FSeed, class FI>
 struct ChainKeys {
 	const FSeed &seed;
-	const FI &cid;
-	int gn, c, t;
+	const FI &cnext;
+	int t;
 	int last_t;
-	HD ChainKeys(const FSeed &s, const FI &id, int n, int chain, int last) : seed(s), cid(id), gn(n), c(chain), t(0), last_t(last) {}
+	HD ChainKeys(const FSeed &s, const FI &nx, int first, int last) : seed(s), cnext(nx), t(first), last_t(last) {}
 	HD bool next(int &pq, int &ps) {
-		while (t < gn && cid[t] != c) t++;
-		if (t >= gn) return false;
-		const uint64_t s = seed(t++);
+		if (t < 0) return false;
+		const uint64_t s = seed(t);
+		t = cnext[t];
 		pq = seed_pq(s), ps = seed_ps(s);
 		return true;
 	}

@@ -136,10 +136,11 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 		struct StrandData {
 			std::vector<uint64_t> seeds;
 			std::vector<int> gstart; // group starts (+ end)
-			std::vector<int32_t> cid, cf, cl, clen;
+			std::vector<int32_t> cnext, cf, cl, clen;
 			std::vector<Group> groups;
-			std::vector<uint32_t> gtask0;
+			std::vector<uint32_t> gtask0, gprog;
 		} sd[2];
+		std::vector<uint32_t> prog;
 		std::vector<DpTask> tasks;
 		std::vector<uint32_t> words_nsteps;
 		uint32_t words_total = 0;
@@ -165,7 +166,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 			std::sort(D.seeds.begin(), D.seeds.end());
 			st_seeds += D.seeds.size();
 			const int n = (int) D.seeds.size();
-			D.cid.assign(n, 0), D.cf.assign(n, 0), D.cl.assign(n, 0), D.clen.assign(n, 0);
+			D.cnext.assign(n, 0), D.cf.assign(n, 0), D.cl.assign(n, 0), D.clen.assign(n, 0);
 			for (int t = 0; t < n; t++)
 				if (t == 0 || seed_qid(D.seeds[t]) != seed_qid(D.seeds[t - 1])) D.gstart.push_back(t);
 			if (o.max_candidates > 0 && (int) D.gstart.size() > o.max_candidates) D.gstart.resize(o.max_candidates);
@@ -176,19 +177,21 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 				while (g1 < n && seed_qid(D.seeds[g1]) == seed_qid(D.seeds[g0])) g1++;
 				Group g;
 				g.seeds = D.seeds.data() + g0;
-				g.cid = D.cid.data() + g0, g.cfirst = D.cf.data() + g0, g.clast = D.cl.data() + g0, g.clen = D.clen.data() + g0;
+				g.cnext = D.cnext.data() + g0, g.cfirst = D.cf.data() + g0, g.clast = D.cl.data() + g0, g.clen = D.clen.data() + g0;
 				g.gn = g1 - g0;
 				chain(o, g);
 				const int64_t q = seed_qid(D.seeds[g0]);
 				PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, strand);
-				int nt, nw;
-				plan_count(o, g, pr, nt, nw);
-				const uint32_t t0 = (uint32_t) tasks.size();
+				int nt, nw, np, nsc;
+				plan_count(o, g, pr, nt, nw, np, nsc);
+				const uint32_t t0 = (uint32_t) tasks.size(), p0 = (uint32_t) prog.size();
 				tasks.resize(tasks.size() + nt);
-				plan_emit(o, g, pr, tasks.data(), t0, words_total);
+				prog.resize(prog.size() + np);
+				plan_emit(o, g, pr, tasks.data(), t0, words_total, prog.data() + p0);
 				words_total += (uint32_t) nw;
 				D.groups.push_back(g);
 				D.gtask0.push_back(t0);
+				D.gprog.push_back(p0);
 			}
 		}
 		// DP
@@ -233,16 +236,15 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 			for (int strand = 0; strand < 2; strand++) {
 				const int gi = strand == 0 ? kv.second.first : kv.second.second;
 				if (gi < 0) continue;
-				PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, strand);
-				score_group(o, sd[strand].groups[gi], pr, R, sd[strand].gtask0[gi], strand, best, st_cells_exec, st_calls_exec, st_built);
+				score_group(o, prog.data(), sd[strand].gprog[gi], R, sd[strand].gtask0[gi], strand, best, st_cells_exec,
+						st_calls_exec, st_built);
 			}
 			if (!best.have) continue;
 			st_kept++;
 			if (!o.call_consensus) continue;
 			PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, best.strand);
-			const int gi = best.strand == 0 ? kv.second.first : kv.second.second;
 			VoteSink sink{tally.data() + 4 * qr->offset[q], pr, &oob};
-			vote_record(o, sd[best.strand].groups[gi], pr, R, best.chain, best.task0, sink);
+			vote_record(prog.data() + best.script, R, best.task0, sink);
 		}
 	}
 	// consensus
