@@ -210,6 +210,7 @@ struct hero_ctx {
 	bool q_contiguous = true; // caller offsets are the back-to-back prefix sums of the lengths
 	int64_t q_bases = 0;
 	DevBuf<uint8_t> cons_out;
+	bool cons_valid = false; // cons_out holds the consensus of the current tallies
 	ReadsDev Qd;
 	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
 	// subjects (separate stream)
@@ -791,6 +792,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 
 int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int64_t *s_uid) {
 	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
+	c->cons_valid = false;
 	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
 	for (int64_t s0 = 0; s0 < n; s0 += chunk) {
 		const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
@@ -981,6 +983,7 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	HIPCHK(c, c->tally.exact((size_t) 4 * qb));
 	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 16 * qb));
 	c->qix.built = false;
+	c->cons_valid = false;
 	c->st_host.queries = (uint64_t) n;
 	return HERO_OK;
 }
@@ -991,6 +994,7 @@ int hero_build_index(hero_ctx *c) {
 	// a new index starts a new correction run: clear the vote tallies
 	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 16 * c->q_bases, c->w.st));
 	const uint64_t P = next_prime_above((uint64_t) c->Qd.n * (uint64_t) c->opt.hash_factor);
+	c->cons_valid = false;
 	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
 	c->st_host.table_size = c->qix.P;
 	c->st_host.total_keys = c->qix.total_keys;
@@ -1030,7 +1034,7 @@ int hero_take_alignments(hero_ctx *c, char *out, int64_t cap, int64_t *len) {
 }
 
 int hero_call_consensus(hero_ctx *c, char *corrected) {
-	if (!c || !corrected) return HERO_ERR_ARG;
+	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	const int64_t n = c->Qd.n;
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
@@ -1038,7 +1042,20 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, n, c->cons_out.p, nullptr,
 			c->w.dstats.p, c->w.st));
 	HIPCHK(c, hipEventRecord(c->w.ev[6].a, c->w.st));
-	// The device writes corrected bases at packed base offsets; the caller's layout usually is that
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
+	float ms = 0;
+	(void) hipEventElapsedTime(&ms, c->w.ev[5].a, c->w.ev[6].a);
+	c->st_host.ms_consensus += ms;
+	c->cons_valid = true;
+	return corrected ? hero_copy_corrected(c, corrected) : HERO_OK;
+}
+
+int hero_copy_corrected(hero_ctx *c, char *corrected) {
+	if (!c || !corrected) return HERO_ERR_ARG;
+	if (!c->cons_valid) return fail(c, HERO_ERR_STATE, "hero_call_consensus has not run on the current tallies");
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	const int64_t n = c->Qd.n;
+	// The device holds corrected bases at packed base offsets; the caller's layout usually is that
 	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
 	if (c->q_contiguous) {
 		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
@@ -1057,9 +1074,6 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 			b += (uint64_t) L;
 		}
 	}
-	float ms = 0;
-	(void) hipEventElapsedTime(&ms, c->w.ev[5].a, c->w.ev[6].a);
-	c->st_host.ms_consensus += ms;
 	return HERO_OK;
 }
 

@@ -1229,36 +1229,51 @@ hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStr
 // Consensus (QueryDataset::callConsensus, src/QueryDataset.cpp:393-412)
 // ---------------------------------------------------------------------------------------------
 // Writes the corrected bases of good reads as ASCII at their base offsets.
-__global__ void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0, int64_t nq,
-		uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats) {
-	const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+// Consensus, one wavefront per read: lanes take 64 positions at a time, the own-base difference
+// array is decoded by a wave-wide inclusive scan (wrapping u32 sums) with a carry between rounds,
+// then every position decides independently (consensus_base).
+__global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally,
+		int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats) {
+	const int64_t i = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int lane = lane_id();
 	if (i >= nq) return;
 	const int64_t q = q0 + i;
 	const uint64_t b = q_boff[q];
 	const int L = Q.len[q];
-	if (!Q.good[q]) { // not corrected (src/QueryDataset.cpp:404-406); the host copies its bytes
-		if (decoded_out) {
-			uint32_t run = 0;
-			for (int p = 0; p < L; p++) {
-				const int own = strand_base(Q, q, p, false);
-				for (int j = 0; j < 4; j++) decoded_out[4 * (b + p) + j] = tally[4 * (b + p) + j];
-				run += tally[4 * (b + p) + own];
-				decoded_out[4 * (b + p) + own] = run;
-			}
+	const bool good = Q.good[q];
+	const uint64_t *w = Q.words + Q.woff[q];
+	uint32_t carry = 0, changed = 0;
+	for (int p0 = 0; p0 < L; p0 += 64) {
+		const int p = p0 + lane;
+		uint32_t c[4] = {0, 0, 0, 0};
+		int own = 0;
+		if (p < L) {
+			const uint4 t = *(const uint4 *) (tally + 4 * (b + p));
+			c[0] = t.x, c[1] = t.y, c[2] = t.z, c[3] = t.w;
+			own = base_at(w, p);
 		}
-		return;
+		uint32_t x = c[0];
+		x = own == 1 ? c[1] : x;
+		x = own == 2 ? c[2] : x;
+		x = own == 3 ? c[3] : x;
+		const uint32_t run = carry + wave_excl_scan(x) + x;
+		carry = __shfl(run, 63, 64);
+		if (p >= L) continue;
+		c[own] = run;
+		if (decoded_out) *(uint4 *) (decoded_out + 4 * (b + p)) = make_uint4(c[0], c[1], c[2], c[3]);
+		if (!good) continue; // not corrected (src/QueryDataset.cpp:404-406); the host copies its bytes
+		c[own] += 1;         // self vote (src/QueryRead.cpp:149)
+		const int outc = consensus_base(o, c, own);
+		changed += outc != own ? 1u : 0u;
+		codes_out[b + p] = (uint8_t) code_ascii(outc);
 	}
-	const int changed = consensus_read(o, tally + 4 * b, L, Q, q, codes_out + b, decoded_out ? decoded_out + 4 * b : nullptr);
-	for (int p = 0; p < L; p++) codes_out[b + p] = (uint8_t) code_ascii(codes_out[b + p]);
 	const uint64_t m = __ballot(changed != 0);
-	if (stats && m && lane_id() == (int) __ffsll((unsigned long long) m) - 1)
-		stat_add(stats, kStReadsChanged, (unsigned long long) __popcll(m));
+	if (stats && good && lane == 0 && m) stat_add(stats, kStReadsChanged, 1ull);
 }
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
 		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st) {
 	if (nq == 0) return hipSuccess;
-	k_consensus<<<(unsigned) ((nq + 255) / 256), 256, 0, st>>>(Q, o, q_boff, tally, q0, nq, codes_out, decoded_out,
-			stats);
+	k_consensus<<<(unsigned) ((nq + 3) / 4), 256, 0, st>>>(Q, o, q_boff, tally, q0, nq, codes_out, decoded_out, stats);
 	return hipGetLastError();
 }
 

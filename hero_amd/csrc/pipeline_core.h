@@ -432,6 +432,19 @@ HD int record_ops(const uint32_t *script, const PairReads &pr, const DpResults &
 // QueryRead::callConsensus (src/QueryRead.cpp:148-180) over a difference-coded tally: the own
 // base's count is the running sum of its slot plus the self vote; aDNA = {A,C,T,G} (:11) is
 // exactly the code order. Writes the corrected read as codes; returns changed bases.
+// One position: counts c (own base already decoded and self-voted) -> the consensus code.
+HD int consensus_base(const Opts &o, const uint32_t c[4], int own) {
+	double sum = 0;
+	for (int j = 0; j < 4; j++) sum += c[j];
+	int id = -1;
+	double largest = 0, larger = 0;
+	for (int j = 0; j < 4; j++) {
+		const double f = ((double) c[j]) / sum;
+		if (largest < f) larger = largest, largest = f, id = j;
+	}
+	return (largest - larger >= o.consensus_freq) ? id : own;
+}
+
 HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed &Q, int64_t r, uint8_t *out_codes,
 		uint32_t *decoded /* optional 4*L, pre-self-vote counts */) {
 	uint32_t run = 0;
@@ -446,16 +459,7 @@ HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed 
 		if (decoded)
 			for (int j = 0; j < 4; j++) decoded[4 * p + j] = c[j];
 		c[own] += 1; // self vote (:149)
-		double sum = 0;
-		for (int j = 0; j < 4; j++) sum += c[j];
-		int id = -1;
-		double largest = 0, larger = 0;
-		for (int j = 0; j < 4; j++) {
-			const double f = ((double) c[j]) / sum;
-			if (largest < f) larger = largest, largest = f, id = j;
-		}
-		int outc = own;
-		if (largest - larger >= o.consensus_freq) outc = id;
+		const int outc = consensus_base(o, c, own);
 		if (outc != own) changed++;
 		out_codes[p] = (uint8_t) outc;
 	}

@@ -71,6 +71,7 @@ SIGNATURES = {
     "hero_align_subjects": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
     "hero_align_self": (ctypes.c_int, [_P]),
     "hero_call_consensus": (ctypes.c_int, [_P, _P]),
+    "hero_copy_corrected": (ctypes.c_int, [_P, _P]),
     "hero_take_alignments": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "hero_get_tally": (ctypes.c_int, [_P, _P]),
     "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
@@ -181,9 +182,18 @@ class Context:
                                              rs.length.ctypes.data, rs.unique_id.ctypes.data, out.ctypes.data))
         return out
 
-    def consensus(self):
+    def consensus(self, device_only=False):
+        """Corrected reads (bytes in the layout of the query seq); device_only keeps them on the GPU."""
+        if device_only:
+            self._check(lib().hero_call_consensus(self.h, None))
+            return None
         out = np.empty_like(self._queries.seq)
         self._check(lib().hero_call_consensus(self.h, out.ctypes.data))
+        return out
+
+    def copy_corrected(self):
+        out = np.empty_like(self._queries.seq)
+        self._check(lib().hero_copy_corrected(self.h, out.ctypes.data))
         return out
 
     def take_alignments(self):

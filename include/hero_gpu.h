@@ -99,8 +99,10 @@ int hero_take_alignments(hero_ctx *ctx, char *out, int64_t cap, int64_t *len);
 
 /* QueryDataset::callConsensus (src/QueryDataset.cpp:393-412, QueryRead.cpp:148-180): writes the
  * corrected reads in the layout of hero_set_queries' `seq` (reads that are not good are copied
- * unchanged, as the reference writes them). */
+ * unchanged, as the reference writes them). With corrected == NULL the corrected reads stay in
+ * device memory; hero_copy_corrected fetches them. */
 int hero_call_consensus(hero_ctx *ctx, char *corrected);
+int hero_copy_corrected(hero_ctx *ctx, char *corrected);
 
 /* The vote matrices (QueryRead::aCoverageMatrix, 4 u32 per base in A,C,T,G order) as they stand
  * after the subject passes, before consensus adds the self vote. */

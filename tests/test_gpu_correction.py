@@ -148,3 +148,20 @@ def test_alignment_records_match_oracle(extra):
         aln = ctx.take_alignments()
     ro = O.run(O.make_cfg(call_consensus=0, **opts), q, s, threads=8, want_aln=True)
     assert aln.decode() == ro["aln"]
+
+
+def test_device_resident_consensus_then_copy():
+    """Consensus kept in HBM and copied afterwards equals the one-call form; a new index invalidates it."""
+    rs = synth.generate(seed=61, genome_len=8000, len_min=100, len_max=140, coverage=25.0, sub_5p=0.01)
+    opts = dict(call_consensus=1, k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02, min_overlap=40)
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        assert ctx.consensus(device_only=True) is None
+        a = ctx.copy_corrected()
+        b = ctx.consensus()
+        assert np.array_equal(a, b)
+        ctx.build_index()
+        with pytest.raises(gpu.HeroError):
+            ctx.align_self() or ctx.copy_corrected()
