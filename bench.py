@@ -72,6 +72,17 @@ def gcups_of(st):
     return st["dp_cells_exec"] / (st["ms_dp"] / 1000.0) / 1e9 if st["ms_dp"] > 0 else 0.0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed counter summary
+    (profiles/c2_hbm_traffic.json: FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.json")
+    try:
+        k = json.load(open(path))["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return int((k["fetch_bytes"] + k["write_bytes"]) / max(k["launches"], 1))
+
+
 def one_form(rs, opts, dp_form):
     with gpu.Context(dp_form=dp_form, **opts) as ctx:
         ctx.set_queries(rs)
@@ -185,7 +196,7 @@ def _drive(srv, batches, arrivals):
 def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_reads=16384, sweep_rates=C4_RATES,
                     prepared=None):
     """Serving form on the resident-subject workload: offline throughput of the same path, a fixed
-    4096-read batch stream at 85% of that offline rate, and Poisson sweeps with log-uniform batch
+    4096-read batch stream at 82% of that offline rate, and Poisson sweeps with log-uniform batch
     sizes in [1024, 65536]."""
     if prepared is None:
         prepared = prepare_serving()
@@ -209,12 +220,15 @@ def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_r
                index_build_s=round(index_s, 3), offline_reads_per_s=round(offline, 1), workers=workers,
                max_step_reads=max_step_reads)
     with gpu.Server(ctx, workers=workers, max_step_reads=max_step_reads) as srv:
-        # fixed 4096-read batches at 85% of the offline rate
-        rate = 0.85 * offline / 4096
+        # fixed 4096-read batches at 82% of the offline rate
+        rate = 0.82 * offline / 4096
         nb = max(20, int(rate * fixed_duration))
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nb))
         batches = [batch(4096, j * 4096) for j in range(nb)]
-        srv.wait(srv.submit(batches[0]))  # one untimed batch warms the workers' buffers
+        # untimed: full-size steps on every worker grow their buffers once
+        warm = [srv.submit(batches[j % nb]) for j in range(4 * workers * max(1, max_step_reads // 4096))]
+        for t in warm:
+            srv.wait(t)
         lat, rps, merged = _drive(srv, batches, arrivals)
         out["fixed_4k"] = dict(rate_batches_per_s=round(rate, 2), batches=nb, p50_ms=round(1e3 * _percentile(lat, 50), 3),
                                p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
@@ -311,15 +325,16 @@ def main():
     achieved = cells_all * AFFINE_OPS_PER_CELL / (dp_ms / 1000.0) / 1e12
     roofline = dict(bound="valu", kernel="k_dp_batched (all band classes)", achieved=round(achieved, 3),
                     peak=round(VALU_PEAK_TOPS, 2), unit="Tops/s (int32 VALU)", frac=round(achieved / VALU_PEAK_TOPS, 4),
-                    traffic=None, ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all),
-                    dp_ms_per_step=round(dp_ms, 3))
+                    traffic=measured_traffic("hero::k_dp_batched<8, true>"), traffic_unit="HBM bytes per launch",
+                    ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all), dp_ms_per_step=round(dp_ms, 3))
     # seed probes: packed subject bases + 24 B per window probe (k > 32) + 8 B per bucket entry read
     probe_ms = st["ms_seed_probe"] / steps
     bases = int(rs.length.astype(np.int64).sum())
     probe_bytes = bases / 4 + (st["lookups"] / steps) * 24 + (st["seeds"] / steps) * 8
     probe_gbs = probe_bytes / (probe_ms / 1000.0) / 1e9 if probe_ms > 0 else 0.0
     roofline_probe = dict(bound="hbm", kernel="k_seed_count", achieved=round(probe_gbs, 1), peak=HBM_PEAK_GBS,
-                          unit="GB/s", frac=round(probe_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                          unit="GB/s", frac=round(probe_gbs / HBM_PEAK_GBS, 4),
+                          traffic=measured_traffic("hero::k_seed_count"), traffic_unit="HBM bytes per launch",
                           bytes_per_step=int(probe_bytes), ms_per_step=round(probe_ms, 3))
 
     gcups = dict(batched=round(gcups_batched, 2))
