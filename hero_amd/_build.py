@@ -60,14 +60,22 @@ def build_gpu(jobs=4):
     os.replace(out + ".tmp", out)
 
 
-def build_cli():
-    out = os.path.join(LIB, "hero_cli")
-    src = os.path.join(CSRC, "hero_cli.cpp")
-    if not os.path.exists(src):
-        return
-    _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-o", out + ".tmp", src, f"-I{os.path.join(ROOT, 'include')}",
-          f"-L{LIB}", "-lhero_gpu", f"-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64"])
+ROCM_LIB = "/opt/rocm/lib"
+
+
+def build_host():
+    """C++ host classes with the reference's class shapes (libhero_host.so over libhero_gpu.so)."""
+    out = os.path.join(LIB, "libhero_host.so")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-o", out + ".tmp",
+          os.path.join(ROOT, "hero_amd", "host", "hero_host.cpp"), f"-L{LIB}", "-lhero_gpu", "-Wl,-rpath,$ORIGIN",
+          f"-Wl,-rpath-link,{ROCM_LIB}"])
     os.replace(out + ".tmp", out)
+    # the test driver of the host classes
+    exe = os.path.join(ROOT, "tests", "native", "host_shim_main")
+    _run(["g++", "-std=c++17", "-O2", "-o", exe + ".tmp", os.path.join(ROOT, "tests", "native", "host_shim_main.cpp"),
+          f"-L{LIB}", "-lhero_host", "-lhero_gpu", "-Wl,-rpath,$ORIGIN/../../hero_amd/_lib",
+          f"-Wl,-rpath-link,{ROCM_LIB}"])
+    os.replace(exe + ".tmp", exe)
 
 
 def build_oracle():
@@ -90,7 +98,7 @@ def build_hostsim():
 def build_all():
     build_synth()
     build_gpu()
-    build_cli()
+    build_host()
     build_oracle()
     build_hostsim()
     build_reference()

@@ -1,0 +1,57 @@
+"""The C++ host classes with the reference's shapes (hero_amd/host/hero_host.h), driven like the
+reference's main.cpp: corrected FASTA and .aln bytes equal the reference's, BandAligner calls equal
+the reference's CIGARs and widened bands."""
+import os
+import subprocess
+
+import pytest
+
+from helpers import load_e2e, load_jsonl, sha256
+from hero_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "native", "host_shim_main")
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not os.path.exists(DRIVER), reason="host_shim_main is not built")]
+
+
+def _flag(flags, name):
+    return flags[flags.index(name) + 1]
+
+
+def _run(mode, case, tmp_path, chunk=0):
+    rs = synth.generate(**case["synth"])
+    fa = str(tmp_path / "reads.fasta")
+    rs.write_fasta(fa)
+    out = str(tmp_path / "out")
+    f = case["flags"]
+    cmd = [DRIVER, mode, fa, _flag(f, "-k"), _flag(f, "-m"), _flag(f, "-i"), _flag(f, "-l"), _flag(f, "-a"), out]
+    if chunk:
+        cmd.append(str(chunk))
+    subprocess.run(cmd, check=True, timeout=120)
+    return open(out, "rb").read()
+
+
+@pytest.mark.parametrize("case", load_e2e()[:3], ids=lambda c: c["name"])
+def test_corrected_fasta_through_host_classes(case, tmp_path):
+    assert sha256(_run("correct", case, tmp_path)) == case["fasta_sha256"]
+
+
+def test_alignment_records_through_host_classes_in_subject_chunks(tmp_path):
+    case = load_e2e()[2]
+    out = _run("aln", case, tmp_path, chunk=400)  # SubjectDataset chunks of the reference's default size
+    assert len(out) == case["aln_bytes"] and sha256(out) == case["aln_sha256"]
+
+
+@pytest.mark.parametrize("aligner", [1, 0], ids=["global", "affine"])
+def test_band_aligner_protocol(aligner, tmp_path):
+    cases = [c for c in load_jsonl("dp_cases.jsonl") if c["aligner"] == aligner][:400]
+    inp = tmp_path / "cases.txt"
+    inp.write_text("".join(f"{c['S'] or '-'} {c['Q'] or '-'} {c['band']} {c['flag']}\n" for c in cases))
+    out = tmp_path / "out.txt"
+    subprocess.run([DRIVER, "band", "a" if aligner == 0 else "g", str(inp), str(out)], check=True, timeout=120)
+    got = out.read_text().splitlines()
+    assert len(got) == len(cases)
+    for c, line in zip(cases, got):
+        band, cigar = (line.split(" ") + [""])[:2]
+        assert (int(band), cigar) == (c["out_band"], c["cigar"]), c
