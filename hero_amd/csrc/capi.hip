@@ -229,6 +229,10 @@ struct hero_ctx {
 	std::string aln;
 	// the context's own pipeline (stream, events, chunk buffers)
 	Worker w;
+	// autotuned DP schedule (hero_autotune): tasks per lane and persistent waves per CU per band class
+	int tuned_per_lane[kNumBandClasses] = {1, 1, 1, 1, 1, 1};
+	int tuned_waves_per_cu[kNumBandClasses] = {0, 0, 0, 0, 0, 0};
+	hero_tune_report *tune = nullptr; // set while hero_autotune runs
 	// stats
 	hero_stats st_host;
 };
@@ -335,6 +339,45 @@ int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, 
 	return HERO_OK;
 }
 
+// Autotuner step for one band class: runs the class's tasks under every candidate schedule (one
+// or two tasks per lane; persistent waves per CU from 1 to the occupancy limit), each timed with
+// HIP events after an untimed run, and keeps the fastest. Every candidate computes the same
+// results; only the schedule differs.
+int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool aff, int rows, int nw) {
+	hero_tune_report &rep = *c->tune;
+	hero_tune_class &tc = rep.cls[cls];
+	tc.band = bm, tc.tasks = (int64_t) Lc.ntasks;
+	float best = -1.0f;
+	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand ? 2 : 1); per_lane++) {
+		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane);
+		for (int wpc = 1; wpc <= occ; wpc *= 2) {
+			const int wpc_eff = wpc * 2 > occ ? occ : wpc; // the last candidate is the occupancy limit
+			const int waves = (int) std::min<uint64_t>((uint64_t) c->cu_count * wpc_eff,
+					(Lc.ntasks + 64 * per_lane - 1) / (64 * per_lane));
+			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64 * per_lane));
+			DpLaunch L2 = Lc;
+			L2.scratch = w->dp_scratch.p;
+			float ms = 0;
+			for (int rep_i = 0; rep_i < 2; rep_i++) {
+				HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
+				HIPCHK(c, hipEventRecord(w->ev[9].a, w->st));
+				HIPCHK(c, launch_dp_batched(L2, bm, per_lane, waves, w->st));
+				HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
+				HIPCHK(c, hipStreamSynchronize(w->st));
+				HIPCHK(c, hipEventElapsedTime(&ms, w->ev[9].a, w->ev[3].a));
+			}
+			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1)) tc.ms_default = ms;
+			if (tc.ncand < (int) (sizeof tc.cand / sizeof tc.cand[0]))
+				tc.cand[tc.ncand++] = hero_tune_candidate{per_lane, wpc_eff, ms};
+			if (best < 0 || ms < best) best = ms, tc.per_lane = per_lane, tc.waves_per_cu = wpc_eff, tc.ms = ms;
+			if (wpc_eff == occ) break;
+		}
+	}
+	c->tuned_per_lane[cls] = tc.per_lane;
+	c->tuned_waves_per_cu[cls] = tc.waves_per_cu;
+	return HERO_OK;
+}
+
 // The DP of a chunk's tasks (cd.tasks): batched form by band class and length bucket, plain form
 // for the rest (or for everything with dp_form 1). Records ev[7] after bucketing, ev[3] at the end.
 int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
@@ -374,14 +417,24 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			}
 			if (b1 == b0) continue;
 			const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
-			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * dp_batched_waves_per_cu(bm, aff);
-			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 63) / 64);
-			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64));
+			// tasks per lane: 2 = packed 16-bit (band classes up to kPk16MaxBand), 1 = 32-bit; the
+			// option, else the tuned choice for this class, else 1
+			const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
+			const int per_lane = (want == 2 && bm <= kPk16MaxBand) ? 2 : 1;
+			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane);
+			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
+			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 64 * per_lane - 1) / (64 * per_lane));
+			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64 * per_lane));
 			DpLaunch Lc = L;
 			Lc.order = w->order.p + b0, Lc.ntasks = b1 - b0;
 			Lc.scratch = w->dp_scratch.p, Lc.rows_max = rows;
+			if (c->tune) {
+				// autotuning: time every (tasks per lane, waves per CU) candidate on this class's tasks
+				if (int r = tune_class(c, w, Lc, cls, bm, aff, rows, nw)) return r;
+				continue;
+			}
 			HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
-			HIPCHK(c, launch_dp_batched(Lc, bm, waves, w->st));
+			HIPCHK(c, launch_dp_batched(Lc, bm, per_lane, waves, w->st));
 		}
 	} else {
 		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
@@ -510,6 +563,7 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	HIPCHK(c, launch_pass_a2(S, Q, c->o, cd, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
 	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, rows_needed)) return r;
+	if (c->tune) return HERO_OK; // autotuning times the DP only: no records, no votes
 	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, stats, w->st));
 	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
@@ -1569,4 +1623,31 @@ extern "C" int hero_server_stop(hero_server *sv, hero_stats *merged) {
 	if (!sv->err.empty()) sv->c->err = sv->err;
 	delete sv;
 	return rc;
+}
+
+extern "C" int hero_autotune(hero_ctx *c, int64_t max_subjects, hero_tune_report *report) {
+	if (!c || max_subjects < 0) return HERO_ERR_ARG;
+	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before hero_autotune");
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	hero_tune_report rep;
+	std::memset(&rep, 0, sizeof rep);
+	rep.classes = kNumBandClasses;
+	for (int i = 0; i < kNumBandClasses; i++) c->tuned_per_lane[i] = 1, c->tuned_waves_per_cu[i] = 0;
+	int ml = 0;
+	for (int32_t l : c->q_len) ml = std::max(ml, l);
+	const int64_t n = std::min<int64_t>(max_subjects > 0 ? max_subjects : 65536, c->Qd.n);
+	// tuning work is not part of the run's statistics: keep the counters and restore them after
+	const hero_stats saved = c->st_host;
+	std::vector<unsigned long long> dsaved((size_t) kStatStripes * kStatStride);
+	HIPCHK(c, hipMemcpy(dsaved.data(), c->w.dstats.p, dsaved.size() * 8, hipMemcpyDeviceToHost));
+	c->tune = &rep;
+	const int rc = n > 0 ? run_chunk(c, &c->w, view(c->Qd, c->q_subject_good.p), 0, (int32_t) n, ml + 1, c->q_uid.data())
+	                     : HERO_OK;
+	c->tune = nullptr;
+	c->st_host = saved;
+	HIPCHK(c, hipMemcpy(c->w.dstats.p, dsaved.data(), dsaved.size() * 8, hipMemcpyHostToDevice));
+	if (rc != HERO_OK) return rc;
+	rep.subjects = n;
+	if (report) *report = rep;
+	return HERO_OK;
 }

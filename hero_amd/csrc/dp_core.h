@@ -54,7 +54,7 @@ HD void cell_update(const Opts &o, int sc, int qc, bool col0, bool colQ, bool le
 		if (left_al && colQ) x = imax_(h_up + o.sx, x_up + o.sx);
 		else x = imax_(h_up + o.dx, x_up + o.ex);
 		const int s = sc != qc ? o.miss : o.hit;
-		h = imax_(imax_(h_diag + s, x_diag + s), y_diag + s);
+		h = imax_(imax_(h_diag, x_diag), y_diag) + s; // = max(h_diag + s, x_diag + s, y_diag + s)
 		p = (h > x && h >= y) ? 2 + (sc != qc) : (y > x ? 1 : 0);
 	}
 }

@@ -4,6 +4,7 @@
 // Every kernel that works per subject runs one 64-lane wavefront per subject read.
 #include <hip/hip_runtime.h>
 
+#include "dp_pk16.h"
 #include "kernels.h"
 #include "pipeline_core.h"
 
@@ -989,6 +990,93 @@ struct LanePtrs {
 	__device__ __forceinline__ uint32_t get(int i, int w) const { return base[((size_t) i * NW + w) * 64 + lane]; }
 };
 
+// Pointer rows of one of SLOTS tasks per wavefront (two per lane in the packed form), interleaved
+// like LanePtrs: word w of row i of slot s at ((i * NW) + w) * SLOTS + s.
+template <int NW, int SLOTS>
+struct SlotPtrs {
+	uint32_t *base;
+	int slot;
+	__device__ __forceinline__ void put_row(int i, const uint32_t *pw) {
+#pragma unroll
+		for (int w = 0; w < NW; w++) base[((size_t) i * NW + w) * SLOTS + slot] = pw[w];
+	}
+	__device__ __forceinline__ uint32_t get(int i, int w) const { return base[((size_t) i * NW + w) * SLOTS + slot]; }
+};
+
+// Traceback of a forward-filled task into its edit words, step count and summary.
+template <class PS>
+__device__ __forceinline__ uint64_t finish_task(const DpLaunch &L, uint32_t t, const DpTask &task, int lenS, int lenQ,
+		const PS &ptrs) {
+	const int cap = task.s_len + task.q_len;
+	StepWriter wr{L.words + task.out_off, cap, -1, 0};
+	const int ns = dp_traceback(lenS, lenQ, task.band, ptrs, wr);
+	wr.flush();
+	L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
+	L.sums[t] = wr.sb.finish();
+	if (ns < 0) stat_add(L.stats, kStDpSkipped, 1ull);
+	return band_cells(lenS, lenQ, task.band);
+}
+
+// Batched form, two tasks per lane: persistent waves pulling 128 tasks at a time; lane l takes
+// tasks 2l and 2l+1 of the batch (neighbours in the length-bucketed order) and runs them together
+// in packed 16-bit arithmetic (dp_pk16.h). A pair with a task beyond the 16-bit bound runs its two
+// tasks one after the other in 32-bit.
+template <int BMAX, bool AFF>
+__global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L) {
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	const int lane = lane_id();
+	uint32_t *wbase = L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 128;
+	SlotPtrs<NW, 128> pa{wbase, 2 * lane}, pb{wbase, 2 * lane + 1};
+	uint64_t cells = 0;
+	for (;;) {
+		uint32_t batch = 0;
+		if (lane == 0) batch = atomicAdd(L.batch_counter, 1u);
+		batch = __shfl(batch, 0, 64);
+		const uint32_t first = batch * 128;
+		if (first >= L.ntasks) break;
+		const uint32_t ia = first + 2 * lane, ib = ia + 1;
+		if (ia >= L.ntasks) continue;
+		const uint32_t ta = L.order ? L.order[ia] : ia;
+		const DpTask A = L.tasks[ta];
+		PackedStream sa, qa;
+		int lsa, lqa;
+		task_streams(A, L.S, L.Q, sa, qa, lsa, lqa);
+		const bool oka = A.band <= BMAX && lsa < L.rows_max;
+		const bool has_b = ib < L.ntasks;
+		uint32_t tb = 0;
+		DpTask B;
+		PackedStream sb, qb;
+		int lsb = 0, lqb = 0;
+		bool okb = false;
+		if (has_b) {
+			tb = L.order ? L.order[ib] : ib;
+			B = L.tasks[tb];
+			task_streams(B, L.S, L.Q, sb, qb, lsb, lqb);
+			okb = B.band <= BMAX && lsb < L.rows_max;
+		}
+		if (oka && okb && pk16_fits(L.o, lsa, lqa) && pk16_fits(L.o, lsb, lqb)) {
+			const Pk16Task PA{lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0, (A.flags & kTaskAlignRight) != 0};
+			const Pk16Task PB{lsb, lqb, B.band, (B.flags & kTaskAlignLeft) != 0, (B.flags & kTaskAlignRight) != 0};
+			dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, sa, qa, sb, qb, pa, pb);
+			cells += finish_task(L, ta, A, lsa, lqa, pa);
+			cells += finish_task(L, tb, B, lsb, lqb, pb);
+		} else {
+			if (oka) {
+				dp_forward<BMAX, AFF>(L.o, lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0,
+						(A.flags & kTaskAlignRight) != 0, sa, qa, pa);
+				cells += finish_task(L, ta, A, lsa, lqa, pa);
+			}
+			if (okb) {
+				dp_forward<BMAX, AFF>(L.o, lsb, lqb, B.band, (B.flags & kTaskAlignLeft) != 0,
+						(B.flags & kTaskAlignRight) != 0, sb, qb, pb);
+				cells += finish_task(L, tb, B, lsb, lqb, pb);
+			}
+		}
+	}
+	cells = wave_sum64(cells);
+	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
+}
+
 // Batched form: one task per lane, persistent waves pulling 64 tasks at a time.
 template <int BMAX, bool AFF>
 __global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
@@ -1027,15 +1115,21 @@ __global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
-int dp_batched_waves_per_cu(int bmax, bool aff) {
+int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane) {
 	int blocks = 0;
 	hipError_t e = hipErrorInvalidValue;
 #define HERO_OCC(BM)                                                                                            \
-	if (bmax == BM)                                                                                             \
+	if (bmax == BM && per_lane == 1)                                                                            \
 		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, true>, 64, 0)          \
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false>, 64, 0);
+#define HERO_OCC2(BM)                                                                                           \
+	if (bmax == BM && per_lane == 2)                                                                            \
+		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64, 0)       \
+		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64, 0);
 	HERO_OCC(4) HERO_OCC(8) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
+	HERO_OCC2(4) HERO_OCC2(8) HERO_OCC2(12) HERO_OCC2(16) HERO_OCC2(24)
 #undef HERO_OCC
+#undef HERO_OCC2
 	return (e == hipSuccess && blocks > 0) ? blocks : 4;
 }
 
@@ -1103,17 +1197,25 @@ hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_ma
 	return hipGetLastError();
 }
 
-hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int nwaves, hipStream_t st) {
+hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwaves, hipStream_t st) {
 	if (L.ntasks == 0) return hipSuccess;
 	const bool aff = L.o.aligner == 0;
 #define HERO_DPB(BM)                                                                               \
-	if (bmax == BM) {                                                                              \
+	if (bmax == BM && per_lane == 1) {                                                             \
 		if (aff) k_dp_batched<BM, true><<<nwaves, 64, 0, st>>>(L);                                   \
 		else k_dp_batched<BM, false><<<nwaves, 64, 0, st>>>(L);                                      \
 		return hipGetLastError();                                                                  \
 	}
+#define HERO_DPB2(BM)                                                                              \
+	if (bmax == BM && per_lane == 2) {                                                             \
+		if (aff) k_dp_batched_pk<BM, true><<<nwaves, 64, 0, st>>>(L);                                \
+		else k_dp_batched_pk<BM, false><<<nwaves, 64, 0, st>>>(L);                                   \
+		return hipGetLastError();                                                                  \
+	}
 	HERO_DPB(4) HERO_DPB(8) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
+	HERO_DPB2(4) HERO_DPB2(8) HERO_DPB2(12) HERO_DPB2(16) HERO_DPB2(24)
 #undef HERO_DPB
+#undef HERO_DPB2
 	return hipErrorInvalidValue;
 }
 

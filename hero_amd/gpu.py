@@ -24,7 +24,8 @@ class Options(ctypes.Structure):
         ("miss", ctypes.c_int32), ("hit", ctypes.c_int32), ("unobserved", ctypes.c_int32),
         ("band_override", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
-        ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -43,6 +44,33 @@ class Stats(ctypes.Structure):
 class BandPair(ctypes.Structure):
     _fields_ = [("s_off", ctypes.c_int64), ("q_off", ctypes.c_int64), ("s_len", ctypes.c_int32),
                 ("q_len", ctypes.c_int32), ("band", ctypes.c_int32), ("flag", ctypes.c_int32)]
+
+
+class TuneCandidate(ctypes.Structure):
+    _fields_ = [("per_lane", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32), ("ms", ctypes.c_float)]
+
+
+class TuneClass(ctypes.Structure):
+    _fields_ = [("band", ctypes.c_int32), ("per_lane", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("ncand", ctypes.c_int32), ("tasks", ctypes.c_int64), ("ms", ctypes.c_float),
+                ("ms_default", ctypes.c_float), ("cand", TuneCandidate * 16)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("classes", ctypes.c_int32), ("_pad", ctypes.c_int32), ("subjects", ctypes.c_int64),
+                ("cls", TuneClass * 6)]
+
+    def as_list(self):
+        out = []
+        for i in range(self.classes):
+            c = self.cls[i]
+            if c.tasks == 0:
+                continue
+            out.append(dict(band=c.band, tasks=c.tasks, per_lane=c.per_lane, waves_per_cu=c.waves_per_cu,
+                            ms=round(c.ms, 4), ms_default=round(c.ms_default, 4),
+                            candidates=[dict(per_lane=c.cand[j].per_lane, waves_per_cu=c.cand[j].waves_per_cu,
+                                             ms=round(c.cand[j].ms, 4)) for j in range(c.ncand)]))
+        return out
 
 
 class ServerOptions(ctypes.Structure):
@@ -79,6 +107,7 @@ SIGNATURES = {
     "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
     "hero_hash_kmers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_uint64, _P]),
     "hero_lookup": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_int64]),
+    "hero_autotune": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(TuneReport)]),
     "hero_set_subjects": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P]),
     "hero_build_subject_index": (ctypes.c_int, [_P]),
     "hero_correct_batch": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P]),
@@ -165,6 +194,12 @@ class Context:
 
     def align_self(self):
         self._check(lib().hero_align_self(self.h))
+
+    def autotune(self, max_subjects=0):
+        """Times the DP schedule candidates per band class on the first subjects; returns the report."""
+        r = TuneReport()
+        self._check(lib().hero_autotune(self.h, max_subjects, ctypes.byref(r)))
+        return r.as_list()
 
     def set_subjects(self, rs):
         """Resident subject set of the serving form."""

@@ -49,7 +49,9 @@ typedef struct {
 	int64_t chunk_subjects; /* subject reads per device pass; 0 = automatic */
 	int32_t dp_waves;       /* persistent DP wavefronts; 0 = automatic */
 	int32_t records;        /* 1: keep the alignment records as .aln text (requires call_consensus = 0) */
-	int32_t reserved[6];
+	int32_t dp_per_lane;    /* batched form: 1 one task per lane (32-bit), 2 two tasks per lane (packed
+	                           16-bit, band classes <= 24); 0 = the hero_autotune choice, else 1 */
+	int32_t reserved[5];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
@@ -134,6 +136,28 @@ int hero_hash_kmers(hero_ctx *ctx, int32_t n, int32_t k, const char *kmers, uint
  * entries[entry_off[i] ..]; entry_cap bounds the total. */
 int hero_lookup(hero_ctx *ctx, int32_t n, const char *windows, int64_t *count, int64_t *entry_off, uint64_t *entries,
 		int64_t entry_cap);
+
+/* Autotuner of the batched DP form: runs the planning of the first max_subjects subjects of the
+ * query set (0 = 65,536) and times, per band class, every schedule candidate (one task per lane in
+ * 32-bit or two per lane in packed 16-bit; persistent wavefronts per CU from 1 to the occupancy
+ * limit), then keeps the fastest for later calls with dp_per_lane = 0 and dp_waves = 0. Casts no
+ * votes and leaves the statistics as they were. Every candidate computes identical results. */
+typedef struct {
+	int32_t per_lane, waves_per_cu;
+	float ms;
+} hero_tune_candidate;
+typedef struct {
+	int32_t band, per_lane, waves_per_cu, ncand;
+	int64_t tasks;
+	float ms, ms_default; /* chosen candidate / one task per lane at the occupancy limit */
+	hero_tune_candidate cand[16];
+} hero_tune_class;
+typedef struct {
+	int32_t classes, _pad;
+	int64_t subjects;
+	hero_tune_class cls[6];
+} hero_tune_report;
+int hero_autotune(hero_ctx *ctx, int64_t max_subjects, hero_tune_report *report);
 
 /* ---- Serving -------------------------------------------------------------------------------
  * The subject set stays resident with an index of its k-mers at every position; batches of query

@@ -11,6 +11,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../hero_amd/csrc/dp_pk16.h"
 #include "../../hero_amd/csrc/pack.h"
 #include "../../hero_amd/csrc/pipeline_core.h"
 
@@ -316,3 +317,68 @@ extern "C" int hostsim_band_align(const hs_cfg *cfg, const char *subj, int ls, c
 }
 
 extern "C" uint64_t hostsim_band_cells(int lenS, int lenQ, int b) { return band_cells(lenS, lenQ, b); }
+
+// Packed two-task DP against the one-task DP on random task pairs: every pointer row of both tasks
+// must be equal. Returns the number of differing tasks (sequences with codes 0..3 as in the
+// pipeline; lengths, bands and flags random; options from cfg).
+struct CodeStream {
+	const std::vector<int> *s;
+	int i;
+	int next() { return i < (int) s->size() ? (*s)[(size_t) i++] : kCodeNone; }
+};
+template <int BMAX, bool AFF>
+static int pk16_check_one(const Opts &o, uint64_t &rng, int max_len) {
+	auto rnd = [&](uint64_t n) {
+		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+		return (int) (rng % n);
+	};
+	std::vector<int> sa, qa, sb, qb;
+	Pk16Task A, B;
+	auto mk = [&](Pk16Task &T, std::vector<int> &S, std::vector<int> &Q) {
+		T.lenS = rnd((uint64_t) max_len + 1);
+		T.lenQ = T.lenS - rnd((uint64_t) (T.lenS < BMAX ? T.lenS + 1 : BMAX + 1));
+		if (rnd(5) == 0) T.lenQ = rnd((uint64_t) T.lenS + 1);
+		T.b = T.lenS - T.lenQ + rnd(3);
+		if (T.b > BMAX) T.b = BMAX, T.lenQ = T.lenS - BMAX > 0 ? T.lenS - BMAX : 0;
+		if (T.b < T.lenS - T.lenQ) T.lenQ = T.lenS - T.b;
+		const int f = rnd(3);
+		T.left_al = f == 1, T.right_al = f == 2;
+		S.resize((size_t) T.lenS), Q.resize((size_t) T.lenQ);
+		for (auto &c : S) c = rnd(4);
+		for (size_t j = 0; j < Q.size(); j++) Q[j] = (j < S.size() && rnd(10)) ? S[j] : rnd(4);
+	};
+	mk(A, sa, qa);
+	mk(B, sb, qb);
+	if (!pk16_fits(o, A.lenS, A.lenQ) || !pk16_fits(o, B.lenS, B.lenQ)) return 0;
+	const int nw = (2 * BMAX + 1 + 15) / 16;
+	RowStore pa, pb, ra, rb;
+	pa.nw = pb.nw = ra.nw = rb.nw = nw;
+	CodeStream SA{&sa, 0}, QA{&qa, 0}, SB{&sb, 0}, QB{&qb, 0};
+	dp_forward_pk16<BMAX, AFF>(o, A, B, SA, QA, SB, QB, pa, pb);
+	CodeStream S1{&sa, 0}, Q1{&qa, 0}, S2{&sb, 0}, Q2{&qb, 0};
+	dp_forward<BMAX, AFF>(o, A.lenS, A.lenQ, A.b, A.left_al, A.right_al, S1, Q1, ra);
+	dp_forward<BMAX, AFF>(o, B.lenS, B.lenQ, B.b, B.left_al, B.right_al, S2, Q2, rb);
+	int bad = 0;
+	for (int i = 0; i <= A.lenS; i++)
+		for (int w = 0; w < nw; w++)
+			if (pa.get(i, w) != ra.get(i, w)) {
+				bad |= 1;
+			}
+	for (int i = 0; i <= B.lenS; i++)
+		for (int w = 0; w < nw; w++)
+			if (pb.get(i, w) != rb.get(i, w)) bad |= 2;
+	return (bad & 1) + ((bad >> 1) & 1);
+}
+
+extern "C" int hostsim_pk16_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len) {
+	const Opts o = to_opts(cfg);
+	uint64_t rng = seed | 1;
+	int bad = 0;
+	for (int t = 0; t < n; t++) {
+#define PK_CASE(BM)                                                                                          \
+	if (bmax == BM) bad += o.aligner == 0 ? pk16_check_one<BM, true>(o, rng, max_len) : pk16_check_one<BM, false>(o, rng, max_len);
+		PK_CASE(4) PK_CASE(8) PK_CASE(12) PK_CASE(16) PK_CASE(24)
+#undef PK_CASE
+	}
+	return bad;
+}

@@ -47,3 +47,11 @@ def band_align(cfg, S, Q, band, flag=0):
     if rc != 0:
         raise RuntimeError(f"hostsim_band_align failed: {rc}")
     return buf.value.decode(), b.value
+
+
+def pk16_check(cfg, n, seed, bmax, max_len):
+    """Task pairs whose packed two-task DP pointers differ from the one-task DP's (0 = all equal)."""
+    f = lib().hostsim_pk16_check
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    return f(ctypes.addressof(cfg), n, seed, bmax, max_len)

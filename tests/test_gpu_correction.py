@@ -165,3 +165,31 @@ def test_device_resident_consensus_then_copy():
         ctx.build_index()
         with pytest.raises(gpu.HeroError):
             ctx.align_self() or ctx.copy_corrected()
+
+
+def test_autotuned_schedule_gives_identical_results():
+    """The autotuner only picks schedules: tallies and corrected reads are unchanged, stats untouched."""
+    rs = synth.generate(seed=303, genome_len=40000, len_min=75, len_max=250, coverage=30.0, sub_5p=0.01)
+    opts = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40)
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        t0, c0 = ctx.tally(), ctx.consensus()
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        before = ctx.stats()
+        report = ctx.autotune(max_subjects=3000)
+        after = ctx.stats()
+        assert report and all(r["tasks"] > 0 and r["ms"] > 0 for r in report)
+        assert any(c["per_lane"] == 2 for r in report for c in r["candidates"])
+        assert before["dp_tasks"] == after["dp_tasks"] and before["dp_cells_all"] == after["dp_cells_all"]
+        ctx.align_self()
+        assert np.array_equal(ctx.tally(), t0) and np.array_equal(ctx.consensus(), c0)
+    for per_lane in (1, 2):
+        with gpu.Context(dp_per_lane=per_lane, **opts) as ctx:
+            ctx.set_queries(rs)
+            ctx.build_index()
+            ctx.align_self()
+            assert np.array_equal(ctx.tally(), t0) and np.array_equal(ctx.consensus(), c0)

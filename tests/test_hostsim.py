@@ -60,3 +60,13 @@ def test_band_cells_closed_form_equals_row_sum():
     for ls, lq, b in [(500, 480, 20), (16383, 16000, 383), (150, 149, 1), (1, 300, 299)]:
         want = sum(min(lq, i + b) - max(0, i - b) + 1 for i in range(ls + 1)) - 1
         assert f(ls, lq, b) == want
+
+
+@pytest.mark.parametrize("bmax", [4, 8, 12, 16, 24])
+@pytest.mark.parametrize("aligner", [0, 1], ids=["affine", "global"])
+def test_packed_two_task_dp_equals_one_task_dp(aligner, bmax):
+    """The packed 16-bit two-tasks-per-lane DP writes the same pointer rows as the 32-bit DP."""
+    costs = dict(dx=-7, dy=-7, ex=-1, miss=-4, hit=1) if aligner == 0 else {}
+    cfg = O.make_cfg(aligner=aligner, **costs)
+    for max_len in (3, 20, 140, 500):
+        assert H.pk16_check(cfg, 400, 12345 + bmax + max_len, bmax, max_len) == 0, (aligner, bmax, max_len)
