@@ -156,6 +156,14 @@ struct Worker {
 	DevBuf<StepSummary> sums;
 	DevBuf<unsigned long long> dstats, dtotal;
 	DevBuf<uint32_t> counter;
+	// the DP stage captured as a graph (bucketing + every band class + plain form), rebuilt when a
+	// buffer it reads moves or the schedule changes; the task count reaches it through ctl[0]
+	DevBuf<uint32_t> ctl;
+	uint32_t *ctl_host = nullptr; // pinned
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	std::vector<uintptr_t> graph_key;
+	uint64_t graph_builds = 0, graph_launches = 0;
 	// serving: one batch of query reads and its seeds
 	ReadsDev bq;
 	DevBuf<uint8_t> bq_good, b_cons;
@@ -170,6 +178,8 @@ struct Worker {
 		if (e == hipSuccess) e = dtotal.exact(1);
 		if (e == hipSuccess) e = counter.exact(1);
 		if (e == hipSuccess) e = hipMemset(dstats.p, 0, (size_t) kStatStripes * kStatStride * 8);
+		if (e == hipSuccess) e = ctl.exact(kCtlWords);
+		if (e == hipSuccess) e = hipHostMalloc((void **) &ctl_host, sizeof(uint32_t) * kCtlWords, hipHostMallocDefault);
 		return e;
 	}
 	void release() {
@@ -187,6 +197,12 @@ struct Worker {
 		b_gflag.release(), b_pflag.release(), b_gx.release(), b_px.release(), b_glist.release(), b_plist.release();
 		b_kref.release();
 		b_keys.release(), b_vals.release(), b_keys2.release(), b_vals2.release(), b_sort_tmp.release();
+		if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+		if (graph) (void) hipGraphDestroy(graph);
+		graph_exec = nullptr, graph = nullptr, graph_key.clear();
+		ctl.release();
+		if (ctl_host) (void) hipHostFree(ctl_host);
+		ctl_host = nullptr;
 		for (auto &t : ev)
 			if (t.a) (void) hipEventDestroy(t.a), t.a = nullptr;
 		if (st) (void) hipStreamDestroy(st);
@@ -380,8 +396,13 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 
 // The DP of a chunk's tasks (cd.tasks): batched form by band class and length bucket, plain form
 // for the rest (or for everything with dp_form 1). Records ev[7] after bucketing, ev[3] at the end.
+int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
+		int rows_needed);
+
 int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
 		int rows_needed) {
+	if (c->opt.dp_form == 0 && !c->opt.dp_eager && !c->tune && ntasks > 0)
+		return run_dp_graph(c, w, S, Q, cd, ntasks, rows_needed);
 	unsigned long long *stats = w->dstats.p;
 	DpLaunch L;
 	std::memset(&L, 0, sizeof L);
@@ -400,13 +421,13 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 		HIPCHK(c, w->bucket_off.ensure(kNumBuckets));
 		HIPCHK(c, w->order.ensure((size_t) ntasks));
 		HIPCHK(c, hipMemsetAsync(w->bucket_hist.p, 0, kNumBuckets * 4, w->st));
-		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, rows, w->bucket_hist.p, w->st));
+		HIPCHK(c, launch_task_histogram(cd.tasks, (uint32_t) ntasks, nullptr, rows, w->bucket_hist.p, w->st));
 		uint64_t total = 0;
 		if (int r = wscan(c, w, w->bucket_hist.p, w->bucket_off.p, kNumBuckets, total)) return r;
 		std::vector<uint32_t> off(kNumBuckets);
 		HIPCHK(c, hipMemcpyAsync(off.data(), w->bucket_off.p, kNumBuckets * 4, hipMemcpyDeviceToHost, w->st));
 		HIPCHK(c, hipStreamSynchronize(w->st));
-		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, rows, w->bucket_off.p, w->order.p, w->st));
+		HIPCHK(c, launch_task_scatter(cd.tasks, (uint32_t) ntasks, nullptr, rows, w->bucket_off.p, w->order.p, w->st));
 		HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
 		for (int cls = 0; cls <= kNumBandClasses; cls++) {
 			const uint32_t b0 = off[(size_t) cls * kLenBins];
@@ -458,6 +479,98 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 		for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
 		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
 	}
+	return HERO_OK;
+}
+
+// The DP stage as one graph launch. Fixed grids (persistent waves per class, grid-stride
+// bucketing over the task buffer's capacity); the task count and the class ranges are read on the
+// device, so one captured graph serves every chunk whose buffers have not moved.
+int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
+		int rows_needed) {
+	const bool aff = c->o.aligner == 0;
+	// rows rounded up so that batches of slightly different read lengths share one graph (the row
+	// limit only routes tasks between the two DP forms, which give identical results)
+	const int rows = std::min((rows_needed + 63) / 64 * 64, 512);
+	const int bmax_plain = 64;
+	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
+	const uint32_t cap = (uint32_t) std::min<size_t>(w->tasks.n, 0xFFFFFFFFu);
+	if (cd.tasks != w->tasks.p || ntasks > cap) return fail(c, HERO_ERR_STATE, "DP graph: task buffer mismatch");
+	HIPCHK(c, w->bucket_hist.ensure(kNumBuckets));
+	HIPCHK(c, w->bucket_off.ensure(kNumBuckets));
+	HIPCHK(c, w->order.ensure(cap));
+	HIPCHK(c, w->scan_tmp.ensure(scan_tmp_words(kNumBuckets)));
+	int per_lane[kNumBandClasses], waves[kNumBandClasses];
+	size_t scratch = 0;
+	for (int cls = 0; cls < kNumBandClasses; cls++) {
+		const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
+		const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
+		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand) ? 2 : 1;
+		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane[cls]);
+		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
+		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);
+	}
+	HIPCHK(c, w->dp_scratch.ensure(scratch));
+	HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+	DpLaunch L;
+	std::memset(&L, 0, sizeof L); // padding too: the key below compares the struct's bytes
+	L.tasks = cd.tasks, L.order = w->order.p, L.S = S, L.Q = Q, L.o = c->o;
+	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = w->dstats.p;
+	L.route_bmax = -1, L.route_rows = 0, L.rows_max = rows;
+	// everything the captured launches take by value
+	std::vector<uintptr_t> key((sizeof L + sizeof(uintptr_t) - 1) / sizeof(uintptr_t), 0);
+	std::memcpy(key.data(), &L, sizeof L);
+	for (uintptr_t v : {(uintptr_t) cap, (uintptr_t) w->dp_scratch.p, (uintptr_t) w->plain_scratch.p,
+				 (uintptr_t) w->bucket_hist.p, (uintptr_t) w->bucket_off.p, (uintptr_t) w->scan_tmp.p,
+				 (uintptr_t) w->scan_tmp.n, (uintptr_t) w->dtotal.p, (uintptr_t) w->ctl.p, (uintptr_t) plain_waves})
+		key.push_back(v);
+	for (int cls = 0; cls < kNumBandClasses; cls++) key.push_back((uintptr_t) (per_lane[cls] * 100000 + waves[cls]));
+	if (!w->graph_exec || key != w->graph_key) {
+		if (w->graph_exec) (void) hipGraphExecDestroy(w->graph_exec);
+		if (w->graph) (void) hipGraphDestroy(w->graph);
+		w->graph_exec = nullptr, w->graph = nullptr, w->graph_key.clear();
+		uint32_t *ctl = w->ctl.p;
+		HIPCHK(c, hipStreamBeginCapture(w->st, hipStreamCaptureModeThreadLocal));
+		hipError_t e = launch_fill_u32(w->bucket_hist.p, kNumBuckets, 0u, w->st);
+		if (e == hipSuccess) e = launch_task_histogram(cd.tasks, cap, ctl, rows, w->bucket_hist.p, w->st);
+		if (e == hipSuccess)
+			e = scan_u32(w->bucket_hist.p, w->bucket_off.p, kNumBuckets, w->dtotal.p, w->scan_tmp.p, w->scan_tmp.n, w->st);
+		if (e == hipSuccess) e = launch_class_ranges(w->bucket_off.p, ctl, w->st);
+		if (e == hipSuccess) e = launch_task_scatter(cd.tasks, cap, ctl, rows, w->bucket_off.p, w->order.p, w->st);
+		for (int cls = 0; cls < kNumBandClasses && e == hipSuccess; cls++) {
+			DpLaunch Lc = L;
+			Lc.range = ctl + kCtlRange + cls, Lc.batch_counter = ctl + kCtlCounter + cls;
+			Lc.scratch = w->dp_scratch.p;
+			e = launch_dp_batched(Lc, kBandClass[cls], per_lane[cls], waves[cls], w->st);
+		}
+		if (e == hipSuccess) {
+			DpLaunch Lp = L;
+			Lp.range = ctl + kCtlRange + kNumBandClasses, Lp.batch_counter = ctl + kCtlCounter + kNumBandClasses;
+			Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
+			e = launch_dp_plain(Lp, bmax_plain, plain_waves, w->st);
+		}
+		hipGraph_t g = nullptr;
+		const hipError_t e2 = hipStreamEndCapture(w->st, &g);
+		if (e != hipSuccess || e2 != hipSuccess) {
+			if (g) (void) hipGraphDestroy(g);
+			return fail(c, HERO_ERR_HIP, "capturing the DP stage: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+		}
+		w->graph = g;
+		HIPCHK(c, hipGraphInstantiate(&w->graph_exec, w->graph, nullptr, nullptr, 0));
+		w->graph_key = key;
+		w->graph_builds++;
+	}
+	w->ctl_host[0] = ntasks;
+	HIPCHK(c, hipMemcpyAsync(w->ctl.p, w->ctl_host, 4, hipMemcpyHostToDevice, w->st));
+	HIPCHK(c, hipEventRecord(w->ev[7].a, w->st));
+	HIPCHK(c, hipGraphLaunch(w->graph_exec, w->st));
+	w->graph_launches++;
+	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
+	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
+	HIPCHK(c, hipMemcpyAsync(stripes.data(), w->dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	unsigned long long skipped = 0;
+	for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
+	if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
 	return HERO_OK;
 }
 
@@ -1172,6 +1285,7 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	out->dp_skipped = s[kStDpSkipped];
 	out->reads_changed = s[kStReadsChanged];
 	out->dp_cells_all = s[kStCellsAll];
+	out->dp_graph_builds = c->w.graph_builds, out->dp_graph_launches = c->w.graph_launches;
 	out->ms_total = out->ms_index + out->ms_seed + out->ms_plan + out->ms_dp + out->ms_score_vote + out->ms_consensus;
 	return HERO_OK;
 }
@@ -1180,6 +1294,7 @@ int hero_reset_stats(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	HIPCHK(c, hipMemset(c->w.dstats.p, 0, (size_t) kStatStripes * kStatStride * 8));
+	c->w.graph_builds = 0, c->w.graph_launches = 0;
 	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;

@@ -104,6 +104,9 @@ struct DpLaunch {
 	int route_bmax, route_rows; // plain form: >= 0 restricts it to tasks the batched form cannot take
 	uint32_t *batch_counter;
 	unsigned long long *stats;
+	// graph launches: the task range [range[0], range[1]) of the order array, read on the device
+	// (null: tasks order[0 .. ntasks))
+	const uint32_t *range;
 };
 
 // launchers (kernels.hip)
@@ -145,9 +148,17 @@ constexpr int kNumBandClasses = 6;
 constexpr int kBandClass[kNumBandClasses] = {4, 8, 12, 16, 24, 32};
 constexpr int kLenBins = 1024;
 constexpr int kNumBuckets = (kNumBandClasses + 1) * kLenBins;
-hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *hist, hipStream_t st);
-hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *cursor, uint32_t *order,
-		hipStream_t st);
+// ntasks_dev (optional) supplies the task count on the device; the grid then covers max_tasks
+hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, const uint32_t *ntasks_dev, int rows_max,
+		uint32_t *hist, hipStream_t st);
+hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, const uint32_t *ntasks_dev, int rows_max,
+		uint32_t *cursor, uint32_t *order, hipStream_t st);
+// Graph form of the DP stage: class task ranges from the bucket offsets (ctl[1 + c], c = 0 ..
+// kNumBandClasses, then ctl[2 + kNumBandClasses] = ntasks = ctl[0]) and zeroed class counters
+// (ctl[16 + c]).
+constexpr int kCtlRange = 1, kCtlCounter = 16, kCtlWords = 32;
+hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
 		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st);

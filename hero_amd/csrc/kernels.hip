@@ -1022,7 +1022,9 @@ __device__ __forceinline__ uint64_t finish_task(const DpLaunch &L, uint32_t t, c
 // in packed 16-bit arithmetic (dp_pk16.h). A pair with a task beyond the 16-bit bound runs its two
 // tasks one after the other in 32-bit.
 template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L) {
+__global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
+	DpLaunch L = L0;
+	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
 	const int lane = lane_id();
 	uint32_t *wbase = L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 128;
@@ -1079,7 +1081,9 @@ __global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L) {
 
 // Batched form: one task per lane, persistent waves pulling 64 tasks at a time.
 template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L) {
+__global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L0) {
+	DpLaunch L = L0;
+	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
 	const int lane = lane_id();
 	LanePtrs<NW> ptrs{L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 64, lane};
@@ -1143,7 +1147,9 @@ __device__ __forceinline__ int task_bucket(const DpTask &t, int rows_max) {
 	return cls * kLenBins + bin;
 }
 
-__global__ __launch_bounds__(1024) void k_task_histogram(const DpTask *tasks, uint32_t n, int rows_max, uint32_t *hist) {
+__global__ __launch_bounds__(1024) void k_task_histogram(const DpTask *tasks, uint32_t n0, const uint32_t *n_dev,
+		int rows_max, uint32_t *hist) {
+	const uint32_t n = n_dev ? *n_dev : n0;
 	__shared__ uint32_t h[kNumBuckets];
 	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) h[i] = 0;
 	__syncthreads();
@@ -1153,10 +1159,11 @@ __global__ __launch_bounds__(1024) void k_task_histogram(const DpTask *tasks, ui
 	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x)
 		if (h[i]) atomicAdd(&hist[i], h[i]);
 }
-hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *hist, hipStream_t st) {
+hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, const uint32_t *ntasks_dev, int rows_max,
+		uint32_t *hist, hipStream_t st) {
 	if (ntasks == 0) return hipSuccess;
 	const unsigned blocks = ntasks / (1024 * 16) + 1 < 512 ? ntasks / (1024 * 16) + 1 : 512;
-	k_task_histogram<<<blocks, 1024, 0, st>>>(tasks, ntasks, rows_max, hist);
+	k_task_histogram<<<blocks, 1024, 0, st>>>(tasks, ntasks, ntasks_dev, rows_max, hist);
 	return hipGetLastError();
 }
 
@@ -1164,8 +1171,9 @@ hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, int rows_
 // bucket's range with one global atomic, then writes: few global atomics even when most tasks
 // share a handful of buckets.
 constexpr int kScatterThreads = 1024, kScatterPerThread = 8;
-__global__ __launch_bounds__(1024) void k_task_scatter(const DpTask *tasks, uint32_t n, int rows_max, uint32_t *cursor,
-		uint32_t *order) {
+__global__ __launch_bounds__(1024) void k_task_scatter(const DpTask *tasks, uint32_t n0, const uint32_t *n_dev,
+		int rows_max, uint32_t *cursor, uint32_t *order) {
+	const uint32_t n = n_dev ? *n_dev : n0;
 	__shared__ uint32_t cnt[kNumBuckets];
 	for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) cnt[i] = 0;
 	__syncthreads();
@@ -1189,16 +1197,36 @@ __global__ __launch_bounds__(1024) void k_task_scatter(const DpTask *tasks, uint
 	for (int i = 0; i < kScatterPerThread; i++)
 		if (bk[i] >= 0) order[cnt[bk[i]] + rk[i]] = base + i * kScatterThreads + threadIdx.x;
 }
-hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, int rows_max, uint32_t *cursor, uint32_t *order,
-		hipStream_t st) {
+hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, const uint32_t *ntasks_dev, int rows_max,
+		uint32_t *cursor, uint32_t *order, hipStream_t st) {
 	if (ntasks == 0) return hipSuccess;
 	const uint32_t per_block = kScatterThreads * kScatterPerThread;
-	k_task_scatter<<<(ntasks + per_block - 1) / per_block, kScatterThreads, 0, st>>>(tasks, ntasks, rows_max, cursor, order);
+	k_task_scatter<<<(ntasks + per_block - 1) / per_block, kScatterThreads, 0, st>>>(tasks, ntasks, ntasks_dev, rows_max,
+			cursor, order);
+	return hipGetLastError();
+}
+
+__global__ void k_class_ranges(const uint32_t *bucket_off, uint32_t *ctl) {
+	const int c = threadIdx.x;
+	if (c <= kNumBandClasses) ctl[kCtlRange + c] = bucket_off[(size_t) c * kLenBins];
+	if (c == kNumBandClasses + 1) ctl[kCtlRange + c] = ctl[0];
+	if (c < kCtlWords - kCtlCounter) ctl[kCtlCounter + c] = 0;
+}
+hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st) {
+	k_class_ranges<<<1, 64, 0, st>>>(bucket_off, ctl);
+	return hipGetLastError();
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st) {
+	if (n == 0) return hipSuccess;
+	k_fill_u32<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, st>>>(p, n, v);
 	return hipGetLastError();
 }
 
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwaves, hipStream_t st) {
-	if (L.ntasks == 0) return hipSuccess;
+	if (L.ntasks == 0 && !L.range) return hipSuccess;
 	const bool aff = L.o.aligner == 0;
 #define HERO_DPB(BM)                                                                               \
 	if (bmax == BM && per_lane == 1) {                                                             \
@@ -1224,7 +1252,9 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 // row, pointers as bytes in per-wave global scratch, traceback by lane 0.
 constexpr int kPlainRows = 512;
 template <bool AFF>
-__global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
+__global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L0, int bmax_plain) {
+	DpLaunch L = L0;
+	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	__shared__ int hb[3][kPlainRows + 1];
 	__shared__ int xb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
 	__shared__ int yb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
@@ -1321,7 +1351,7 @@ __global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L, int bmax_plain) {
 }
 
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st) {
-	if (L.ntasks == 0) return hipSuccess;
+	if (L.ntasks == 0 && !L.range) return hipSuccess;
 	if (L.o.aligner == 0) k_dp_plain<true><<<nwaves, 64, 0, st>>>(L, bmax_plain);
 	else k_dp_plain<false><<<nwaves, 64, 0, st>>>(L, bmax_plain);
 	return hipGetLastError();
@@ -1447,7 +1477,8 @@ static hipError_t scan_rec(const uint32_t *in, uint32_t *out, int64_t n, uint32_
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long long *total_dev, uint32_t *tmp,
 		size_t tmp_words, hipStream_t st) {
 	if (tmp_words < scan_tmp_words(n)) return hipErrorInvalidValue;
-	hipError_t e = hipMemsetAsync(total_dev, 0, sizeof(unsigned long long), st);
+	// zeroed by a kernel rather than a memset, so the scan can be captured in a graph
+	hipError_t e = launch_fill_u32((uint32_t *) total_dev, 2, 0u, st);
 	if (e != hipSuccess) return e;
 	if (n > 0) {
 		k_sum64<<<(unsigned) (n / 4096 + 1 < 1024 ? n / 4096 + 1 : 1024), 256, 0, st>>>(in, n, total_dev);

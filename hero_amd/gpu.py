@@ -25,7 +25,7 @@ class Options(ctypes.Structure):
         ("band_override", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("dp_eager", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -35,7 +35,7 @@ class Stats(ctypes.Structure):
         "dp_cells_exec", "dp_cells_all", "records_built", "records_kept", "dp_skipped", "vote_oob", "subjects",
         "queries", "reads_changed")] + [(n, ctypes.c_double) for n in (
             "ms_index", "ms_seed", "ms_plan", "ms_dp", "ms_score_vote", "ms_consensus", "ms_total",
-            "ms_seed_probe")]
+            "ms_seed_probe")] + [(n, ctypes.c_uint64) for n in ("dp_graph_builds", "dp_graph_launches")]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -51,7 +51,8 @@ typedef struct {
 	int32_t records;        /* 1: keep the alignment records as .aln text (requires call_consensus = 0) */
 	int32_t dp_per_lane;    /* batched form: 1 one task per lane (32-bit), 2 two tasks per lane (packed
 	                           16-bit, band classes <= 24); 0 = the hero_autotune choice, else 1 */
-	int32_t reserved[5];
+	int32_t dp_eager;       /* 1: launch the DP stage kernel by kernel instead of as one captured graph */
+	int32_t reserved[4];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
@@ -61,6 +62,8 @@ typedef struct {
 	uint64_t records_built, records_kept, dp_skipped, vote_oob, subjects, queries, reads_changed;
 	double ms_index, ms_seed, ms_plan, ms_dp, ms_score_vote, ms_consensus, ms_total;
 	double ms_seed_probe; /* the seed-count probe kernel alone (part of ms_seed) */
+	uint64_t dp_graph_builds;   /* DP stage graphs captured (a buffer moved or the schedule changed) */
+	uint64_t dp_graph_launches; /* DP stage graph launches (one per subject chunk) */
 } hero_stats;
 
 void hero_default_options(hero_options *opt);

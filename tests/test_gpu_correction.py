@@ -193,3 +193,41 @@ def test_autotuned_schedule_gives_identical_results():
             ctx.build_index()
             ctx.align_self()
             assert np.array_equal(ctx.tally(), t0) and np.array_equal(ctx.consensus(), c0)
+
+
+def test_dp_graph_replays_equal_eager_launches():
+    """The DP stage captured once as a graph and replayed for every subject chunk (task count and
+    class ranges read on the device) gives the kernel-by-kernel launches' tallies, corrected reads
+    and DP counts; eager work and synchronisations run between the replays."""
+    rs = synth.generate(seed=404, genome_len=20000, len_min=75, len_max=250, coverage=30.0, sub_5p=0.01,
+                        indel_rate=0.002)
+    opts = dict(call_consensus=1, k=25, aligner=0, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40,
+                chunk_subjects=250)
+    res = []
+    for eager in (1, 0):
+        with gpu.Context(dp_eager=eager, **opts) as ctx:
+            ctx.set_queries(rs)
+            ctx.build_index()
+            ctx.align_self()
+            res.append((ctx.tally(), ctx.consensus(), ctx.stats()))
+    (t0, c0, s0), (t1, c1, s1) = res
+    assert s0["dp_graph_launches"] == 0
+    assert s1["dp_graph_launches"] >= 10 and s1["dp_graph_builds"] < s1["dp_graph_launches"]
+    assert np.array_equal(t0, t1) and np.array_equal(c0, c1)
+    for k in ("dp_calls_exec", "dp_cells_exec", "records_kept", "dp_skipped"):
+        assert s0[k] == s1[k], k
+
+
+def test_dp_graph_records_equal_oracle_over_chunks():
+    """Record mode through the graph-replayed DP stage, small chunks: the .aln text equals the oracle's."""
+    rs = synth.generate(seed=405, genome_len=9000, len_min=80, len_max=220, coverage=25.0, sub_5p=0.01)
+    opts = dict(k=31, aligner=1, mismatch_rate=0.04, indel_rate=0.02, min_overlap=40)
+    with gpu.Context(call_consensus=0, records=1, chunk_subjects=180, **opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        aln = ctx.take_alignments()
+        st = ctx.stats()
+    assert st["dp_graph_launches"] >= 5
+    ro = O.run(O.make_cfg(call_consensus=0, **opts), rs, threads=8, want_aln=True)
+    assert aln.decode() == ro["aln"]
