@@ -198,7 +198,7 @@ def _drive(srv, batches, arrivals):
     return lat, reads / span if span > 0 else 0.0, merged
 
 
-def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_reads=16384, sweep_rates=C4_RATES,
+def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_reads=32768, sweep_rates=C4_RATES,
                     prepared=None):
     """Serving form on the resident-subject workload: offline throughput of the same path, a fixed
     4096-read batch stream at 82% of that offline rate, and Poisson sweeps with log-uniform batch

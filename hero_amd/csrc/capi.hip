@@ -101,6 +101,29 @@ struct DevBuf {
 	}
 };
 
+// Pinned host staging, grow-only like DevBuf.
+template <class T>
+struct HostBuf {
+	T *p = nullptr;
+	size_t n = 0;
+	void release() {
+		if (p) (void) hipHostFree(p);
+		p = nullptr, n = 0;
+	}
+	hipError_t ensure(size_t want) {
+		if (want <= n && p) return hipSuccess;
+		release();
+		const size_t alloc = want + want / 4 + 64;
+		hipError_t e = hipHostMalloc((void **) &p, alloc * sizeof(T), hipHostMallocDefault);
+		if (e != hipSuccess) {
+			p = nullptr;
+			return e;
+		}
+		n = alloc;
+		return hipSuccess;
+	}
+};
+
 struct ReadsDev {
 	DevBuf<uint64_t> words, woff, boff;
 	DevBuf<int32_t> len;
@@ -166,7 +189,11 @@ struct Worker {
 	uint64_t graph_builds = 0, graph_launches = 0;
 	// serving: one batch of query reads and its seeds
 	ReadsDev bq;
-	DevBuf<uint8_t> bq_good, b_cons;
+	DevBuf<uint8_t> b_cons;
+	DevBuf<char> b_raw;
+	DevBuf<uint64_t> b_koff;
+	HostBuf<uint64_t> h_meta; // per-read base / word / key offsets of the batch
+	HostBuf<char> h_bytes;    // gather / scatter of batches whose reads are not back to back
 	DevBuf<uint32_t> b_tally, b_cnt, b_off, b_gflag, b_pflag, b_gx, b_px, b_glist, b_plist;
 	DevBuf<uint64_t> b_kref, b_keys, b_vals, b_keys2, b_vals2;
 	DevBuf<uint8_t> b_sort_tmp;
@@ -193,7 +220,8 @@ struct Worker {
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
 		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
 		sub_pairs.release(), sub_pair_off.release(), votes.release();
-		bq.release(), bq_good.release(), b_cons.release(), b_tally.release(), b_cnt.release(), b_off.release();
+		bq.release(), b_cons.release(), b_raw.release(), b_koff.release(), h_meta.release(), h_bytes.release();
+		b_tally.release(), b_cnt.release(), b_off.release();
 		b_gflag.release(), b_pflag.release(), b_gx.release(), b_px.release(), b_glist.release(), b_plist.release();
 		b_kref.release();
 		b_keys.release(), b_vals.release(), b_keys2.release(), b_vals2.release(), b_sort_tmp.release();
@@ -824,29 +852,58 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
 	if (n >= (1 << 22)) return fail(c, HERO_ERR_LIMIT, "at most 2^22 reads per batch");
 	const int k = c->o.k;
-	std::vector<uint8_t> good((size_t) n);
-	std::vector<uint64_t> kref;
+	const uint32_t min_ov = (uint16_t) c->o.min_overlap;
+	// per-read offsets of the batch: bases (boff), packed words (woff) and key references (koff);
+	// reads not longer than the minimum overlap can never be good and get no keys
+	HIPCHK(c, w->h_meta.ensure((size_t) 3 * (n + 1)));
+	uint64_t *hb = w->h_meta.p, *hw = hb + (n + 1), *hk = hw + (n + 1);
+	uint64_t bases = 0, rwords = 0, nkeys64 = 0;
+	bool contiguous = true;
 	int max_len = c->r_max_len;
 	for (int64_t i = 0; i < n; i++) {
-		good[(size_t) i] = query_good(seq + off[i], len[i], c->o.min_overlap);
+		hb[i] = bases, hw[i] = rwords, hk[i] = nkeys64;
+		contiguous = contiguous && off[i] == off[0] + (int64_t) bases;
+		bases += (uint64_t) len[i], rwords += (uint64_t) (len[i] + 31) / 32;
+		if ((uint64_t) len[i] > min_ov) nkeys64 += (uint64_t) num_keys(len[i], k);
 		max_len = std::max(max_len, len[i]);
-		if (!good[(size_t) i]) continue;
-		const int nk = num_keys(len[i], k);
-		for (int t = 0; t < nk; t++) kref.push_back(((uint64_t) i << 14) | (uint64_t) key_pos(len[i], k, t));
 	}
-	if (kref.size() >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "batch has too many keys");
-	const uint32_t nkeys = (uint32_t) kref.size();
-	if (int r = upload_reads_on(c, w->st, w->bq, n, seq, off, len, uid, good)) return r;
-	int64_t bases = 0;
-	for (int64_t i = 0; i < n; i++) bases += len[i];
-	HIPCHK(c, w->bq_good.ensure((size_t) n + 1));
+	hb[n] = bases, hw[n] = rwords, hk[n] = nkeys64;
+	if (nkeys64 >= (1ull << 31)) return fail(c, HERO_ERR_LIMIT, "batch has too many keys");
+	const uint32_t nkeys = (uint32_t) nkeys64;
+	// the batch's bytes go to the device as they are (one copy when its reads are back to back);
+	// packing, the quality filter and the key references are computed there
+	const char *src = n ? seq + off[0] : nullptr;
+	if (!contiguous) {
+		HIPCHK(c, w->h_bytes.ensure((size_t) bases + 1));
+		for (int64_t i = 0; i < n; i++) std::memcpy(w->h_bytes.p + hb[i], seq + off[i], (size_t) len[i]);
+		src = w->h_bytes.p;
+	}
+	ReadsDev &d = w->bq;
+	HIPCHK(c, d.words.ensure((size_t) rwords + 2));
+	HIPCHK(c, d.woff.ensure((size_t) n + 1));
+	HIPCHK(c, d.boff.ensure((size_t) n + 1));
+	HIPCHK(c, d.len.ensure((size_t) n + 1));
+	HIPCHK(c, d.uid.ensure((size_t) n + 1));
+	HIPCHK(c, d.good.ensure((size_t) n + 1));
+	HIPCHK(c, w->b_koff.ensure((size_t) n + 1));
+	HIPCHK(c, w->b_raw.ensure((size_t) bases + 1));
 	HIPCHK(c, w->b_tally.ensure((size_t) 4 * bases + 4));
 	HIPCHK(c, w->b_cons.ensure((size_t) bases + 1));
 	HIPCHK(c, w->b_kref.ensure((size_t) nkeys + 1));
 	HIPCHK(c, w->b_cnt.ensure((size_t) nkeys + 1));
 	HIPCHK(c, w->b_off.ensure((size_t) nkeys + 1));
+	if (bases) HIPCHK(c, hipMemcpyAsync(w->b_raw.p, src, (size_t) bases, hipMemcpyHostToDevice, w->st));
+	if (n) {
+		HIPCHK(c, hipMemcpyAsync(d.boff.p, hb, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
+		HIPCHK(c, hipMemcpyAsync(d.woff.p, hw, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
+		HIPCHK(c, hipMemcpyAsync(w->b_koff.p, hk, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
+		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, w->st));
+		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
+	}
+	d.n = n;
+	HIPCHK(c, launch_read_intake(w->b_raw.p, d.boff.p, d.len.p, d.woff.p, w->b_koff.p, n, k, min_ov, d.words.p, rwords,
+			d.good.p, w->b_kref.p, w->st));
 	HIPCHK(c, hipMemsetAsync(w->b_tally.p, 0, (size_t) 16 * bases, w->st));
-	if (nkeys) HIPCHK(c, hipMemcpyAsync(w->b_kref.p, kref.data(), (size_t) nkeys * 8, hipMemcpyHostToDevice, w->st));
 	const Packed Q = view(w->bq, w->bq.good.p);
 	const Packed S = view(c->Rd, c->Rd.good.p);
 	const DevIndex six = c->six.view(k);
@@ -940,18 +997,19 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	w->acc->ms_score_vote += elapsed(w, 3, 4);
 	w->acc->dp_tasks += ntasks;
 	// consensus of the batch, corrected bases back to the caller's layout
+	// corrected bytes: the batch's own bytes, overwritten by the consensus of its good reads
 	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
+	if (bases) HIPCHK(c, hipMemcpyAsync(w->b_cons.p, w->b_raw.p, (size_t) bases, hipMemcpyDeviceToDevice, w->st));
 	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
-	std::vector<char> h((size_t) bases);
-	if (bases) HIPCHK(c, hipMemcpyAsync(h.data(), w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
-	HIPCHK(c, hipStreamSynchronize(w->st));
-	uint64_t b = 0;
-	for (int64_t i = 0; i < n; i++) {
-		const int L = len[i];
-		std::memcpy(corrected + off[i], good[(size_t) i] ? h.data() + b : seq + off[i], (size_t) L);
-		b += (uint64_t) L;
+	if (bases) {
+		// back in the caller's layout: one copy when its reads are back to back
+		char *dst = contiguous ? corrected + off[0] : w->h_bytes.p;
+		HIPCHK(c, hipMemcpyAsync(dst, w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
 	}
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	if (!contiguous)
+		for (int64_t i = 0; i < n; i++) std::memcpy(corrected + off[i], w->h_bytes.p + hb[i], (size_t) len[i]);
 	w->acc->ms_consensus += elapsed(w, 5, 6);
 	w->acc->queries += (uint64_t) n;
 	return HERO_OK;
@@ -1664,7 +1722,7 @@ extern "C" int hero_server_start(hero_ctx *c, const hero_server_options *opt, he
 	sv->c = c;
 	if (opt) sv->opt = *opt;
 	if (sv->opt.workers <= 0) sv->opt.workers = 2;
-	if (sv->opt.max_step_reads <= 0) sv->opt.max_step_reads = 16384;
+	if (sv->opt.max_step_reads <= 0) sv->opt.max_step_reads = 32768;
 	if (sv->opt.hbm_budget <= 0) {
 		size_t fr = 0, tot = 0;
 		(void) hipMemGetInfo(&fr, &tot);

@@ -167,6 +167,9 @@ hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_bo
 // query << 14 | position) and its reverse complement probe them. Count pass (keys_out == nullptr)
 // writes per-key seed counts; emit pass writes, at off[key], keys_out = strand << 28 | posQ << 14 |
 // posS and vals_out = subject << 22 | query.
+hipError_t launch_read_intake(const char *raw, const uint64_t *boff, const int32_t *len, const uint64_t *woff,
+		const uint64_t *koff, int64_t n, int k, uint32_t min_ov, uint64_t *words, uint64_t nwords, uint8_t *good,
+		uint64_t *kref, hipStream_t st);
 hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &six, const Opts &o, const uint64_t *kref,
 		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st);
 // A serving batch's sorted seeds (hi = subject << 22 | query, lo = strand << 28 | posQ << 14 |

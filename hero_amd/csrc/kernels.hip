@@ -605,6 +605,58 @@ hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, co
 // reverse complement matching s at p is strand-1 window Ls - p - k (searchHashTable2,
 // src/SubjectReadAligner.cpp:444-491, with the index and the probe sides exchanged).
 // ---------------------------------------------------------------------------------------------
+// Batch intake on the device: one wavefront per read packs its bytes into 2-bit MSB-first words
+// (lane per 32-base word; every word written in full, so no clearing is needed), evaluates the
+// query predicate len > (uint16) minOverlap && qualityFilter (src/QueryDataset.cpp:60-75,226) with
+// wave-reduced base counts, and writes the read's key references (pos 0, k, 2k, ... and L - k,
+// src/HashTableLongKey.cpp:279-289) at koff. Keys are written for every read longer than the
+// minimum overlap; the probe skips those of reads that fail the quality filter.
+__global__ __launch_bounds__(256) void k_read_intake(const char *raw, const uint64_t *boff, const int32_t *len,
+		const uint64_t *woff, const uint64_t *koff, int64_t n, int k, uint32_t min_ov, uint64_t *words, uint64_t nwords,
+		uint8_t *good, uint64_t *kref) {
+	const int lane = lane_id();
+	if (blockIdx.x == 0 && threadIdx.x < 2) words[nwords + threadIdx.x] = 0; // tail padding
+	for (int64_t r = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6); r < n; r += (int64_t) gridDim.x * 4) {
+		const int L = len[r];
+		const char *s = raw + boff[r];
+		uint64_t *wd = words + woff[r];
+		uint32_t cnt[4] = {0, 0, 0, 0}, invalid = 0;
+		for (int wi = lane; wi * 32 < L; wi += 64) {
+			const int m = L - wi * 32 < 32 ? L - wi * 32 : 32;
+			uint64_t x = 0;
+			for (int t = 0; t < m; t++) {
+				const char ch = s[wi * 32 + t];
+				const int code = ascii_code(ch);
+				x |= (uint64_t) code << (62 - 2 * t);
+				const bool ok = ch == 'A' || ch == 'C' || ch == 'G' || ch == 'T';
+				invalid += ok ? 0u : 1u;
+				cnt[code] += ok ? 1u : 0u;
+			}
+			wd[wi] = x;
+		}
+		invalid = wave_sum(invalid);
+#pragma unroll
+		for (int c = 0; c < 4; c++) cnt[c] = wave_sum(cnt[c]);
+		const bool longer = (uint64_t) L > (uint64_t) min_ov;
+		if (lane == 0) {
+			const uint64_t thr = (uint64_t) ((double) L * .8);
+			const bool skewed = cnt[0] >= thr || cnt[1] >= thr || cnt[2] >= thr || cnt[3] >= thr;
+			good[r] = longer && invalid == 0 && !skewed ? 1 : 0;
+		}
+		if (longer) {
+			const int nk = num_keys(L, k);
+			for (int t = lane; t < nk; t += 64) kref[koff[r] + t] = ((uint64_t) r << 14) | (uint64_t) key_pos(L, k, t);
+		}
+	}
+}
+hipError_t launch_read_intake(const char *raw, const uint64_t *boff, const int32_t *len, const uint64_t *woff,
+		const uint64_t *koff, int64_t n, int k, uint32_t min_ov, uint64_t *words, uint64_t nwords, uint8_t *good,
+		uint64_t *kref, hipStream_t st) {
+	const int64_t blocks = n / 4 + 1 < 16384 ? n / 4 + 1 : 16384;
+	k_read_intake<<<(unsigned) blocks, 256, 0, st>>>(raw, boff, len, woff, koff, n, k, min_ov, words, nwords, good, kref);
+	return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevIndex six, Opts o, const uint64_t *kref,
 		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out) {
 	const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -612,6 +664,10 @@ __global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevInde
 	const uint64_t r = kref[t];
 	const int64_t q = (int64_t) (r >> 14);
 	const int pos = (int) (r & 0x3FFF);
+	if (!Q.good[q]) { // a read that fails the quality filter has no seeds
+		if (!keys_out) cnt[t] = 0;
+		return;
+	}
 	const int k = o.k;
 	uint64_t fh, fl;
 	kmer_fwd(Q.words + Q.woff[q], pos, k, fh, fl);

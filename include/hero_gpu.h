@@ -185,7 +185,7 @@ int hero_correct_batch(hero_ctx *ctx, int64_t n, const char *seq, const int64_t 
 typedef struct hero_server hero_server;
 typedef struct {
 	int32_t workers;        /* concurrent steps (streams); 0 = 2 */
-	int32_t max_step_reads; /* reads merged into one step at most; 0 = 16384 */
+	int32_t max_step_reads; /* reads merged into one step at most; 0 = 32768 */
 	int64_t hbm_budget;     /* device bytes for in-flight steps; 0 = half of the free memory */
 	int32_t reserved[8];
 } hero_server_options;

@@ -42,6 +42,41 @@ def test_batch_matches_oracle(served):
     assert ro["stats"]["records_kept"] > 0
 
 
+def test_batch_layout_and_filtered_reads(served):
+    """Reads the quality filter drops (an N, one base at >= 80%, not longer than the minimum overlap)
+    come back unchanged and the rest match the oracle; the same reads laid out in reverse order
+    with gaps between them give the same bytes at their own offsets."""
+    rs, resident, fresh, again, ctx = served
+    seqs = [rs.read(int(i)) for i in fresh[:240]]
+    uids = [int(rs.unique_id[i]) for i in fresh[:240]]
+    seqs[3] = seqs[3][:50] + "N" + seqs[3][51:]
+    cut = int(len(seqs[7]) * 0.85)
+    seqs[7] = "A" * cut + seqs[7][cut:]
+    seqs[11] = seqs[11][:40]  # equal to the minimum overlap: not longer
+    seqs[12] = seqs[12][:41]
+    flat = synth.ReadSet.from_strings(seqs, uids)
+    out = ctx.correct_batch(flat)
+    ro = O.run(O.make_cfg(**OPTS), flat, resident, threads=8)
+    assert np.array_equal(out, ro["corrected"])
+    for i in (3, 7, 11):
+        o, n = int(flat.offset[i]), int(flat.length[i])
+        assert out[o:o + n].tobytes().decode() == seqs[i]
+    n = flat.n
+    offs = np.zeros(n, dtype=np.int64)
+    pos = 0
+    for i in range(n - 1, -1, -1):
+        offs[i] = pos
+        pos += int(flat.length[i]) + 7
+    seq = np.full(pos, ord("x"), dtype=np.uint8)
+    for i in range(n):
+        seq[offs[i]:offs[i] + flat.length[i]] = flat.seq[flat.offset[i]:flat.offset[i] + flat.length[i]]
+    gapped = synth.ReadSet(seq, offs, flat.length.copy(), flat.unique_id.copy())
+    out2 = ctx.correct_batch(gapped)
+    for i in range(n):
+        a, b, m = int(offs[i]), int(flat.offset[i]), int(flat.length[i])
+        assert np.array_equal(out2[a:a + m], out[b:b + m]), i
+
+
 def test_batch_alone_equals_batch_in_a_mix(served):
     rs, resident, fresh, again, ctx = served
     a = rs.subset(fresh[:300])
