@@ -370,7 +370,8 @@ double elapsed(Worker *w, int a, int b) {
 // One subject chunk [s0, s0 + nsub) of the packed subject set S.
 int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64_t *s_uid, uint32_t nslots);
 int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const DevIndex &ix, ChunkDev &cd,
-		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, const int64_t *s_uid, bool rec_mode);
+		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, const int64_t *s_uid,
+		bool rec_mode);
 
 // Exclusive scan of n u32 on the worker's stream; the total comes back to the host.
 int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, uint64_t &total) {
@@ -625,13 +626,15 @@ int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub,
 	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
 	uint64_t nseeds = 0;
 	if (int r = wscan(c, w, cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
-	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->Qd.boff.p, c->tally.p, s_uid, rec_mode);
+	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->Qd.boff.p, c->tally.p, c->tally.p + 4 * c->q_bases,
+			s_uid, rec_mode);
 }
 
 // The rest of a chunk once its seeds are counted per (subject, strand) segment: seed emission,
 // chaining, planning, DP, scoring, and votes or records.
 int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const DevIndex &ix, ChunkDev &cd,
-		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, const int64_t *s_uid, bool rec_mode) {
+		uint64_t nseeds, int rows_needed, const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, const int64_t *s_uid,
+		bool rec_mode) {
 	const int32_t nsub = cd.nsub;
 	const int64_t s0 = cd.s_begin;
 	unsigned long long *stats = w->dstats.p;
@@ -706,7 +709,7 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, rows_needed)) return r;
 	if (c->tune) return HERO_OK; // autotuning times the DP only: no records, no votes
 	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, stats, w->st));
-	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, stats, w->st));
+	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, diff, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
 	w->acc->ms_seed += elapsed(w, 0, 1);
@@ -887,7 +890,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, d.good.ensure((size_t) n + 1));
 	HIPCHK(c, w->b_koff.ensure((size_t) n + 1));
 	HIPCHK(c, w->b_raw.ensure((size_t) bases + 1));
-	HIPCHK(c, w->b_tally.ensure((size_t) 4 * bases + 4));
+	HIPCHK(c, w->b_tally.ensure((size_t) 5 * bases + 5)); // 4 mismatch counters + 1 run difference per base
 	HIPCHK(c, w->b_cons.ensure((size_t) bases + 1));
 	HIPCHK(c, w->b_kref.ensure((size_t) nkeys + 1));
 	HIPCHK(c, w->b_cnt.ensure((size_t) nkeys + 1));
@@ -903,7 +906,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	d.n = n;
 	HIPCHK(c, launch_read_intake(w->b_raw.p, d.boff.p, d.len.p, d.woff.p, w->b_koff.p, n, k, min_ov, d.words.p, rwords,
 			d.good.p, w->b_kref.p, w->st));
-	HIPCHK(c, hipMemsetAsync(w->b_tally.p, 0, (size_t) 16 * bases, w->st));
+	HIPCHK(c, hipMemsetAsync(w->b_tally.p, 0, (size_t) 20 * bases, w->st));
 	const Packed Q = view(w->bq, w->bq.good.p);
 	const Packed S = view(c->Rd, c->Rd.good.p);
 	const DevIndex six = c->six.view(k);
@@ -987,7 +990,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, launch_serve_a2(S, Q, c->o, cd, v, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[2].a, w->st));
 	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, max_len + 1)) return r;
-	HIPCHK(c, launch_serve_b(S, Q, c->o, cd, v, w->bq.boff.p, w->b_tally.p, stats, w->st));
+	HIPCHK(c, launch_serve_b(S, Q, c->o, cd, v, w->bq.boff.p, w->b_tally.p, w->b_tally.p + 4 * bases, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
 	w->acc->ms_seed += elapsed(w, 0, 1);
@@ -1000,7 +1003,8 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	// corrected bytes: the batch's own bytes, overwritten by the consensus of its good reads
 	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
 	if (bases) HIPCHK(c, hipMemcpyAsync(w->b_cons.p, w->b_raw.p, (size_t) bases, hipMemcpyDeviceToDevice, w->st));
-	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, 0, n, w->b_cons.p, nullptr, w->dstats.p, w->st));
+	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, w->b_tally.p + 4 * bases, 0, n, w->b_cons.p, nullptr,
+			w->dstats.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
 	if (bases) {
 		// back in the caller's layout: one copy when its reads are back to back
@@ -1205,8 +1209,9 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	int64_t qb = 0;
 	for (int64_t i = 0; i < n; i++) qb += len[i];
 	c->q_bases = qb;
-	HIPCHK(c, c->tally.exact((size_t) 4 * qb));
-	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 16 * qb));
+	// 4 mismatch counters per base, then the matched-run difference array (1 per base)
+	HIPCHK(c, c->tally.exact((size_t) 5 * qb));
+	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 20 * qb));
 	c->qix.built = false;
 	c->cons_valid = false;
 	c->st_host.queries = (uint64_t) n;
@@ -1217,7 +1222,7 @@ int hero_build_index(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	// a new index starts a new correction run: clear the vote tallies
-	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 16 * c->q_bases, c->w.st));
+	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * c->q_bases, c->w.st));
 	const uint64_t P = next_prime_above((uint64_t) c->Qd.n * (uint64_t) c->opt.hash_factor);
 	c->cons_valid = false;
 	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
@@ -1264,7 +1269,8 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 	const int64_t n = c->Qd.n;
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
 	HIPCHK(c, hipEventRecord(c->w.ev[5].a, c->w.st));
-	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, n, c->cons_out.p, nullptr,
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p,
+			c->tally.p + 4 * c->q_bases, 0, n, c->cons_out.p, nullptr,
 			c->w.dstats.p, c->w.st));
 	HIPCHK(c, hipEventRecord(c->w.ev[6].a, c->w.st));
 	HIPCHK(c, hipStreamSynchronize(c->w.st));
@@ -1309,7 +1315,8 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 	DevBuf<uint32_t> dec;
 	HIPCHK(c, codes.exact((size_t) c->q_bases));
 	HIPCHK(c, dec.exact((size_t) 4 * c->q_bases));
-	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p, 0, c->Qd.n, codes.p, dec.p,
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p,
+			c->tally.p + 4 * c->q_bases, 0, c->Qd.n, codes.p, dec.p,
 			nullptr, c->w.st));
 	std::vector<uint32_t> h((size_t) 4 * c->q_bases);
 	HIPCHK(c, hipMemcpyAsync(h.data(), dec.p, h.size() * 4, hipMemcpyDeviceToHost, c->w.st));

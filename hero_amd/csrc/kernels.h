@@ -131,7 +131,7 @@ hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const
 hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, unsigned long long *stats,
 		hipStream_t st);
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st);
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st);
 // Record mode: op count (ops == nullptr) or ops of every accepted record slot; each op is
 // (count << 8) | type, type 'M' / 'I' / 'D' or the query base letter of a mismatch run.
 hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,
@@ -160,16 +160,21 @@ constexpr int kCtlRange = 1, kCtlCounter = 16, kCtlWords = 32;
 hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
-hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
-		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st);
+// Tallies: 4 counters per base (mismatch votes by base code) at 4 * q_boff[q], and the matched-run
+// difference array, 1 counter per base, at q_boff[q].
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally,
+		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats,
+		hipStream_t st);
 
+// Serving batch intake: packs raw bytes into 2-bit words, evaluates the query predicate into good
+// and writes key references (query << 14 | position) at koff for reads longer than min_ov.
+hipError_t launch_read_intake(const char *raw, const uint64_t *boff, const int32_t *len, const uint64_t *woff,
+		const uint64_t *koff, int64_t n, int k, uint32_t min_ov, uint64_t *words, uint64_t nwords, uint8_t *good,
+		uint64_t *kref, hipStream_t st);
 // Serving orientation: the subjects' k-mers at every position are indexed; each query key (kref =
 // query << 14 | position) and its reverse complement probe them. Count pass (keys_out == nullptr)
 // writes per-key seed counts; emit pass writes, at off[key], keys_out = strand << 28 | posQ << 14 |
 // posS and vals_out = subject << 22 | query.
-hipError_t launch_read_intake(const char *raw, const uint64_t *boff, const int32_t *len, const uint64_t *woff,
-		const uint64_t *koff, int64_t n, int k, uint32_t min_ov, uint64_t *words, uint64_t nwords, uint8_t *good,
-		uint64_t *kref, hipStream_t st);
 hipError_t launch_query_probe(const Packed &Q, const Packed &S, const DevIndex &six, const Opts &o, const uint64_t *kref,
 		uint32_t nkeys, uint32_t *cnt, const uint32_t *off, uint64_t *keys_out, uint64_t *vals_out, hipStream_t st);
 // A serving batch's sorted seeds (hi = subject << 22 | query, lo = strand << 28 | posQ << 14 |
@@ -190,7 +195,7 @@ hipError_t launch_serve_a1(const Packed &S, const Packed &Q, const Opts &o, cons
 hipError_t launch_serve_a2(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
 		hipStream_t st);
 hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st);
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st);
 
 // Radix sort of (key, value) u64 pairs on the low key_bits bits (serve_sort.hip).
 hipError_t sort_pairs_u64(const uint64_t *keys_in, uint64_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,

@@ -556,7 +556,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 // Votes of the kept records (SubjectRead::updateQueryCoverage, src/SubjectRead.cpp:208-330): one
 // lane per record; the record is replayed from its chain and DP results.
 __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkDev c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats) {
 	const uint32_t i = blockIdx.x * 64u + threadIdx.x;
 	if (i >= nvotes) return;
 	const VoteItem v = c.votes[i];
@@ -564,13 +564,14 @@ __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkD
 	const int64_t s = c.s_begin + v.sub;
 	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) v.q, v.strand);
 	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
-	VoteSink sink{tally + 4 * q_boff[v.q], pr, &stats[kStOob]};
+	const uint64_t b = q_boff[v.q];
+	VoteSink sink{tally + 4 * b, diff + b, pr, &stats[kStOob]};
 	vote_record(c.prog + v.script, R, v.task0, sink);
 }
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st) {
 	if (nvotes == 0) return hipSuccess;
-	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, stats);
+	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, diff, stats);
 	return hipGetLastError();
 }
 
@@ -918,7 +919,7 @@ __global__ __launch_bounds__(64) void k_serve_a2(Packed S, Packed Q, Opts o, Chu
 	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g], c.prog + c.g_prog_off[g]);
 }
 __global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats) {
 	const uint32_t p = blockIdx.x * 64u + threadIdx.x;
 	PassBCounters k;
 	if (p < v.npairs) {
@@ -943,7 +944,8 @@ __global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, Chun
 			const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
 			const int64_t q = (int64_t) (v.hi[base] & kServeQMask);
 			const PairReads prb = make_pair(S, Q, o, s, S.len[s], q, best.strand);
-			VoteSink sink{tally + 4 * q_boff[q], prb, &stats[kStOob]};
+			const uint64_t b = q_boff[q];
+			VoteSink sink{tally + 4 * b, diff + b, prb, &stats[kStOob]};
 			vote_record(c.prog + best.script, R, best.task0, sink);
 		}
 	}
@@ -968,9 +970,9 @@ hipError_t launch_serve_a2(const Packed &S, const Packed &Q, const Opts &o, cons
 	return hipGetLastError();
 }
 hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, const ServeDev &v,
-		const uint64_t *q_boff, uint32_t *tally, unsigned long long *stats, hipStream_t st) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st) {
 	if (v.npairs == 0) return hipSuccess;
-	k_serve_b<<<(v.npairs + 63) / 64, 64, 0, st>>>(S, Q, o, c, v, q_boff, tally, stats);
+	k_serve_b<<<(v.npairs + 63) / 64, 64, 0, st>>>(S, Q, o, c, v, q_boff, tally, diff, stats);
 	return hipGetLastError();
 }
 
@@ -1417,11 +1419,13 @@ hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStr
 // Consensus (QueryDataset::callConsensus, src/QueryDataset.cpp:393-412)
 // ---------------------------------------------------------------------------------------------
 // Writes the corrected bases of good reads as ASCII at their base offsets.
-// Consensus, one wavefront per read: lanes take 64 positions at a time, the own-base difference
-// array is decoded by a wave-wide inclusive scan (wrapping u32 sums) with a carry between rounds,
-// then every position decides independently (consensus_base).
+// Consensus, one wavefront per read: lanes take 64 positions at a time, the matched-run
+// difference array is decoded by a wave-wide inclusive scan (wrapping u32 sums) with a carry
+// between rounds and added to the own base's counter, then every position decides independently
+// (consensus_base).
 __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally,
-		int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats) {
+		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
+		unsigned long long *stats) {
 	const int64_t i = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
 	const int lane = lane_id();
 	if (i >= nq) return;
@@ -1433,21 +1437,21 @@ __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint6
 	uint32_t carry = 0, changed = 0;
 	for (int p0 = 0; p0 < L; p0 += 64) {
 		const int p = p0 + lane;
-		uint32_t c[4] = {0, 0, 0, 0};
+		uint32_t c[4] = {0, 0, 0, 0}, x = 0;
 		int own = 0;
 		if (p < L) {
 			const uint4 t = *(const uint4 *) (tally + 4 * (b + p));
 			c[0] = t.x, c[1] = t.y, c[2] = t.z, c[3] = t.w;
+			x = diff[b + p];
 			own = base_at(w, p);
 		}
-		uint32_t x = c[0];
-		x = own == 1 ? c[1] : x;
-		x = own == 2 ? c[2] : x;
-		x = own == 3 ? c[3] : x;
 		const uint32_t run = carry + wave_excl_scan(x) + x;
 		carry = __shfl(run, 63, 64);
 		if (p >= L) continue;
-		c[own] = run;
+		c[0] += own == 0 ? run : 0u;
+		c[1] += own == 1 ? run : 0u;
+		c[2] += own == 2 ? run : 0u;
+		c[3] += own == 3 ? run : 0u;
 		if (decoded_out) *(uint4 *) (decoded_out + 4 * (b + p)) = make_uint4(c[0], c[1], c[2], c[3]);
 		if (!good) continue; // not corrected (src/QueryDataset.cpp:404-406); the host copies its bytes
 		c[own] += 1;         // self vote (src/QueryRead.cpp:149)
@@ -1458,10 +1462,12 @@ __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint6
 	const uint64_t m = __ballot(changed != 0);
 	if (stats && good && lane == 0 && m) stat_add(stats, kStReadsChanged, 1ull);
 }
-hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally, int64_t q0,
-		int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats, hipStream_t st) {
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally,
+		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats,
+		hipStream_t st) {
 	if (nq == 0) return hipSuccess;
-	k_consensus<<<(unsigned) ((nq + 3) / 4), 256, 0, st>>>(Q, o, q_boff, tally, q0, nq, codes_out, decoded_out, stats);
+	k_consensus<<<(unsigned) ((nq + 3) / 4), 256, 0, st>>>(Q, o, q_boff, tally, diff, q0, nq, codes_out, decoded_out,
+			stats);
 	return hipGetLastError();
 }
 

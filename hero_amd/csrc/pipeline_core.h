@@ -304,8 +304,13 @@ HD void replay_script(const uint32_t *sc, uint32_t task, V &v) {
 // Tally of one query read: 4 u32 per base at index 4*p + code. The slot of the read's own base
 // holds a difference array of M-run votes (+1 at a run's first base, -1 after its last), the
 // other three slots hold mismatch votes; consensus_read decodes it.
+// A query's votes: `tally` holds 4 counters per base (votes for each base code cast by
+// mismatches), `diff` one counter per base, the difference array of the matched-run coverage
+// (+1 where a run of matches starts, -1 one past its end), so a run costs two updates and needs
+// no lookup of the query's own bases. Consensus adds its prefix sum to the own base's counter.
 struct VoteSink {
-	uint32_t *tally; // this query's tally
+	uint32_t *tally; // 4 per base
+	uint32_t *diff;  // 1 per base
 	PairReads pr;
 	unsigned long long *oob; // counts votes that would leave a read (the reference throws out_of_range)
 	HD void add_u32(uint32_t *p, uint32_t v) const {
@@ -328,8 +333,8 @@ struct VoteSink {
 			flag_oob();
 			return;
 		}
-		add_u32(&tally[4 * p + pr.qbase(p)], 1u);
-		if (p + c < pr.Lq) add_u32(&tally[4 * (p + c) + pr.qbase(p + c)], 0xFFFFFFFFu);
+		add_u32(&diff[p], 1u);
+		if (p + c < pr.Lq) add_u32(&diff[p + c], 0xFFFFFFFFu);
 	}
 	HD void mismatch(int pq, int ps) const {
 		if (pq < 0 || pq >= pr.Lq || ps < 0 || ps >= pr.Ls) {
@@ -445,8 +450,8 @@ HD int consensus_base(const Opts &o, const uint32_t c[4], int own) {
 	return (largest - larger >= o.consensus_freq) ? id : own;
 }
 
-HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed &Q, int64_t r, uint8_t *out_codes,
-		uint32_t *decoded /* optional 4*L, pre-self-vote counts */) {
+HD int consensus_read(const Opts &o, const uint32_t *tally, const uint32_t *diff, int L, const Packed &Q, int64_t r,
+		uint8_t *out_codes, uint32_t *decoded /* optional 4*L, pre-self-vote counts */) {
 	uint32_t run = 0;
 	int changed = 0;
 	const uint64_t *w = Q.words + Q.woff[r];
@@ -454,8 +459,8 @@ HD int consensus_read(const Opts &o, const uint32_t *tally, int L, const Packed 
 		const int own = base_at(w, p);
 		uint32_t c[4];
 		for (int j = 0; j < 4; j++) c[j] = tally[4 * p + j];
-		run += c[own];
-		c[own] = run;
+		run += diff[p];
+		c[own] += run;
 		if (decoded)
 			for (int j = 0; j < 4; j++) decoded[4 * p + j] = c[j];
 		c[own] += 1; // self vote (:149)

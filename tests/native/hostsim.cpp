@@ -127,7 +127,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 	// tally per query (4 u32 per base at the query's base offset)
 	int64_t qbases = 0;
 	for (int64_t q = 0; q < qr->n; q++) qbases = std::max<int64_t>(qbases, qr->offset[q] + qr->length[q]);
-	std::vector<uint32_t> tally((size_t) 4 * qbases, 0);
+	std::vector<uint32_t> tally((size_t) 4 * qbases, 0), diff((size_t) qbases, 0);
 	uint64_t st_seeds = 0, st_tasks = 0, st_cells_exec = 0, st_calls_exec = 0, st_built = 0, st_kept = 0, st_cells_all = 0;
 	unsigned long long oob = 0;
 	std::vector<uint32_t> steps;
@@ -244,7 +244,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 			st_kept++;
 			if (!o.call_consensus) continue;
 			PairReads pr = make_pair_reads(S.P, Q.P, o.call_consensus, s, Ls, q, best.strand);
-			VoteSink sink{tally.data() + 4 * qr->offset[q], pr, &oob};
+			VoteSink sink{tally.data() + 4 * qr->offset[q], diff.data() + qr->offset[q], pr, &oob};
 			vote_record(prog.data() + best.script, R, best.task0, sink);
 		}
 	}
@@ -255,7 +255,8 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 	for (int64_t q = 0; q < qr->n; q++) {
 		const int L = qr->length[q];
 		codes.resize(L);
-		consensus_read(o, tally.data() + 4 * qr->offset[q], L, Q.P, q, codes.data(), decoded.data() + 4 * qr->offset[q]);
+		consensus_read(o, tally.data() + 4 * qr->offset[q], diff.data() + qr->offset[q], L, Q.P, q, codes.data(),
+				decoded.data() + 4 * qr->offset[q]);
 		if (qg[q] && corrected)
 			for (int p = 0; p < L; p++) corrected[qr->offset[q] + p] = code_ascii(codes[p]);
 	}
