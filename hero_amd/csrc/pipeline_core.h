@@ -181,7 +181,8 @@ HD int step_type(uint32_t code, bool swapped) {
 
 // Feeds one DP call's steps in order as record ops (addAlignment, src/AlignmentRecord.cpp:50-54),
 // as runs: op(count, type) for each maximal run of equal op types (a mismatch run is one call
-// with its length). A 16-step word that is all matches extends the current run in one step.
+// with its length). Within a 16-step word the loop visits only the non-match steps (match = 0, so
+// the lowest set bit pair is the next one); the matches between them extend the current run.
 template <class F>
 HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 	const DpTask &t = R.tasks[task];
@@ -190,31 +191,31 @@ HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 	const uint32_t *w = R.words + t.out_off;
 	int e = cap - n;
 	int run_t = -1, run_c = 0;
+	auto push = [&](int c, int ty) {
+		if (ty == run_t) {
+			run_c += c;
+		} else {
+			if (run_c) op(run_c, run_t);
+			run_t = ty, run_c = c;
+		}
+	};
+	static_assert(kStepMatch == 0, "match steps are zero bits");
 	while (e < cap) {
 		const int stop = (e | 15) + 1 < cap ? (e | 15) + 1 : cap;
 		const int m = stop - e;
 		uint32_t word = w[e >> 4] >> (2 * (e & 15));
-		const uint32_t mask = m == 16 ? 0xFFFFFFFFu : ((1u << (2 * m)) - 1u);
-		static_assert(kStepMatch == 0, "an all-zero word is all matches");
-		if ((word & mask) == 0u) { // m matches
-			if (run_t == kOpM) {
-				run_c += m;
-			} else {
-				if (run_c) op(run_c, run_t);
-				run_t = kOpM, run_c = m;
-			}
-			e = stop;
-			continue;
+		if (m < 16) word &= (1u << (2 * m)) - 1u;
+		int done = 0;
+		while (word) {
+			const int j = __builtin_ctz(word) >> 1; // matches before the next non-match step
+			if (j) push(j, kOpM);
+			word >>= 2 * j;
+			push(1, step_type(word & 3u, swapped));
+			word >>= 2;
+			done += j + 1;
 		}
-		for (; e < stop; e++, word >>= 2) {
-			const int ty = step_type(word & 3u, swapped);
-			if (ty == run_t) {
-				run_c++;
-			} else {
-				if (run_c) op(run_c, run_t);
-				run_t = ty, run_c = 1;
-			}
-		}
+		if (m > done) push(m - done, kOpM);
+		e = stop;
 	}
 	if (run_c) op(run_c, run_t);
 }
