@@ -9,6 +9,8 @@
 // cell here, and sentinel-derived values only ever compete against real values (which stay far
 // above -MAXDUL), so every comparison the reference makes has the same outcome.
 #pragma once
+#include <type_traits>
+
 #include "hd.h"
 
 namespace hero {
@@ -18,6 +20,23 @@ constexpr int kCodeN = 4;    // 'N' (BandAligner::CWILD) in the standalone align
 constexpr int kCodeNone = 7; // beyond the end of a sequence (never equal to a base)
 
 HD int imax_(int a, int b) { return a > b ? a : b; }
+
+// Wave-uniform row decisions of the per-lane DP (one task per lane): true when the predicate
+// holds in any / every active lane. A single task (host code) is its own wave.
+HD bool wave_any(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __ballot(v) != 0;
+#else
+	return v;
+#endif
+}
+HD bool wave_all(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __ballot(!v) == 0;
+#else
+	return v;
+#endif
+}
 
 // Diagonal score of the global aligner (src/BandAligner.cpp:89-97): a mismatch involving N
 // scores unobserved_match; equal characters (N==N included) score hit.
@@ -76,6 +95,37 @@ HD bool row0_cell(const Opts &o, int j, int b, int lenQ, int &h, int &x, int &y)
 	return in;
 }
 
+// Query window of a row: the code of cell d is Q[i - b - 1 + d]. Streams whose codes are 2-bit
+// bases (Stream::kTwoBit) keep the window packed, 2 bits per cell, in one 64-bit value (band
+// width up to 32 cells); others keep one register per cell. A code past either end of the query
+// is read only by cells outside the matrix, so the packed form may keep it as any base.
+template <int W, bool TWO_BIT>
+struct QWin {
+	int q[W];
+	HD int at(int d) const { return q[d]; }
+	HD void set(int d, int c) { q[d] = c; }
+	HD void slide(int next) {
+#pragma unroll
+		for (int d = 0; d + 1 < W; d++) q[d] = q[d + 1];
+		q[W - 1] = next;
+	}
+	HD uint32_t differs(int d, int sc) const { return q[d] != sc ? 1u : 0u; }
+	HD uint64_t differ_bits(int) const { return 0; }
+};
+template <int W>
+struct QWin<W, true> {
+	static_assert(W <= 32, "packed window holds 32 cells");
+	uint64_t v = 0; // cell d at bits 2d, 2d+1
+	HD int at(int d) const { return (int) ((v >> (2 * d)) & 3u); }
+	HD void set(int d, int c) { v |= (uint64_t) (c & 3) << (2 * d); }
+	HD void slide(int next) { v = (v >> 2) | ((uint64_t) (next & 3) << (2 * (W - 1))); }
+	// bit 2d set where cell d's code differs from sc
+	HD uint64_t differ_bits(int sc) const {
+		const uint64_t x = v ^ (0x5555555555555555ull * (uint64_t) sc);
+		return (x | (x >> 1)) & 0x5555555555555555ull;
+	}
+};
+
 // Forward pass. S.next() / Q.next() yield the aligner subject / query codes in order and
 // kCodeNone past the end. ptrs.put_row(i, words) receives row i's pointers, 16 cells per word,
 // cell d at bits 2*(d%16); codes are the reference's BM1 values: 2 diagonal, 1 left, 0 up.
@@ -85,7 +135,8 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 	constexpr int NW = (W + 15) / 16;
 	int H[W + 1];
 	int X[AFFINE ? W + 1 : 1], Y[AFFINE ? W + 1 : 1];
-	int qw[W]; // qw[d] = Q[i - b - 1 + d] for the current row i
+	constexpr bool TWO_BIT = QS::kTwoBit && W <= 32;
+	QWin<W, TWO_BIT> qw; // qw.at(d) = Q[i - b - 1 + d] for the current row i
 	// Row 0 (global: src/BandAligner.cpp:63-111 at i = 0; affine: :140-171).
 	{
 		uint32_t pw[NW];
@@ -102,38 +153,87 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 		ptrs.put_row(0, pw);
 	}
 #pragma unroll
-	for (int d = 0; d < W; d++) qw[d] = d < b ? kCodeNone : Q.next(); // row 1: Q[d - b]
-	for (int i = 1; i <= lenS; i++) {
-		const int sc = S.next();
-		const int d0 = b - i;        // column j = 0
-		const int dQ = lenQ - i + b; // column j = lenQ
-		const int dlo = d0 > 0 ? d0 : 0;
-		const int dhi = dQ < 2 * b ? dQ : 2 * b;
-		uint32_t pw[NW];
+	for (int d = 0; d < W; d++) qw.set(d, d < b ? kCodeNone : Q.next()); // row 1: Q[d - b]
+	// Rows with column 0 in the band (i <= b) or, for an aligned-left call, column lenQ in the band
+	// (i >= lenQ - b) apply the edge rules and clear every cell outside the matrix. Other rows
+	// ("inner") compute every band cell by the interior recurrence and clear nothing: a cell of
+	// the matrix reads only cells of the matrix (column j - 1 or j of rows i - 1 and i), so values
+	// computed beyond column lenQ are never read and those rows need no masks. Cells beyond the
+	// band edge 2b stay sentinel-derived as long as the left chain is cut at 2b + 1 (needed only
+	// when b < BMAX: otherwise the band edge is the register row's end).
+	auto row = [&](int i, int sc, uint32_t *pw, auto mode_tag) {
+		constexpr int MODE = decltype(mode_tag)::value; // 0 edge, 1 inner, 2 inner with the cut
 #pragma unroll
 		for (int w = 0; w < NW; w++) pw[w] = 0;
 		int hl = kSent, yl = kSent; // this row's cell at d-1
+		if (MODE == 0) {
+			const int d0 = b - i;        // column j = 0
+			const int dQ = lenQ - i + b; // column j = lenQ
+			const int dlo = d0 > 0 ? d0 : 0;
+			const int dhi = dQ < 2 * b ? dQ : 2 * b;
+#pragma unroll
+			for (int d = 0; d < W; d++) {
+				const bool valid = d >= dlo && d <= dhi;
+				int h, x, y, p;
+				cell_update<AFFINE>(o, sc, qw.at(d), d == d0, d == dQ, left_al, right_al, lenQ, H[d],
+						AFFINE ? X[d] : kSent, AFFINE ? Y[d] : kSent, H[d + 1], AFFINE ? X[d + 1] : kSent, hl, yl, h, x,
+						y, p);
+				h = valid ? h : kSent;
+				H[d] = h;
+				hl = h;
+				if (AFFINE) {
+					X[d] = valid ? x : kSent;
+					Y[d] = valid ? y : kSent;
+					yl = Y[d];
+				}
+				pw[d >> 4] |= (valid ? (uint32_t) p : 0u) << (2 * (d & 15));
+			}
+			return;
+		}
+		const int edge_d = 2 * b; // last band cell
+		const uint64_t mbits = qw.differ_bits(sc);
 #pragma unroll
 		for (int d = 0; d < W; d++) {
-			const bool valid = d >= dlo && d <= dhi;
-			int h, x, y, p;
-			cell_update<AFFINE>(o, sc, qw[d], d == d0, d == dQ, left_al, right_al, lenQ, H[d], AFFINE ? X[d] : kSent,
-					AFFINE ? Y[d] : kSent, H[d + 1], AFFINE ? X[d + 1] : kSent, hl, yl, h, x, y, p);
-			h = valid ? h : kSent;
-			H[d] = h;
-			hl = h;
-			if (AFFINE) {
-				X[d] = valid ? x : kSent;
-				Y[d] = valid ? y : kSent;
-				yl = Y[d];
+			// mismatch flag of the cell: from the packed window's bits, or by comparing codes
+			const uint32_t m1 = TWO_BIT ? (uint32_t) (mbits >> (2 * d)) & 1u : (sc != qw.at(d) ? 1u : 0u);
+			const bool mm = m1 != 0;
+			uint32_t p;
+			if (!AFFINE) { // src/BandAligner.cpp:63-111, interior column
+				const int up = H[d + 1] + o.dx;
+				const int lf = hl + o.dy;
+				const int dg = H[d] + (TWO_BIT ? (mm ? o.miss : o.hit) : diag_score_global(o, sc, qw.at(d)));
+				int h;
+				if (dg >= lf && dg >= up) h = dg, p = 2u + m1;
+				else if (lf >= up) h = lf, p = 1u;
+				else h = up, p = 0u;
+				if (MODE == 2) h = d <= edge_d ? h : kSent;
+				H[d] = h;
+				hl = h;
+			} else { // src/BandAligner.cpp:176-222, interior column
+				int y = imax_(hl + o.dy, yl + o.ex);
+				if (MODE == 2) y = d <= edge_d ? y : kSent;
+				const int x = imax_(H[d + 1] + o.dx, X[d + 1] + o.ex);
+				const int h = imax_(imax_(H[d], X[d]), Y[d]) + (mm ? o.miss : o.hit);
+				p = (h > x && h >= y) ? 2u + m1 : (y > x ? 1u : 0u);
+				H[d] = h, X[d] = x, Y[d] = y;
+				hl = h, yl = y;
 			}
-			pw[d >> 4] |= (valid ? (uint32_t) p : 0u) << (2 * (d & 15));
+			pw[d >> 4] |= p << (2 * (d & 15));
 		}
+	};
+	// the widest bands keep one row form: a second one would not fit the register file
+	constexpr bool kInnerRows = BMAX <= 24;
+	const bool full_band = b == BMAX;
+	for (int i = 1; i <= lenS; i++) {
+		const int sc = S.next();
+		uint32_t pw[NW];
+		const bool edge = i <= b || (left_al && lenQ - i <= b);
+		if (!kInnerRows || wave_any(edge)) row(i, sc, pw, std::integral_constant<int, 0>());
+		else if (wave_all(full_band)) row(i, sc, pw, std::integral_constant<int, 1>());
+		else row(i, sc, pw, std::integral_constant<int, 2>());
 		ptrs.put_row(i, pw);
-		// slide the query window to row i+1: qw[d] = Q[i - b + d]
-#pragma unroll
-		for (int d = 0; d + 1 < W; d++) qw[d] = qw[d + 1];
-		qw[W - 1] = (i - b + W - 1 >= 0) ? Q.next() : kCodeNone;
+		// slide the query window to row i+1: cell d reads Q[i - b + d]
+		qw.slide((i - b + W - 1 >= 0) ? Q.next() : kCodeNone);
 	}
 }
 

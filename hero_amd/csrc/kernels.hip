@@ -983,6 +983,7 @@ hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const
 // A read segment as a stream of 2-bit codes (forward, or reverse complement when rc). The current
 // 32-base word stays in a register: one load per 32 bases instead of one per base.
 struct PackedStream {
+	static constexpr bool kTwoBit = true;
 	const uint64_t *w;
 	int L, start, len, i;
 	bool rc;
@@ -1137,9 +1138,16 @@ __global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
+// Occupancy the register allocator plans the one-task-per-lane DP for (waves per SIMD): the most
+// that holds a band row in registers without spilling.
+constexpr int dp_waves_per_simd(int bmax) {
+	return bmax <= 4 ? 4 : bmax <= 8 ? 3 : bmax <= 12 ? 2 : 1;
+}
+
 // Batched form: one task per lane, persistent waves pulling 64 tasks at a time.
 template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) void k_dp_batched(DpLaunch L0) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per_simd(BMAX), 8))) void k_dp_batched(
+		DpLaunch L0) {
 	DpLaunch L = L0;
 	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
@@ -1554,6 +1562,7 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long 
 // Standalone BandAligner batch over bytes (A/C/G/T/N)
 // ---------------------------------------------------------------------------------------------
 struct ByteStreamDev {
+	static constexpr bool kTwoBit = false; // 'N' is code 4
 	const char *s;
 	int len, i;
 	__device__ __forceinline__ int next() {

@@ -62,6 +62,7 @@ struct PackedSet {
 };
 
 struct Stream {
+	static constexpr bool kTwoBit = true;
 	const Packed *P;
 	int64_t r;
 	int start, len, i;
@@ -270,6 +271,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 
 // One BandAligner call on ASCII strings (A/C/G/T/N), BandAligner CIGAR as text.
 struct ByteStream {
+	static constexpr bool kTwoBit = false; // 'N' is code 4
 	const char *s;
 	int len, i;
 	int next() {
@@ -319,10 +321,11 @@ extern "C" int hostsim_band_align(const hs_cfg *cfg, const char *subj, int ls, c
 
 extern "C" uint64_t hostsim_band_cells(int lenS, int lenQ, int b) { return band_cells(lenS, lenQ, b); }
 
-// Packed two-task DP against the one-task DP on random task pairs: every pointer row of both tasks
+// Packed two-task DP against the one-task DP on random task pairs: the tracebacks of both tasks
 // must be equal. Returns the number of differing tasks (sequences with codes 0..3 as in the
 // pipeline; lengths, bands and flags random; options from cfg).
 struct CodeStream {
+	static constexpr bool kTwoBit = true;
 	const std::vector<int> *s;
 	int i;
 	int next() { return i < (int) s->size() ? (*s)[(size_t) i++] : kCodeNone; }
@@ -359,16 +362,18 @@ static int pk16_check_one(const Opts &o, uint64_t &rng, int max_len) {
 	CodeStream S1{&sa, 0}, Q1{&qa, 0}, S2{&sb, 0}, Q2{&qb, 0};
 	dp_forward<BMAX, AFF>(o, A.lenS, A.lenQ, A.b, A.left_al, A.right_al, S1, Q1, ra);
 	dp_forward<BMAX, AFF>(o, B.lenS, B.lenQ, B.b, B.left_al, B.right_al, S2, Q2, rb);
+	// the tracebacks (what every consumer reads) must agree step for step; pointer bits of cells
+	// outside the matrix are never read and may differ
+	auto trace = [](const RowStore &r, const Pk16Task &T, std::vector<int> &steps) {
+		steps.clear();
+		auto emit = [&](int st) { steps.push_back(st); };
+		return dp_traceback(T.lenS, T.lenQ, T.b, r, emit);
+	};
+	std::vector<int> x, y;
 	int bad = 0;
-	for (int i = 0; i <= A.lenS; i++)
-		for (int w = 0; w < nw; w++)
-			if (pa.get(i, w) != ra.get(i, w)) {
-				bad |= 1;
-			}
-	for (int i = 0; i <= B.lenS; i++)
-		for (int w = 0; w < nw; w++)
-			if (pb.get(i, w) != rb.get(i, w)) bad |= 2;
-	return (bad & 1) + ((bad >> 1) & 1);
+	if (trace(pa, A, x) != trace(ra, A, y) || x != y) bad++;
+	if (trace(pb, B, x) != trace(rb, B, y) || x != y) bad++;
+	return bad;
 }
 
 extern "C" int hostsim_pk16_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len) {
