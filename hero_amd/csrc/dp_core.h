@@ -154,83 +154,86 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 	}
 #pragma unroll
 	for (int d = 0; d < W; d++) qw.set(d, d < b ? kCodeNone : Q.next()); // row 1: Q[d - b]
-	// Rows with column 0 in the band (i <= b) or, for an aligned-left call, column lenQ in the band
-	// (i >= lenQ - b) apply the edge rules and clear every cell outside the matrix. Other rows
-	// ("inner") compute every band cell by the interior recurrence and clear nothing: a cell of
-	// the matrix reads only cells of the matrix (column j - 1 or j of rows i - 1 and i), so values
-	// computed beyond column lenQ are never read and those rows need no masks. Cells beyond the
-	// band edge 2b stay sentinel-derived as long as the left chain is cut at 2b + 1 (needed only
-	// when b < BMAX: otherwise the band edge is the register row's end).
+	// Every row computes all band cells by the interior recurrence and clears nothing outside the
+	// matrix: a cell of the matrix reads only cells of the matrix (columns j - 1 and j of rows
+	// i - 1 and i), except the column-0 cell, whose own rule reads only the cell above it. So
+	// values computed left of column 0 or right of column lenQ are never read. The edge rules are
+	// applied where they can occur, by per-cell selects:
+	//   kC0 - column 0 is in the band (i <= b): that cell takes the column-0 rule, whose values
+	//         have a closed form in i (a vertical gap from the origin: the affine x is i * sx
+	//         after setAlignRight, else dx + (i - 1) * ex, with y = dy or x; the global h is i * g0);
+	//   kCQ - an aligned-left call has column lenQ in the band (i >= lenQ - b): that cell's
+	//         vertical move costs sx (src/BandAligner.cpp:89-97,196-200);
+	//   kCut - some band is narrower than BMAX: cells past the band edge 2b are cut from the left
+	//         chain, so they stay sentinel-derived (with b = BMAX the edge is the register row's end).
+	// Row variants are chosen per wavefront (any lane that needs a rule), so lanes stay together;
+	// rows with an edge rule take all three (fewer variants keep the register allocation lean).
+	constexpr int kC0 = 1, kCQ = 2, kCut = 4;
+	const int d0base = b, dQbase = lenQ + b; // d0 = b - i, dQ = lenQ - i + b
+	const int edge_d = 2 * b;                // last band cell
+	const int g0 = (right_al || (lenQ == 0 && left_al)) ? o.sx : o.dx; // global: column-0 vertical cost
 	auto row = [&](int i, int sc, uint32_t *pw, auto mode_tag) {
-		constexpr int MODE = decltype(mode_tag)::value; // 0 edge, 1 inner, 2 inner with the cut
+		constexpr int MODE = decltype(mode_tag)::value;
 #pragma unroll
 		for (int w = 0; w < NW; w++) pw[w] = 0;
 		int hl = kSent, yl = kSent; // this row's cell at d-1
-		if (MODE == 0) {
-			const int d0 = b - i;        // column j = 0
-			const int dQ = lenQ - i + b; // column j = lenQ
-			const int dlo = d0 > 0 ? d0 : 0;
-			const int dhi = dQ < 2 * b ? dQ : 2 * b;
-#pragma unroll
-			for (int d = 0; d < W; d++) {
-				const bool valid = d >= dlo && d <= dhi;
-				int h, x, y, p;
-				cell_update<AFFINE>(o, sc, qw.at(d), d == d0, d == dQ, left_al, right_al, lenQ, H[d],
-						AFFINE ? X[d] : kSent, AFFINE ? Y[d] : kSent, H[d + 1], AFFINE ? X[d + 1] : kSent, hl, yl, h, x,
-						y, p);
-				h = valid ? h : kSent;
-				H[d] = h;
-				hl = h;
-				if (AFFINE) {
-					X[d] = valid ? x : kSent;
-					Y[d] = valid ? y : kSent;
-					yl = Y[d];
-				}
-				pw[d >> 4] |= (valid ? (uint32_t) p : 0u) << (2 * (d & 15));
-			}
-			return;
-		}
-		const int edge_d = 2 * b; // last band cell
+		const int d0 = d0base - i, dQ = left_al ? dQbase - i : -1;
+		const int x0 = AFFINE ? (right_al ? i * o.sx : o.dx + (i - 1) * o.ex) : i * g0; // column-0 values
+		const int y0 = right_al ? o.dy : x0;
 		const uint64_t mbits = qw.differ_bits(sc);
 #pragma unroll
 		for (int d = 0; d < W; d++) {
 			// mismatch flag of the cell: from the packed window's bits, or by comparing codes
 			const uint32_t m1 = TWO_BIT ? (uint32_t) (mbits >> (2 * d)) & 1u : (sc != qw.at(d) ? 1u : 0u);
 			const bool mm = m1 != 0;
+			const bool c0 = (MODE & kC0) && d == d0;
+			const bool cq = (MODE & kCQ) && d == dQ;
 			uint32_t p;
-			if (!AFFINE) { // src/BandAligner.cpp:63-111, interior column
-				const int up = H[d + 1] + o.dx;
+			if (!AFFINE) { // src/BandAligner.cpp:63-111
+				const int g = (MODE & kCQ) ? (cq ? o.sx : o.dx) : o.dx;
+				const int up = H[d + 1] + g;
 				const int lf = hl + o.dy;
 				const int dg = H[d] + (TWO_BIT ? (mm ? o.miss : o.hit) : diag_score_global(o, sc, qw.at(d)));
 				int h;
 				if (dg >= lf && dg >= up) h = dg, p = 2u + m1;
 				else if (lf >= up) h = lf, p = 1u;
 				else h = up, p = 0u;
-				if (MODE == 2) h = d <= edge_d ? h : kSent;
+				if (MODE & kC0) { // column 0: the vertical move only
+					h = c0 ? x0 : h;
+					p = c0 ? 0u : p;
+				}
+				if (MODE & kCut) h = d <= edge_d ? h : kSent;
 				H[d] = h;
 				hl = h;
-			} else { // src/BandAligner.cpp:176-222, interior column
+			} else { // src/BandAligner.cpp:150-222
 				int y = imax_(hl + o.dy, yl + o.ex);
-				if (MODE == 2) y = d <= edge_d ? y : kSent;
-				const int x = imax_(H[d + 1] + o.dx, X[d + 1] + o.ex);
-				const int h = imax_(imax_(H[d], X[d]), Y[d]) + (mm ? o.miss : o.hit);
+				if (MODE & kCut) y = d <= edge_d ? y : kSent;
+				int x = imax_(H[d + 1] + o.dx, X[d + 1] + o.ex);
+				if (MODE & kCQ) x = cq ? imax_(H[d + 1], X[d + 1]) + o.sx : x;
+				int h = imax_(imax_(H[d], X[d]), Y[d]) + (mm ? o.miss : o.hit);
 				p = (h > x && h >= y) ? 2u + m1 : (y > x ? 1u : 0u);
+				if (MODE & kC0) { // column 0 (src/BandAligner.cpp:150-163)
+					x = c0 ? x0 : x;
+					y = c0 ? y0 : y;
+					h = c0 ? x0 : h;
+					p = c0 ? 0u : p;
+				}
 				H[d] = h, X[d] = x, Y[d] = y;
 				hl = h, yl = y;
 			}
 			pw[d >> 4] |= p << (2 * (d & 15));
 		}
 	};
-	// the widest bands keep one row form: a second one would not fit the register file
-	constexpr bool kInnerRows = BMAX <= 24;
-	const bool full_band = b == BMAX;
+	const int cut = wave_all(b == BMAX) ? 0 : kCut;
 	for (int i = 1; i <= lenS; i++) {
 		const int sc = S.next();
 		uint32_t pw[NW];
-		const bool edge = i <= b || (left_al && lenQ - i <= b);
-		if (!kInnerRows || wave_any(edge)) row(i, sc, pw, std::integral_constant<int, 0>());
-		else if (wave_all(full_band)) row(i, sc, pw, std::integral_constant<int, 1>());
-		else row(i, sc, pw, std::integral_constant<int, 2>());
+		const int mode = cut | (wave_any(i <= b) ? kC0 : 0) | (wave_any(left_al && lenQ - i <= b) ? kCQ : 0);
+		switch (mode) {
+		case 0: row(i, sc, pw, std::integral_constant<int, 0>()); break;
+		case kCut: row(i, sc, pw, std::integral_constant<int, kCut>()); break;
+		default: row(i, sc, pw, std::integral_constant<int, kC0 | kCQ | kCut>()); break;
+		}
 		ptrs.put_row(i, pw);
 		// slide the query window to row i+1: cell d reads Q[i - b + d]
 		qw.slide((i - b + W - 1 >= 0) ? Q.next() : kCodeNone);
