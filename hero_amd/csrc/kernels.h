@@ -146,7 +146,7 @@ constexpr int kPk16MaxBand = 24; // widest band class of the two-tasks-per-lane 
 // longest rows first, so the 64 lanes of a wavefront run tasks of similar length.
 constexpr int kNumBandClasses = 6;
 constexpr int kBandClass[kNumBandClasses] = {4, 8, 12, 16, 24, 32};
-constexpr int kLenBins = 1024;
+constexpr int kLenBins = 1024; // per band class: 512 row counts x (aligned-left or not)
 constexpr int kNumBuckets = (kNumBandClasses + 1) * kLenBins;
 // ntasks_dev (optional) supplies the task count on the device; the grid then covers max_tasks
 hipError_t launch_task_histogram(const DpTask *tasks, uint32_t ntasks, const uint32_t *ntasks_dev, int rows_max,

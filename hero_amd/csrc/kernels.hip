@@ -1209,7 +1209,10 @@ __device__ __forceinline__ int task_bucket(const DpTask &t, int rows_max) {
 	if (lenS < rows_max)
 		for (int c = kNumBandClasses - 1; c >= 0; c--)
 			if (t.band <= kBandClass[c]) cls = c;
-	const int bin = kLenBins - 1 - (lenS < kLenBins ? lenS : kLenBins - 1);
+	// descending length, and within a length the aligned-left calls together: a wavefront of
+	// aligned-left calls is the only kind whose last rows need the column-lenQ rule
+	constexpr int kLenRows = kLenBins / 2;
+	const int bin = 2 * (kLenRows - 1 - (lenS < kLenRows ? lenS : kLenRows - 1)) + ((t.flags & kTaskAlignLeft) ? 1 : 0);
 	return cls * kLenBins + bin;
 }
 
