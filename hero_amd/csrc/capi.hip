@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/hero_gpu.h"
+#include "dp_pk16.h"
 #include "kernels.h"
 #include "pack.h"
 #include "pipeline_core.h"
@@ -393,7 +394,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	hero_tune_class &tc = rep.cls[cls];
 	tc.band = bm, tc.tasks = (int64_t) Lc.ntasks;
 	float best = -1.0f;
-	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand ? 2 : 1); per_lane++) {
+	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand && pk16_class_fits(c->o, rows) ? 2 : 1); per_lane++) {
 		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane);
 		for (int wpc = 1; wpc <= occ; wpc *= 2) {
 			const int wpc_eff = wpc * 2 > occ ? occ : wpc; // the last candidate is the occupancy limit
@@ -470,7 +471,7 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			// tasks per lane: 2 = packed 16-bit (band classes up to kPk16MaxBand), 1 = 32-bit; the
 			// option, else the tuned choice for this class, else 1
 			const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
-			const int per_lane = (want == 2 && bm <= kPk16MaxBand) ? 2 : 1;
+			const int per_lane = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
 			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane);
 			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 64 * per_lane - 1) / (64 * per_lane));
@@ -533,7 +534,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	for (int cls = 0; cls < kNumBandClasses; cls++) {
 		const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
 		const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
-		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand) ? 2 : 1;
+		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
 		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane[cls]);
 		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);

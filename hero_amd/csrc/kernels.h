@@ -136,10 +136,10 @@ hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const C
 // (count << 8) | type, type 'M' / 'I' / 'D' or the query base letter of a mismatch run.
 hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,
 		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops, hipStream_t st);
-// per_lane: tasks per lane, 1 (32-bit) or 2 (packed 16-bit, band classes up to 24)
+// per_lane: tasks per lane, 1 (32-bit) or 2 (packed 16-bit, band classes up to kPk16MaxBand)
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwaves, hipStream_t st);
 int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane); // occupancy of one batched instantiation
-constexpr int kPk16MaxBand = 24; // widest band class of the two-tasks-per-lane form
+constexpr int kPk16MaxBand = 12; // widest band class of the two-tasks-per-lane form (windows of 32 cells)
 
 // Band classes of the batched form; tasks above the last class (or with rows beyond the pointer
 // scratch) go to the plain form. Buckets are (class, 1023 - min(lenS, 1023)): one class at a time,

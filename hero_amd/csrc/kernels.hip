@@ -1106,7 +1106,7 @@ __global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
 		const bool has_b = ib < L.ntasks;
 		uint32_t tb = 0;
 		DpTask B;
-		PackedStream sb, qb;
+		PackedStream sb{nullptr, 0, 0, 0, 0, false}, qb{nullptr, 0, 0, 0, 0, false}; // empty unless a task
 		int lsb = 0, lqb = 0;
 		bool okb = false;
 		if (has_b) {
@@ -1115,24 +1115,20 @@ __global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
 			task_streams(B, L.S, L.Q, sb, qb, lsb, lqb);
 			okb = B.band <= BMAX && lsb < L.rows_max;
 		}
-		if (oka && okb && pk16_fits(L.o, lsa, lqa) && pk16_fits(L.o, lsb, lqb)) {
-			const Pk16Task PA{lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0, (A.flags & kTaskAlignRight) != 0};
-			const Pk16Task PB{lsb, lqb, B.band, (B.flags & kTaskAlignLeft) != 0, (B.flags & kTaskAlignRight) != 0};
-			dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, sa, qa, sb, qb, pa, pb);
-			cells += finish_task(L, ta, A, lsa, lqa, pa);
-			cells += finish_task(L, tb, B, lsb, lqb, pb);
-		} else {
-			if (oka) {
-				dp_forward<BMAX, AFF>(L.o, lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0,
-						(A.flags & kTaskAlignRight) != 0, sa, qa, pa);
-				cells += finish_task(L, ta, A, lsa, lqa, pa);
-			}
-			if (okb) {
-				dp_forward<BMAX, AFF>(L.o, lsb, lqb, B.band, (B.flags & kTaskAlignLeft) != 0,
-						(B.flags & kTaskAlignRight) != 0, sb, qb, pb);
-				cells += finish_task(L, tb, B, lsb, lqb, pb);
-			}
+		// the host selects this form only when every task of the class is within the 16-bit bound
+		// (pk16_class_fits); a pair that is not (never expected) is counted as skipped
+		const bool fit = oka && (!has_b || okb) && pk16_fits(L.o, lsa, lqa) && (!has_b || pk16_fits(L.o, lsb, lqb));
+		if (!fit) {
+			stat_add(L.stats, kStDpSkipped, has_b ? 2ull : 1ull);
+			continue;
 		}
+		if (!has_b) lsb = 0, lqb = 0; // an empty second task: one row of sentinels, never read
+		const Pk16Task PA{lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0, (A.flags & kTaskAlignRight) != 0};
+		const Pk16Task PB{lsb, lqb, has_b ? B.band : A.band, has_b && (B.flags & kTaskAlignLeft) != 0,
+				has_b && (B.flags & kTaskAlignRight) != 0};
+		dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, sa, qa, sb, qb, pa, pb);
+		cells += finish_task(L, ta, A, lsa, lqa, pa);
+		if (has_b) cells += finish_task(L, tb, B, lsb, lqb, pb);
 	}
 	cells = wave_sum64(cells);
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
@@ -1197,7 +1193,7 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane) {
 		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64, 0)       \
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64, 0);
 	HERO_OCC(4) HERO_OCC(8) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
-	HERO_OCC2(4) HERO_OCC2(8) HERO_OCC2(12) HERO_OCC2(16) HERO_OCC2(24)
+	HERO_OCC2(4) HERO_OCC2(8) HERO_OCC2(12)
 #undef HERO_OCC
 #undef HERO_OCC2
 	return (e == hipSuccess && blocks > 0) ? blocks : 4;
@@ -1310,7 +1306,7 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 		return hipGetLastError();                                                                  \
 	}
 	HERO_DPB(4) HERO_DPB(8) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
-	HERO_DPB2(4) HERO_DPB2(8) HERO_DPB2(12) HERO_DPB2(16) HERO_DPB2(24)
+	HERO_DPB2(4) HERO_DPB2(8) HERO_DPB2(12)
 #undef HERO_DPB
 #undef HERO_DPB2
 	return hipErrorInvalidValue;

@@ -50,7 +50,7 @@ typedef struct {
 	int32_t dp_waves;       /* persistent DP wavefronts; 0 = automatic */
 	int32_t records;        /* 1: keep the alignment records as .aln text (requires call_consensus = 0) */
 	int32_t dp_per_lane;    /* batched form: 1 one task per lane (32-bit), 2 two tasks per lane (packed
-	                           16-bit, band classes <= 24); 0 = the hero_autotune choice, else 1 */
+	                           16-bit, band classes <= 12); 0 = the hero_autotune choice, else 1 */
 	int32_t dp_eager;       /* 1: launch the DP stage kernel by kernel instead of as one captured graph */
 	int32_t reserved[4];
 } hero_options;

@@ -383,7 +383,7 @@ extern "C" int hostsim_pk16_check(const hs_cfg *cfg, int n, uint64_t seed, int b
 	for (int t = 0; t < n; t++) {
 #define PK_CASE(BM)                                                                                          \
 	if (bmax == BM) bad += o.aligner == 0 ? pk16_check_one<BM, true>(o, rng, max_len) : pk16_check_one<BM, false>(o, rng, max_len);
-		PK_CASE(4) PK_CASE(8) PK_CASE(12) PK_CASE(16) PK_CASE(24)
+		PK_CASE(4) PK_CASE(8) PK_CASE(12)
 #undef PK_CASE
 	}
 	return bad;

@@ -62,7 +62,7 @@ def test_band_cells_closed_form_equals_row_sum():
         assert f(ls, lq, b) == want
 
 
-@pytest.mark.parametrize("bmax", [4, 8, 12, 16, 24])
+@pytest.mark.parametrize("bmax", [4, 8, 12])
 @pytest.mark.parametrize("aligner", [0, 1], ids=["affine", "global"])
 def test_packed_two_task_dp_equals_one_task_dp(aligner, bmax):
     """The packed 16-bit two-tasks-per-lane DP writes the same pointer rows as the 32-bit DP."""
