@@ -141,18 +141,19 @@ struct Timer {
 	hipEvent_t a = nullptr, b = nullptr;
 };
 
-// A device seed index: open-addressing slots over CSR buckets of (pos << 40) | read entries.
+// A device seed index: open-addressing slots over CSR buckets of (tag << 54) | (pos << 40) | read
+// entries.
 struct IndexBufs {
 	DevBuf<Slot> slots;
-	DevBuf<uint64_t> key_lo, entries;
+	DevBuf<uint64_t> entries;
 	uint64_t P = 0, total_keys = 0;
 	uint32_t max_probe = 0;
 	bool built = false;
 	DevIndex view(int k) const {
-		return DevIndex{slots.p, key_lo.p, entries.p, P, max_probe, k, make_fastmod(P ? P : 1)};
+		return DevIndex{slots.p, entries.p, P, max_probe, k, make_fastmod(P ? P : 1)};
 	}
 	void release() {
-		slots.release(), key_lo.release(), entries.release();
+		slots.release(), entries.release();
 		P = total_keys = 0, max_probe = 0, built = false;
 	}
 };
@@ -1082,7 +1083,6 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(errf.exact(1));
 		IXCHK(cursor.exact(1));
 		IXCHK(ix.slots.exact(ix.P));
-		IXCHK(ix.key_lo.exact(k > 32 ? ix.P : 1));
 		IXCHK(ix.entries.exact(nocc));
 		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
 		IXCHK(hipMemsetAsync(owner.p, 0, ix.P * 4, c->w.st));
@@ -1094,7 +1094,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(launch_index_gen(Rp, dkoff.p, k, all_positions, occ_hi.p, occ_lo.p, occ_entry.p, c->w.st));
 		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, ix.P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->w.st));
 		IXCHK(launch_index_count(occ_slot.p, nocc, ix.slots.p, occ_rank.p, c->w.st));
-		IXCHK(launch_index_alloc(owner.p, ix.P, occ_hi.p, occ_lo.p, k, ix.slots.p, ix.key_lo.p, cursor.p, c->w.st));
+		IXCHK(launch_index_alloc(owner.p, ix.P, occ_hi.p, occ_lo.p, k, ix.slots.p, cursor.p, c->w.st));
 		IXCHK(launch_index_fill(occ_slot.p, occ_rank.p, occ_entry.p, nocc, ix.slots.p, ix.entries.p, c->w.st));
 		IXCHK(launch_index_sort_buckets(ix.slots.p, ix.P, ix.entries.p, c->w.st));
 		IXCHK(hipEventRecord(e1, c->w.st));
@@ -1501,7 +1501,7 @@ __global__ void k_lookup(DevIndex ix, const char *win, int n, int64_t *count, ui
 	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
 		const Slot s = ix.slots[h];
 		if (s.count == 0) return;
-		if (s.key == hi && (k <= 32 || ix.key_lo[h] == lo)) {
+		if (s.key == hi && (k <= 32 || s.key_lo == lo)) {
 			count[i] = s.count;
 			first_entry_idx[i] = s.start;
 			return;
@@ -1554,6 +1554,7 @@ extern "C" int hero_lookup(hero_ctx *c, int32_t n, const char *windows, int64_t 
 		if (count[i] > 0 && entries) {
 			if (pos + count[i] > entry_cap) return fail(c, HERO_ERR_LIMIT, "entry_cap too small");
 			HIPCHK(c, hipMemcpy(entries + pos, c->qix.entries.p + first[(size_t) i], (size_t) count[i] * 8, hipMemcpyDeviceToHost));
+			for (int64_t j = pos; j < pos + count[i]; j++) entries[j] &= kEntryMask; // (pos << 40) | innerId
 		}
 		pos += count[i];
 	}

@@ -6,18 +6,28 @@
 
 namespace hero {
 
-// Open-addressing slot of the seed index: the k-mer (first 32 bases), its bucket in `entries`.
-// count == 0 marks an empty slot. k-mers longer than 32 keep their tail in key_lo[slot].
-struct Slot {
+// Open-addressing slot of the seed index: the k-mer (first 32 bases in key, the rest of a longer
+// one in key_lo), its bucket in `entries`; count == 0 marks an empty slot. 32 bytes, aligned, so
+// a probe reads one cache line.
+struct __align__(32) Slot {
 	uint64_t key;
+	uint64_t key_lo;
 	uint32_t start;
 	uint32_t count;
 };
 
+// Bucket entry: (tag << 54) | (pos << 40) | innerId (src/HashTableLongKey.cpp:334-336 without
+// the tag). The tag hashes the read's uniqueId, so the self-hit test (same uniqueId as the
+// subject) loads the uniqueId only when the tags agree.
+constexpr uint64_t kIdMask = (1ull << 40) - 1; // iMask for iBitOfReadId = 40
+constexpr uint64_t kEntryMask = (1ull << 54) - 1;
+HD uint32_t uid_tag(int64_t uid) { return (uint32_t) (((uint64_t) uid * 0x9E3779B97F4A7C15ull) >> 54); }
+HD int entry_pos(uint64_t e) { return (int) ((e >> 40) & 0x3FFFu); }
+HD uint32_t entry_tag(uint64_t e) { return (uint32_t) (e >> 54); }
+
 struct DevIndex {
 	const Slot *slots;
-	const uint64_t *key_lo;
-	const uint64_t *entries; // (pos << 40) | innerId (src/HashTableLongKey.cpp:334-336)
+	const uint64_t *entries;
 	uint64_t P;
 	uint32_t max_probe;
 	int32_t k;
@@ -116,7 +126,7 @@ hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, ui
 		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st);
 hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st);
 hipError_t launch_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
-		Slot *slots, uint64_t *key_lo, unsigned long long *cursor, hipStream_t st);
+		Slot *slots, unsigned long long *cursor, hipStream_t st);
 hipError_t launch_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank, const uint64_t *occ_entry, uint64_t nocc,
 		const Slot *slots, uint64_t *entries, hipStream_t st);
 hipError_t launch_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries, hipStream_t st);

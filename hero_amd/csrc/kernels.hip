@@ -10,7 +10,6 @@
 
 namespace hero {
 
-constexpr uint64_t kIdMask = (1ull << 40) - 1; // iMask for iBitOfReadId = 40
 
 __device__ __forceinline__ int lane_id() { return (int) threadIdx.x & 63; }
 
@@ -70,7 +69,7 @@ __global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, int all_posit
 		kmer_fwd(w, p, k, hi, lo);
 		occ_hi[o] = hi;
 		occ_lo[o] = lo;
-		occ_entry[o] = ((uint64_t) p << 40) | (uint64_t) q;
+		occ_entry[o] = ((uint64_t) uid_tag(Q.uid[q]) << 54) | ((uint64_t) p << 40) | (uint64_t) q;
 		o++;
 	}
 }
@@ -128,7 +127,7 @@ hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slo
 // get consecutive ranges; where buckets lie in `entries` is invisible to lookups).
 constexpr int kAllocPerThread = 16;
 __global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi,
-		const uint64_t *occ_lo, int k, Slot *slots, uint64_t *key_lo, unsigned long long *cursor) {
+		const uint64_t *occ_lo, int k, Slot *slots, unsigned long long *cursor) {
 	__shared__ uint32_t wave_tot[4];
 	__shared__ uint32_t block_base;
 	const uint64_t base = (uint64_t) blockIdx.x * 256 * kAllocPerThread + threadIdx.x;
@@ -160,17 +159,17 @@ __global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint
 		const uint64_t o = own - 1;
 		Slot sl = slots[s];
 		sl.key = occ_hi[o];
+		sl.key_lo = k > 32 ? occ_lo[o] : 0;
 		sl.start = b;
 		b += sl.count;
 		slots[s] = sl;
-		if (k > 32) key_lo[s] = occ_lo[o];
 	}
 }
 hipError_t launch_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi, const uint64_t *occ_lo, int k,
-		Slot *slots, uint64_t *key_lo, unsigned long long *cursor, hipStream_t st) {
+		Slot *slots, unsigned long long *cursor, hipStream_t st) {
 	const uint64_t per_block = 256ull * kAllocPerThread;
 	k_index_alloc<<<(unsigned) ((P + per_block - 1) / per_block), 256, 0, st>>>(owner, P, occ_hi, occ_lo, k, slots,
-			key_lo, cursor);
+			cursor);
 	return hipGetLastError();
 }
 
@@ -195,7 +194,7 @@ __global__ void k_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *en
 	const Slot sl = slots[s];
 	if (sl.count < 2 || sl.count > 512) return;
 	uint64_t *e = entries + sl.start;
-	auto key = [](uint64_t v) { return ((v & kIdMask) << 24) | (v >> 40); };
+	auto key = [](uint64_t v) { return ((v & kIdMask) << 14) | (uint64_t) entry_pos(v); };
 	for (uint32_t i = 1; i < sl.count; i++) {
 		const uint64_t v = e[i], kv = key(v);
 		int j = (int) i - 1;
@@ -216,7 +215,7 @@ __device__ __forceinline__ bool ix_lookup(const DevIndex &ix, uint64_t hi, uint6
 	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
 		const Slot s = ix.slots[h];
 		if (s.count == 0) return false;
-		if (s.key == hi && (ix.k <= 32 || ix.key_lo[h] == lo)) {
+		if (s.key == hi && (ix.k <= 32 || s.key_lo == lo)) {
 			start = s.start;
 			count = s.count;
 			return true;
@@ -249,11 +248,12 @@ __device__ __forceinline__ uint32_t bucket_seeds(const Packed &S, const Packed &
 		int Ls, int64_t uid, int w, int strand, uint32_t start, uint32_t count, F &&emit) {
 	(void) S;
 	uint32_t n = 0;
+	const uint32_t tag = uid_tag(uid);
 	for (uint32_t e = start; e < start + count; e++) {
 		const uint64_t data = ix.entries[e];
 		const uint64_t qid = data & kIdMask;
-		if (Q.uid[qid] == uid) continue;
-		int pq = (int) (data >> 40), ps = w;
+		if (entry_tag(data) == tag && Q.uid[qid] == uid) continue;
+		int pq = entry_pos(data), ps = w;
 		if (!o.call_consensus && strand == 1) ps = Ls - (ps + o.k), pq = Q.len[qid] - (pq + o.k);
 		emit(seed_pack(qid, pq, ps));
 		n++;
@@ -673,6 +673,7 @@ __global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevInde
 	uint64_t fh, fl;
 	kmer_fwd(Q.words + Q.woff[q], pos, k, fh, fl);
 	const int64_t quid = Q.uid[q];
+	const uint32_t qtag = uid_tag(quid);
 	const uint32_t base = keys_out ? off[t] : 0u;
 	uint32_t n = 0;
 	for (int strand = 0; strand < 2; strand++) {
@@ -683,9 +684,9 @@ __global__ __launch_bounds__(256) void k_query_probe(Packed Q, Packed S, DevInde
 		for (uint32_t e = start; e < start + count; e++) {
 			const uint64_t d = six.entries[e];
 			const uint64_t sr = d & kIdMask;
-			if (S.uid[sr] == quid) continue;
+			if (entry_tag(d) == qtag && S.uid[sr] == quid) continue;
 			if (keys_out) {
-				const int p = (int) (d >> 40);
+				const int p = entry_pos(d);
 				const int ps = strand ? S.len[sr] - p - k : p;
 				keys_out[base + n] = ((uint64_t) strand << 28) | ((uint64_t) pos << 14) | (uint64_t) ps;
 				vals_out[base + n] = (sr << 22) | (uint64_t) q;
