@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-form and mixed-length runs")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (latency) runs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default DP schedules instead of the autotuner's (profiles of exactly the step's kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (the reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -312,8 +314,10 @@ def main():
     ctx = gpu.Context(**C2_OPTS)
     ctx.set_queries(rs)  # packed reads resident in HBM before any timed step
     ctx.build_index()
-    tune_c2 = ctx.autotune()  # DP schedule per band class, timed on the first 65,536 subjects
-    log("autotune:", json.dumps([{k: v for k, v in t.items() if k != "candidates"} for t in tune_c2]))
+    tune_c2 = []
+    if not args.no_autotune:
+        tune_c2 = ctx.autotune()  # DP schedule per band class, timed on the first 65,536 subjects
+        log("autotune:", json.dumps([{k: v for k, v in t.items() if k != "candidates"} for t in tune_c2]))
     for _ in range(args.warmup):
         run_steps(ctx, 1)
     ms, st = run_steps(ctx, args.steps)
@@ -335,10 +339,10 @@ def main():
                     peak=round(VALU_PEAK_TOPS, 2), unit="Tops/s (int32 VALU)", frac=round(achieved / VALU_PEAK_TOPS, 4),
                     traffic=measured_traffic("hero::k_dp_batched<8, true>"), traffic_unit="HBM bytes per launch",
                     ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all), dp_ms_per_step=round(dp_ms, 3))
-    # seed probes: packed subject bases + 24 B per window probe (k > 32) + 8 B per bucket entry read
+    # seed probes: packed subject bases + one 32 B slot per window probe + 8 B per bucket entry read
     probe_ms = st["ms_seed_probe"] / steps
     bases = int(rs.length.astype(np.int64).sum())
-    probe_bytes = bases / 4 + (st["lookups"] / steps) * 24 + (st["seeds"] / steps) * 8
+    probe_bytes = bases / 4 + (st["lookups"] / steps) * 32 + (st["seeds"] / steps) * 8
     probe_gbs = probe_bytes / (probe_ms / 1000.0) / 1e9 if probe_ms > 0 else 0.0
     roofline_probe = dict(bound="hbm", kernel="k_seed_count", achieved=round(probe_gbs, 1), peak=HBM_PEAK_GBS,
                           unit="GB/s", frac=round(probe_gbs / HBM_PEAK_GBS, 4),
