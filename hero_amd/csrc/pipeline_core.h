@@ -183,6 +183,9 @@ HD int step_type(uint32_t code, bool swapped) {
 // as runs: op(count, type) for each maximal run of equal op types (a mismatch run is one call
 // with its length). Within a 16-step word the loop visits only the non-match steps (match = 0, so
 // the lowest set bit pair is the next one); the matches between them extend the current run.
+// A call of up to kFeedWords words (128 steps) has all its words loaded before any is used, so
+// their loads overlap instead of each waiting for the previous one.
+constexpr int kFeedWords = 8;
 template <class F>
 HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 	const DpTask &t = R.tasks[task];
@@ -200,10 +203,10 @@ HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 		}
 	};
 	static_assert(kStepMatch == 0, "match steps are zero bits");
-	while (e < cap) {
+	auto steps_of = [&](uint32_t raw) { // the steps e .. up to the end of e's word
 		const int stop = (e | 15) + 1 < cap ? (e | 15) + 1 : cap;
 		const int m = stop - e;
-		uint32_t word = w[e >> 4] >> (2 * (e & 15));
+		uint32_t word = raw >> (2 * (e & 15));
 		if (m < 16) word &= (1u << (2 * m)) - 1u;
 		int done = 0;
 		while (word) {
@@ -216,6 +219,17 @@ HD void feed_runs(const DpResults &R, uint32_t task, bool swapped, F &&op) {
 		}
 		if (m > done) push(m - done, kOpM);
 		e = stop;
+	};
+	const int w0 = e >> 4, w1 = (cap - 1) >> 4; // words holding steps e .. cap-1
+	if (w1 - w0 < kFeedWords) {
+		uint32_t pre[kFeedWords];
+#pragma unroll
+		for (int k = 0; k < kFeedWords; k++) pre[k] = w0 + k <= w1 ? w[w0 + k] : 0u;
+#pragma unroll
+		for (int k = 0; k < kFeedWords; k++)
+			if (e < cap) steps_of(pre[k]);
+	} else {
+		while (e < cap) steps_of(w[e >> 4]);
 	}
 	if (run_c) op(run_c, run_t);
 }
