@@ -59,16 +59,6 @@ HD I16x2 pk_sub(I16x2 a, I16x2 b) { return pk2(pk_lo(a) - pk_lo(b), pk_hi(a) - p
 HD I16x2 pk_max(I16x2 a, I16x2 b) { return pk2(imax_(pk_lo(a), pk_lo(b)), imax_(pk_hi(a), pk_hi(b))); }
 HD uint32_t pk_neg(I16x2 a) { return (pk_lo(a) < 0 ? 0xFFFFu : 0u) | (pk_hi(a) < 0 ? 0xFFFF0000u : 0u); }
 #endif
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __forceinline__ I16x2 pk_min(I16x2 a, I16x2 b) { return pk_w(__builtin_elementwise_min(pk_v(a), pk_v(b))); }
-// a * b + c in each half, modulo 2^16
-__device__ __forceinline__ I16x2 pk_mad(I16x2 a, I16x2 b, I16x2 c) { return pk_w(pk_v(a) * pk_v(b) + pk_v(c)); }
-#else
-HD I16x2 pk_min(I16x2 a, I16x2 b) { return pk2(pk_lo(a) < pk_lo(b) ? pk_lo(a) : pk_lo(b), pk_hi(a) < pk_hi(b) ? pk_hi(a) : pk_hi(b)); }
-HD I16x2 pk_mad(I16x2 a, I16x2 b, I16x2 c) {
-	return pk2(pk_lo(a) * pk_lo(b) + pk_lo(c), pk_hi(a) * pk_hi(b) + pk_hi(c));
-}
-#endif
 // a < b in each half (no overflow: both operands stay within the range above)
 HD uint32_t pk_lt(I16x2 a, I16x2 b) { return pk_neg(pk_sub(a, b)); }
 HD I16x2 pk_sel(uint32_t m, I16x2 a, I16x2 b) { return I16x2{(a.v & m) | (b.v & ~m)}; }
@@ -117,7 +107,7 @@ HD void dp_forward_pk16(const Opts &o, const Pk16Task &A, const Pk16Task &B, SS 
 	static_assert(W <= 32, "packed query windows hold 32 cells");
 	using M = uint32_t;
 	const I16x2 K2 = pk2(kSent16, kSent16);
-	const I16x2 hit2 = pk2(o.hit, o.hit), dmh2 = pk2(o.miss - o.hit, o.miss - o.hit);
+	const I16x2 hit2 = pk2(o.hit, o.hit), miss2 = pk2(o.miss, o.miss);
 	const I16x2 dx2 = pk2(o.dx, o.dx), dy2 = pk2(o.dy, o.dy), ex2 = pk2(o.ex, o.ex), sx2 = pk2(o.sx, o.sx);
 	const int g0a = (A.right_al || (A.lenQ == 0 && A.left_al)) ? o.sx : o.dx;
 	const int g0b = (B.right_al || (B.lenQ == 0 && B.left_al)) ? o.sx : o.dx;
@@ -178,7 +168,7 @@ HD void dp_forward_pk16(const Opts &o, const Pk16Task &A, const Pk16Task &B, SS 
 		for (int d = 0; d < W; d++) {
 			// 1 in each half whose codes differ
 			const I16x2 m01{((uint32_t) (ma >> (2 * d)) & 1u) | (((uint32_t) (mb >> (2 * d)) & 1u) << 16)};
-			const I16x2 s = pk_mad(m01, dmh2, hit2);
+			const I16x2 s = pk_sel(m01.v * 0xFFFFu, miss2, hit2); // 0xFFFF in each differing half
 			const uint32_t pdiag = 0x00020002u | m01.v;
 			I16x2 h;
 			uint32_t p;

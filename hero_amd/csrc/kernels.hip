@@ -983,30 +983,54 @@ hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const
 // Sequential reader of one side of a task in its frame (reverse complement when rc).
 // A read segment as a stream of 2-bit codes (forward, or reverse complement when rc). The current
 // 32-base word stays in a register: one load per 32 bases instead of one per base.
+// Codes of a read segment in strand frame, one at a time: the current word sits pre-shifted so the
+// next code is in its top bits (a reverse-complement word is reversed in 2-bit groups and
+// complemented once, when loaded). Seven registers per stream.
 struct PackedStream {
 	static constexpr bool kTwoBit = true;
-	const uint64_t *w;
-	int L, start, len, i;
-	bool rc;
-	int word_idx = -1;
-	uint64_t word = 0;
-	__device__ __forceinline__ int next() {
-		if (i >= len) return kCodeNone;
-		const int p = start + i++;
-		const int q = rc ? L - 1 - p : p;
-		if ((q >> 5) != word_idx) {
-			word_idx = q >> 5;
-			word = w[word_idx];
+	const uint64_t *p; // next word to load
+	uint64_t cur;      // codes not yet returned, next at the top
+	int nleft;         // codes left in cur
+	int remain;        // codes left in the segment
+	int step;          // +1 forward, -1 reverse complement
+	__device__ static __forceinline__ uint64_t rc_word(uint64_t x) {
+		x = __builtin_bitreverse64(x);
+		x = ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1); // back to 2-bit codes
+		return x ^ 0xAAAAAAAAAAAAAAAAull;                                                 // complement: c ^ 2
+	}
+	// the segment [start, start + len) of a read of L bases at w; rc: of its reverse complement
+	__device__ static __forceinline__ PackedStream make(const uint64_t *w, int L, int start, int len, bool rc) {
+		PackedStream s;
+		s.remain = len, s.step = rc ? -1 : 1, s.p = w, s.cur = 0, s.nleft = 0;
+		if (len <= 0) return s;
+		if (!rc) {
+			const uint64_t *q = w + (start >> 5);
+			s.cur = *q << (2 * (start & 31)), s.nleft = 32 - (start & 31), s.p = q + 1;
+		} else {
+			const int f = L - 1 - start; // forward position of the first code
+			const uint64_t *q = w + (f >> 5);
+			s.cur = rc_word(*q) << (2 * (31 - (f & 31))), s.nleft = (f & 31) + 1, s.p = q - 1; // read only if needed
 		}
-		const int c = (int) ((word >> (62 - 2 * (q & 31))) & 3ull);
-		return rc ? c ^ 2 : c;
+		return s;
+	}
+	__device__ __forceinline__ int next() {
+		if (remain <= 0) return kCodeNone;
+		if (nleft == 0) {
+			cur = step > 0 ? *p : rc_word(*p);
+			p += step, nleft = 32;
+		}
+		const int c = (int) (cur >> 62);
+		cur <<= 2, nleft--, remain--;
+		return c;
 	}
 };
 
 __device__ __forceinline__ void task_streams(const DpTask &t, const Packed &S, const Packed &Q, PackedStream &as,
 		PackedStream &aq, int &lenS, int &lenQ) {
-	PackedStream subj{S.words + S.woff[t.s_read], S.len[t.s_read], t.s0, t.s_len, 0, (t.flags & kTaskSubjRC) != 0};
-	PackedStream qry{Q.words + Q.woff[t.q_read], Q.len[t.q_read], t.q0, t.q_len, 0, (t.flags & kTaskQryRC) != 0};
+	const PackedStream subj = PackedStream::make(S.words + S.woff[t.s_read], S.len[t.s_read], t.s0, t.s_len,
+			(t.flags & kTaskSubjRC) != 0);
+	const PackedStream qry = PackedStream::make(Q.words + Q.woff[t.q_read], Q.len[t.q_read], t.q0, t.q_len,
+			(t.flags & kTaskQryRC) != 0);
 	const bool sw = (t.flags & kTaskSwapped) != 0;
 	as = sw ? qry : subj;
 	aq = sw ? subj : qry;
@@ -1082,7 +1106,8 @@ __device__ __forceinline__ uint64_t finish_task(const DpLaunch &L, uint32_t t, c
 // in packed 16-bit arithmetic (dp_pk16.h). A pair with a task beyond the 16-bit bound runs its two
 // tasks one after the other in 32-bit.
 template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 3 : 1, 8))) void k_dp_batched_pk(
+		DpLaunch L0) {
 	DpLaunch L = L0;
 	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
@@ -1107,7 +1132,7 @@ __global__ __launch_bounds__(64) void k_dp_batched_pk(DpLaunch L0) {
 		const bool has_b = ib < L.ntasks;
 		uint32_t tb = 0;
 		DpTask B;
-		PackedStream sb{nullptr, 0, 0, 0, 0, false}, qb{nullptr, 0, 0, 0, 0, false}; // empty unless a task
+		PackedStream sb = PackedStream::make(nullptr, 0, 0, 0, false), qb = sb; // empty unless a task
 		int lsb = 0, lqb = 0;
 		bool okb = false;
 		if (has_b) {
@@ -1345,16 +1370,20 @@ __global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L0, int bmax_plain) {
 		}
 		const int W = 2 * b + 1;
 		// stage the codes in LDS (each lane reads a strided slice of the packed sides)
-		for (int i = lane; i < lenS; i += 64) {
-			PackedStream x = as;
-			x.i = i;
-			sc[i] = (uint8_t) x.next();
-		}
-		for (int j = lane; j < lenQ; j += 64) {
-			PackedStream x = aq;
-			x.i = j;
-			qc[j] = (uint8_t) x.next();
-		}
+		struct SegRef {
+			const uint64_t *w;
+			int L, start;
+			bool rc;
+			__device__ int at(int p) const {
+				const int q = start + p;
+				return rc ? (base_at(w, L - 1 - q) ^ 2) : base_at(w, q);
+			}
+		};
+		const SegRef rs{L.S.words + L.S.woff[task.s_read], L.S.len[task.s_read], task.s0, (task.flags & kTaskSubjRC) != 0};
+		const SegRef rq{L.Q.words + L.Q.woff[task.q_read], L.Q.len[task.q_read], task.q0, (task.flags & kTaskQryRC) != 0};
+		const bool swp = (task.flags & kTaskSwapped) != 0;
+		for (int i = lane; i < lenS; i += 64) sc[i] = (uint8_t) (swp ? rq : rs).at(i);
+		for (int j = lane; j < lenQ; j += 64) qc[j] = (uint8_t) (swp ? rs : rq).at(j);
 		__syncthreads();
 		const bool la = (task.flags & kTaskAlignLeft) != 0, ra = (task.flags & kTaskAlignRight) != 0;
 		auto valid = [&](int i, int j) { return i >= 0 && i <= lenS && j >= 0 && j <= lenQ && j - i <= b && i - j <= b; };
