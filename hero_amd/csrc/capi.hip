@@ -396,7 +396,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	tc.band = bm, tc.tasks = (int64_t) Lc.ntasks;
 	float best = -1.0f;
 	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand && pk16_class_fits(c->o, rows) ? 2 : 1); per_lane++) {
-		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane);
+		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane, rows);
 		for (int wpc = 1; wpc <= occ; wpc *= 2) {
 			const int wpc_eff = wpc * 2 > occ ? occ : wpc; // the last candidate is the occupancy limit
 			const int waves = (int) std::min<uint64_t>((uint64_t) c->cu_count * wpc_eff,
@@ -413,7 +413,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 				HIPCHK(c, hipStreamSynchronize(w->st));
 				HIPCHK(c, hipEventElapsedTime(&ms, w->ev[9].a, w->ev[3].a));
 			}
-			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1)) tc.ms_default = ms;
+			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1, rows)) tc.ms_default = ms;
 			if (tc.ncand < (int) (sizeof tc.cand / sizeof tc.cand[0]))
 				tc.cand[tc.ncand++] = hero_tune_candidate{per_lane, wpc_eff, ms};
 			if (best < 0 || ms < best) best = ms, tc.per_lane = per_lane, tc.waves_per_cu = wpc_eff, tc.ms = ms;
@@ -473,7 +473,8 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			// option, else the tuned choice for this class, else 1
 			const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
 			const int per_lane = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
-			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane);
+			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
+			                                               : dp_batched_waves_per_cu(bm, aff, per_lane, rows);
 			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 64 * per_lane - 1) / (64 * per_lane));
 			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64 * per_lane));
@@ -536,7 +537,8 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
 		const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
 		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
-		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls] : dp_batched_waves_per_cu(bm, aff, per_lane[cls]);
+		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
+		                                               : dp_batched_waves_per_cu(bm, aff, per_lane[cls], rows);
 		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);
 	}

@@ -148,8 +148,13 @@ hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, co
 		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops, hipStream_t st);
 // per_lane: tasks per lane, 1 (32-bit) or 2 (packed 16-bit, band classes up to kPk16MaxBand)
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwaves, hipStream_t st);
-int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane); // occupancy of one batched instantiation
+// occupancy of one batched instantiation (rows_max sizes the packed form's LDS)
+int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane, int rows_max);
 constexpr int kPk16MaxBand = 12; // widest band class of the two-tasks-per-lane form (windows of 32 cells)
+// The packed form stages each lane's four segments (< rows_max codes each) in LDS: words per
+// segment (odd, so the lanes' segments start in different banks) and bytes per wavefront.
+inline __host__ __device__ int pk_lds_words(int rows_max) { return ((rows_max + 15) / 16) | 1; }
+inline __host__ __device__ size_t pk_lds_bytes(int rows_max) { return (size_t) 64 * 4 * pk_lds_words(rows_max) * 4; }
 
 // Band classes of the batched form; tasks above the last class (or with rows beyond the pointer
 // scratch) go to the plain form. Buckets are (class, 1023 - min(lenS, 1023)): one class at a time,

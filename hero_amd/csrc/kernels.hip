@@ -1101,10 +1101,33 @@ __device__ __forceinline__ uint64_t finish_task(const DpLaunch &L, uint32_t t, c
 	return band_cells(lenS, lenQ, task.band);
 }
 
+// A segment's codes staged in LDS by its lane, 16 per word (code p at bits 2 * (p & 15) of word
+// p >> 4): three registers per stream instead of a packed-word reader's seven.
+struct LdsStream {
+	static constexpr bool kTwoBit = true;
+	const uint32_t *w; // LDS
+	int pos, len;
+	__device__ __forceinline__ int next() {
+		if (pos >= len) return kCodeNone;
+		const int c = (int) ((w[pos >> 4] >> (2 * (pos & 15))) & 3u);
+		pos++;
+		return c;
+	}
+};
+__device__ __forceinline__ LdsStream stage_codes(PackedStream s, int len, uint32_t *dst) {
+	uint32_t acc = 0;
+	for (int p = 0; p < len; p++) {
+		acc |= (uint32_t) s.next() << (2 * (p & 15));
+		if ((p & 15) == 15) dst[p >> 4] = acc, acc = 0;
+	}
+	if (len & 15) dst[len >> 4] = acc;
+	return LdsStream{dst, 0, len};
+}
+
 // Batched form, two tasks per lane: persistent waves pulling 128 tasks at a time; lane l takes
 // tasks 2l and 2l+1 of the batch (neighbours in the length-bucketed order) and runs them together
-// in packed 16-bit arithmetic (dp_pk16.h). A pair with a task beyond the 16-bit bound runs its two
-// tasks one after the other in 32-bit.
+// in packed 16-bit arithmetic (dp_pk16.h), its four segments staged in LDS. The host selects this
+// form only when every task of the class is within the 16-bit bound.
 template <int BMAX, bool AFF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 3 : 1, 8))) void k_dp_batched_pk(
 		DpLaunch L0) {
@@ -1114,6 +1137,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 
 	const int lane = lane_id();
 	uint32_t *wbase = L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 128;
 	SlotPtrs<NW, 128> pa{wbase, 2 * lane}, pb{wbase, 2 * lane + 1};
+	// the lane's four segments (A's and B's aligner subject and query) in LDS
+	extern __shared__ uint32_t lds_codes[];
+	const int wps = pk_lds_words(L.rows_max);
+	uint32_t *mine = lds_codes + (size_t) lane * 4 * wps;
 	uint64_t cells = 0;
 	for (;;) {
 		uint32_t batch = 0;
@@ -1152,7 +1179,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 
 		const Pk16Task PA{lsa, lqa, A.band, (A.flags & kTaskAlignLeft) != 0, (A.flags & kTaskAlignRight) != 0};
 		const Pk16Task PB{lsb, lqb, has_b ? B.band : A.band, has_b && (B.flags & kTaskAlignLeft) != 0,
 				has_b && (B.flags & kTaskAlignRight) != 0};
-		dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, sa, qa, sb, qb, pa, pb);
+		LdsStream ls_sa = stage_codes(sa, lsa, mine), ls_qa = stage_codes(qa, lqa, mine + wps);
+		LdsStream ls_sb = stage_codes(sb, lsb, mine + 2 * wps), ls_qb = stage_codes(qb, lqb, mine + 3 * wps);
+		dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, ls_sa, ls_qa, ls_sb, ls_qb, pa, pb);
 		cells += finish_task(L, ta, A, lsa, lqa, pa);
 		if (has_b) cells += finish_task(L, tb, B, lsb, lqb, pb);
 	}
@@ -1207,7 +1236,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
-int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane) {
+int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max) {
 	int blocks = 0;
 	hipError_t e = hipErrorInvalidValue;
 #define HERO_OCC(BM)                                                                                            \
@@ -1216,8 +1245,10 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane) {
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false>, 64, 0);
 #define HERO_OCC2(BM)                                                                                           \
 	if (bmax == BM && per_lane == 2)                                                                            \
-		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64, 0)       \
-		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64, 0);
+		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64,           \
+		                  pk_lds_bytes(rows_max))                                                          \
+		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64,          \
+		                  pk_lds_bytes(rows_max));
 	HERO_OCC(4) HERO_OCC(8) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
 	HERO_OCC2(4) HERO_OCC2(8) HERO_OCC2(12)
 #undef HERO_OCC
@@ -1327,8 +1358,8 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 	}
 #define HERO_DPB2(BM)                                                                              \
 	if (bmax == BM && per_lane == 2) {                                                             \
-		if (aff) k_dp_batched_pk<BM, true><<<nwaves, 64, 0, st>>>(L);                                \
-		else k_dp_batched_pk<BM, false><<<nwaves, 64, 0, st>>>(L);                                   \
+		if (aff) k_dp_batched_pk<BM, true><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);         \
+		else k_dp_batched_pk<BM, false><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);            \
 		return hipGetLastError();                                                                  \
 	}
 	HERO_DPB(4) HERO_DPB(8) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
