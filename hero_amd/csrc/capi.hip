@@ -275,6 +275,12 @@ struct hero_ctx {
 	std::string aln;
 	// the context's own pipeline (stream, events, chunk buffers)
 	Worker w;
+	// a second pipeline for subject chunks aligned concurrently with w's (created on first use;
+	// its host timings accumulate in st_w2, its device counters in its own stripes)
+	Worker w2;
+	bool w2_ready = false;
+	hero_stats st_w2;
+	std::mutex err_mu; // fail() may run on a chunk worker's thread
 	// autotuned DP schedule (hero_autotune): tasks per lane and persistent waves per CU per band class
 	int tuned_per_lane[kNumBandClasses] = {1, 1, 1, 1, 1, 1};
 	int tuned_waves_per_cu[kNumBandClasses] = {0, 0, 0, 0, 0, 0};
@@ -295,7 +301,10 @@ int fail(hero_ctx *c, int code, const char *fmt, ...) {
 	va_start(ap, fmt);
 	vsnprintf(buf, sizeof buf, fmt, ap);
 	va_end(ap);
-	if (c) c->err = buf;
+	if (c) {
+		std::lock_guard<std::mutex> lk(c->err_mu);
+		c->err = buf;
+	}
 	return code;
 }
 #define HIPCHK(c, expr)                                                                                     \
@@ -1027,11 +1036,48 @@ int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int
 	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
 	c->cons_valid = false;
 	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
-	for (int64_t s0 = 0; s0 < n; s0 += chunk) {
-		const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
-		if (int r = run_chunk(c, &c->w, S, s0, m, max_len + 1, s_uid)) return r;
+	const int64_t nchunks = (n + chunk - 1) / chunk;
+	// Chunks in flight at once: votes are integer atomic sums into the shared tallies, so their
+	// order does not matter; records are written in subject order and autotuning times one chunk
+	// at a time, so those run one chunk after the other.
+	const bool concurrent = (c->opt.chunk_workers == 0 || c->opt.chunk_workers >= 2) && !c->opt.records && !c->tune &&
+			nchunks > 1;
+	if (concurrent && !c->w2_ready) {
+		c->w2.acc = &c->st_w2;
+		if (hipError_t e = c->w2.init(); e != hipSuccess) {
+			c->w2.release();
+			return fail(c, HERO_ERR_HIP, "second chunk worker: %s", hipGetErrorString(e));
+		}
+		c->w2_ready = true;
 	}
-	return HERO_OK;
+	if (!concurrent) {
+		for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+			const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
+			if (int r = run_chunk(c, &c->w, S, s0, m, max_len + 1, s_uid)) return r;
+		}
+		return HERO_OK;
+	}
+	// two workers, each on its own stream and host thread, take the next chunk as they finish
+	std::atomic<int64_t> next{0};
+	std::atomic<bool> stop{false};
+	auto work = [&](Worker *w) {
+		(void) hipSetDevice(c->opt.device);
+		for (;;) {
+			const int64_t k = next.fetch_add(1);
+			if (k >= nchunks || stop.load()) return HERO_OK;
+			const int64_t s0 = k * chunk;
+			const int32_t m = (int32_t) std::min<int64_t>(chunk, n - s0);
+			if (int r = run_chunk(c, w, S, s0, m, max_len + 1, s_uid)) {
+				stop.store(true);
+				return r;
+			}
+		}
+	};
+	int r2 = HERO_OK;
+	std::thread t([&] { r2 = work(&c->w2); });
+	const int r1 = work(&c->w);
+	t.join();
+	return r1 != HERO_OK ? r1 : r2;
 }
 
 // Seed index over the good reads of R: keys at 0, k, 2k, ... and L - k (the reference's query
@@ -1155,6 +1201,7 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 	c->opt = *opt;
 	c->o = make_opts(*opt);
 	std::memset(&c->st_host, 0, sizeof c->st_host);
+	std::memset(&c->st_w2, 0, sizeof c->st_w2);
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, opt->device) != hipSuccess) {
 		delete c;
@@ -1179,6 +1226,7 @@ void hero_destroy(hero_ctx *c) {
 	c->qix.release(), c->tally.release(), c->cons_out.release();
 	c->Rd.release(), c->six.release();
 	c->w.release();
+	if (c->w2_ready) c->w2.release();
 	delete c;
 }
 
@@ -1338,11 +1386,20 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	if (!c || !out) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
-	HIPCHK(c, hipMemcpy(stripes.data(), c->w.dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost));
 	unsigned long long s[kStCount] = {};
-	for (int i = 0; i < kStatStripes; i++)
-		for (int j = 0; j < kStCount; j++) s[j] += stripes[(size_t) i * kStatStride + j];
+	for (Worker *w : {&c->w, &c->w2}) {
+		if (w == &c->w2 && !c->w2_ready) continue;
+		HIPCHK(c, hipMemcpy(stripes.data(), w->dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost));
+		for (int i = 0; i < kStatStripes; i++)
+			for (int j = 0; j < kStCount; j++) s[j] += stripes[(size_t) i * kStatStride + j];
+	}
 	*out = c->st_host;
+	// the second chunk worker's device time adds to the first's (the workers overlap, so stage
+	// times are GPU time summed over both streams)
+	const hero_stats &b = c->st_w2;
+	out->ms_seed += b.ms_seed, out->ms_seed_probe += b.ms_seed_probe, out->ms_plan += b.ms_plan;
+	out->ms_dp += b.ms_dp, out->ms_score_vote += b.ms_score_vote;
+	out->dp_tasks += b.dp_tasks, out->subjects += b.subjects;
 	out->lookups = s[kStLookups];
 	out->seeds = s[kStSeeds];
 	out->dp_calls_exec = s[kStCallsExec];
@@ -1353,7 +1410,8 @@ int hero_get_stats(hero_ctx *c, hero_stats *out) {
 	out->dp_skipped = s[kStDpSkipped];
 	out->reads_changed = s[kStReadsChanged];
 	out->dp_cells_all = s[kStCellsAll];
-	out->dp_graph_builds = c->w.graph_builds, out->dp_graph_launches = c->w.graph_launches;
+	out->dp_graph_builds = c->w.graph_builds + c->w2.graph_builds;
+	out->dp_graph_launches = c->w.graph_launches + c->w2.graph_launches;
 	out->ms_total = out->ms_index + out->ms_seed + out->ms_plan + out->ms_dp + out->ms_score_vote + out->ms_consensus;
 	return HERO_OK;
 }
@@ -1362,7 +1420,9 @@ int hero_reset_stats(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	HIPCHK(c, hipMemset(c->w.dstats.p, 0, (size_t) kStatStripes * kStatStride * 8));
-	c->w.graph_builds = 0, c->w.graph_launches = 0;
+	if (c->w2_ready) HIPCHK(c, hipMemset(c->w2.dstats.p, 0, (size_t) kStatStripes * kStatStride * 8));
+	c->w.graph_builds = 0, c->w.graph_launches = 0, c->w2.graph_builds = 0, c->w2.graph_launches = 0;
+	std::memset(&c->st_w2, 0, sizeof c->st_w2);
 	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;

@@ -52,7 +52,9 @@ typedef struct {
 	int32_t dp_per_lane;    /* batched form: 1 one task per lane (32-bit), 2 two tasks per lane (packed
 	                           16-bit, band classes <= 12); 0 = the hero_autotune choice, else 1 */
 	int32_t dp_eager;       /* 1: launch the DP stage kernel by kernel instead of as one captured graph */
-	int32_t reserved[4];
+	int32_t chunk_workers;  /* subject chunks in flight at once (own streams and buffers; votes only,
+	                           record mode and autotuning run one); 0 = 2 */
+	int32_t reserved[3];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */

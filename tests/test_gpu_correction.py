@@ -55,15 +55,21 @@ def test_build_knobs_match_oracle(opts):
 def test_plain_form_equals_batched_and_chunking_is_invisible():
     rs = synth.generate(seed=88, genome_len=10000, len_min=100, len_max=200, coverage=30.0, sub_5p=0.01)
     base = dict(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40)
-    outs = []
-    for extra in (dict(dp_form=0), dict(dp_form=1), dict(dp_form=0, chunk_subjects=333)):
+    outs, stats = [], []
+    # one chunk; plain form; many chunks on two concurrent workers (the default); the same chunks
+    # one after the other
+    for extra in (dict(dp_form=0), dict(dp_form=1), dict(dp_form=0, chunk_subjects=333),
+                  dict(dp_form=0, chunk_subjects=333, chunk_workers=1)):
         with gpu.Context(**base, **extra) as ctx:
             ctx.set_queries(rs)
             ctx.build_index()
             ctx.align_self()
             outs.append((ctx.tally(), ctx.consensus()))
+            stats.append(ctx.stats())
     for t, c in outs[1:]:
         assert np.array_equal(t, outs[0][0]) and np.array_equal(c, outs[0][1])
+    for key in ("seeds", "dp_tasks", "dp_calls_exec", "dp_cells_exec", "records_kept", "subjects"):
+        assert stats[2][key] == stats[3][key] == stats[0][key], key
 
 
 def test_separate_subject_set_matches_oracle():
