@@ -405,7 +405,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	tc.band = bm, tc.tasks = (int64_t) Lc.ntasks;
 	float best = -1.0f;
 	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand && pk16_class_fits(c->o, rows) ? 2 : 1); per_lane++) {
-		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane, rows);
+		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane, rows, Lc.tagged != 0);
 		for (int wpc = 1; wpc <= occ; wpc *= 2) {
 			const int wpc_eff = wpc * 2 > occ ? occ : wpc; // the last candidate is the occupancy limit
 			const int waves = (int) std::min<uint64_t>((uint64_t) c->cu_count * wpc_eff,
@@ -422,7 +422,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 				HIPCHK(c, hipStreamSynchronize(w->st));
 				HIPCHK(c, hipEventElapsedTime(&ms, w->ev[9].a, w->ev[3].a));
 			}
-			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1, rows)) tc.ms_default = ms;
+			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1, rows, Lc.tagged != 0)) tc.ms_default = ms;
 			if (tc.ncand < (int) (sizeof tc.cand / sizeof tc.cand[0]))
 				tc.cand[tc.ncand++] = hero_tune_candidate{per_lane, wpc_eff, ms};
 			if (best < 0 || ms < best) best = ms, tc.per_lane = per_lane, tc.waves_per_cu = wpc_eff, tc.ms = ms;
@@ -453,6 +453,7 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 	const int bmax_plain = 64;
 	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
 	L.route_bmax = -1, L.route_rows = 0;
+	L.tagged = !c->opt.dp_untagged && dp_tagged_form(c->o, rows) ? 1 : 0;
 	uint32_t plain_begin = 0, plain_n = (uint32_t) ntasks;
 	if (c->opt.dp_form == 0 && ntasks > 0) {
 		// batched form: bucket tasks by (band class, descending length), then one persistent launch
@@ -483,7 +484,7 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
 			const int per_lane = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
 			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
-			                                               : dp_batched_waves_per_cu(bm, aff, per_lane, rows);
+			                                               : dp_batched_waves_per_cu(bm, aff, per_lane, rows, L.tagged != 0);
 			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 			waves = (int) std::min<uint64_t>((uint64_t) waves, (b1 - b0 + 64 * per_lane - 1) / (64 * per_lane));
 			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64 * per_lane));
@@ -542,12 +543,13 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	HIPCHK(c, w->scan_tmp.ensure(scan_tmp_words(kNumBuckets)));
 	int per_lane[kNumBandClasses], waves[kNumBandClasses];
 	size_t scratch = 0;
+	const bool tagged = !c->opt.dp_untagged && dp_tagged_form(c->o, rows);
 	for (int cls = 0; cls < kNumBandClasses; cls++) {
 		const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
 		const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
 		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
 		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
-		                                               : dp_batched_waves_per_cu(bm, aff, per_lane[cls], rows);
+		                                               : dp_batched_waves_per_cu(bm, aff, per_lane[cls], rows, tagged);
 		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
 		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);
 	}
@@ -558,6 +560,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	L.tasks = cd.tasks, L.order = w->order.p, L.S = S, L.Q = Q, L.o = c->o;
 	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = w->dstats.p;
 	L.route_bmax = -1, L.route_rows = 0, L.rows_max = rows;
+	L.tagged = tagged ? 1 : 0;
 	// everything the captured launches take by value
 	std::vector<uintptr_t> key((sizeof L + sizeof(uintptr_t) - 1) / sizeof(uintptr_t), 0);
 	std::memcpy(key.data(), &L, sizeof L);

@@ -240,18 +240,178 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 	}
 }
 
+// Tagged affine forward pass: the same recurrence, results and pointers as dp_forward<BMAX, true>
+// for gap opens of equal cost (o.dx == o.dy) and 2-bit codes, in fewer operations per cell.
+//
+// A value v of cell (i, j) is kept as 8 v + t - 4 hit (i + j): scaled by 8, a tag t in the two low
+// bits, and an offset that moves by 4 hit per up or left step and by 8 hit per diagonal step, so a
+// diagonal move adds 8 (s - hit): nothing on a match. Values of one cell share the offset, so
+// comparing them compares the values, and equal values compare by tag. The tags rank the three
+// states as the reference breaks ties (src/BandAligner.cpp:176-222: the diagonal needs h > x and
+// h >= y, then the left gap needs y > x): 3 the vertical gap (up), 2 / 1 the diagonal (match /
+// mismatch), 0 the horizontal gap (left). The max3 of a cell's three states is therefore at once
+// the next row's diagonal input and, in its low bits, the cell's pointer, stored as it is.
+//
+// A cell's h plus a gap open, G = 8 h - off + 8 dx - 4 hit (untagged), is its up neighbour's and
+// its left neighbour's gap-open candidate at once (dx == dy; both moves add 4 hit to the offset);
+// the extended vertical gap is XE = x + 8 ex - 4 hit (tag 3 kept), the horizontal one
+// y + 8 ex - 4 hit (tag 0). Per band cell the rows carry two values to the next row:
+//   MK - the diagonal base: 8 max(h, x, y) - off + 2 (the match tag preset), so the next row's h
+//        is MK on a match and MK + 8 (miss - hit) - 1 on a mismatch;
+//   U  - the vertical gap of the cell below: max(G, XE) | 3.
+// G and y + E of the cell to the left run along the row. The aligned column lenQ's rule needs G
+// and XE of the cell above separately: each lane keeps them for its next row's column lenQ.
+// Sentinel cells hold kSentT; the caller keeps every real value within the scaled range
+// (dp_tagged_fits). Pointer codes: 3 up, 2 match, 1 mismatch, 0 left (dp_traceback<true>).
+constexpr int kSentT = -(1 << 29);
+constexpr int kTagRange = 1 << 27;
+HD int max3_(int a, int b, int c) { return imax_(imax_(a, b), c); }
+// bit p of w (0 or 1)
+HD uint32_t bit_u32(uint32_t w, int p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __builtin_amdgcn_ubfe(w, (uint32_t) p, 1u);
+#else
+	return (w >> p) & 1u;
+#endif
+}
+// c + m * a for m in {0, 1} and |a| < 2^23: one v_mad_i32_i24 (written out, because the compiler
+// turns a product with a one-bit value into a bit test and a select: three instructions)
+HD int mad01(int m, int a, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	int r;
+	asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(c));
+	return r;
+#else
+	return c + m * a;
+#endif
+}
+// Every scaled value of a task with rows_max rows stays within +-kTagRange: |v| <= s (lenS + lenQ)
+// for the largest step cost s, times 8, plus the offset 4 hit (lenS + lenQ) and the tag.
+HD bool dp_tagged_fits(const Opts &o, int rows_max) {
+	int s = o.hit < 0 ? -o.hit : o.hit;
+	const int c[5] = {o.dx, o.ex, o.sx, o.miss, o.dy};
+	for (int k = 0; k < 5; k++) s = imax_(s, c[k] < 0 ? -c[k] : c[k]);
+	return (int64_t) (12 * (int64_t) s + 4) * (int64_t) (2 * rows_max + 2) < (int64_t) kTagRange;
+}
+
+// mode_or: row variants taken in addition to those the task needs (a lane in a wavefront with
+// other tasks takes theirs too); the kernels pass 0.
+template <int BMAX, class SS, class QS, class PS>
+HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool right_al, SS &S, QS &Q,
+		PS &ptrs, int mode_or = 0) {
+	static_assert(SS::kTwoBit && QS::kTwoBit, "the tagged form reads 2-bit codes");
+	constexpr int W = 2 * BMAX + 1;
+	static_assert(W <= 32, "the packed query window holds 32 cells");
+	constexpr int NW = (W + 15) / 16;
+	constexpr int kLast = W - 16 * (NW - 1); // cells in the last pointer word
+	const int step = 4 * o.hit;                 // offset of one up or left move
+	const int C = 8 * o.dx - step;              // G = 8 h - off + C
+	const int E = 8 * o.ex - step;              // XE = x + E, and y + E along the row
+	const int DM = 8 * (o.miss - o.hit) - 1;    // h on a mismatch: MK + DM (tag 2 -> 1)
+	const int CQx = 8 * (o.sx - o.dx), CQe = 8 * (o.sx - o.ex); // aligned column lenQ: vertical cost sx
+	int MK[W + 1], U[W + 1];
+	int Gq = kSentT, XEq = kSentT; // G and XE of the cell above the next row's column lenQ
+	QWin<W, true> qw;              // qw.at(d) = Q[i - b - 1 + d] for the current row i
+	{ // row 0 (src/BandAligner.cpp:140-171): pointers 0 (left) everywhere
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+#pragma unroll
+		for (int d = 0; d <= W; d++) {
+			int h, x, y;
+			const int j = d - b;
+			const bool in = row0_cell<true>(o, j, b, lenQ, h, x, y) && d < W;
+			const int off = step * j;
+			const int g = in ? 8 * h - off + C : kSentT, xe = in ? 8 * x + 3 - off + E : kSentT;
+			MK[d] = in ? 8 * max3_(h, x, y) - off + 2 : kSentT;
+			U[d] = in ? imax_(g, xe) | 3 : kSentT;
+			if (left_al && d == lenQ + b) Gq = g, XEq = xe; // above row 1's column lenQ
+		}
+		ptrs.put_row(0, pw);
+	}
+#pragma unroll
+	for (int d = 0; d < W; d++) qw.set(d, d < b ? kCodeNone : Q.next()); // row 1: Q[d - b]
+	// Rows as in dp_forward: every band cell takes the interior recurrence, and the edge rules
+	// (column 0, the aligned column lenQ, the cut past a narrower band's edge) are selects in
+	// the rows where some lane needs them.
+	constexpr int kC0 = 1, kCQ = 2, kCut = 4;
+	const int d0base = b, dQbase = lenQ + b;
+	const int edge_d = 2 * b;
+	auto row = [&](int i, int sc, uint32_t *pw, auto mode_tag) {
+		constexpr int MODE = decltype(mode_tag)::value;
+		const int d0 = d0base - i, dQ = left_al ? dQbase - i : -1;
+		// column 0 (src/BandAligner.cpp:150-163): a vertical gap from the origin; off = step * i
+		int h0 = 0, x0 = 0, y0 = 0, g0 = 0;
+		if (MODE & kC0) {
+			const int xv = right_al ? i * o.sx : o.dx + (i - 1) * o.ex;
+			const int yv = right_al ? o.dy : xv;
+			const int off = step * i;
+			h0 = 8 * xv + 2 - off, x0 = 8 * xv + 3 - off, y0 = 8 * yv - off, g0 = 8 * xv - off + C;
+		}
+		const uint64_t mbits = qw.differ_bits(sc);
+		const uint32_t mlo = (uint32_t) mbits, mhi = (uint32_t) (mbits >> 32);
+		int gl = kSentT, yel = kSentT; // G and y + E of this row's cell d - 1
+		int gq = Gq, xeq = XEq;         // of this row's column lenQ, for the next row
+#pragma unroll
+		for (int d = 0; d < W; d++) {
+			const int m = (int) bit_u32(d < 16 ? mlo : mhi, 2 * (d & 15));
+			int h = mad01(m, DM, MK[d]);
+			int x = U[d + 1];
+			int y = imax_(gl, yel);
+			if (MODE & kCQ) x = d == dQ ? imax_(Gq + CQx, XEq + CQe) | 3 : x;
+			if (MODE & kCut) y = d <= edge_d ? y : kSentT;
+			int g = h + m + (C - 2); // 8 h - off + C: the tag 2 - m taken off
+			int mx;
+			if (MODE & kC0) {
+				const bool c0 = d == d0;
+				h = c0 ? h0 : h, x = c0 ? x0 : x, y = c0 ? y0 : y, g = c0 ? g0 : g;
+				mx = max3_(h, x, y);
+				const int pv = c0 ? 3 : mx; // column 0 points up
+				pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) pv << 30);
+			} else {
+				mx = max3_(h, x, y);
+				pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) mx << 30);
+			}
+			MK[d] = (mx & ~3) | 2;
+			const int xe = x + E;
+			U[d] = imax_(g, xe) | 3;
+			if (MODE & kCQ) gq = d == dQ ? g : gq, xeq = d == dQ ? xe : xeq;
+			gl = g, yel = y + E;
+		}
+		if (MODE & kCQ) Gq = gq, XEq = xeq;
+		if (kLast < 16) pw[NW - 1] >>= 2 * (16 - kLast);
+	};
+	const int cut = mode_or | (wave_all(b == BMAX) ? 0 : kCut);
+	for (int i = 1; i <= lenS; i++) {
+		const int sc = S.next();
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+		const int mode = cut | (wave_any(i <= b) ? kC0 : 0) | (wave_any(left_al && lenQ - i <= b) ? kCQ : 0);
+		switch (mode) {
+		case 0: row(i, sc, pw, std::integral_constant<int, 0>()); break;
+		case kCut: row(i, sc, pw, std::integral_constant<int, kCut>()); break;
+		default: row(i, sc, pw, std::integral_constant<int, kC0 | kCQ | kCut>()); break;
+		}
+		ptrs.put_row(i, pw);
+		qw.slide((i - b + W - 1 >= 0) ? Q.next() : kCodeNone);
+	}
+}
+
 // Traceback (src/BandAligner.cpp:114-125,227-238): from (lenS, lenQ) follow the pointers to the
 // origin. Steps come last-to-first through emit(step): kStepMatch / kStepMismatch for a diagonal,
 // kStepLeft (j-1), kStepUp (i-1). Returns the step count, or -1 if the path leaves the band
-// (never for pointers written by dp_forward).
-template <class PS, class EM>
+// (never for pointers written by dp_forward). TAGGED: the pointers of dp_forward_tagged.
+template <bool TAGGED = false, class PS, class EM>
 HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 	int i = lenS, j = lenQ, n = 0;
 	while (!(i == 0 && j == 0)) {
 		const int d = j - i + b;
 		if (i < 0 || j < 0 || d < 0 || d > 2 * b) return -1; // left the band: corrupt pointers
 		const uint32_t word = ptrs.get(i, d >> 4);
-		const int p = (int) ((word >> (2 * (d & 15))) & 3u);
+		const int t = (int) ((word >> (2 * (d & 15))) & 3u);
+		// tagged codes 0 left, 1 mismatch, 2 match, 3 up -> 1, 3, 2, 0
+		const int p = TAGGED ? (int) ((0x2Du >> (2 * t)) & 3u) : t;
 		if (p >= 2) {
 			emit(p == 2 ? kStepMatch : kStepMismatch);
 			i--, j--;

@@ -112,6 +112,7 @@ struct DpLaunch {
 	uint32_t *scratch;
 	int rows_max;
 	int route_bmax, route_rows; // plain form: >= 0 restricts it to tasks the batched form cannot take
+	int tagged;                 // one task per lane, affine: rows in the tagged form (dp_tagged_form)
 	uint32_t *batch_counter;
 	unsigned long long *stats;
 	// graph launches: the task range [range[0], range[1]) of the order array, read on the device
@@ -148,9 +149,14 @@ hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, co
 		uint32_t *op_n, const uint32_t *op_off, int32_t *pos, uint32_t *ops, hipStream_t st);
 // per_lane: tasks per lane, 1 (32-bit) or 2 (packed 16-bit, band classes up to kPk16MaxBand)
 hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwaves, hipStream_t st);
-// occupancy of one batched instantiation (rows_max sizes the packed form's LDS)
-int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane, int rows_max);
-constexpr int kPk16MaxBand = 12; // widest band class of the two-tasks-per-lane form (windows of 32 cells)
+// occupancy of one batched instantiation (rows_max sizes the packed form's LDS; tagged: the
+// one-task-per-lane affine rows in the tagged form)
+int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane, int rows_max, bool tagged);
+constexpr int kPk16MaxBand = 12;   // widest band class of the two-tasks-per-lane form (windows of 32 cells)
+constexpr int kTaggedMaxBand = 12; // widest band class of the tagged affine rows (windows of 32 cells)
+// Whether one-task-per-lane affine rows may run in the tagged form: gap opens of equal cost both
+// ways and every scaled value of a task of up to rows_max rows within range (dp_core.h).
+bool dp_tagged_form(const Opts &o, int rows_max);
 // The packed form stages each lane's four segments (< rows_max codes each) in LDS: words per
 // segment (odd, so the lanes' segments start in different banks) and bytes per wavefront.
 inline __host__ __device__ int pk_lds_words(int rows_max) { return ((rows_max + 15) / 16) | 1; }

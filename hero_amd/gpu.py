@@ -25,7 +25,8 @@ class Options(ctypes.Structure):
         ("band_override", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
-        ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -54,7 +54,9 @@ typedef struct {
 	int32_t dp_eager;       /* 1: launch the DP stage kernel by kernel instead of as one captured graph */
 	int32_t chunk_workers;  /* subject chunks in flight at once (own streams and buffers; votes only,
 	                           record mode and autotuning run one); 0 = 2 */
-	int32_t reserved[3];
+	int32_t dp_untagged;    /* 1: one-task-per-lane affine rows in the general form; 0: the tagged form
+	                           (pointer in the scores' low bits) where dx == dy and the band is <= 12 */
+	int32_t reserved[2];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */

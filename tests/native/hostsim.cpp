@@ -376,6 +376,56 @@ static int pk16_check_one(const Opts &o, uint64_t &rng, int max_len) {
 	return bad;
 }
 
+// Tagged affine DP against dp_forward on random tasks (as pk16_check_one): the tracebacks must
+// agree. mode_or forces row variants on every row (a lane among other tasks). Returns the number
+// of differing tasks; -1 when the options do not admit the tagged form.
+template <int BMAX>
+static int tagged_check_one(const Opts &o, uint64_t &rng, int max_len, int mode_or) {
+	auto rnd = [&](uint64_t n) {
+		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+		return (int) (rng % n);
+	};
+	Pk16Task T;
+	std::vector<int> s, q;
+	T.lenS = rnd((uint64_t) max_len + 1);
+	T.lenQ = T.lenS - rnd((uint64_t) (T.lenS < BMAX ? T.lenS + 1 : BMAX + 1));
+	if (rnd(5) == 0) T.lenQ = rnd((uint64_t) T.lenS + 1);
+	T.b = T.lenS - T.lenQ + rnd(3);
+	if (T.b > BMAX) T.b = BMAX, T.lenQ = T.lenS - BMAX > 0 ? T.lenS - BMAX : 0;
+	if (T.b < T.lenS - T.lenQ) T.lenQ = T.lenS - T.b;
+	const int f = rnd(3);
+	T.left_al = f == 1, T.right_al = f == 2;
+	s.resize((size_t) T.lenS), q.resize((size_t) T.lenQ);
+	for (auto &c : s) c = rnd(4);
+	for (size_t j = 0; j < q.size(); j++) q[j] = (j < s.size() && rnd(10)) ? s[j] : rnd(4);
+	const int nw = (2 * BMAX + 1 + 15) / 16;
+	RowStore rt, rr;
+	rt.nw = rr.nw = nw;
+	CodeStream S1{&s, 0}, Q1{&q, 0}, S2{&s, 0}, Q2{&q, 0};
+	dp_forward_tagged<BMAX>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S1, Q1, rt, mode_or);
+	dp_forward<BMAX, true>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S2, Q2, rr);
+	std::vector<int> x, y;
+	auto emx = [&](int st) { x.push_back(st); };
+	auto emy = [&](int st) { y.push_back(st); };
+	const int nx = dp_traceback<true>(T.lenS, T.lenQ, T.b, rt, emx);
+	const int ny = dp_traceback(T.lenS, T.lenQ, T.b, rr, emy);
+	return (nx != ny || x != y) ? 1 : 0;
+}
+
+extern "C" int hostsim_tagged_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len, int mode_or) {
+	const Opts o = to_opts(cfg);
+	if (o.aligner != 0 || o.dx != o.dy || !dp_tagged_fits(o, max_len + 1)) return -1;
+	uint64_t rng = seed | 1;
+	int bad = 0;
+	for (int t = 0; t < n; t++) {
+#define TG_CASE(BM) \
+	if (bmax == BM) bad += tagged_check_one<BM>(o, rng, max_len, mode_or);
+		TG_CASE(4) TG_CASE(8) TG_CASE(12)
+#undef TG_CASE
+	}
+	return bad;
+}
+
 extern "C" int hostsim_pk16_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len) {
 	const Opts o = to_opts(cfg);
 	uint64_t rng = seed | 1;

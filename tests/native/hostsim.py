@@ -55,3 +55,12 @@ def pk16_check(cfg, n, seed, bmax, max_len):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     return f(ctypes.addressof(cfg), n, seed, bmax, max_len)
+
+
+def tagged_check(cfg, n, seed, bmax, max_len, mode_or=0):
+    """Tasks whose tagged affine DP traceback differs from the plain one-task DP's (0 = all
+    equal; -1 = options outside the tagged form). mode_or forces row variants on every row."""
+    f = lib().hostsim_tagged_check
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return f(ctypes.addressof(cfg), n, seed, bmax, max_len, mode_or)

@@ -201,6 +201,33 @@ def test_autotuned_schedule_gives_identical_results():
             assert np.array_equal(ctx.tally(), t0) and np.array_equal(ctx.consensus(), c0)
 
 
+@pytest.mark.parametrize("costs", [dict(), dict(dx=-6, dy=-6, ex=-2, miss=-3, hit=2)], ids=["default", "hit2"])
+def test_tagged_affine_rows_equal_general_rows(costs):
+    """The tagged affine rows (pointer in the low bits of scaled scores; band classes up to 12)
+    give the general rows' tallies, corrected reads and DP counts, and both equal the oracle;
+    reads of 60-260 bases put tasks in every band class, graph and eager launches alike."""
+    rs = synth.generate(seed=515, genome_len=16000, len_min=60, len_max=260, coverage=30.0, sub_5p=0.01,
+                        indel_rate=0.003)
+    opts = dict(call_consensus=1, k=25, aligner=0, mismatch_rate=0.05, indel_rate=0.03, min_overlap=40,
+                chunk_subjects=700, **costs)
+    res = []
+    for untagged, eager in ((1, 0), (0, 0), (0, 1)):
+        with gpu.Context(dp_untagged=untagged, dp_eager=eager, **opts) as ctx:
+            ctx.set_queries(rs)
+            ctx.build_index()
+            ctx.align_self()
+            res.append((ctx.tally(), ctx.consensus(), ctx.stats()))
+    t0, c0, s0 = res[0]
+    assert s0["dp_skipped"] == 0 and s0["dp_calls_exec"] > 0
+    for t, cns, s in res[1:]:
+        assert np.array_equal(t, t0) and np.array_equal(cns, c0)
+        for k in ("dp_calls_exec", "dp_cells_exec", "records_kept", "dp_skipped"):
+            assert s[k] == s0[k], k
+    ro = O.run(O.make_cfg(**{k: v for k, v in opts.items() if k != "chunk_subjects"}), rs, threads=8,
+               want_tally=True)
+    assert np.array_equal(t0, ro["tally"]) and np.array_equal(c0, ro["corrected"])
+
+
 def test_dp_graph_replays_equal_eager_launches():
     """The DP stage captured once as a graph and replayed for every subject chunk (task count and
     class ranges read on the device) gives the kernel-by-kernel launches' tallies, corrected reads

@@ -70,3 +70,25 @@ def test_packed_two_task_dp_equals_one_task_dp(aligner, bmax):
     cfg = O.make_cfg(aligner=aligner, **costs)
     for max_len in (3, 20, 140, 500):
         assert H.pk16_check(cfg, 400, 12345 + bmax + max_len, bmax, max_len) == 0, (aligner, bmax, max_len)
+
+
+@pytest.mark.parametrize("bmax", [4, 8, 12])
+@pytest.mark.parametrize(
+    "costs",
+    [
+        dict(dx=-4, dy=-4, ex=-1, sx=0, miss=-4, hit=1),
+        dict(dx=-7, dy=-7, ex=-2, sx=-1, miss=-3, hit=2),
+        dict(dx=-3, dy=-3, ex=-3, sx=0, miss=-1, hit=0),
+        dict(dx=-5, dy=-5, ex=-1, sx=-2, miss=-6, hit=3),
+    ],
+    ids=["default", "hit2", "hit0", "hit3"],
+)
+def test_tagged_affine_dp_equals_plain_dp(costs, bmax):
+    """The tagged affine DP (pointer in the low bits of scaled, offset scores) traces the same
+    paths as the plain 32-bit DP, with each task's own row variants and with every row forced
+    through the cut / all-edge variants (as a lane among other tasks is)."""
+    cfg = O.make_cfg(aligner=0, **costs)
+    for max_len in (3, 20, 140, 500):
+        for mode_or in (0, 4, 7):
+            seed = 777 + bmax + max_len + 13 * mode_or
+            assert H.tagged_check(cfg, 300, seed, bmax, max_len, mode_or) == 0, (costs, bmax, max_len, mode_or)
