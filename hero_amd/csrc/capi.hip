@@ -1142,7 +1142,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(hipMemsetAsync(errf.p, 0, 4, c->w.st));
 		IXCHK(hipMemsetAsync(cursor.p, 0, 8, c->w.st));
 		const Packed Rp = view(R, R.good.p);
-		IXCHK(launch_index_gen(Rp, dkoff.p, k, all_positions, occ_hi.p, occ_lo.p, occ_entry.p, c->w.st));
+		IXCHK(launch_index_gen(Rp, dkoff.p, k, all_positions, occ_hi.p, occ_lo.p, occ_entry.p, occ_rank.p, c->w.st));
 		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, ix.P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->w.st));
 		IXCHK(launch_index_count(occ_slot.p, nocc, ix.slots.p, occ_rank.p, c->w.st));
 		IXCHK(launch_index_alloc(owner.p, ix.P, occ_hi.p, occ_lo.p, k, ix.slots.p, cursor.p, c->w.st));
@@ -1560,19 +1560,10 @@ __global__ void k_lookup(DevIndex ix, const char *win, int n, int64_t *count, ui
 		if (t < 32) hi = (hi << 2) | c;
 		else lo = (lo << 2) | c;
 	}
-	uint64_t h = hash_kmer(k, hi, lo, ix.fm);
-	count[i] = 0;
-	first_entry_idx[i] = 0;
-	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
-		const Slot s = ix.slots[h];
-		if (s.count == 0) return;
-		if (s.key == hi && (k <= 32 || s.key_lo == lo)) {
-			count[i] = s.count;
-			first_entry_idx[i] = s.start;
-			return;
-		}
-		h = (h + 1 == ix.P) ? 0 : h + 1;
-	}
+	uint32_t start = 0, cnt = 0;
+	const bool hit = ix_lookup(ix, hi, lo, start, cnt);
+	count[i] = hit ? cnt : 0;
+	first_entry_idx[i] = hit ? start : 0;
 }
 } // namespace
 

@@ -6,15 +6,29 @@
 
 namespace hero {
 
-// Open-addressing slot of the seed index: the k-mer (first 32 bases in key, the rest of a longer
-// one in key_lo), its bucket in `entries`; count == 0 marks an empty slot. 32 bytes, aligned, so
-// a probe reads one cache line.
+// Open-addressing slot of the seed index. The table is keyed by canonical k-mers: the smaller
+// (as the number (key, key_lo)) of a k-mer and its reverse complement, first 32 bases in key, the
+// rest of a longer one in key_lo. Its bucket in `entries` is [start, start + count): the nfwd
+// occurrences of the canonical k-mer itself, then the nrc occurrences of its reverse complement,
+// so one probe answers a window on both strands. count == 0 marks an empty slot. 32 bytes,
+// aligned, so a probe reads one cache line.
 struct __align__(32) Slot {
 	uint64_t key;
 	uint64_t key_lo;
 	uint32_t start;
 	uint32_t count;
+	uint32_t nfwd, nrc;
 };
+
+// The canonical form (chi, clo) of k-mer (hi, lo); true when the k-mer is its own canonical form
+// (as a palindrome of even k is).
+HD bool kmer_canonical(int k, uint64_t hi, uint64_t lo, uint64_t &chi, uint64_t &clo) {
+	uint64_t rhi, rlo;
+	kmer_revcomp(k, hi, lo, rhi, rlo);
+	const bool fwd = hi < rhi || (hi == rhi && lo <= rlo);
+	chi = fwd ? hi : rhi, clo = fwd ? lo : rlo;
+	return fwd;
+}
 
 // Bucket entry: (tag << 54) | (pos << 40) | innerId (src/HashTableLongKey.cpp:334-336 without
 // the tag). The tag hashes the read's uniqueId, so the self-hit test (same uniqueId as the
@@ -33,6 +47,39 @@ struct DevIndex {
 	int32_t k;
 	FastMod fm; // P with its reciprocal
 };
+
+#if defined(__HIPCC__)
+// The slot of canonical k-mer (chi, clo): from its home slot to the first empty one.
+__device__ __forceinline__ bool ix_probe(const DevIndex &ix, uint64_t chi, uint64_t clo, Slot &out) {
+	uint64_t h = hash_kmer(ix.k, chi, clo, ix.fm);
+	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
+		const Slot s = ix.slots[h];
+		if (s.count == 0) return false;
+		if (s.key == chi && (ix.k <= 32 || s.key_lo == clo)) {
+			out = s;
+			return true;
+		}
+		h = (h + 1 == ix.P) ? 0 : h + 1;
+	}
+	return false;
+}
+// The half of a canonical bucket holding the occurrences of the k-mer whose canonical-form test
+// gave fwd (kmer_canonical).
+__device__ __forceinline__ void ix_half(const Slot &s, bool fwd, uint32_t &start, uint32_t &count) {
+	start = fwd ? s.start : s.start + s.nfwd;
+	count = fwd ? s.nfwd : s.count - s.nfwd;
+}
+// Occurrences of exactly the k-mer (hi, lo) (HashTableLongKey::getListOfReads,
+// src/HashTableLongKey.cpp:79-99): false when it has none.
+__device__ __forceinline__ bool ix_lookup(const DevIndex &ix, uint64_t hi, uint64_t lo, uint32_t &start, uint32_t &count) {
+	uint64_t chi, clo;
+	const bool fwd = kmer_canonical(ix.k, hi, lo, chi, clo);
+	Slot s;
+	if (!ix_probe(ix, chi, clo, s)) return false;
+	ix_half(s, fwd, start, count);
+	return count != 0;
+}
+#endif
 
 // One accepted alignment record of a chunk (record mode, without -c). Slots are laid out in the
 // reference's processing order: subject, strand, query group (innerId), chain index; a slot whose
@@ -122,7 +169,7 @@ struct DpLaunch {
 
 // launchers (kernels.hip)
 hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool all_positions, uint64_t *occ_hi,
-		uint64_t *occ_lo, uint64_t *occ_entry, hipStream_t st);
+		uint64_t *occ_lo, uint64_t *occ_entry, uint32_t *occ_rank, hipStream_t st);
 hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
 		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st);
 hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st);

@@ -51,12 +51,16 @@ __device__ __forceinline__ uint64_t lanes_below_mask() {
 
 // ---------------------------------------------------------------------------------------------
 // Seed index (roles of HashTableLongKey::createHashTables / insertQueryDataset / getListOfReads,
-// src/HashTableLongKey.cpp:41-99,250-342). The home slot is the reference hash; linear probing.
-// Buckets hold exactly the occurrences of one k-mer, so lookups return what getListOfReads
-// returns, whatever the slot placement.
+// src/HashTableLongKey.cpp:41-99,250-342). Slots are keyed by canonical k-mers (Slot); the home
+// slot is the reference hash of the canonical form; linear probing. Each half of a bucket holds
+// exactly the occurrences of one k-mer, so lookups return what getListOfReads returns, whatever
+// the slot placement.
 // ---------------------------------------------------------------------------------------------
+// Build-time occurrence rank: bit 31 marks an occurrence of the reverse complement of its slot's
+// canonical k-mer (set by k_index_gen), the low bits are its rank in its half (k_index_count).
+constexpr uint32_t kOccRc = 0x80000000u;
 __global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, int all_positions, uint64_t *occ_hi, uint64_t *occ_lo,
-		uint64_t *occ_entry) {
+		uint64_t *occ_entry, uint32_t *occ_rank) {
 	const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= Q.n || !Q.good[q]) return;
 	const int L = Q.len[q];
@@ -65,18 +69,21 @@ __global__ void k_index_gen(Packed Q, const uint64_t *koff, int k, int all_posit
 	const int nk = all_positions ? (L >= k ? L - k + 1 : 0) : num_keys(L, k);
 	for (int t = 0; t < nk; t++) {
 		const int p = all_positions ? t : key_pos(L, k, t);
-		uint64_t hi, lo;
+		uint64_t hi, lo, chi, clo;
 		kmer_fwd(w, p, k, hi, lo);
-		occ_hi[o] = hi;
-		occ_lo[o] = lo;
+		const bool fwd = kmer_canonical(k, hi, lo, chi, clo);
+		occ_hi[o] = chi;
+		occ_lo[o] = clo;
 		occ_entry[o] = ((uint64_t) uid_tag(Q.uid[q]) << 54) | ((uint64_t) p << 40) | (uint64_t) q;
+		occ_rank[o] = fwd ? 0u : kOccRc;
 		o++;
 	}
 }
 hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool all_positions, uint64_t *occ_hi,
-		uint64_t *occ_lo, uint64_t *occ_entry, hipStream_t st) {
+		uint64_t *occ_lo, uint64_t *occ_entry, uint32_t *occ_rank, hipStream_t st) {
 	if (Q.n == 0) return hipSuccess;
-	k_index_gen<<<(unsigned) ((Q.n + 255) / 256), 256, 0, st>>>(Q, koff, k, all_positions ? 1 : 0, occ_hi, occ_lo, occ_entry);
+	k_index_gen<<<(unsigned) ((Q.n + 255) / 256), 256, 0, st>>>(Q, koff, k, all_positions ? 1 : 0, occ_hi, occ_lo,
+			occ_entry, occ_rank);
 	return hipGetLastError();
 }
 
@@ -115,7 +122,9 @@ hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, ui
 __global__ void k_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank) {
 	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= nocc) return;
-	occ_rank[o] = atomicAdd(&slots[occ_slot[o]].count, 1u);
+	const uint32_t rc = occ_rank[o] & kOccRc;
+	Slot &s = slots[occ_slot[o]];
+	occ_rank[o] = rc | atomicAdd(rc ? &s.nrc : &s.nfwd, 1u);
 }
 hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slots, uint32_t *occ_rank, hipStream_t st) {
 	if (nocc == 0) return hipSuccess;
@@ -134,7 +143,7 @@ __global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint
 	uint32_t cnt = 0;
 	for (int i = 0; i < kAllocPerThread; i++) {
 		const uint64_t s = base + (uint64_t) i * 256;
-		if (s < P && owner[s]) cnt += slots[s].count;
+		if (s < P && owner[s]) cnt += slots[s].nfwd + slots[s].nrc;
 	}
 	const uint32_t pre = wave_excl_scan(cnt);
 	const int wv = threadIdx.x >> 6;
@@ -161,6 +170,7 @@ __global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint
 		sl.key = occ_hi[o];
 		sl.key_lo = k > 32 ? occ_lo[o] : 0;
 		sl.start = b;
+		sl.count = sl.nfwd + sl.nrc;
 		b += sl.count;
 		slots[s] = sl;
 	}
@@ -177,7 +187,9 @@ __global__ void k_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank,
 		const Slot *slots, uint64_t *entries) {
 	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= nocc) return;
-	entries[(uint64_t) slots[occ_slot[o]].start + occ_rank[o]] = occ_entry[o];
+	const uint32_t r = occ_rank[o];
+	const Slot &s = slots[occ_slot[o]];
+	entries[(uint64_t) s.start + ((r & kOccRc) ? s.nfwd : 0u) + (r & ~kOccRc)] = occ_entry[o];
 }
 hipError_t launch_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank, const uint64_t *occ_entry, uint64_t nocc,
 		const Slot *slots, uint64_t *entries, hipStream_t st) {
@@ -186,43 +198,31 @@ hipError_t launch_index_fill(const uint32_t *occ_slot, const uint32_t *occ_rank,
 	return hipGetLastError();
 }
 
-// Puts each bucket in the reference's insertion order (innerId, then position). Seeds are
+// Puts each half bucket in the reference's insertion order (innerId, then position). Seeds are
 // re-sorted by (innerId, posQ, posS) downstream, so this only makes the index deterministic.
 __global__ void k_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries) {
 	const uint64_t s = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= P) return;
 	const Slot sl = slots[s];
-	if (sl.count < 2 || sl.count > 512) return;
-	uint64_t *e = entries + sl.start;
 	auto key = [](uint64_t v) { return ((v & kIdMask) << 14) | (uint64_t) entry_pos(v); };
-	for (uint32_t i = 1; i < sl.count; i++) {
-		const uint64_t v = e[i], kv = key(v);
-		int j = (int) i - 1;
-		while (j >= 0 && key(e[j]) > kv) {
-			e[j + 1] = e[j];
-			j--;
+	for (int half = 0; half < 2; half++) {
+		const uint32_t n = half ? sl.count - sl.nfwd : sl.nfwd;
+		if (n < 2 || n > 512) continue;
+		uint64_t *e = entries + sl.start + (half ? sl.nfwd : 0u);
+		for (uint32_t i = 1; i < n; i++) {
+			const uint64_t v = e[i], kv = key(v);
+			int j = (int) i - 1;
+			while (j >= 0 && key(e[j]) > kv) {
+				e[j + 1] = e[j];
+				j--;
+			}
+			e[j + 1] = v;
 		}
-		e[j + 1] = v;
 	}
 }
 hipError_t launch_index_sort_buckets(const Slot *slots, uint64_t P, uint64_t *entries, hipStream_t st) {
 	k_index_sort_buckets<<<(unsigned) ((P + 255) / 256), 256, 0, st>>>(slots, P, entries);
 	return hipGetLastError();
-}
-
-__device__ __forceinline__ bool ix_lookup(const DevIndex &ix, uint64_t hi, uint64_t lo, uint32_t &start, uint32_t &count) {
-	uint64_t h = hash_kmer(ix.k, hi, lo, ix.fm);
-	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
-		const Slot s = ix.slots[h];
-		if (s.count == 0) return false;
-		if (s.key == hi && (ix.k <= 32 || s.key_lo == lo)) {
-			start = s.start;
-			count = s.count;
-			return true;
-		}
-		h = (h + 1 == ix.P) ? 0 : h + 1;
-	}
-	return false;
 }
 
 // The k-mer of subject window w in strand frame (reverse strand: window of the RC).
@@ -285,20 +285,35 @@ __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex 
 	const int Ls = S.len[s];
 	const int64_t uid = S.uid[s];
 	const int nwin = Ls - o.k + 1;
+	const uint64_t *ws = S.words + S.woff[s];
+	// Window w of the forward strand and window nwin - 1 - w of the reverse strand are the two
+	// orientations of one k-mer: one probe of its canonical form finds both windows' buckets (the
+	// reverse-strand window is the lane's too; pass A1 follows the same order).
+	uint32_t cnt[2] = {0, 0};
+	for (int w = lane; w < nwin; w += 64) {
+		uint64_t hi, lo, rhi, rlo;
+		kmer_fwd(ws, w, o.k, hi, lo);
+		kmer_revcomp(o.k, hi, lo, rhi, rlo); // the reverse strand's window nwin - 1 - w
+		const bool fwd = hi < rhi || (hi == rhi && lo <= rlo);
+		const bool fwd_r = !fwd || (hi == rhi && lo == rlo); // the reverse window's test
+		Slot sl_ix;
+		const bool found = ix_probe(ix, fwd ? hi : rhi, fwd ? lo : rlo, sl_ix);
+#pragma unroll
+		for (int strand = 0; strand < 2; strand++) {
+			const int wi = strand ? nwin - 1 - w : w;
+			uint32_t start = 0, count = 0;
+			if (found) ix_half(sl_ix, strand ? fwd_r : fwd, start, count);
+			if (c.win_info)
+				c.win_info[(int64_t) (2 * sl + strand) * c.win_stride + wi] = count ? ((uint64_t) start << 32) | count : 0;
+			if (count) cnt[strand] += bucket_seeds(S, Q, ix, o, Ls, uid, wi, strand, start, count, [](uint64_t) {});
+		}
+	}
+#pragma unroll
 	for (int strand = 0; strand < 2; strand++) {
 		const int slot = 2 * sl + strand;
-		uint32_t cnt = 0;
-		for (int w = lane; w < nwin; w += 64) {
-			uint64_t hi, lo;
-			window_key(S, s, Ls, w, strand, o.k, hi, lo);
-			uint32_t start = 0, count = 0;
-			const bool hit = ix_lookup(ix, hi, lo, start, count);
-			if (c.win_info) c.win_info[(int64_t) slot * c.win_stride + w] = hit ? ((uint64_t) start << 32) | count : 0;
-			if (hit) cnt += bucket_seeds(S, Q, ix, o, Ls, uid, w, strand, start, count, [](uint64_t) {});
-		}
-		if (c.win_info) c.lane_n[(int64_t) slot * 64 + lane] = cnt;
-		cnt = wave_sum(cnt);
-		if (lane == 0) c.seg_n[slot] = cnt;
+		if (c.win_info) c.lane_n[(int64_t) slot * 64 + lane] = cnt[strand];
+		const uint32_t t = wave_sum(cnt[strand]);
+		if (lane == 0) c.seg_n[slot] = t;
 	}
 	if (lane == 0) stat_add(stats, kStLookups, (unsigned long long) (2 * nwin));
 }
@@ -396,7 +411,9 @@ __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix,
 			const int slot = 2 * sl + strand;
 			uint32_t pos = wave_excl_scan(c.lane_n[(int64_t) slot * 64 + lane]);
 			const uint64_t *wi = c.win_info + (int64_t) slot * c.win_stride;
-			for (int w = lane; w < nwin; w += 64) {
+			// the lane's windows as k_seed_count counted them: reverse-strand windows from the end
+			for (int wl = lane; wl < nwin; wl += 64) {
+				const int w = strand ? nwin - 1 - wl : wl;
 				const uint64_t b = wi[w];
 				if (b) bucket_seeds(S, Q, ix, o, Ls, uid, w, strand, (uint32_t) (b >> 32), (uint32_t) b,
 						[&](uint64_t v) { dst[pos++] = v; });
@@ -1593,11 +1610,19 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t *out, int64_t n, cons
 	const int64_t i = (int64_t) blockIdx.x * kScanThreads + threadIdx.x;
 	if (i < n) out[i] += block_offsets[i / kScanBlock];
 }
-__global__ void k_sum64(const uint32_t *in, int64_t n, unsigned long long *total) {
+// 64-bit total of n words: a block's four wavefronts add their sums in LDS, then one atomic per
+// block (atomics of every wavefront on the one address serialize: 70 us for 4096 of them)
+__global__ __launch_bounds__(256) void k_sum64(const uint32_t *in, int64_t n, unsigned long long *total) {
+	__shared__ uint64_t part[4];
 	uint64_t s = 0;
 	for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) s += in[i];
 	s = wave_sum64(s);
-	if (lane_id() == 0 && s) atomicAdd(total, (unsigned long long) s);
+	if (lane_id() == 0) part[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const uint64_t t = part[0] + part[1] + part[2] + part[3];
+		if (t) atomicAdd(total, (unsigned long long) t);
+	}
 }
 
 size_t scan_tmp_words(int64_t n) {
@@ -1630,7 +1655,7 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long 
 	hipError_t e = launch_fill_u32((uint32_t *) total_dev, 2, 0u, st);
 	if (e != hipSuccess) return e;
 	if (n > 0) {
-		k_sum64<<<(unsigned) (n / 4096 + 1 < 1024 ? n / 4096 + 1 : 1024), 256, 0, st>>>(in, n, total_dev);
+		k_sum64<<<(unsigned) (n / 8192 + 1 < 512 ? n / 8192 + 1 : 512), 256, 0, st>>>(in, n, total_dev);
 		e = hipGetLastError();
 		if (e != hipSuccess) return e;
 	}
