@@ -343,12 +343,14 @@ def main():
     achieved = cells_all * AFFINE_OPS_PER_CELL / (dp_ms / 1000.0) / 1e12
     roofline = dict(bound="valu", kernel="k_dp_batched (all band classes)", achieved=round(achieved, 3),
                     peak=round(VALU_PEAK_TOPS, 2), unit="Tops/s (int32 VALU)", frac=round(achieved / VALU_PEAK_TOPS, 4),
-                    traffic=measured_traffic("hero::k_dp_batched<8, true>"), traffic_unit="HBM bytes per launch",
+                    traffic=measured_traffic("hero::k_dp_batched<8, true, true>"), traffic_unit="HBM bytes per launch",
                     ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all), dp_ms_per_step=round(dp_ms, 3))
-    # seed probes: packed subject bases + one 32 B slot per window probe + 8 B per bucket entry read
+    # seed probes: packed subject bases + one 32 B slot per canonical probe (a forward window and
+    # its reverse-strand partner share one; "lookups" counts the reference's per-strand lookups)
+    # + 8 B per bucket entry read
     probe_ms = s1["ms_seed_probe"]
     bases = int(rs.length.astype(np.int64).sum())
-    probe_bytes = bases / 4 + s1["lookups"] * 32 + s1["seeds"] * 8
+    probe_bytes = bases / 4 + s1["lookups"] / 2 * 32 + s1["seeds"] * 8
     probe_gbs = probe_bytes / (probe_ms / 1000.0) / 1e9 if probe_ms > 0 else 0.0
     roofline_probe = dict(bound="hbm", kernel="k_seed_count", achieved=round(probe_gbs, 1), peak=HBM_PEAK_GBS,
                           unit="GB/s", frac=round(probe_gbs / HBM_PEAK_GBS, 4),
