@@ -1552,10 +1552,12 @@ __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint6
 		const uint32_t run = carry + wave_excl_scan(x) + x;
 		carry = __shfl(run, 63, 64);
 		if (p >= L) continue;
-		c[0] += own == 0 ? run : 0u;
-		c[1] += own == 1 ? run : 0u;
-		c[2] += own == 2 ? run : 0u;
-		c[3] += own == 3 ? run : 0u;
+		// records aligning a base to p, less the mismatch votes: the match votes for the own base
+		const uint32_t m = run - (c[0] + c[1] + c[2] + c[3]);
+		c[0] += own == 0 ? m : 0u;
+		c[1] += own == 1 ? m : 0u;
+		c[2] += own == 2 ? m : 0u;
+		c[3] += own == 3 ? m : 0u;
 		if (decoded_out) *(uint4 *) (decoded_out + 4 * (b + p)) = make_uint4(c[0], c[1], c[2], c[3]);
 		if (!good) continue; // not corrected (src/QueryDataset.cpp:404-406); the host copies its bytes
 		c[own] += 1;         // self vote (src/QueryRead.cpp:149)

@@ -316,13 +316,14 @@ HD void replay_script(const uint32_t *sc, uint32_t task, V &v) {
 }
 
 // ---- votes --------------------------------------------------------------------------------
-// Tally of one query read: 4 u32 per base at index 4*p + code. The slot of the read's own base
-// holds a difference array of M-run votes (+1 at a run's first base, -1 after its last), the
-// other three slots hold mismatch votes; consensus_read decodes it.
 // A query's votes: `tally` holds 4 counters per base (votes for each base code cast by
-// mismatches), `diff` one counter per base, the difference array of the matched-run coverage
-// (+1 where a run of matches starts, -1 one past its end), so a run costs two updates and needs
-// no lookup of the query's own bases. Consensus adds its prefix sum to the own base's counter.
+// mismatches), `diff` one counter per base, a difference array whose prefix sum at p counts the
+// records that align a subject base to query base p (by a match or a mismatch). Every query base
+// of a record's span [0, end) is a match, a mismatch or inserted (aligned to nothing), so a record
+// adds +1 at 0 and -1 at end, and an insertion run -1 at its first base and +1 after its last:
+// two updates per record and per insertion, none per mismatch, and no lookup of the query's own
+// bases. The match votes at p (the reference's votes for the own base) are then the prefix sum
+// minus the mismatch votes at p; consensus_read and k_consensus decode it.
 struct VoteSink {
 	uint32_t *tally; // 4 per base
 	uint32_t *diff;  // 1 per base
@@ -342,14 +343,22 @@ struct VoteSink {
 		__atomic_fetch_add(oob, 1ull, __ATOMIC_RELAXED);
 #endif
 	}
-	HD void m_run(int p, int c) const {
+	// a record's span: query bases [0, end)
+	HD void span(int end) const {
+		if (end <= 0) return;
+		if (end > pr.Lq) flag_oob(), end = pr.Lq;
+		add_u32(&diff[0], 1u);
+		if (end < pr.Lq) add_u32(&diff[end], 0xFFFFFFFFu);
+	}
+	// query bases [p, p + c) inserted (aligned to no subject base)
+	HD void hole(int p, int c) const {
 		if (c <= 0) return;
 		if (p < 0 || p + c > pr.Lq) {
 			flag_oob();
 			return;
 		}
-		add_u32(&diff[p], 1u);
-		if (p + c < pr.Lq) add_u32(&diff[p + c], 0xFFFFFFFFu);
+		add_u32(&diff[p], 0xFFFFFFFFu);
+		if (p + c < pr.Lq) add_u32(&diff[p + c], 1u);
 	}
 	HD void mismatch(int pq, int ps) const {
 		if (pq < 0 || pq >= pr.Lq || ps < 0 || ps >= pr.Ls) {
@@ -363,18 +372,12 @@ struct VoteSink {
 struct VoteV {
 	const DpResults &R;
 	const VoteSink &sink;
-	int ps, pq;       // record frames (iPosOnSubjectRead, iPosOnQueryRead)
-	int run_p, run_c; // pending M run
-	HD void flush() {
-		if (run_c) sink.m_run(run_p, run_c);
-		run_c = 0;
-	}
+	int ps, pq; // record frames (iPosOnSubjectRead, iPosOnQueryRead)
 	HD void op(int c, int t) {
 		if (t == kOpM) {
-			if (run_c && run_p + run_c == pq) run_c += c;
-			else flush(), run_p = pq, run_c = c;
 			pq += c, ps += c;
 		} else if (t == kOpI) {
+			sink.hole(pq, c);
 			pq += c;
 		} else if (t == kOpD) {
 			ps += c;
@@ -386,11 +389,11 @@ struct VoteV {
 	HD void start(int pos) { ps = pos, pq = 0; }
 	HD void add(int c, int t) { op(c, t); }
 	HD void run(uint32_t task, bool swapped) { feed_runs(R, task, swapped, [&](int cnt, int t) { op(cnt, t); }); }
-	HD void finish() { flush(); }
+	HD void finish() { sink.span(pq); }
 };
 
 HD void vote_record(const uint32_t *script, const DpResults &R, uint32_t task0, const VoteSink &sink) {
-	VoteV vv{R, sink, 0, 0, 0, 0};
+	VoteV vv{R, sink, 0, 0};
 	replay_script(script, task0, vv);
 }
 
@@ -475,7 +478,7 @@ HD int consensus_read(const Opts &o, const uint32_t *tally, const uint32_t *diff
 		uint32_t c[4];
 		for (int j = 0; j < 4; j++) c[j] = tally[4 * p + j];
 		run += diff[p];
-		c[own] += run;
+		c[own] += run - (c[0] + c[1] + c[2] + c[3]); // aligned bases less mismatches: the matches
 		if (decoded)
 			for (int j = 0; j < 4; j++) decoded[4 * p + j] = c[j];
 		c[own] += 1; // self vote (:149)
