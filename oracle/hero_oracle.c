@@ -188,7 +188,15 @@ static int index_build(const oracle_cfg *c, const oracle_reads *q, index_t *ix) 
 	int64_t k = c->k;
 	memset(ix, 0, sizeof *ix);
 	ix->q = q;
-	ix->P = oracle_table_size((uint64_t) q->n, (uint64_t) c->hash_factor);
+	/* --mapping sizes the table by the k-mers of the bases, not by the reads
+	 * (src/HashTableLongKey.cpp:52-60) */
+	uint64_t units = (uint64_t) q->n;
+	if (c->mapping) {
+		uint64_t total = 0;
+		for (int64_t i = 0; i < q->n; i++) total += (uint64_t) q->length[i];
+		units = total / (uint64_t) k;
+	}
+	ix->P = oracle_table_size(units, (uint64_t) c->hash_factor);
 	ix->slot = (int64_t *) xmalloc(sizeof(int64_t) * ix->P);
 	for (uint64_t s = 0; s < ix->P; s++) ix->slot[s] = -1;
 	int64_t cap_b = 1024;
@@ -872,6 +880,53 @@ static void reclist_add(reclist *l, rec_t *r) {
 	}
 }
 
+/* AlignmentRecord::isSubjectCoveredByQuery (src/AlignmentRecord.cpp:343-370): the query starts at
+ * the subject's first base, neither end op is a deletion, and the non-insertion ops span at least
+ * the subject. */
+static int rec_covers_subject(const rec_t *r, int64_t Ls) {
+	if (r->pos != 0 || r->n == 0) return 0;
+	int64_t len = 0;
+	if (r->typ[0] == 'D') return 0;
+	if (r->typ[0] != 'I') len += r->cnt[0];
+	if (r->n > 1) {
+		for (int i = 1; i < r->n - 1; i++)
+			if (r->typ[i] != 'I') len += r->cnt[i];
+		if (r->typ[r->n - 1] == 'D') return 0;
+		if (r->typ[r->n - 1] != 'I') len += r->cnt[r->n - 1];
+	}
+	return len >= Ls;
+}
+/* AlignmentRecord::getAccuracy (:372-381): substituted bases (ops other than I, D, M). */
+static double rec_substitutions(const rec_t *r) {
+	double s = 0;
+	for (int i = 0; i < r->n; i++)
+		if (r->typ[i] != 'I' && r->typ[i] != 'D' && r->typ[i] != 'M') s += r->cnt[i];
+	return s;
+}
+/* SubjectRead::chooseQueryWithBestAlignment (src/SubjectRead.cpp:463-499), --mapping: no record
+ * covers the subject -> none votes; several -> only the first with the fewest substitutions
+ * votes; exactly one -> every record stays (the reference keeps the list as it is). */
+static void choose_best(reclist *l, int64_t Ls) {
+	if (l->n <= 0) return;
+	int nq = 0, best = -1;
+	double least = (double) Ls;
+	for (int i = 0; i < l->n; i++) {
+		if (!rec_covers_subject(&l->v[i], Ls)) continue;
+		nq++;
+		const double s = rec_substitutions(&l->v[i]);
+		if (s < least) least = s, best = i;
+	}
+	if (nq == 0 || (nq > 1 && best < 0)) {
+		reclist_clear(l);
+		return;
+	}
+	if (nq == 1) return;
+	for (int i = 0; i < l->n; i++)
+		if (i != best) rec_free(&l->v[i]);
+	l->v[0] = l->v[best];
+	l->n = 1;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Chaining (src/SubjectReadAligner.cpp:391-408,711-751,805-852)                               */
 /* ------------------------------------------------------------------------------------------ */
@@ -1186,6 +1241,7 @@ int oracle_run(const oracle_cfg *cfg, const oracle_reads *q, const oracle_reads 
 				process_strand(cfg, q, subj, Ls, orient, &sv, &w, &cw, &rc, &recs, qrcbuf);
 			}
 			kept += (uint64_t) recs.n;
+			if (cfg->call_consensus && cfg->mapping) choose_best(&recs, Ls); /* :557-560 */
 			if (cfg->call_consensus) {
 				for (int r = 0; r < recs.n; r++) vote_record(q, tally, &recs.v[r], (recs.v[r].orient & 1) ? rcbuf : fw);
 			} else if (aln_parts) {
@@ -1245,7 +1301,7 @@ int oracle_run(const oracle_cfg *cfg, const oracle_reads *q, const oracle_reads 
 	/* QueryDataset::callConsensus (src/QueryDataset.cpp:393-412): good reads only; the others are
 	 * written unchanged (isGoodRead() returns true, src/QueryRead.cpp:214-215). */
 	uint64_t changed = 0;
-	if (cfg->call_consensus && corrected) {
+	if (cfg->call_consensus && !cfg->mapping && corrected) { /* --mapping stops before consensus (main.cpp:82-87) */
 		memcpy(corrected, q->seq, (size_t) qbases);
 		uint32_t *tw = tally;
 		uint32_t *scratch = NULL;
@@ -1267,4 +1323,34 @@ int oracle_run(const oracle_cfg *cfg, const oracle_reads *q, const oracle_reads 
 	st->t_consensus = omp_get_wtime() - t2;
 	if (own_tally) free(tally);
 	return 0;
+}
+
+/* QueryDataset::getCoverageAndWriteResults (src/QueryDataset.cpp:560-600), --mapping: a base's
+ * coverage is the sum of its four counters (QueryRead::getCoverageAt, src/QueryRead.cpp:435-442),
+ * over every base of every query read; the text is "COV\tFRE" and one line per coverage value that
+ * some base has, ascending. Returns malloc'd text (free with oracle_free). */
+char *oracle_coverage_text(const oracle_reads *q, const uint32_t *tally) {
+	uint32_t maxc = 0;
+	for (int64_t i = 0; i < q->n; i++)
+		for (int32_t j = 0; j < q->length[i]; j++) {
+			const uint32_t *t = tally + 4 * (q->offset[i] + j);
+			const uint32_t c = t[0] + t[1] + t[2] + t[3];
+			if (c > maxc) maxc = c;
+		}
+	maxc++;
+	uint64_t *h = (uint64_t *) calloc((size_t) maxc, sizeof(uint64_t));
+	for (int64_t i = 0; i < q->n; i++)
+		for (int32_t j = 0; j < q->length[i]; j++) {
+			const uint32_t *t = tally + 4 * (q->offset[i] + j);
+			h[t[0] + t[1] + t[2] + t[3]]++;
+		}
+	strbuf sb = {0};
+	sb_put(&sb, "COV\tFRE\n", 8);
+	for (uint32_t c = 0; c < maxc; c++) {
+		if (!h[c]) continue;
+		sb_printf(&sb, "%lld\t", (long long) c);
+		sb_printf(&sb, "%lld\n", (long long) (uint32_t) h[c]);
+	}
+	free(h);
+	return sb.s;
 }

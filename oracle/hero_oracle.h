@@ -31,6 +31,7 @@ typedef struct {
 	int32_t band_override;  /* build-only: >0 replaces setBand(2*extra) by setBand(band_override) */
 	int32_t max_candidates; /* build-only: >0 keeps only the first N query groups per subject strand */
 	int32_t dx, dy, ex, sx, miss, hit, unobserved; /* BandAligner::setCost */
+	int32_t mapping;        /* Config::bMappingVersion (--mapping) */
 } oracle_cfg;
 
 /* A read set as the datasets hold it after loading: upper-case ASCII, concatenated. */
@@ -100,6 +101,11 @@ int oracle_run(const oracle_cfg *cfg, const oracle_reads *queries, const oracle_
  * or -1 when cap is too small. The index is rebuilt per call (test helper). */
 int64_t oracle_seeds(const oracle_cfg *cfg, const oracle_reads *queries, const char *subject, int32_t len,
 		int64_t subject_unique_id, int reverse, int64_t *out, int64_t cap);
+
+/* --mapping output (QueryDataset::getCoverageAndWriteResults, src/QueryDataset.cpp:560-600) from
+ * the vote tally oracle_run returns: "COV\tFRE\n" then "<coverage>\t<bases>\n" for each coverage
+ * some base has. malloc'd; free with oracle_free. */
+char *oracle_coverage_text(const oracle_reads *queries, const uint32_t *tally);
 
 void oracle_free(void *p);
 

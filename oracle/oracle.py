@@ -19,6 +19,7 @@ class Cfg(ctypes.Structure):
         ("indel_rate", ctypes.c_double), ("consensus_freq", ctypes.c_double), ("band_override", ctypes.c_int32),
         ("max_candidates", ctypes.c_int32), ("dx", ctypes.c_int32), ("dy", ctypes.c_int32), ("ex", ctypes.c_int32),
         ("sx", ctypes.c_int32), ("miss", ctypes.c_int32), ("hit", ctypes.c_int32), ("unobserved", ctypes.c_int32),
+        ("mapping", ctypes.c_int32),
     ]
 
 
@@ -139,6 +140,19 @@ def seeds(cfg, queries, subject, subject_uid, reverse):
     if n < 0:
         raise ValueError(f"oracle_seeds failed: {n}")
     return out[:3 * n].reshape(n, 3)
+
+
+def coverage_text(queries, tally):
+    """--mapping output text (COV / FRE histogram) from a vote tally of run(..., want_tally=True)."""
+    f = lib().oracle_coverage_text
+    f.restype = ctypes.c_void_p
+    f.argtypes = [ctypes.POINTER(Reads), ctypes.c_void_p]
+    t = np.ascontiguousarray(tally, dtype=np.uint32)
+    r = _reads(queries)
+    p = f(ctypes.byref(r), t.ctypes.data)
+    s = ctypes.string_at(p).decode()
+    lib().oracle_free(ctypes.c_void_p(p))
+    return s
 
 
 def run(cfg, queries, subjects=None, threads=0, want_tally=False, want_aln=False, want_records=False):

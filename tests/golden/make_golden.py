@@ -12,6 +12,8 @@ Run `make -C oracle ref` first. The fixtures are data only (inputs and expected 
   consensus_cases.jsonl QueryRead::callConsensus on given tallies
   e2e.json             reference program runs: synthetic read-set parameters, flags, and sha256 /
                        per-read digests of the corrected FASTA and the .aln output
+  mapping.json         reference program runs with -c --mapping: the coverage histogram text
+                       (`make_golden.py mapping` writes this file alone)
 """
 import hashlib
 import json
@@ -194,7 +196,45 @@ def e2e_cases():
     return out
 
 
+MAPPING = [
+    dict(name="map_affine_k31", synth=dict(seed=11, genome_len=8000, len_min=100, len_max=150, coverage=20.0,
+                                           sub_5p=0.01),
+         flags=["-k", "31", "-m", "4", "-i", "2", "-l", "40", "-a", "a"]),
+    dict(name="map_global_k25", synth=dict(seed=612, genome_len=12000, len_min=75, len_max=220, coverage=25.0,
+                                           sub_5p=0.01, indel_rate=0.003),
+         flags=["-k", "25", "-m", "5", "-i", "3", "-l", "40", "-a", "g"]),
+    dict(name="map_repeats_k39", synth=dict(seed=713, genome_len=15100, len_min=150, coverage=30.0, sub_5p=0.02,
+                                            indel_rate=0.002, repeat_frac=0.2, repeat_seg_len=1500,
+                                            repeat_copies_per_family=4, repeat_div=0.01),
+         flags=["-k", "39", "-m", "4", "-i", "3", "-l", "40", "-a", "a"]),
+]
+
+
+def mapping_cases():
+    """--mapping runs of the reference (-c): the coverage histogram it writes to the -o path."""
+    from hero_amd import synth
+    out = []
+    for case in MAPPING:
+        rs = synth.generate(**case["synth"])
+        assert rs.n % 400 != 0, "the reference's subject streamer aborts on multiples of 400 reads"
+        d = tempfile.mkdtemp(prefix="golden_map_", dir="/tmp")
+        fa = os.path.join(d, "reads.fasta")
+        rs.write_fasta(fa)
+        prefix = os.path.join(d, "out_map")
+        cmd = [HERO_REF, "-q", fa, "-s", fa, "-o", prefix, "-t", "8"] + case["flags"] + ["-c", "--singleReads",
+                                                                                        "--mapping"]
+        subprocess.run(cmd, capture_output=True, check=True)
+        text = open(prefix).read()
+        out.append(dict(name=case["name"], synth=case["synth"], flags=case["flags"], n_reads=rs.n, coverage_text=text))
+        print(case["name"], rs.n, len(text), flush=True)
+    return out
+
+
 def main():
+    if sys.argv[1:] == ["mapping"]:
+        with open(os.path.join(HERE, "mapping.json"), "w") as f:
+            json.dump(mapping_cases(), f, separators=(",", ":"))
+        return
     rng = random.Random(20261203)
     with open(os.path.join(HERE, "dp_cases.jsonl"), "w") as f:
         for c in dp_cases(rng):

@@ -18,6 +18,12 @@ def load_e2e():
         return json.load(f)
 
 
+def load_mapping():
+    """Reference --mapping runs (-c): synthetic read-set parameters, flags, coverage histogram text."""
+    with open(os.path.join(GOLDEN, "mapping.json")) as f:
+        return json.load(f)
+
+
 def flags_to_opts(flags):
     """Reference CLI flags (-k -m -i -l -a) -> option dict (rates as fractions, like Config.cpp:165-170)."""
     o = {}

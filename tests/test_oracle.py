@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from helpers import fasta_text, flags_to_opts, load_e2e, load_jsonl, read_digests, sha256
+from helpers import fasta_text, flags_to_opts, load_e2e, load_jsonl, load_mapping, read_digests, sha256
 from hero_amd import synth
 from oracle import oracle as O
 
@@ -63,6 +63,18 @@ def test_end_to_end_matches_reference(case):
     assert sha256(fasta_text(r["corrected"], rs)) == case["fasta_sha256"]
     a = O.run(O.make_cfg(call_consensus=0, **opts), rs, threads=4, want_aln=True)
     assert sha256(a["aln"].encode()) == case["aln_sha256"]
+
+
+@pytest.mark.parametrize("case", load_mapping(), ids=lambda c: c["name"])
+def test_mapping_histogram_matches_reference(case):
+    """--mapping: per subject the best record covering it (chooseQueryWithBestAlignment, including
+    the keep-all case of exactly one covering record), then the coverage histogram; each of the
+    selection rules changes these outputs when altered (checked by planting them in the oracle)."""
+    rs = synth.generate(**case["synth"])
+    assert rs.n == case["n_reads"]
+    opts = flags_to_opts(case["flags"])
+    r = O.run(O.make_cfg(call_consensus=1, mapping=1, **opts), rs, threads=4, want_tally=True)
+    assert O.coverage_text(rs, r["tally"]) == case["coverage_text"]
 
 
 def test_seed_search_brute_force():
