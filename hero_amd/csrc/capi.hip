@@ -322,6 +322,7 @@ Opts make_opts(const hero_options &p) {
 	o.min_overlap = p.min_overlap;
 	o.mismatch_rate = p.mismatch_rate, o.indel_rate = p.indel_rate, o.consensus_freq = p.consensus_freq;
 	o.dx = p.dx, o.dy = p.dy, o.ex = p.ex, o.sx = p.sx, o.miss = p.miss, o.hit = p.hit, o.unobserved = p.unobserved;
+	o.mapping = p.mapping;
 	return o;
 }
 
@@ -725,6 +726,7 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	if (int r = run_dp(c, w, S, Q, cd, (uint32_t) ntasks, rows_needed)) return r;
 	if (c->tune) return HERO_OK; // autotuning times the DP only: no records, no votes
 	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, stats, w->st));
+	if (vote_mode && c->o.mapping) HIPCHK(c, launch_map_select(S, cd, w->st)); // SubjectReadAligner.cpp:558-560
 	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, diff, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
@@ -1277,7 +1279,9 @@ int hero_build_index(hero_ctx *c) {
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	// a new index starts a new correction run: clear the vote tallies
 	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * c->q_bases, c->w.st));
-	const uint64_t P = next_prime_above((uint64_t) c->Qd.n * (uint64_t) c->opt.hash_factor);
+	// --mapping sizes the table by the bases' k-mers (src/HashTableLongKey.cpp:52-60)
+	const uint64_t units = c->opt.mapping ? (uint64_t) c->q_bases / (uint64_t) c->opt.k : (uint64_t) c->Qd.n;
+	const uint64_t P = next_prime_above(units * (uint64_t) c->opt.hash_factor);
 	c->cons_valid = false;
 	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
 	c->st_host.table_size = c->qix.P;
@@ -1382,6 +1386,27 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 		std::memcpy(tally + 4 * c->q_off[(size_t) i], h.data() + 4 * b, (size_t) L * 16);
 		b += (uint64_t) L;
 	}
+	return HERO_OK;
+}
+
+extern "C" int hero_coverage_histogram(hero_ctx *c, uint64_t *counts, int32_t cap, int32_t *max_coverage) {
+	if (!c || !counts || cap < 1 || !max_coverage) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	DevBuf<unsigned long long> hist;
+	DevBuf<uint32_t> mx;
+	HIPCHK(c, hist.exact((size_t) cap));
+	HIPCHK(c, mx.exact(1));
+	HIPCHK(c, hipMemsetAsync(hist.p, 0, (size_t) cap * 8, c->w.st));
+	HIPCHK(c, hipMemsetAsync(mx.p, 0, 4, c->w.st));
+	HIPCHK(c, launch_coverage_hist(c->Qd.len.p, c->Qd.boff.p, c->tally.p + 4 * c->q_bases, c->Qd.n, (uint32_t) cap,
+			hist.p, mx.p, c->w.st));
+	uint32_t m = 0;
+	HIPCHK(c, hipMemcpyAsync(counts, hist.p, (size_t) cap * 8, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipMemcpyAsync(&m, mx.p, 4, hipMemcpyDeviceToHost, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
+	*max_coverage = (int32_t) (m < 0x7FFFFFFFu ? m : 0x7FFFFFFFu);
+	if ((int64_t) m >= (int64_t) cap)
+		return fail(c, HERO_ERR_LIMIT, "coverage %u needs a histogram of %u bins (cap %d)", m, m + 1, cap);
 	return HERO_OK;
 }
 

@@ -21,7 +21,7 @@ struct Opts {
 	int32_t call_consensus; // 1: vote and correct (-c); 0: alignment records only
 	int32_t band_override;  // >0 replaces setBand(2*extra)
 	int32_t max_candidates; // >0 keeps the first N query groups per subject strand
-	int32_t _pad;
+	int32_t mapping;        // --mapping: per subject only the best record covering it votes
 	int64_t min_overlap;
 	double mismatch_rate, indel_rate, consensus_freq;
 	int32_t dx, dy, ex, sx, miss, hit, unobserved, _pad2;

@@ -101,6 +101,9 @@ struct VoteItem {
 	uint32_t task0;  // first DP task of the record
 	int32_t sub;     // subject index within the chunk
 	int32_t strand;
+	// 1: the query's entry in the subject's record list (SubjectRead::vAlignment) came from a forward
+	// record, so it precedes every entry first added by a reverse record (src/SubjectRead.cpp:25-79)
+	int32_t first_fwd;
 };
 
 // Per-chunk device buffers. Seed-parallel arrays hold, at a group's first seed, that group's
@@ -190,6 +193,12 @@ hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const 
 		hipStream_t st);
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
 		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st);
+// --mapping: per subject of the chunk, keep the votes of the best record covering it (see the kernel)
+hipError_t launch_map_select(const Packed &S, const ChunkDev &c, hipStream_t st);
+// --mapping output: hist[v] += bases of the nq reads with coverage v (bins 0 .. cap-1, the last one
+// taking every larger value), *max_cov = max(*max_cov, largest coverage)
+hipError_t launch_coverage_hist(const int32_t *len, const uint64_t *q_boff, const uint32_t *diff, int64_t nq, uint32_t cap,
+		unsigned long long *hist, uint32_t *max_cov, hipStream_t st);
 // Record mode: op count (ops == nullptr) or ops of every accepted record slot; each op is
 // (count << 8) | type, type 'M' / 'I' / 'D' or the query base letter of a mismatch run.
 hipError_t launch_record_ops(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nslots,

@@ -18,6 +18,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
 	return v;
 }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		const uint32_t o = __shfl_xor(v, off, 64);
+		v = o > v ? o : v;
+	}
+	return v;
+}
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
 	const int lane = lane_id();
 	uint32_t x = v;
@@ -540,9 +548,11 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 		uint32_t vbase, uint32_t *vcount, PassBCounters &k) {
 	Best best;
 	best.have = false, best.rate_sum = 0, best.len = 0, best.strand = 0, best.script = 0, best.task0 = 0;
+	bool fwd_seen = false; // a forward record was accepted: the query's entry sits among the forward ones
 #pragma unroll 1
 	for (int strand = 0; strand < 2; strand++) {
 		const int gn = strand == 0 ? gn0 : gn1;
+		if (strand == 1) fwd_seen = best.have;
 		if (gn <= 0) continue;
 		const uint32_t base = strand == 0 ? base0 : base1;
 		const uint32_t gp = c.g_prog_off[base], task0 = c.g_task_off[base];
@@ -566,7 +576,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 	// the winner's votes are cast by k_votes
 	VoteItem v;
 	v.q = (uint32_t) q, v.script = best.script, v.task0 = best.task0;
-	v.sub = (int32_t) (s - c.s_begin), v.strand = best.strand;
+	v.sub = (int32_t) (s - c.s_begin), v.strand = best.strand, v.first_fwd = fwd_seen ? 1 : 0;
 	c.votes[vbase + atomicAdd(vcount, 1u)] = v;
 }
 
@@ -589,6 +599,117 @@ hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const C
 		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st) {
 	if (nvotes == 0) return hipSuccess;
 	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, diff, stats);
+	return hipGetLastError();
+}
+
+// --mapping (SubjectRead::chooseQueryWithBestAlignment, src/SubjectRead.cpp:463-499), one wavefront
+// per subject over its kept records, before the votes. A record covers the subject when it starts
+// at the subject's first base, neither end op of its CIGAR is a deletion and its non-insertion ops
+// span the subject (AlignmentRecord::isSubjectCoveredByQuery, src/AlignmentRecord.cpp:343-370).
+// None covers: no record votes. Several: only the first one, in record-list order, with the fewest
+// substituted bases (getAccuracy, :372-381) votes. Exactly one: every record keeps its votes (the
+// reference leaves the list as it is then).
+// The CIGAR facts come from the op stream the record writer merges (RecordV): first and last op
+// type (a zero-length op counts, except a mismatch run of no bases, which adds no op), the
+// non-insertion length and the mismatched bases.
+struct MapStatsV {
+	const DpResults &R;
+	int first, last, pos;
+	int64_t len, subs;
+	__device__ void note(int c, int t) {
+		if (t == kOpX && c == 0) return;
+		if (first < 0) first = t;
+		last = t;
+		if (t != kOpI) len += c;
+		if (t == kOpX) subs += c;
+	}
+	__device__ void start(int p) { pos = p; }
+	__device__ void add(int c, int t) { note(c, t); }
+	__device__ void run(uint32_t task, bool swapped) {
+		feed_runs(R, task, swapped, [&](int cnt, int t) { note(cnt, t); });
+	}
+	__device__ void finish() {}
+};
+__global__ __launch_bounds__(64) void k_map_select(Packed S, ChunkDev c) {
+	const int sl = blockIdx.x;
+	const int lane = lane_id();
+	const uint32_t b0 = c.sub_pair_off[sl], n = c.sub_pairs[sl];
+	if (n == 0) return;
+	const int Ls = S.len[c.s_begin + sl];
+	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
+	// order of a covering record: fewest substitutions, then the list order (entries first added by
+	// a forward record, by innerId, then those first added by a reverse record, by innerId)
+	uint32_t nq = 0;
+	uint64_t best = ~0ull;
+	for (uint32_t i = lane; i < n; i += 64) {
+		const VoteItem v = c.votes[b0 + i];
+		if (v.q == kNoRecord) continue;
+		MapStatsV m{R, -1, -1, -1, 0, 0};
+		replay_script(c.prog + v.script, v.task0, m);
+		if (!(m.pos == 0 && m.first >= 0 && m.first != kOpD && m.last != kOpD && m.len >= Ls)) continue;
+		nq++;
+		const uint64_t key = ((uint64_t) (m.subs < 0x7FFFFFFF ? m.subs : 0x7FFFFFFF) << 33) |
+				((uint64_t) (v.first_fwd ? 0u : 1u) << 32) | v.q;
+		best = key < best ? key : best;
+	}
+	nq = wave_sum(nq);
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		const uint64_t o2 = __shfl_xor(best, off, 64);
+		best = o2 < best ? o2 : best;
+	}
+	if (nq == 1) return;
+	for (uint32_t i = lane; i < n; i += 64) {
+		const uint32_t q = c.votes[b0 + i].q;
+		if (q != kNoRecord && (nq == 0 || q != (uint32_t) best)) c.votes[b0 + i].q = kNoRecord;
+	}
+}
+hipError_t launch_map_select(const Packed &S, const ChunkDev &c, hipStream_t st) {
+	if (c.nsub == 0) return hipSuccess;
+	k_map_select<<<c.nsub, 64, 0, st>>>(S, c);
+	return hipGetLastError();
+}
+
+// --mapping output (QueryDataset::getCoverageAndWriteResults, src/QueryDataset.cpp:560-600): bases
+// by coverage, the sum of a base's four counters (QueryRead::getCoverageAt, src/QueryRead.cpp:435-442).
+// With the vote layout of VoteSink that sum is the prefix sum of the difference array (the records
+// aligning a subject base to it: its matches plus its mismatches). A wavefront per read; counts below
+// kCovLds go to a block histogram in LDS, the rest straight to the device histogram (cap bins, the
+// last taking every larger coverage); *max_cov receives the largest coverage.
+constexpr int kCovLds = 2048;
+__global__ __launch_bounds__(256) void k_coverage_hist(const int32_t *len, const uint64_t *q_boff, const uint32_t *diff,
+		int64_t nq, uint32_t cap, unsigned long long *hist, uint32_t *max_cov) {
+	__shared__ uint32_t h[kCovLds];
+	for (int i = threadIdx.x; i < kCovLds; i += blockDim.x) h[i] = 0;
+	__syncthreads();
+	const int lane = lane_id();
+	uint32_t mx = 0;
+	for (int64_t r = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6); r < nq; r += (int64_t) gridDim.x * 4) {
+		const int L = len[r];
+		const uint64_t b = q_boff[r];
+		uint32_t carry = 0;
+		for (int p0 = 0; p0 < L; p0 += 64) {
+			const int p = p0 + lane;
+			const uint32_t x = p < L ? diff[b + p] : 0u;
+			const uint32_t run = carry + wave_excl_scan(x) + x;
+			carry = __shfl(run, 63, 64);
+			if (p >= L) continue;
+			mx = run > mx ? run : mx;
+			if (run < (uint32_t) kCovLds) atomicAdd(&h[run], 1u);
+			else atomicAdd(&hist[run < cap ? run : cap - 1], 1ull);
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < kCovLds; i += blockDim.x)
+		if (h[i]) atomicAdd(&hist[(uint32_t) i < cap ? (uint32_t) i : cap - 1], (unsigned long long) h[i]);
+	mx = wave_max_u32(mx);
+	if (lane == 0) atomicMax(max_cov, mx);
+}
+hipError_t launch_coverage_hist(const int32_t *len, const uint64_t *q_boff, const uint32_t *diff, int64_t nq, uint32_t cap,
+		unsigned long long *hist, uint32_t *max_cov, hipStream_t st) {
+	if (nq == 0) return hipSuccess;
+	const int64_t blocks = (nq + 3) / 4 < 2048 ? (nq + 3) / 4 : 2048;
+	k_coverage_hist<<<(unsigned) blocks, 256, 0, st>>>(len, q_boff, diff, nq, cap, hist, max_cov);
 	return hipGetLastError();
 }
 

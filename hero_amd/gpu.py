@@ -26,7 +26,7 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
         ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("mapping", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -103,6 +103,7 @@ SIGNATURES = {
     "hero_copy_corrected": (ctypes.c_int, [_P, _P]),
     "hero_take_alignments": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "hero_get_tally": (ctypes.c_int, [_P, _P]),
+    "hero_coverage_histogram": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "hero_reset_stats": (ctypes.c_int, [_P]),
     "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
@@ -244,6 +245,18 @@ class Context:
         t = np.zeros(4 * len(self._queries.seq), dtype=np.uint32)
         self._check(lib().hero_get_tally(self.h, t.ctypes.data))
         return t
+
+    def coverage_histogram(self, cap=1 << 16):
+        """--mapping output: bases by coverage (counts[v], v = 0 .. max) via hero_coverage_histogram."""
+        counts = np.zeros(cap, dtype=np.uint64)
+        m = ctypes.c_int32(0)
+        self._check(lib().hero_coverage_histogram(self.h, counts.ctypes.data, cap, ctypes.byref(m)))
+        return counts[:m.value + 1]
+
+    def coverage_text(self, cap=1 << 16):
+        """The text the reference's --mapping run writes (QueryDataset::getCoverageAndWriteResults)."""
+        h = self.coverage_histogram(cap)
+        return "COV\tFRE\n" + "".join(f"{v}\t{int(n)}\n" for v, n in enumerate(h) if n)
 
     def stats(self):
         s = Stats()

@@ -56,7 +56,11 @@ typedef struct {
 	                           record mode and autotuning run one); 0 = 2 */
 	int32_t dp_untagged;    /* 1: one-task-per-lane affine rows in the general form; 0: the tagged form
 	                           (pointer in the scores' low bits) where dx == dy and the band is <= 12 */
-	int32_t reserved[2];
+	int32_t mapping;        /* Config::bMappingVersion (--mapping): with call_consensus, per subject only
+	                           the best record covering it votes (SubjectRead::chooseQueryWithBestAlignment);
+	                           the index is sized by bases / k; read the result with
+	                           hero_coverage_histogram instead of calling consensus */
+	int32_t reserved[1];
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
@@ -116,6 +120,13 @@ int hero_copy_corrected(hero_ctx *ctx, char *corrected);
 /* The vote matrices (QueryRead::aCoverageMatrix, 4 u32 per base in A,C,T,G order) as they stand
  * after the subject passes, before consensus adds the self vote. */
 int hero_get_tally(hero_ctx *ctx, uint32_t *tally);
+
+/* --mapping output (QueryDataset::getCoverageAndWriteResults, src/QueryDataset.cpp:560-600): counts[v]
+ * = bases of the query reads whose coverage (QueryRead::getCoverageAt: the sum of the base's four
+ * vote counters) is v, for v < cap; *max_coverage = the largest coverage. The reference writes
+ * "COV\tFRE" and "v\tcounts[v]" for every v <= max_coverage with counts[v] > 0. Returns
+ * HERO_ERR_LIMIT when max_coverage >= cap (counts[cap-1] then also holds every larger value). */
+int hero_coverage_histogram(hero_ctx *ctx, uint64_t *counts, int32_t cap, int32_t *max_coverage);
 
 int hero_get_stats(hero_ctx *ctx, hero_stats *out);
 int hero_reset_stats(hero_ctx *ctx);

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from helpers import fasta_text, flags_to_opts, load_e2e, read_digests, sha256
+from helpers import fasta_text, flags_to_opts, load_e2e, load_mapping, read_digests, sha256
 from hero_amd import gpu, synth
 from oracle import oracle as O
 
@@ -199,6 +199,24 @@ def test_autotuned_schedule_gives_identical_results():
             ctx.build_index()
             ctx.align_self()
             assert np.array_equal(ctx.tally(), t0) and np.array_equal(ctx.consensus(), c0)
+
+
+@pytest.mark.parametrize("case", load_mapping(), ids=lambda c: c["name"])
+def test_mapping_histogram_matches_reference(case):
+    """--mapping: the coverage histogram equals the reference's output file, and the vote tally the
+    oracle's, in one subject chunk and in small chunks on two workers."""
+    rs = synth.generate(**case["synth"])
+    opts = flags_to_opts(case["flags"])
+    ro = O.run(O.make_cfg(call_consensus=1, mapping=1, **opts), rs, threads=8, want_tally=True)
+    for chunk in (0, 300):
+        with gpu.Context(call_consensus=1, mapping=1, chunk_subjects=chunk, **opts) as ctx:
+            ctx.set_queries(rs)
+            ctx.build_index()
+            ctx.align_self()
+            assert np.array_equal(ctx.tally(), ro["tally"]), chunk
+            assert ctx.coverage_text() == case["coverage_text"], chunk
+            st = ctx.stats()
+        assert st["vote_oob"] == 0 and st["dp_skipped"] == 0
 
 
 @pytest.mark.parametrize("costs", [dict(), dict(dx=-6, dy=-6, ex=-2, miss=-3, hit=2)], ids=["default", "hit2"])
