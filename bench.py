@@ -14,6 +14,8 @@ Prints one JSON line (see the keys below). Extra measurements:
                    workload and on a mixed-length workload (75-250 bp, band 5% of length)
   roofline         DP kernel: integer VALU ops (18 per affine cell) per second vs the VALU peak
   roofline_probe   seed-probe kernel: algorithmic HBM bytes per second vs 8 TB/s
+  mapping          the same reads with -c --mapping (best covering record per subject, coverage
+                   histogram): reads/s of index + subjects + histogram
   cpu_baseline     the reference program (oracle/_ref/hero_ref) or the CPU oracle on host cores
 """
 import argparse
@@ -66,6 +68,26 @@ def run_steps(ctx, steps):
         ctx.consensus(device_only=True)
     t1 = time.perf_counter()
     return (t1 - t0) * 1000.0 / steps, ctx.stats()
+
+
+def mapping_run(rs, steps=2):
+    """--mapping on the same reads: index build, every subject (only the best record covering each
+    subject votes), coverage histogram; after one untimed step."""
+    with gpu.Context(mapping=1, **C2_OPTS) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        ctx.coverage_histogram()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.build_index()
+            ctx.align_self()
+            h = ctx.coverage_histogram()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+    return dict(workload="c2 reads with -c --mapping: per subject the best record covering it votes; output the "
+                         "coverage histogram (QueryDataset::getCoverageAndWriteResults)",
+                reads_per_s=round(rs.n / (ms / 1000.0), 1), ms_per_step=round(ms, 3), steps=steps,
+                max_coverage=int(len(h) - 1), bases_covered=int(h[1:].sum()))
 
 
 def gcups_of(st):
@@ -372,6 +394,11 @@ def main():
                                      autotune=tune_c3)
         log("gcups:", json.dumps(gcups))
 
+    mapping = None
+    if not args.no_compare:
+        mapping = mapping_run(rs)
+        log("mapping:", json.dumps(mapping))
+
     latency = None
     if not args.no_serve:
         prepared = prepare_serving()
@@ -395,7 +422,8 @@ def main():
                                       "+1/-4/-6/-1, band 8, cap 64 candidates, self-correction"),
                             reads=rs.n, bases=bases, parallelism="1 GPU"),
                 pcie_inclusive=dict(copy_ms=round(copy_ms, 3), reads_per_s=round(rs.n / ((ms + copy_ms) / 1000.0), 1)),
-                gcups=gcups, autotune=tune_c2, latency=latency, roofline=roofline, roofline_probe=roofline_probe, cpu_baseline=cpu,
+                gcups=gcups, autotune=tune_c2, latency=latency, mapping=mapping, roofline=roofline,
+                roofline_probe=roofline_probe, cpu_baseline=cpu,
                 cpu_baseline_port=port,
                 stages_ms=dict(index=round(s1["ms_index"], 3), seed=round(s1["ms_seed"], 3),
                                plan=round(s1["ms_plan"], 3), dp=round(dp_ms, 3),

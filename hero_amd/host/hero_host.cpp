@@ -93,6 +93,27 @@ void QueryDataset::writeQueryOut(const std::string &filename) {
 	f << queryOutText();
 }
 
+std::string QueryDataset::coverageText() const {
+	// the histogram grows until it holds the largest coverage (hero_coverage_histogram reports it)
+	std::vector<uint64_t> counts(1 << 12);
+	int32_t maxc = 0;
+	for (;;) {
+		const int rc = hero_coverage_histogram(dev_.ctx(), counts.data(), (int32_t) counts.size(), &maxc);
+		if (rc == HERO_OK) break;
+		if (rc != HERO_ERR_LIMIT || maxc >= (1 << 30)) dev_.check(rc, "hero_coverage_histogram");
+		counts.assign((size_t) maxc + 1, 0);
+	}
+	std::string out = "COV\tFRE\n";
+	for (int32_t v = 0; v <= maxc; v++)
+		if (counts[(size_t) v]) out += std::to_string(v) + "\t" + std::to_string((uint32_t) counts[(size_t) v]) + "\n";
+	return out;
+}
+
+void QueryDataset::getCoverageAndWriteResults(const std::string &filename) {
+	std::ofstream f(filename, std::ios_base::app | std::ios_base::binary);
+	f << coverageText();
+}
+
 HashTableLongKey::HashTableLongKey(QueryDataset *q) : dataSet(q), hashKeyLength(q->device().options().k), dev_(q->device()) {}
 
 bool HashTableLongKey::insertQueryDataset(QueryDataset *q) {

@@ -32,6 +32,7 @@ public:
 	void setMinimumOverlap(int64_t l) { opt.min_overlap = l; }
 	void setAffine(bool affine) { opt.aligner = affine ? 0 : 1; }
 	void setCallConsensus(bool c) { opt.call_consensus = c ? 1 : 0; }
+	void setMapping(bool m) { opt.mapping = m ? 1 : 0; } // --mapping (Config::bMappingVersion)
 };
 
 // The device context; every dataset / aligner object borrows one.
@@ -77,6 +78,10 @@ public:
 	// Corrected FASTA as QueryDataset::writeQueryOut writes it without --merge (QueryDataset.cpp:414-487).
 	void writeQueryOut(const std::string &filename);
 	std::string queryOutText() const;
+	// --mapping output (QueryDataset::getCoverageAndWriteResults, QueryDataset.cpp:560-600): the
+	// bases by coverage, appended to the file as "COV\tFRE" and "coverage\tbases" lines.
+	void getCoverageAndWriteResults(const std::string &filename);
+	std::string coverageText() const;
 	Device &device() const { return dev_; }
 	const ReadBatch &reads() const { return reads_; }
 

@@ -14,7 +14,9 @@
 
 using namespace hero_host;
 
-static int run_path(bool correct, int argc, char **argv) {
+// correct: -c, corrected FASTA; aln: the .aln text; mapping: -c --mapping, the coverage histogram
+// file (main.cpp:82-87 writes it instead of calling consensus)
+static int run_path(bool correct, bool mapping, int argc, char **argv) {
 	if (argc < 9) return 2;
 	ReadBatch reads;
 	if (!ReadBatch::loadFasta(argv[2], reads)) return 3;
@@ -25,6 +27,7 @@ static int run_path(bool correct, int argc, char **argv) {
 	cfg.setMinimumOverlap(std::atoll(argv[6]));
 	cfg.setAffine(argv[7][0] == 'a');
 	cfg.setCallConsensus(correct);
+	cfg.setMapping(mapping);
 	const int64_t chunk = argc > 9 ? std::atoll(argv[9]) : 0;
 	Device dev(cfg);
 	QueryDataset q(dev, reads);
@@ -37,6 +40,11 @@ static int run_path(bool correct, int argc, char **argv) {
 		aligner.start(subjects);
 	} else {
 		aligner.startSelf();
+	}
+	if (mapping) {
+		std::ofstream(argv[8], std::ios_base::binary | std::ios_base::trunc).close();
+		q.getCoverageAndWriteResults(argv[8]);
+		return 0;
 	}
 	std::ofstream out(argv[8], std::ios_base::binary | std::ios_base::trunc);
 	if (correct) {
@@ -81,8 +89,9 @@ int main(int argc, char **argv) {
 	if (argc < 2) return 2;
 	const std::string mode = argv[1];
 	try {
-		if (mode == "correct") return run_path(true, argc, argv);
-		if (mode == "aln") return run_path(false, argc, argv);
+		if (mode == "correct") return run_path(true, false, argc, argv);
+		if (mode == "aln") return run_path(false, false, argc, argv);
+		if (mode == "mapping") return run_path(true, true, argc, argv);
 		if (mode == "band") return run_band(argc, argv);
 	} catch (const std::exception &e) {
 		std::fprintf(stderr, "%s\n", e.what());

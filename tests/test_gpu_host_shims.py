@@ -6,7 +6,7 @@ import subprocess
 
 import pytest
 
-from helpers import load_e2e, load_jsonl, sha256
+from helpers import load_e2e, load_jsonl, load_mapping, sha256
 from hero_amd import synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -35,6 +35,12 @@ def _run(mode, case, tmp_path, chunk=0):
 @pytest.mark.parametrize("case", load_e2e()[:3], ids=lambda c: c["name"])
 def test_corrected_fasta_through_host_classes(case, tmp_path):
     assert sha256(_run("correct", case, tmp_path)) == case["fasta_sha256"]
+
+
+@pytest.mark.parametrize("case", load_mapping(), ids=lambda c: c["name"])
+def test_mapping_histogram_through_host_classes(case, tmp_path):
+    """-c --mapping through the host classes: the histogram file equals the reference's."""
+    assert _run("mapping", case, tmp_path).decode() == case["coverage_text"]
 
 
 def test_alignment_records_through_host_classes_in_subject_chunks(tmp_path):
