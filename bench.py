@@ -327,7 +327,8 @@ def main():
                     help="also run a 1 GiB device copy (the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, 'Sharding').",
+        print("bench.py measures one GPU only: the library has no multi-GPU path; a subject set "
+              "shards across GPUs by subject chunks with a vote-tally reduction, which is not built.",
               file=sys.stderr)
         sys.exit(2)
 

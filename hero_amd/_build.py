@@ -15,8 +15,9 @@ LIB = os.path.join(ROOT, "hero_amd", "_lib")
 OBJ = os.path.join(ROOT, "hero_amd", "_lib", "obj")
 
 ARCH = os.environ.get("HERO_OFFLOAD_ARCH", "gfx950")
-# Packed-FP32 instructions stay off ("Build"); doubles keep IEEE semantics and no
-# contraction, which the bit-exact chaining / rate / consensus tests rely on.
+# Packed-FP32 instructions stay off (some of their forms have given wrong results on gfx950 with
+# no error); doubles keep IEEE semantics and no contraction, which the bit-exact chaining / rate /
+# consensus tests rely on.
 HIP_FLAGS = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
@@ -83,7 +84,7 @@ def build_oracle():
 
 
 def build_reference():
-    """Reference binary + probe, only where the reference sources exist (not on the GPU box)."""
+    """Reference binary + probe, only where the reference sources exist (the build host)."""
     if os.path.isdir("/root/reference/src"):
         _run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle"), "ref"])
 
