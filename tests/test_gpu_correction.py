@@ -123,6 +123,46 @@ def test_empty_and_tiny_sets():
         assert ctx.consensus().tobytes() == b"ACGT" * 20
 
 
+def _deep_stack(seed=9, n=2400, dups=100):
+    """n 100-bp reads over a 300-bp template, half reverse-complemented, 1% substitutions, plus
+    dups exact copies of one read: about 1,450 seeds per subject strand on average."""
+    rng = np.random.default_rng(seed)
+    tpl = "".join(rng.choice(list("ACGT"), 300))
+    comp = str.maketrans("ACGT", "TGCA")
+    reads = []
+    for _ in range(n):
+        p = int(rng.integers(0, 201))
+        r = list(tpl[p:p + 100])
+        for i in np.nonzero(rng.random(100) < 0.01)[0]:
+            r[i] = "ACGT"[("ACGT".index(r[i]) + int(rng.integers(1, 4))) % 4]
+        r = "".join(r)
+        reads.append(r.translate(comp)[::-1] if rng.random() < 0.5 else r)
+    reads += [reads[0]] * dups
+    return synth.ReadSet.from_strings(reads)
+
+
+@pytest.mark.parametrize("aligner,mapping", [(1, 0), (0, 0), (1, 1)], ids=["affine", "global", "mapping"])
+def test_deep_stack_matches_oracle(aligner, mapping):
+    """Deep coverage: strands with more seeds than a wavefront sorts in LDS (the HBM sort path),
+    buckets of hundreds of entries and exact duplicate reads; tally and output equal the oracle's."""
+    rs = _deep_stack()
+    opts = dict(call_consensus=1, k=21, aligner=aligner, mismatch_rate=0.04, indel_rate=0.02, min_overlap=40,
+                mapping=mapping)
+    ro = O.run(O.make_cfg(**opts), rs, threads=8, want_tally=True)
+    with gpu.Context(**opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        assert np.array_equal(ctx.tally(), ro["tally"])
+        if mapping:
+            assert ctx.coverage_text() == O.coverage_text(rs, ro["tally"])
+        else:
+            assert np.array_equal(ctx.consensus(), ro["corrected"])
+        st = ctx.stats()
+    assert st["seeds"] > 2 * 1024 * rs.n, st["seeds"]  # mean per subject strand above the LDS sort's 1024
+    assert st["vote_oob"] == 0 and st["dp_skipped"] == 0
+
+
 @pytest.mark.parametrize("case", load_e2e(), ids=lambda c: c["name"])
 def test_alignment_records_match_reference(case):
     """Record mode (no -c): the .aln text is byte-identical to the reference's."""
