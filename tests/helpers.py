@@ -73,3 +73,26 @@ def cigar_ops(cig):
             ops.append((int(num), ch))
             num = ""
     return ops
+
+
+def load_scale():
+    """Fixtures at the workloads' stated sizes (tests/golden/make_scale_golden.py)."""
+    with open(os.path.join(GOLDEN, "scale.json")) as f:
+        return {c["name"]: c for c in json.load(f)}
+
+
+def block_digests(seq_bytes, rs, block):
+    """blake2b-64 of each block of `block` reads' bases, reads joined by newlines."""
+    out = []
+    for b0 in range(0, rs.n, block):
+        h = hashlib.blake2b(digest_size=8)
+        for i in range(b0, min(rs.n, b0 + block)):
+            o = int(rs.offset[i])
+            h.update(seq_bytes[o:o + int(rs.length[i])].tobytes())
+            h.update(b"\n")
+        out.append(h.hexdigest())
+    return out
+
+
+def first_differing_block(got, want):
+    return next((i for i, (a, b) in enumerate(zip(got, want)) if a != b), None)

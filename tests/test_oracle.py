@@ -103,3 +103,15 @@ def test_seed_search_brute_force():
                             exp.append((q, ps, pq))
             exp.sort(key=lambda x: (x[0], x[2], x[1]))
             assert [tuple(map(int, x)) for x in got] == exp
+
+
+@pytest.mark.parametrize("name", ["c1_full_global_i2", "c1_full_global_i3"])
+def test_oracle_at_c1_full_size_matches_reference(name):
+    """The oracle on the whole 29,703-read C1 set reproduces the reference's corrected FASTA."""
+    from helpers import block_digests, load_scale
+    case = load_scale()[name]
+    rs = synth.generate(**case["synth"])
+    assert rs.n == case["n_reads"]
+    r = O.run(O.make_cfg(call_consensus=1, **flags_to_opts(case["flags"])), rs, threads=8)
+    assert block_digests(r["corrected"], rs, case["block"]) == case["block_digests"]
+    assert sha256(fasta_text(r["corrected"], rs)) == case["fasta_sha256"]
