@@ -170,7 +170,7 @@ struct Worker {
 	DevBuf<uint8_t> gflag;
 	DevBuf<uint32_t> g_tasks, g_task_off, g_words, g_word_off, g_prog, g_prog_off, prog;
 	DevBuf<DpTask> tasks;
-	DevBuf<uint32_t> words, nsteps, dp_scratch, plain_scratch;
+	DevBuf<uint32_t> words, nsteps, dp_scratch, wave_scratch, wave_over;
 	DevBuf<uint32_t> bucket_hist, bucket_off, order;
 	DevBuf<int32_t> g_nch, g_best;
 	DevBuf<uint32_t> g_recs, g_rec_off, rec_opn, rec_opoff, rec_ops;
@@ -189,6 +189,7 @@ struct Worker {
 	hipGraphExec_t graph_exec = nullptr;
 	std::vector<uintptr_t> graph_key;
 	uint64_t graph_builds = 0, graph_launches = 0;
+	unsigned long long skip_seen = 0; // DP tasks skipped as beyond the limits, as last checked
 	// serving: one batch of query reads and its seeds
 	ReadsDev bq;
 	DevBuf<uint8_t> b_cons;
@@ -217,7 +218,7 @@ struct Worker {
 		cnext.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
 		g_tasks.release(), g_task_off.release(), g_words.release(), g_word_off.release();
 		g_prog.release(), g_prog_off.release(), prog.release();
-		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), plain_scratch.release();
+		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), wave_scratch.release(), wave_over.release();
 		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
 		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
@@ -285,6 +286,12 @@ struct hero_ctx {
 	int tuned_per_lane[kNumBandClasses] = {1, 1, 1, 1, 1, 1};
 	int tuned_waves_per_cu[kNumBandClasses] = {0, 0, 0, 0, 0, 0};
 	hero_tune_report *tune = nullptr; // set while hero_autotune runs
+	// BandAligner drop-in (hero_band_align_batch): grow-only device buffers
+	DevBuf<hero_band_pair> ba_pairs;
+	DevBuf<char> ba_seq, ba_types;
+	DevBuf<int64_t> ba_off;
+	DevBuf<int32_t> ba_band, ba_n, ba_counts;
+	DevBuf<uint32_t> ba_scratch;
 	// stats
 	hero_stats st_host;
 };
@@ -435,6 +442,48 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	return HERO_OK;
 }
 
+// Plain-form (wave) launches for calls whose sides have at most max_len codes: both sides' codes
+// and, when they fit the LDS share, the pointers of a call in LDS; otherwise its pointers go to the
+// wave's slice of wave_scratch, sized for the widest band the plain form takes (kWaveMaxBand).
+// Waves are limited so that the scratch stays within kWaveScratchBytes.
+constexpr size_t kWaveLdsShare = 32768;
+constexpr size_t kWaveScratchBytes = (size_t) 1 << 30;
+WaveSizing wave_sizing(int max_len, int max_band, int want_waves) {
+	WaveSizing z;
+	std::memset(&z, 0, sizeof z);
+	z.seq_cap = (2 * std::max(max_len, 1) + 15) & ~15;
+	const int bmax = std::min(std::max(max_band, 0), kWaveMaxBand);
+	const size_t share = std::max(kWaveLdsShare, (size_t) z.seq_cap + 8192) - (size_t) z.seq_cap;
+	z.ptr_lds_words = (int32_t) std::min(share / 4, wave_call_ptr_words(max_len, std::min(bmax, wave_band_hi(0))));
+	const size_t need = wave_call_ptr_words(max_len, bmax);
+	z.scratch_words = need > (size_t) z.ptr_lds_words ? need : 0;
+	z.nwaves = std::max(1, want_waves);
+	if (z.scratch_words)
+		z.nwaves = (int) std::max<size_t>(1, std::min<size_t>((size_t) z.nwaves, kWaveScratchBytes / (z.scratch_words * 4)));
+	return z;
+}
+
+// The three band-range launches of the plain form over Lp's tasks; ctl_wave (kWaveCtlWords) must be
+// zero before the first.
+hipError_t launch_wave_stage(const DpLaunch &Lp, const WaveSizing &z, uint32_t *ctl_wave, uint32_t *over, hipStream_t st) {
+	for (int l = 0; l < kWaveLaunches; l++)
+		if (hipError_t e = launch_dp_wave_tasks(Lp, z, l, ctl_wave, over, st); e != hipSuccess) return e;
+	return hipSuccess;
+}
+
+// Tasks the DP skipped since the last check (beyond kWaveMaxBand or the read-length limits).
+int check_dp_skipped(hero_ctx *c, Worker *w) {
+	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
+	HIPCHK(c, hipMemcpyAsync(stripes.data(), w->dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	unsigned long long skipped = 0;
+	for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
+	const unsigned long long fresh = skipped - w->skip_seen;
+	w->skip_seen = skipped;
+	if (fresh) return fail(c, HERO_ERR_LIMIT, "%llu DP calls have a band above %d", fresh, kWaveMaxBand);
+	return HERO_OK;
+}
+
 // The DP of a chunk's tasks (cd.tasks): batched form by band class and length bucket, plain form
 // for the rest (or for everything with dp_form 1). Records ev[7] after bucketing, ev[3] at the end.
 int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const ChunkDev &cd, uint32_t ntasks,
@@ -451,9 +500,7 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = stats, L.batch_counter = w->counter.p;
 	const bool aff = c->o.aligner == 0;
 	const int rows = std::min(rows_needed, 512);
-	const int bmax_plain = 64;
 	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
-	L.route_bmax = -1, L.route_rows = 0;
 	L.tagged = !c->opt.dp_untagged && dp_tagged_form(c->o, rows) ? 1 : 0;
 	uint32_t plain_begin = 0, plain_n = (uint32_t) ntasks;
 	if (c->opt.dp_form == 0 && ntasks > 0) {
@@ -505,24 +552,20 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 	}
 	if (plain_n > 0) {
 		// plain form: every task (dp_form 1), or the bucket beyond the batched band classes
-		HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+		const WaveSizing z = wave_sizing(rows_needed - 1, kWaveMaxBand, plain_waves);
+		HIPCHK(c, w->wave_scratch.ensure(z.scratch_words * (size_t) z.nwaves + 1));
+		HIPCHK(c, w->wave_over.ensure((size_t) plain_n));
+		WaveSizing zz = z;
+		zz.scratch = z.scratch_words ? w->wave_scratch.p : nullptr;
 		DpLaunch Lp = L;
 		Lp.order = c->opt.dp_form == 0 ? w->order.p + plain_begin : nullptr;
 		Lp.ntasks = plain_n;
-		Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
-		HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
-		HIPCHK(c, launch_dp_plain(Lp, bmax_plain, (int) std::min<uint64_t>((uint64_t) plain_waves, plain_n), w->st));
+		uint32_t *ctl_wave = w->ctl.p + kCtlWave;
+		HIPCHK(c, hipMemsetAsync(ctl_wave, 0, kWaveCtlWords * 4, w->st));
+		HIPCHK(c, launch_wave_stage(Lp, zz, ctl_wave, w->wave_over.p, w->st));
 	}
 	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
-	{
-		std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
-		HIPCHK(c, hipMemcpyAsync(stripes.data(), stats, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
-		HIPCHK(c, hipStreamSynchronize(w->st));
-		unsigned long long skipped = 0;
-		for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
-		if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
-	}
-	return HERO_OK;
+	return check_dp_skipped(c, w);
 }
 
 // The DP stage as one graph launch. Fixed grids (persistent waves per class, grid-stride
@@ -534,7 +577,6 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	// rows rounded up so that batches of slightly different read lengths share one graph (the row
 	// limit only routes tasks between the two DP forms, which give identical results)
 	const int rows = std::min((rows_needed + 63) / 64 * 64, 512);
-	const int bmax_plain = 64;
 	const int plain_waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * 8;
 	const uint32_t cap = (uint32_t) std::min<size_t>(w->tasks.n, 0xFFFFFFFFu);
 	if (cd.tasks != w->tasks.p || ntasks > cap) return fail(c, HERO_ERR_STATE, "DP graph: task buffer mismatch");
@@ -555,17 +597,21 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);
 	}
 	HIPCHK(c, w->dp_scratch.ensure(scratch));
-	HIPCHK(c, w->plain_scratch.ensure((size_t) plain_waves * 512 * (2 * bmax_plain + 1) / 4 + 64));
+	WaveSizing z = wave_sizing((rows_needed + 63) / 64 * 64 - 1, kWaveMaxBand, plain_waves);
+	HIPCHK(c, w->wave_scratch.ensure(z.scratch_words * (size_t) z.nwaves + 1));
+	HIPCHK(c, w->wave_over.ensure(cap));
+	z.scratch = z.scratch_words ? w->wave_scratch.p : nullptr;
 	DpLaunch L;
 	std::memset(&L, 0, sizeof L); // padding too: the key below compares the struct's bytes
 	L.tasks = cd.tasks, L.order = w->order.p, L.S = S, L.Q = Q, L.o = c->o;
 	L.words = cd.words, L.nsteps = cd.nsteps, L.sums = cd.sums, L.stats = w->dstats.p;
-	L.route_bmax = -1, L.route_rows = 0, L.rows_max = rows;
+	L.rows_max = rows;
 	L.tagged = tagged ? 1 : 0;
 	// everything the captured launches take by value
 	std::vector<uintptr_t> key((sizeof L + sizeof(uintptr_t) - 1) / sizeof(uintptr_t), 0);
 	std::memcpy(key.data(), &L, sizeof L);
-	for (uintptr_t v : {(uintptr_t) cap, (uintptr_t) w->dp_scratch.p, (uintptr_t) w->plain_scratch.p,
+	for (uintptr_t v : {(uintptr_t) cap, (uintptr_t) w->dp_scratch.p, (uintptr_t) w->wave_scratch.p, (uintptr_t) w->wave_over.p,
+				 (uintptr_t) z.seq_cap, (uintptr_t) z.ptr_lds_words, (uintptr_t) z.scratch_words, (uintptr_t) z.nwaves,
 				 (uintptr_t) w->bucket_hist.p, (uintptr_t) w->bucket_off.p, (uintptr_t) w->scan_tmp.p,
 				 (uintptr_t) w->scan_tmp.n, (uintptr_t) w->dtotal.p, (uintptr_t) w->ctl.p, (uintptr_t) plain_waves})
 		key.push_back(v);
@@ -590,9 +636,8 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		}
 		if (e == hipSuccess) {
 			DpLaunch Lp = L;
-			Lp.range = ctl + kCtlRange + kNumBandClasses, Lp.batch_counter = ctl + kCtlCounter + kNumBandClasses;
-			Lp.scratch = w->plain_scratch.p, Lp.rows_max = 512;
-			e = launch_dp_plain(Lp, bmax_plain, plain_waves, w->st);
+			Lp.range = ctl + kCtlRange + kNumBandClasses;
+			e = launch_wave_stage(Lp, z, ctl + kCtlWave, w->wave_over.p, w->st);
 		}
 		hipGraph_t g = nullptr;
 		const hipError_t e2 = hipStreamEndCapture(w->st, &g);
@@ -611,13 +656,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	HIPCHK(c, hipGraphLaunch(w->graph_exec, w->st));
 	w->graph_launches++;
 	HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
-	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
-	HIPCHK(c, hipMemcpyAsync(stripes.data(), w->dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
-	HIPCHK(c, hipStreamSynchronize(w->st));
-	unsigned long long skipped = 0;
-	for (int i = 0; i < kStatStripes; i++) skipped += stripes[(size_t) i * kStatStride + kStDpSkipped];
-	if (skipped) return fail(c, HERO_ERR_LIMIT, "%llu DP tasks exceed band 64 or 511 rows", skipped);
-	return HERO_OK;
+	return check_dp_skipped(c, w);
 }
 
 int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub, int rows_needed, const int64_t *s_uid) {
@@ -1181,6 +1220,147 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 
 } // namespace
 
+namespace {
+
+// Test route of hero_band_align_batch (hero_options.band_align_form 1-3): the calls become DP
+// tasks over a packed read set (each call's two sides as two reads) and run through one batched
+// kernel of the correction pipeline: 1 one task per lane in the general rows, 2 the tagged affine
+// rows, 3 two tasks per lane in packed 16-bit. Returns per call its edit words (at woff[i]) and
+// step count.
+int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const char *seq,
+		std::vector<int64_t> &woff, std::vector<uint32_t> &words, std::vector<uint32_t> &nsteps) {
+	const int form = c->opt.band_align_form;
+	if (form < 1 || form > 3) return fail(c, HERO_ERR_ARG, "band_align_form %d (0 wave, 1 general, 2 tagged, 3 packed)", form);
+	std::vector<int64_t> roff((size_t) 2 * n);
+	std::vector<int32_t> rlen((size_t) 2 * n);
+	std::vector<DpTask> tasks((size_t) n);
+	woff.assign((size_t) n, 0);
+	int64_t nw = 0;
+	int bmax = 0, rows = 1;
+	for (int i = 0; i < n; i++) {
+		const hero_band_pair &p = pairs[i];
+		for (int64_t k = 0; k < p.s_len; k++)
+			if (seq[p.s_off + k] == 'N') return fail(c, HERO_ERR_ARG, "pair %d: the batched forms take A/C/G/T only", i);
+		for (int64_t k = 0; k < p.q_len; k++)
+			if (seq[p.q_off + k] == 'N') return fail(c, HERO_ERR_ARG, "pair %d: the batched forms take A/C/G/T only", i);
+		roff[(size_t) 2 * i] = p.s_off, rlen[(size_t) 2 * i] = p.s_len;
+		roff[(size_t) 2 * i + 1] = p.q_off, rlen[(size_t) 2 * i + 1] = p.q_len;
+		const bool sw = !(p.q_len <= p.s_len);
+		const int lenS = sw ? p.q_len : p.s_len, lenQ = sw ? p.s_len : p.q_len;
+		DpTask &t = tasks[(size_t) i];
+		std::memset(&t, 0, sizeof t);
+		t.s_read = (uint32_t) (2 * i), t.q_read = (uint32_t) (2 * i + 1);
+		t.s0 = 0, t.s_len = (uint16_t) p.s_len, t.q0 = 0, t.q_len = (uint16_t) p.q_len;
+		t.band = (uint16_t) std::max(p.band, lenS - lenQ);
+		t.flags = (uint8_t) ((p.flag == 1 ? kTaskAlignLeft : p.flag == 2 ? kTaskAlignRight : 0) | (sw ? kTaskSwapped : 0));
+		t.out_off = (uint32_t) nw;
+		woff[(size_t) i] = nw;
+		nw += edit_words(p.s_len, p.q_len);
+		bmax = std::max(bmax, (int) t.band), rows = std::max(rows, lenS + 1);
+	}
+	int cls = -1;
+	for (int k = kNumBandClasses - 1; k >= 0; k--)
+		if (bmax <= kBandClass[k]) cls = k;
+	const int limit = form == 1 ? kBandClass[kNumBandClasses - 1] : (form == 2 ? kTaggedMaxBand : kPk16MaxBand);
+	if (cls < 0 || kBandClass[cls] > limit) return fail(c, HERO_ERR_ARG, "band %d beyond the batched form's classes", bmax);
+	if (rows > 512) return fail(c, HERO_ERR_ARG, "the batched forms take at most 511 rows");
+	if (form == 2 && !dp_tagged_form(c->o, rows)) return fail(c, HERO_ERR_ARG, "the tagged rows need affine costs with dx == dy");
+	if (form == 3 && !pk16_class_fits(c->o, rows)) return fail(c, HERO_ERR_ARG, "the calls exceed the 16-bit range");
+	std::vector<uint64_t> pw, pwoff;
+	pack_reads(seq, roff.data(), rlen.data(), 2 * (int64_t) n, pw, pwoff);
+	DevBuf<uint64_t> dw, dwoff;
+	DevBuf<int32_t> dlen;
+	DevBuf<DpTask> dt;
+	DevBuf<uint32_t> dwords, dns, dscr;
+	DevBuf<StepSummary> dsums;
+	const int bm = kBandClass[cls], per_lane = form == 3 ? 2 : 1, nwd = (2 * bm + 1 + 15) / 16;
+	const int waves = (int) std::max<int64_t>(1, std::min<int64_t>((int64_t) c->cu_count * 2, (n + 64 * per_lane - 1) / (64 * per_lane)));
+	hipStream_t st = c->w.st;
+	HIPCHK(c, dw.exact(pw.size()));
+	HIPCHK(c, dwoff.exact((size_t) 2 * n));
+	HIPCHK(c, dlen.exact((size_t) 2 * n));
+	HIPCHK(c, dt.exact((size_t) n));
+	HIPCHK(c, dwords.exact((size_t) nw + 1));
+	HIPCHK(c, dns.exact((size_t) n));
+	HIPCHK(c, dsums.exact((size_t) n));
+	HIPCHK(c, dscr.exact((size_t) waves * rows * nwd * 64 * per_lane));
+	HIPCHK(c, hipMemcpyAsync(dw.p, pw.data(), pw.size() * 8, hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(dwoff.p, pwoff.data(), (size_t) 2 * n * 8, hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(dlen.p, rlen.data(), (size_t) 2 * n * 4, hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(dt.p, tasks.data(), (size_t) n * sizeof(DpTask), hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemsetAsync(c->w.counter.p, 0, 4, st));
+	Packed R;
+	std::memset(&R, 0, sizeof R);
+	R.words = dw.p, R.woff = dwoff.p, R.len = dlen.p, R.n = 2 * (int64_t) n;
+	DpLaunch L;
+	std::memset(&L, 0, sizeof L);
+	L.tasks = dt.p, L.ntasks = (uint32_t) n, L.S = R, L.Q = R, L.o = c->o;
+	L.words = dwords.p, L.nsteps = dns.p, L.sums = dsums.p, L.scratch = dscr.p, L.rows_max = rows;
+	L.tagged = form == 2 ? 1 : 0, L.batch_counter = c->w.counter.p, L.stats = c->w.dstats.p;
+	HIPCHK(c, launch_dp_batched(L, bm, per_lane, waves, st));
+	words.resize((size_t) nw + 1);
+	nsteps.resize((size_t) n);
+	HIPCHK(c, hipMemcpyAsync(words.data(), dwords.p, words.size() * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipMemcpyAsync(nsteps.data(), dns.p, (size_t) n * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipStreamSynchronize(st));
+	dw.release(), dwoff.release(), dlen.release(), dt.release(), dwords.release(), dns.release(), dsums.release(), dscr.release();
+	return check_dp_skipped(c, &c->w);
+}
+
+// BandAligner::calculateCigarAndPosition[Reversed] (src/BandAligner.cpp:258-419) from edit words:
+// M runs merge, every mismatch is an op of its own typed by the original query's base, I / D
+// swap when the aligner's sides were swapped.
+int steps_to_cigars(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const char *seq, const std::vector<int64_t> &woff,
+		const std::vector<uint32_t> &hw, const std::vector<uint32_t> &hn, int32_t *band_out, int64_t *cigar_off,
+		int32_t *cigar_n, int32_t *cigar_counts, char *cigar_types, int64_t cigar_cap) {
+	int64_t pos = 0;
+	for (int i = 0; i < n; i++) {
+		const hero_band_pair &p = pairs[i];
+		const bool sw = !(p.q_len <= p.s_len);
+		const char *A = seq + (sw ? p.q_off : p.s_off), *B = seq + (sw ? p.s_off : p.q_off);
+		band_out[i] = std::max(p.band, std::abs(p.s_len - p.q_len));
+		const int cap = p.s_len + p.q_len, ns = (int) hn[(size_t) i];
+		cigar_off[i] = pos;
+		int nops = 0, cnt = 0, ii = 0, jj = 0;
+		char cur = 0;
+		bool cur_mis = false;
+		auto put = [&]() -> bool {
+			if (pos + nops >= cigar_cap) return false;
+			if (cigar_counts) cigar_counts[pos + nops] = cnt;
+			if (cigar_types) cigar_types[pos + nops] = cur;
+			nops++;
+			return true;
+		};
+		bool ok = true;
+		for (int e = cap - ns; e < cap && ok; e++) {
+			const uint32_t stp = (hw[(size_t) woff[(size_t) i] + (e >> 4)] >> (2 * (e & 15))) & 3u;
+			char t;
+			bool mis = false;
+			if (stp == kStepLeft) t = sw ? 'D' : 'I', jj++;
+			else if (stp == kStepUp) t = sw ? 'I' : 'D', ii++;
+			else {
+				const char a = A[ii], q = B[jj];
+				mis = a != q;
+				t = mis ? (sw ? a : q) : 'M';
+				ii++, jj++;
+			}
+			if (cnt > 0 && t == cur && !mis && !cur_mis) {
+				cnt++;
+				continue;
+			}
+			if (cnt > 0) ok = put();
+			cur = t, cur_mis = mis, cnt = 1;
+		}
+		if (ok && cnt > 0) ok = put();
+		if (!ok) return fail(c, HERO_ERR_LIMIT, "cigar_cap %lld too small", (long long) cigar_cap);
+		cigar_n[i] = nops;
+		pos += nops;
+	}
+	return HERO_OK;
+}
+
+} // namespace
+
 extern "C" {
 
 void hero_default_options(hero_options *p) {
@@ -1230,6 +1410,8 @@ void hero_destroy(hero_ctx *c) {
 	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
 	c->qix.release(), c->tally.release(), c->cons_out.release();
 	c->Rd.release(), c->six.release();
+	c->ba_pairs.release(), c->ba_seq.release(), c->ba_types.release(), c->ba_off.release();
+	c->ba_band.release(), c->ba_n.release(), c->ba_counts.release(), c->ba_scratch.release();
 	c->w.release();
 	if (c->w2_ready) c->w2.release();
 	delete c;
@@ -1462,100 +1644,79 @@ int hero_band_align_batch(hero_ctx *c, int32_t n, const hero_band_pair *pairs, c
 	if (!c || n < 0 || (n > 0 && (!pairs || !seq || !band_out || !cigar_off || !cigar_n))) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	if (n == 0) return HERO_OK;
-	// setSequences: the longer side is the aligner's subject (query side keeps it on ties)
-	std::vector<BytePair> bp((size_t) n);
-	int64_t seq_end = 0;
-	uint32_t words = 0;
-	int bmax = 0, rows = 0;
+	// setSequences: the longer side is the aligner's subject (the query side keeps it on ties);
+	// m_band = max(band, lenS - lenQ)
+	int64_t seq_end = 0, total = 0;
+	int max_len = 1, max_band = 0;
+	std::vector<int64_t> off((size_t) n);
 	for (int i = 0; i < n; i++) {
 		const hero_band_pair &p = pairs[i];
 		if (p.s_len < 0 || p.q_len < 0 || p.s_off < 0 || p.q_off < 0) return fail(c, HERO_ERR_REF_ABORT, "pair %d: negative start (the reference exits)", i);
 		if (p.s_len == 0 && p.q_len == 0) return fail(c, HERO_ERR_ARG, "pair %d: both sequences empty", i);
 		seq_end = std::max<int64_t>(seq_end, std::max(p.s_off + p.s_len, p.q_off + p.q_len));
-		const bool sw = !(p.q_len <= p.s_len);
-		BytePair &b = bp[(size_t) i];
-		b.s_off = (uint64_t) (sw ? p.q_off : p.s_off), b.s_len = sw ? p.q_len : p.s_len;
-		b.q_off = (uint64_t) (sw ? p.s_off : p.q_off), b.q_len = sw ? p.s_len : p.q_len;
-		b.band = std::max(p.band, b.s_len - b.q_len);
-		b.flags = p.flag == 1 ? 1 : (p.flag == 2 ? 2 : 0);
-		b.out_off = words;
-		b._r = sw;
-		words += (uint32_t) edit_words(b.s_len, b.q_len);
-		bmax = std::max(bmax, b.band);
-		rows = std::max(rows, b.s_len + 1);
-		band_out[i] = b.band;
+		const int lenS = std::max(p.s_len, p.q_len), lenQ = std::min(p.s_len, p.q_len);
+		const int b = std::max(p.band, lenS - lenQ);
+		if (b > kWaveMaxBand) return fail(c, HERO_ERR_LIMIT, "pair %d: band %d above %d", i, b, kWaveMaxBand);
+		max_len = std::max(max_len, lenS), max_band = std::max(max_band, b);
+		off[(size_t) i] = total;
+		total += (int64_t) p.s_len + p.q_len;
 	}
 	for (int64_t i = 0; i < seq_end; i++) {
 		const char ch = seq[i];
 		if (ch != 'A' && ch != 'C' && ch != 'G' && ch != 'T' && ch != 'N')
 			return fail(c, HERO_ERR_ARG, "sequence byte %lld is '%c'; only A/C/G/T/N are accepted", (long long) i, ch);
 	}
-	if (bmax > 64) return fail(c, HERO_ERR_LIMIT, "band %d above 64", bmax);
-	DevBuf<BytePair> dpairs;
-	DevBuf<char> dseq;
-	DevBuf<uint32_t> dwords, dns, dscr;
-	const int bsel = bmax <= 8 ? 8 : bmax <= 16 ? 16 : bmax <= 32 ? 32 : 64;
-	const int nw = (2 * bsel + 1 + 15) / 16;
-	HIPCHK(c, dpairs.exact((size_t) n));
-	HIPCHK(c, dseq.exact((size_t) seq_end + 1));
-	HIPCHK(c, dwords.exact((size_t) words + 1));
-	HIPCHK(c, dns.exact((size_t) n));
-	HIPCHK(c, dscr.exact((size_t) ((n + 63) / 64) * rows * nw * 64));
-	HIPCHK(c, hipMemcpyAsync(dpairs.p, bp.data(), bp.size() * sizeof(BytePair), hipMemcpyHostToDevice, c->w.st));
-	HIPCHK(c, hipMemcpyAsync(dseq.p, seq, (size_t) seq_end, hipMemcpyHostToDevice, c->w.st));
-	HIPCHK(c, launch_band_align_bytes(dpairs.p, n, dseq.p, c->o, dwords.p, dns.p, dscr.p, rows, bsel, c->w.st));
-	std::vector<uint32_t> hw((size_t) words + 1), hn((size_t) n);
-	HIPCHK(c, hipMemcpyAsync(hw.data(), dwords.p, hw.size() * 4, hipMemcpyDeviceToHost, c->w.st));
-	HIPCHK(c, hipMemcpyAsync(hn.data(), dns.p, hn.size() * 4, hipMemcpyDeviceToHost, c->w.st));
-	HIPCHK(c, hipStreamSynchronize(c->w.st));
-	// BandAligner::calculateCigarAndPosition[Reversed] (src/BandAligner.cpp:258-419)
+	if (c->opt.band_align_form != 0) {
+		// test route through the pipeline's batched DP kernels
+		std::vector<uint32_t> words, nsteps;
+		std::vector<int64_t> woff;
+		if (int r = band_align_via_batched(c, n, pairs, seq, woff, words, nsteps)) return r;
+		return steps_to_cigars(c, n, pairs, seq, woff, words, nsteps, band_out, cigar_off, cigar_n, cigar_counts, cigar_types,
+				cigar_cap);
+	}
+	hipStream_t st = c->w.st;
+	HIPCHK(c, c->ba_pairs.ensure((size_t) n));
+	HIPCHK(c, c->ba_seq.ensure((size_t) seq_end + 1));
+	HIPCHK(c, c->ba_off.ensure((size_t) n));
+	HIPCHK(c, c->ba_band.ensure((size_t) n));
+	HIPCHK(c, c->ba_n.ensure((size_t) n));
+	HIPCHK(c, c->ba_counts.ensure((size_t) total));
+	HIPCHK(c, c->ba_types.ensure((size_t) total));
+	WaveSizing z = wave_sizing(max_len, max_band, (int) std::min<int64_t>(n, (int64_t) c->cu_count * 16));
+	HIPCHK(c, c->ba_scratch.ensure(z.scratch_words * (size_t) z.nwaves + 1));
+	z.scratch = z.scratch_words ? c->ba_scratch.p : nullptr;
+	HIPCHK(c, hipMemcpyAsync(c->ba_pairs.p, pairs, (size_t) n * sizeof(hero_band_pair), hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(c->ba_seq.p, seq, (size_t) seq_end, hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(c->ba_off.p, off.data(), (size_t) n * 8, hipMemcpyHostToDevice, st));
+	uint32_t *ctl_wave = c->w.ctl.p + kCtlWave;
+	HIPCHK(c, hipMemsetAsync(ctl_wave, 0, kWaveCtlWords * 4, st));
+	WavePairs P;
+	std::memset(&P, 0, sizeof P);
+	P.pairs = c->ba_pairs.p, P.n = n, P.max_len = max_len, P.max_band = max_band, P.seq = c->ba_seq.p;
+	P.band_out = c->ba_band.p, P.cigar_off = c->ba_off.p, P.cigar_n = c->ba_n.p, P.cigar_counts = c->ba_counts.p;
+	P.cigar_types = c->ba_types.p;
+	for (int l = 0; l < kWaveLaunches; l++) {
+		if (l > 0 && max_band <= wave_band_hi(l - 1)) break;
+		P.counter = ctl_wave + kWaveCtlCounter + l;
+		HIPCHK(c, launch_dp_wave_pairs(P, c->o, z, l, st));
+	}
+	std::vector<int32_t> hn((size_t) n), hcnt((size_t) total);
+	std::vector<char> htyp((size_t) total);
+	HIPCHK(c, hipMemcpyAsync(band_out, c->ba_band.p, (size_t) n * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipMemcpyAsync(hn.data(), c->ba_n.p, (size_t) n * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipMemcpyAsync(hcnt.data(), c->ba_counts.p, (size_t) total * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipMemcpyAsync(htyp.data(), c->ba_types.p, (size_t) total, hipMemcpyDeviceToHost, st));
+	HIPCHK(c, hipStreamSynchronize(st));
+	// the device wrote each call's ops at the front of its region; pack them back to back
 	int64_t pos = 0;
 	for (int i = 0; i < n; i++) {
-		const BytePair &b = bp[(size_t) i];
-		const bool sw = b._r != 0;
-		const char *A = seq + b.s_off, *B = seq + b.q_off;
-		const int cap = b.s_len + b.q_len, ns = (int) hn[(size_t) i];
-		cigar_off[i] = pos;
-		int nops = 0;
-		char cur = 0;
-		int cnt = 0;
-		int ii = 0, jj = 0;
-		auto push = [&](char t, bool mismatch) -> bool {
-			if (cnt > 0 && t == cur && !mismatch) {
-				cnt++;
-				return true;
-			}
-			if (cnt > 0) {
-				if (pos + nops >= cigar_cap) return false;
-				if (cigar_counts) cigar_counts[pos + nops] = cnt;
-				if (cigar_types) cigar_types[pos + nops] = cur;
-				nops++;
-			}
-			cur = t, cnt = 1;
-			return true;
-		};
-		bool ok = true;
-		for (int e = cap - ns; e < cap && ok; e++) {
-			const uint32_t st = (hw[b.out_off + (e >> 4)] >> (2 * (e & 15))) & 3u;
-			if (st == kStepLeft) ok = push(sw ? 'D' : 'I', false), jj++;
-			else if (st == kStepUp) ok = push(sw ? 'I' : 'D', false), ii++;
-			else {
-				const char a = A[ii], q = B[jj];
-				ok = a == q ? push('M', false) : push(sw ? a : q, true);
-				ii++, jj++;
-			}
-		}
-		if (ok && cnt > 0) {
-			if (pos + nops >= cigar_cap) ok = false;
-			else {
-				if (cigar_counts) cigar_counts[pos + nops] = cnt;
-				if (cigar_types) cigar_types[pos + nops] = cur;
-				nops++;
-			}
-		}
-		if (!ok) return fail(c, HERO_ERR_LIMIT, "cigar_cap %lld too small", (long long) cigar_cap);
-		cigar_n[i] = nops;
-		pos += nops;
+		const int k = hn[(size_t) i];
+		if (k < 0) return fail(c, k, "pair %d: the device aligner returned %d", i, k);
+		if (pos + k > cigar_cap) return fail(c, HERO_ERR_LIMIT, "cigar_cap %lld too small", (long long) cigar_cap);
+		cigar_off[i] = pos, cigar_n[i] = k;
+		if (cigar_counts) std::memcpy(cigar_counts + pos, hcnt.data() + off[(size_t) i], (size_t) k * 4);
+		if (cigar_types) std::memcpy(cigar_types + pos, htyp.data() + off[(size_t) i], (size_t) k);
+		pos += k;
 	}
 	return HERO_OK;
 }

@@ -232,6 +232,29 @@ struct SummaryBuilder {
 	}
 };
 
+// Writes traceback steps last-to-first into a task's word range: step m goes to slot cap-1-m.
+struct StepWriter {
+	uint32_t *w;
+	int e;
+	int cur;
+	uint32_t acc;
+	SummaryBuilder sb;
+	HD void operator()(uint32_t st) {
+		sb.add(st);
+		e--;
+		const int wi = e >> 4;
+		if (wi != cur) {
+			if (cur >= 0) w[cur] = acc;
+			acc = 0;
+			cur = wi;
+		}
+		acc |= st << (2 * (e & 15));
+	}
+	HD void flush() {
+		if (cur >= 0) w[cur] = acc;
+	}
+};
+
 // In-band cells of one DP call: sum over rows of the row width, minus the origin.
 // Cells of one banded DP call: sum over rows i = 0..lenS of (min(lenQ, i+b) - max(0, i-b) + 1),
 // minus the origin cell. Closed form of that sum (each part summed separately, so the identity holds

@@ -161,7 +161,6 @@ struct DpLaunch {
 	StepSummary *sums;
 	uint32_t *scratch;
 	int rows_max;
-	int route_bmax, route_rows; // plain form: >= 0 restricts it to tasks the batched form cannot take
 	int tagged;                 // one task per lane, affine: rows in the tagged form (dp_tagged_form)
 	uint32_t *batch_counter;
 	unsigned long long *stats;
@@ -234,9 +233,9 @@ hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, const uint3
 // kNumBandClasses, then ctl[2 + kNumBandClasses] = ntasks = ctl[0]) and zeroed class counters
 // (ctl[16 + c]).
 constexpr int kCtlRange = 1, kCtlCounter = 16, kCtlWords = 32;
+constexpr int kCtlWave = 24; // the plain form's kWaveCtlWords control words (zeroed with the counters)
 hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
-hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st);
 // Tallies: 4 counters per base (mismatch votes by base code) at 4 * q_boff[q], and the matched-run
 // difference array, 1 counter per base, at q_boff[q].
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally,
@@ -283,14 +282,46 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long 
 		size_t tmp_words, hipStream_t st);
 size_t scan_tmp_words(int64_t n);
 
-// Standalone BandAligner batch over byte sequences (A/C/G/T/N).
-struct BytePair {
-	uint64_t s_off, q_off; // aligner subject / query bytes in the sequence buffer
-	int32_t s_len, q_len, band, flags; // flags: 1 left, 2 right
-	uint32_t out_off;      // edit words
-	int32_t _r;
+// Plain form of the DP (dp_wave.hip): one wavefront per call, bands up to kWaveMaxBand in
+// kWaveLaunches launches by band range (bands 0-63, 64-255, 256-1023). Per wave: the codes of both
+// sides in LDS (seq_cap bytes) and the traceback pointers in LDS (ptr_lds_words) or, for a call
+// whose pointers do not fit there, in the wave's global scratch (scratch_words per wave, wave w at
+// scratch + w * scratch_words).
+constexpr int kWaveLaunches = 3;
+constexpr int kWaveMaxBand = 1023;
+struct WaveSizing {
+	int32_t seq_cap;       // LDS bytes for the codes of one call (lenS + lenQ)
+	int32_t ptr_lds_words; // LDS words for one call's pointers
+	uint32_t *scratch;     // global pointer words (may be null)
+	size_t scratch_words;  // per wave
+	int32_t nwaves;
 };
-hipError_t launch_band_align_bytes(const BytePair *pairs, int n, const char *seq, const Opts &o, uint32_t *words,
-		uint32_t *nsteps, uint32_t *scratch, int rows_max, int bmax, hipStream_t st);
+int wave_band_hi(int launch);          // widest band of a launch (its narrowest is the previous one's + 1)
+size_t wave_lds_bytes(const WaveSizing &z);
+size_t wave_call_ptr_words(int lenS, int b);
+// The pipeline's DP tasks (DpLaunch: tasks, order / range, words, nsteps, sums, stats) whose band
+// lies in the launch's range. ctl_wave holds kWaveCtlWords words, zero before launch 0: the launches'
+// work counters and the overflow list's range; launch 0 runs the given tasks and lists those wider
+// than its range in `over` (capacity: the task count) for launches 1 and 2.
+constexpr int kWaveCtlCounter = 0, kWaveCtlOver = 4, kWaveCtlWords = 8;
+hipError_t launch_dp_wave_tasks(const DpLaunch &L, const WaveSizing &z, int launch, uint32_t *ctl_wave, uint32_t *over,
+		hipStream_t st);
+
+// BandAligner calls over bytes A/C/G/T/N (the C-ABI drop-in); pairs are hero_band_pair records
+// (include/hero_gpu.h). Per call: band_out, and cigar_n ops (or a negative status) written at
+// cigar_off[t], whose region holds s_len + q_len ops. Calls beyond max_len / max_band get
+// HERO_ERR_LIMIT from launch 0.
+struct WavePairs {
+	const void *pairs; // hero_band_pair[n]
+	int32_t n;
+	int32_t max_len, max_band;
+	const char *seq;
+	int32_t *band_out;
+	const int64_t *cigar_off;
+	int32_t *cigar_n, *cigar_counts;
+	char *cigar_types;
+	uint32_t *counter; // work counter, zero at launch
+};
+hipError_t launch_dp_wave_pairs(const WavePairs &P, const Opts &o, const WaveSizing &z, int launch, hipStream_t st);
 
 } // namespace hero

@@ -1176,29 +1176,6 @@ __device__ __forceinline__ void task_streams(const DpTask &t, const Packed &S, c
 	lenQ = sw ? t.s_len : t.q_len;
 }
 
-// Writes traceback steps last-to-first into a task's word range: step m goes to slot cap-1-m.
-struct StepWriter {
-	uint32_t *w;
-	int e;
-	int cur;
-	uint32_t acc;
-	SummaryBuilder sb;
-	__device__ __forceinline__ void operator()(uint32_t st) {
-		sb.add(st);
-		e--;
-		const int wi = e >> 4;
-		if (wi != cur) {
-			if (cur >= 0) w[cur] = acc;
-			acc = 0;
-			cur = wi;
-		}
-		acc |= st << (2 * (e & 15));
-	}
-	__device__ __forceinline__ void flush() {
-		if (cur >= 0) w[cur] = acc;
-	}
-};
-
 // Pointer rows of the lanes of one wavefront, interleaved so a row store is one coalesced
 // 256-byte write: word w of row i of lane l at ((i * NW) + w) * 64 + l.
 template <int NW>
@@ -1526,120 +1503,6 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 	return hipErrorInvalidValue;
 }
 
-// Plain form: one wavefront per task, one anti-diagonal per step. Lane l owns the cell of row
-// i_lo(t) + l on anti-diagonal t = i + j; the two previous anti-diagonals live in LDS indexed by
-// row, pointers as bytes in per-wave global scratch, traceback by lane 0.
-constexpr int kPlainRows = 512;
-template <bool AFF>
-__global__ __launch_bounds__(64) void k_dp_plain(DpLaunch L0, int bmax_plain) {
-	DpLaunch L = L0;
-	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
-	__shared__ int hb[3][kPlainRows + 1];
-	__shared__ int xb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
-	__shared__ int yb[AFF ? 3 : 1][AFF ? kPlainRows + 1 : 1];
-	__shared__ uint8_t sc[kPlainRows], qc[kPlainRows];
-	const int lane = lane_id();
-	uint8_t *ptr = (uint8_t *) (L.scratch) + (size_t) blockIdx.x * (size_t) kPlainRows * (2 * bmax_plain + 1);
-	for (;;) {
-		uint32_t ti = 0;
-		if (lane == 0) ti = atomicAdd(L.batch_counter, 1u);
-		ti = __shfl(ti, 0, 64);
-		if (ti >= L.ntasks) break;
-		const uint32_t t = L.order ? L.order[ti] : ti;
-		const DpTask task = L.tasks[t];
-		PackedStream as, aq;
-		int lenS, lenQ;
-		task_streams(task, L.S, L.Q, as, aq, lenS, lenQ);
-		const int b = task.band;
-		if (L.route_bmax >= 0 && b <= L.route_bmax && lenS < L.route_rows) continue; // done by the batched form
-		if (b > bmax_plain || lenS >= kPlainRows || lenQ >= kPlainRows) {
-			if (lane == 0) stat_add(L.stats, kStDpSkipped, 1ull);
-			continue;
-		}
-		const int W = 2 * b + 1;
-		// stage the codes in LDS (each lane reads a strided slice of the packed sides)
-		struct SegRef {
-			const uint64_t *w;
-			int L, start;
-			bool rc;
-			__device__ int at(int p) const {
-				const int q = start + p;
-				return rc ? (base_at(w, L - 1 - q) ^ 2) : base_at(w, q);
-			}
-		};
-		const SegRef rs{L.S.words + L.S.woff[task.s_read], L.S.len[task.s_read], task.s0, (task.flags & kTaskSubjRC) != 0};
-		const SegRef rq{L.Q.words + L.Q.woff[task.q_read], L.Q.len[task.q_read], task.q0, (task.flags & kTaskQryRC) != 0};
-		const bool swp = (task.flags & kTaskSwapped) != 0;
-		for (int i = lane; i < lenS; i += 64) sc[i] = (uint8_t) (swp ? rq : rs).at(i);
-		for (int j = lane; j < lenQ; j += 64) qc[j] = (uint8_t) (swp ? rs : rq).at(j);
-		__syncthreads();
-		const bool la = (task.flags & kTaskAlignLeft) != 0, ra = (task.flags & kTaskAlignRight) != 0;
-		auto valid = [&](int i, int j) { return i >= 0 && i <= lenS && j >= 0 && j <= lenQ && j - i <= b && i - j <= b; };
-		for (int ad = 0; ad <= lenS + lenQ; ad++) {
-			const int cur = ad % 3, prv = (ad + 2) % 3, pp = (ad + 1) % 3;
-			int ilo = ad - lenQ > 0 ? ad - lenQ : 0;
-			int ihi = ad < lenS ? ad : lenS;
-			const int blo = (ad - b + 1) >> 1; // ceil((ad-b)/2) for ad-b >= -1; clamp below
-			const int bhi = (ad + b) >> 1;
-			if (blo > ilo) ilo = blo;
-			if (bhi < ihi) ihi = bhi;
-			for (int i = ilo + lane; i <= ihi; i += 64) {
-				const int j = ad - i;
-				int h, x, y, p;
-				if (i == 0) {
-					row0_cell<AFF>(L.o, j, b, lenQ, h, x, y);
-					p = 1;
-				} else {
-					const int hd = valid(i - 1, j - 1) ? hb[pp][i - 1] : kSent;
-					const int hu = valid(i - 1, j) ? hb[prv][i - 1] : kSent;
-					const int hl = valid(i, j - 1) ? hb[prv][i] : kSent;
-					int xd = kSent, yd = kSent, xu = kSent, yl = kSent;
-					if (AFF) {
-						xd = valid(i - 1, j - 1) ? xb[pp][i - 1] : kSent;
-						yd = valid(i - 1, j - 1) ? yb[pp][i - 1] : kSent;
-						xu = valid(i - 1, j) ? xb[prv][i - 1] : kSent;
-						yl = valid(i, j - 1) ? yb[prv][i] : kSent;
-					}
-					cell_update<AFF>(L.o, sc[i - 1], j >= 1 ? qc[j - 1] : kCodeNone, j == 0, j == lenQ, la, ra, lenQ, hd, xd, yd,
-							hu, xu, hl, yl, h, x, y, p);
-				}
-				hb[cur][i] = h;
-				if (AFF) xb[cur][i] = x, yb[cur][i] = y;
-				ptr[(size_t) i * W + (j - i + b)] = (uint8_t) p;
-			}
-			__syncthreads();
-		}
-		if (lane == 0) {
-			struct BytePtrs {
-				const uint8_t *p;
-				int W, b;
-				__device__ uint32_t get(int i, int w) const {
-					// repack the 16 cells of word w of row i into the 2-bit layout the traceback reads
-					uint32_t v = 0;
-					for (int d = 16 * w; d < 16 * w + 16 && d < W; d++) v |= (uint32_t) p[(size_t) i * W + d] << (2 * (d & 15));
-					return v;
-				}
-			} bp{ptr, W, b};
-			const int cap = task.s_len + task.q_len;
-			StepWriter wr{L.words + task.out_off, cap, -1, 0};
-			const int ns = dp_traceback(lenS, lenQ, b, bp, wr);
-			wr.flush();
-			L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
-				L.sums[t] = wr.sb.finish();
-			if (ns < 0) stat_add(L.stats, kStDpSkipped, 1ull);
-			stat_add(L.stats, kStCellsAll, (unsigned long long) band_cells(lenS, lenQ, b));
-		}
-		__syncthreads();
-	}
-}
-
-hipError_t launch_dp_plain(const DpLaunch &L, int bmax_plain, int nwaves, hipStream_t st) {
-	if (L.ntasks == 0 && !L.range) return hipSuccess;
-	if (L.o.aligner == 0) k_dp_plain<true><<<nwaves, 64, 0, st>>>(L, bmax_plain);
-	else k_dp_plain<false><<<nwaves, 64, 0, st>>>(L, bmax_plain);
-	return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------------------------
 // Consensus (QueryDataset::callConsensus, src/QueryDataset.cpp:393-412)
 // ---------------------------------------------------------------------------------------------
@@ -1783,57 +1646,6 @@ hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long 
 		if (e != hipSuccess) return e;
 	}
 	return scan_rec(in, out, n, tmp, st);
-}
-
-// ---------------------------------------------------------------------------------------------
-// Standalone BandAligner batch over bytes (A/C/G/T/N)
-// ---------------------------------------------------------------------------------------------
-struct ByteStreamDev {
-	static constexpr bool kTwoBit = false; // 'N' is code 4
-	const char *s;
-	int len, i;
-	__device__ __forceinline__ int next() {
-		if (i >= len) return kCodeNone;
-		const char c = s[i++];
-		return c == 'N' ? kCodeN : ascii_code(c);
-	}
-};
-
-template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) void k_band_align_bytes(const BytePair *pairs, int n, const char *seq, Opts o,
-		uint32_t *words, uint32_t *nsteps, uint32_t *scratch, int rows_max) {
-	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
-	const int lane = lane_id();
-	const int t = blockIdx.x * 64 + lane;
-	LanePtrs<NW> ptrs{scratch + (size_t) blockIdx.x * rows_max * NW * 64, lane};
-	if (t >= n) return;
-	const BytePair p = pairs[t];
-	if (p.band > BMAX || p.s_len >= rows_max) {
-		nsteps[t] = 0xFFFFFFFFu;
-		return;
-	}
-	ByteStreamDev as{seq + p.s_off, p.s_len, 0}, aq{seq + p.q_off, p.q_len, 0};
-	dp_forward<BMAX, AFF>(o, p.s_len, p.q_len, p.band, (p.flags & 1) != 0, (p.flags & 2) != 0, as, aq, ptrs);
-	StepWriter wr{words + p.out_off, p.s_len + p.q_len, -1, 0};
-	const int ns = dp_traceback(p.s_len, p.q_len, p.band, ptrs, wr);
-	wr.flush();
-	nsteps[t] = ns < 0 ? 0xFFFFFFFFu : (uint32_t) ns;
-}
-
-hipError_t launch_band_align_bytes(const BytePair *pairs, int n, const char *seq, const Opts &o, uint32_t *words,
-		uint32_t *nsteps, uint32_t *scratch, int rows_max, int bmax, hipStream_t st) {
-	if (n == 0) return hipSuccess;
-	const unsigned blocks = (unsigned) ((n + 63) / 64);
-	const bool aff = o.aligner == 0;
-#define HERO_BAB(BM)                                                                                                  \
-	if (bmax <= BM) {                                                                                                 \
-		if (aff) k_band_align_bytes<BM, true><<<blocks, 64, 0, st>>>(pairs, n, seq, o, words, nsteps, scratch, rows_max);  \
-		else k_band_align_bytes<BM, false><<<blocks, 64, 0, st>>>(pairs, n, seq, o, words, nsteps, scratch, rows_max);     \
-		return hipGetLastError();                                                                                     \
-	}
-	HERO_BAB(8) HERO_BAB(16) HERO_BAB(32) HERO_BAB(64)
-#undef HERO_BAB
-	return hipErrorInvalidValue;
 }
 
 } // namespace hero

@@ -26,7 +26,7 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
         ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
-        ("mapping", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1),
+        ("mapping", ctypes.c_int32), ("band_align_form", ctypes.c_int32),
     ]
 
 

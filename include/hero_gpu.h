@@ -60,7 +60,9 @@ typedef struct {
 	                           the best record covering it votes (SubjectRead::chooseQueryWithBestAlignment);
 	                           the index is sized by bases / k; read the result with
 	                           hero_coverage_histogram instead of calling consensus */
-	int32_t reserved[1];
+	int32_t band_align_form; /* hero_band_align_batch: 0 the plain form (one wavefront per call); for tests
+	                            of the pipeline's kernels on given calls (A/C/G/T, bands <= 32, < 512 rows):
+	                            1 batched general rows, 2 batched tagged affine rows, 3 batched packed 16-bit */
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */

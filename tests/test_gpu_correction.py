@@ -322,3 +322,48 @@ def test_dp_graph_records_equal_oracle_over_chunks():
     assert st["dp_graph_launches"] >= 5
     ro = O.run(O.make_cfg(call_consensus=0, **opts), rs, threads=8, want_aln=True)
     assert aln.decode() == ro["aln"]
+
+
+@pytest.mark.parametrize("dp_form", [0, 1], ids=["batched", "plain"])
+@pytest.mark.parametrize("aligner", [0, 1], ids=["affine", "global"])
+def test_long_reads_match_oracle(aligner, dp_form):
+    """Reads of 800-3000 bp: DP calls beyond 511 rows and bands beyond 64 (indel rate 3 %) run in
+    the plain form's wide launches."""
+    rs = synth.generate(seed=99, genome_len=30000, len_min=800, len_max=3000, coverage=8.0, sub_5p=0.01,
+                        indel_rate=0.003)
+    # -f 128: at 32 slots per read the reference's table would be full (about 65 keys per read)
+    opts = dict(call_consensus=1, k=31, aligner=aligner, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40,
+                hash_factor=128)
+    with gpu.Context(dp_form=dp_form, **opts) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_self()
+        tally = ctx.tally()
+        out = ctx.consensus()
+        st = ctx.stats()
+    assert st["dp_skipped"] == 0 and st["vote_oob"] == 0
+    ro = O.run(O.make_cfg(**opts), rs, threads=8, want_tally=True)
+    assert np.array_equal(tally, ro["tally"])
+    assert np.array_equal(out, ro["corrected"])
+    assert st["dp_cells_exec"] == ro["stats"]["dp_cells"]
+
+
+def test_call_beyond_band_limit_fails_and_the_context_recovers():
+    """With the reference's default indel rate (1.0) a 1,500 bp overlap asks for a band of about
+    3,000: the call fails with HERO_ERR_LIMIT, and the next call on the same context succeeds."""
+    long = synth.generate(seed=5, genome_len=6000, len_min=1500, coverage=4.0, sub_5p=0.01)
+    short = synth.generate(seed=6, genome_len=4000, len_min=100, coverage=10.0, sub_5p=0.01)
+    with gpu.Context(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=1.0, min_overlap=40,
+                     hash_factor=128) as ctx:
+        ctx.set_queries(long)
+        ctx.build_index()
+        with pytest.raises(gpu.HeroError) as e:
+            ctx.align_self()
+        assert e.value.code == -5
+        ctx.set_queries(short)
+        ctx.build_index()
+        ctx.align_self()
+        out = ctx.consensus()
+    ro = O.run(O.make_cfg(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=1.0, min_overlap=40,
+                          hash_factor=128), short, threads=8)
+    assert np.array_equal(out, ro["corrected"])

@@ -18,6 +18,68 @@ def test_band_align_batch_known_answers(aligner):
     assert not bad, bad[:3]
 
 
+FORMS = [(1, 32), (2, 12), (3, 12)]  # band_align_form, widest band class it takes
+
+
+@pytest.mark.parametrize("form,bmax", FORMS, ids=["batched_general", "batched_tagged", "batched_pk16"])
+def test_timed_dp_kernels_on_known_answers(form, bmax):
+    """The pipeline's batched DP kernels (the ones the correction step times) on every affine
+    reference DP case they can take: A/C/G/T only, widened band within the kernel's classes."""
+    cases = [c for c in load_jsonl("dp_cases.jsonl") if c["aligner"] == 0 and "N" not in c["S"] + c["Q"]
+             and max(c["band"], abs(len(c["S"]) - len(c["Q"]))) <= bmax]
+    assert len(cases) > 1000
+    with gpu.Context(aligner=0, band_align_form=form) as ctx:
+        res = ctx.band_align([(c["S"], c["Q"], c["band"], c["flag"]) for c in cases])
+    bad = [(c, r) for c, r in zip(cases, res) if r != (c["cigar"], c["out_band"])]
+    assert not bad, bad[:3]
+
+
+def _long_pairs(rng, n, len_lo, len_hi, bands):
+    pairs = []
+    for t in range(n):
+        L = int(rng.integers(len_lo, len_hi))
+        base = "".join(rng.choice(list("ACGT"), L))
+        other = list(base)
+        for _ in range(int(L * 0.03)):  # substitutions, then a few 1-3 bp indels
+            p = int(rng.integers(0, len(other)))
+            other[p] = str(rng.choice(list("ACGT")))
+        for _ in range(int(rng.integers(0, 6))):
+            p = int(rng.integers(0, len(other)))
+            if rng.random() < 0.5:
+                del other[p:p + int(rng.integers(1, 4))]
+            else:
+                other[p:p] = list(rng.choice(list("ACGT"), int(rng.integers(1, 4))))
+        other = "".join(other) or "A"
+        if t % 5 == 4:
+            other = other[: len(other) // 3]  # a much shorter side: the band widens to the length gap
+        band = int(rng.choice(bands))
+        S, Q = (base, other) if t % 2 else (other, base)
+        pairs.append((S, Q, band, int(rng.integers(0, 3))))
+    return pairs
+
+
+@pytest.mark.parametrize("aligner", [1, 0], ids=["global", "affine"])
+def test_band_align_long_calls_match_oracle(aligner):
+    """Calls beyond 511 rows and bands beyond 64 (up to the 1023 limit) through the plain form."""
+    rng = np.random.default_rng(7 + aligner)
+    pairs = _long_pairs(rng, 40, 500, 2500, [4, 16, 40, 70, 200, 400])
+    pairs = [p for p in pairs if max(p[2], abs(len(p[0]) - len(p[1]))) <= 1023]
+    with gpu.Context(aligner=aligner) as ctx:
+        got = ctx.band_align(pairs)
+    cfg = O.make_cfg(aligner=aligner)
+    for (S, Q, b, f), g in zip(pairs, got):
+        cig, ob, _ = O.band_align(cfg, S, Q, b, f)
+        assert g == (cig, ob), (len(S), len(Q), b, f)
+
+
+def test_band_align_rejects_band_above_limit():
+    with gpu.Context() as ctx:
+        with pytest.raises(gpu.HeroError) as e:
+            ctx.band_align([("A" * 1100, "A", 4, 0)])
+        assert e.value.code == -5  # HERO_ERR_LIMIT: the widened band is 1099
+        assert ctx.band_align([("ACGT", "ACGT", 2, 0)]) == [("4M", 2)]
+
+
 def test_hash_kmers_known_answers():
     cases = load_jsonl("hash_cases.jsonl")
     with gpu.Context() as ctx:
