@@ -298,6 +298,36 @@ struct hero_ctx {
 
 namespace {
 
+// Device scratch of hero_band_align_batch_noalloc: a fixed array in every device's copy of the
+// code object, so that entry point allocates nothing. The first kNoAllocCtlBytes hold the plain
+// form's work counters (cleared by a kernel on each call); the rest holds the traceback pointers
+// of calls that do not fit in LDS. Host code uses the per-device address from
+// hipGetSymbolAddress (noalloc_base), never the array's name.
+constexpr size_t kNoAllocScratchBytes = (size_t) 64 << 20;
+constexpr size_t kNoAllocCtlBytes = 256;
+static __device__ __align__(256) uint8_t noalloc_scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex noalloc_mu;
+uint8_t *noalloc_base[kNoAllocMaxDevices] = {};
+
+// Looks up this device's address of noalloc_scratch once; the device must be current.
+hipError_t noalloc_setup(int device) {
+	if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lk(noalloc_mu);
+	if (noalloc_base[device]) return hipSuccess;
+	void *ptr = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+	if (e != hipSuccess) return e;
+	noalloc_base[device] = (uint8_t *) ptr;
+	return hipSuccess;
+}
+
+uint8_t *noalloc_scratch_of(int device) {
+	if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+	std::lock_guard<std::mutex> lk(noalloc_mu);
+	return noalloc_base[device];
+}
+
 // Largest per-chunk window-bucket buffer (8 B per subject window and strand); a chunk beyond it
 // probes the index a second time in pass A1 instead.
 constexpr uint64_t kWinInfoBytes = 1ull << 30;
@@ -1393,6 +1423,10 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 		return HERO_ERR_HIP;
 	}
 	c->cu_count = prop.multiProcessorCount;
+	if (hipError_t e = noalloc_setup(opt->device); e != hipSuccess) {
+		delete c;
+		return HERO_ERR_HIP;
+	}
 	c->w.acc = &c->st_host;
 	if (hipError_t e = c->w.init(); e != hipSuccess) {
 		c->w.release();
@@ -1657,6 +1691,7 @@ int hero_band_align_batch(hero_ctx *c, int32_t n, const hero_band_pair *pairs, c
 		const int lenS = std::max(p.s_len, p.q_len), lenQ = std::min(p.s_len, p.q_len);
 		const int b = std::max(p.band, lenS - lenQ);
 		if (b > kWaveMaxBand) return fail(c, HERO_ERR_LIMIT, "pair %d: band %d above %d", i, b, kWaveMaxBand);
+		if (lenS > kMaxReadLen) return fail(c, HERO_ERR_LIMIT, "pair %d: %d bases (limit %d)", i, lenS, kMaxReadLen);
 		max_len = std::max(max_len, lenS), max_band = std::max(max_band, b);
 		off[(size_t) i] = total;
 		total += (int64_t) p.s_len + p.q_len;
@@ -1717,6 +1752,45 @@ int hero_band_align_batch(hero_ctx *c, int32_t n, const hero_band_pair *pairs, c
 		if (cigar_counts) std::memcpy(cigar_counts + pos, hcnt.data() + off[(size_t) i], (size_t) k * 4);
 		if (cigar_types) std::memcpy(cigar_types + pos, htyp.data() + off[(size_t) i], (size_t) k);
 		pos += k;
+	}
+	return HERO_OK;
+}
+
+int hero_band_align_batch_noalloc(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const char *seq, int32_t max_len,
+		int32_t max_band, int32_t *band_out, const int64_t *cigar_off, int32_t *cigar_n, int32_t *cigar_counts,
+		char *cigar_types, void *stream) {
+	if (!c || n < 0) return HERO_ERR_ARG;
+	if (n > 0 && (!pairs || !seq || !band_out || !cigar_off || !cigar_n || !cigar_counts || !cigar_types))
+		return fail(c, HERO_ERR_ARG, "null device array");
+	if (max_len < 1 || max_len > kMaxReadLen) return fail(c, HERO_ERR_LIMIT, "max_len %d outside 1..%d", max_len, kMaxReadLen);
+	if (max_band < 0 || max_band > kWaveMaxBand)
+		return fail(c, HERO_ERR_LIMIT, "max_band %d outside 0..%d", max_band, kWaveMaxBand);
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	uint8_t *base = noalloc_scratch_of(c->opt.device);
+	if (!base) return fail(c, HERO_ERR_STATE, "no scratch address for device %d", c->opt.device);
+	if (n == 0) return HERO_OK;
+	WaveSizing z = wave_sizing(max_len, max_band, (int) std::min<int64_t>(n, (int64_t) c->cu_count * 16));
+	const size_t avail_words = (kNoAllocScratchBytes - kNoAllocCtlBytes) / 4;
+	if (z.scratch_words) {
+		if (z.scratch_words > avail_words)
+			return fail(c, HERO_ERR_LIMIT,
+					"calls of up to %d bases with bands up to %d need %zu bytes of scratch per wavefront; the fixed scratch holds %zu",
+					max_len, max_band, z.scratch_words * 4, avail_words * 4);
+		z.nwaves = (int) std::min<size_t>((size_t) z.nwaves, avail_words / z.scratch_words);
+		z.scratch = (uint32_t *) (base + kNoAllocCtlBytes);
+	}
+	hipStream_t st = stream ? (hipStream_t) stream : c->w.st;
+	uint32_t *ctl = (uint32_t *) base;
+	HIPCHK(c, launch_fill_u32(ctl, kWaveCtlWords, 0u, st)); // a kernel, not a memset: capturable
+	WavePairs P;
+	std::memset(&P, 0, sizeof P);
+	P.pairs = pairs, P.n = n, P.max_len = max_len, P.max_band = max_band, P.seq = seq;
+	P.band_out = band_out, P.cigar_off = cigar_off, P.cigar_n = cigar_n, P.cigar_counts = cigar_counts;
+	P.cigar_types = cigar_types;
+	for (int l = 0; l < kWaveLaunches; l++) {
+		if (l > 0 && max_band <= wave_band_hi(l - 1)) break;
+		P.counter = ctl + kWaveCtlCounter + l;
+		HIPCHK(c, launch_dp_wave_pairs(P, c->o, z, l, st));
 	}
 	return HERO_OK;
 }

@@ -107,6 +107,8 @@ SIGNATURES = {
     "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "hero_reset_stats": (ctypes.c_int, [_P]),
     "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
+    "hero_band_align_batch_noalloc": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
+                                                     _P, _P, _P, _P]),
     "hero_hash_kmers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_uint64, _P]),
     "hero_lookup": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_int64]),
     "hero_autotune": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(TuneReport)]),

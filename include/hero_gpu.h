@@ -147,6 +147,21 @@ typedef struct {
 int hero_band_align_batch(hero_ctx *ctx, int32_t n, const hero_band_pair *pairs, const char *seq, int32_t *band_out,
 		int64_t *cigar_off, int32_t *cigar_n, int32_t *cigar_counts, char *cigar_types, int64_t cigar_cap);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): hero_band_align_batch on
+ * device-resident arrays, stream-ordered on `stream` (a hipStream_t; null = the context's stream),
+ * allocating nothing and never synchronizing. Every pointer is device memory: pairs, seq (bytes
+ * A/C/G/T/N), band_out, cigar_n, and cigar_counts / cigar_types, where call i's ops go at
+ * cigar_off[i] (an input: the caller reserves s_len + q_len ops there). max_len bounds both sides
+ * of every call (<= 16383) and max_band its widened band (<= 1023). cigar_n[i] < 0 reports a call
+ * the blocking entry rejects: HERO_ERR_REF_ABORT (negative start), HERO_ERR_ARG (both sides
+ * empty), HERO_ERR_LIMIT (beyond max_len / max_band). Work counters and the traceback pointers of
+ * calls too large for LDS live in one fixed 64 MiB device array per GPU (HERO_ERR_LIMIT when calls
+ * of max_len x max_band need more); calls that overlap on one GPU share it and race. */
+int hero_band_align_batch_noalloc(hero_ctx *ctx, int32_t n, const hero_band_pair *pairs, const char *seq,
+		int32_t max_len, int32_t max_band, int32_t *band_out, const int64_t *cigar_off, int32_t *cigar_n,
+		int32_t *cigar_counts, char *cigar_types, void *stream);
+
 /* HashTableLongKey::hashFunction (src/HashTableLongKey.cpp:188-206) on the device for n k-mers
  * of length k packed back to back in `kmers` (A/C/G/T). */
 int hero_hash_kmers(hero_ctx *ctx, int32_t n, int32_t k, const char *kmers, uint64_t table_size, uint64_t *out);
