@@ -19,6 +19,7 @@ Prints one JSON line (see the keys below). Extra measurements:
   cpu_baseline     the reference program (oracle/_ref/hero_ref) or the CPU oracle on host cores
 """
 import argparse
+import hashlib
 import json
 import os
 import re
@@ -48,6 +49,9 @@ C4_RESIDENT = 8_000_000
 C4_POOL = 262_144
 C4_READS = dict(seed=404, genome_len=20_000_000, len_min=150, n_reads=C4_RESIDENT + C4_POOL, sub_5p=0.005,
                 sub_3p=0.02, indel_rate=0.001)
+# the same reads at the reference program's own flags (-c -k 39 -m 4 -i 2.67 -l 40 -a a: open -3 / extend -1,
+# band 2*round(0.0267*overlap), no candidate cap): the work its CPU leg times
+C2_REF_OPTS = dict(call_consensus=1, k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267, min_overlap=40)
 C4_OPTS = dict(C2_OPTS, max_candidates=0)  # a per-subject candidate cap would depend on the batch mix
 C4_RATES = (5, 20, 50, 100, 200)
 C3_READS = dict(seed=303, genome_len=1_000_000, len_min=75, len_max=250, coverage=40.0, sub_5p=0.01)
@@ -56,6 +60,12 @@ C3_OPTS = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def check_clean(st, what):
+    """Every DP call ran and every vote landed in its read's tally."""
+    if st["vote_oob"] or st["dp_skipped"]:
+        raise SystemExit(f"{what}: vote_oob={st['vote_oob']} dp_skipped={st['dp_skipped']}")
 
 
 def run_steps(ctx, steps):
@@ -67,7 +77,55 @@ def run_steps(ctx, steps):
         ctx.align_self()
         ctx.consensus(device_only=True)
     t1 = time.perf_counter()
-    return (t1 - t0) * 1000.0 / steps, ctx.stats()
+    st = ctx.stats()
+    check_clean(st, "timed steps")
+    return (t1 - t0) * 1000.0 / steps, st
+
+
+def host_to_host(rs, opts, steps):
+    """Reads/s from ASCII reads in host memory to corrected ASCII bases in host memory: each step
+    packs and uploads the reads (hero_set_queries), builds the index, aligns every subject and
+    copies the consensus back (hero_call_consensus into a host buffer). After one untimed step."""
+    out = np.empty_like(rs.seq)
+    phases = []
+    with gpu.Context(**opts) as ctx:
+        for i in range(steps + 1):
+            t = [time.perf_counter()]
+            ctx.set_queries(rs)
+            t.append(time.perf_counter())
+            ctx.build_index()
+            t.append(time.perf_counter())
+            ctx.align_self()
+            t.append(time.perf_counter())
+            ctx.consensus(out=out)
+            t.append(time.perf_counter())
+            if i:
+                phases.append(np.diff(np.array(t)) * 1000.0)
+        check_clean(ctx.stats(), "host-to-host steps")
+    ph = np.mean(np.array(phases), axis=0)
+    ms = float(ph.sum())
+    return dict(reads_per_s=round(rs.n / (ms / 1000.0), 1), ms_per_step=round(ms, 3), steps=steps,
+                phases_ms=dict(pack_upload=round(ph[0], 3), index=round(ph[1], 3), align=round(ph[2], 3),
+                               consensus_copy_back=round(ph[3], 3)),
+                note="ASCII reads in host memory to corrected ASCII in host memory, wall clock per step")
+
+
+def sample_parity(rs):
+    """The timed workload's knobs on its C2 sample (all reads as queries, the first 49,999 as
+    subjects) against the committed oracle tally (tests/golden/scale.json)."""
+    path = os.path.join(ROOT, "tests", "golden", "scale.json")
+    case = {c["name"]: c for c in json.load(open(path))}["c2_sample_knobs"]
+    with gpu.Context(**case["options"]) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        ctx.align_subjects(rs.subset(np.arange(case["n_subjects"])))
+        tally = ctx.tally()
+        st = ctx.stats()
+    sha = hashlib.sha256(tally.astype("<u4").tobytes()).hexdigest()
+    ok = sha == case["tally_sha256"] and st["dp_cells_exec"] == case["dp_cells"]
+    return dict(sample=f"all {rs.n} reads indexed, first {case['n_subjects']} subjects, workload knobs",
+                tally_sha256_equal=sha == case["tally_sha256"], dp_cells_equal=st["dp_cells_exec"] == case["dp_cells"],
+                result="bit-exact" if ok else "MISMATCH")
 
 
 def mapping_run(rs, steps=2):
@@ -363,11 +421,14 @@ def main():
         f"{s1['ms_seed']:.1f}, plan {s1['ms_plan']:.1f}, dp {dp_ms:.1f}, score+votes {s1['ms_score_vote']:.1f}, "
         f"consensus {s1['ms_consensus']:.1f} ms; {gcups_batched:.1f} GCUPS")
 
-    achieved = cells_all * AFFINE_OPS_PER_CELL / (dp_ms / 1000.0) / 1e12
+    cells_exec = s1["dp_cells_exec"]
+    achieved = cells_exec * AFFINE_OPS_PER_CELL / (dp_ms / 1000.0) / 1e12
     roofline = dict(bound="valu", kernel="k_dp_batched (all band classes)", achieved=round(achieved, 3),
                     peak=round(VALU_PEAK_TOPS, 2), unit="Tops/s (int32 VALU)", frac=round(achieved / VALU_PEAK_TOPS, 4),
                     traffic=measured_traffic("hero::k_dp_batched<8, true, true>"), traffic_unit="HBM bytes per launch",
-                    ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_all), dp_ms_per_step=round(dp_ms, 3))
+                    ops_per_cell=AFFINE_OPS_PER_CELL, cells_per_step=int(cells_exec),
+                    cells_computed_per_step=int(cells_all), dp_ms_per_step=round(dp_ms, 3),
+                    note="cells of the DP calls the reference's control flow executes")
     # seed probes: packed subject bases + one 32 B slot per canonical probe (a forward window and
     # its reverse-strand partner share one; "lookups" counts the reference's per-strand lookups)
     # + 8 B per bucket entry read
@@ -400,6 +461,19 @@ def main():
         mapping = mapping_run(rs)
         log("mapping:", json.dumps(mapping))
 
+    h2h = host_to_host(rs, C2_OPTS, 2)
+    log("host to host:", json.dumps(h2h))
+    with gpu.Context(**C2_REF_OPTS) as rctx:
+        rctx.set_queries(rs)
+        run_steps(rctx, 1)
+        ref_ms, ref_st = run_steps(rctx, 2)
+    ref_flags = dict(reads_per_s=round(rs.n / (ref_ms / 1000.0), 1), ms_per_step=round(ref_ms, 3),
+                     dp_cells_per_step=ref_st["dp_cells_exec"] // 2,
+                     flags="-c -k 39 -m 4 -i 2.67 -l 40 -a a (open -3 / extend -1, no candidate cap)")
+    log("reference flags:", json.dumps(ref_flags))
+    parity = sample_parity(rs)
+    log("parity:", json.dumps(parity))
+
     latency = None
     if not args.no_serve:
         prepared = prepare_serving()
@@ -422,6 +496,8 @@ def main():
                 config=dict(workload=("c2: 5 Mb genome, 150 bp, 50x, 0.5->2% substitutions, 0.1% indels, k=39, affine "
                                       "+1/-4/-6/-1, band 8, cap 64 candidates, self-correction"),
                             reads=rs.n, bases=bases, parallelism="1 GPU"),
+                value_note="device-resident: packed reads in HBM at the start of each step, corrected reads left in HBM",
+                host_to_host=h2h, reference_flags=ref_flags, parity=parity,
                 pcie_inclusive=dict(copy_ms=round(copy_ms, 3), reads_per_s=round(rs.n / ((ms + copy_ms) / 1000.0), 1)),
                 gcups=gcups, autotune=tune_c2, latency=latency, mapping=mapping, roofline=roofline,
                 roofline_probe=roofline_probe, cpu_baseline=cpu,

@@ -254,6 +254,8 @@ struct hero_ctx {
 	std::vector<int32_t> q_len;
 	std::vector<uint8_t> q_good, q_active_as_subject;
 	std::vector<int64_t> q_not_good, q_uid;
+	std::vector<int64_t> q_keep_off; // without a full copy: where each not-good read's bytes sit in q_seq
+	bool q_full_copy = false;        // q_seq holds every byte of the caller's layout
 	bool q_contiguous = true; // caller offsets are the back-to-back prefix sums of the lengths
 	int64_t q_bases = 0;
 	DevBuf<uint8_t> cons_out;
@@ -292,6 +294,7 @@ struct hero_ctx {
 	DevBuf<int64_t> ba_off;
 	DevBuf<int32_t> ba_band, ba_n, ba_counts;
 	DevBuf<uint32_t> ba_scratch;
+	HostBuf<uint64_t> stage; // pinned staging of packed reads and their offsets (upload_reads)
 	// stats
 	hero_stats st_host;
 };
@@ -377,22 +380,32 @@ int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int
 }
 int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
 		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good) {
-	std::vector<uint64_t> words, woff;
-	pack_reads(seq, off, len, n, words, woff);
-	std::vector<uint64_t> boff((size_t) n);
-	uint64_t acc = 0;
-	for (int64_t i = 0; i < n; i++) boff[(size_t) i] = acc, acc += (uint64_t) len[i];
+	// word and base offsets, then the 2-bit words packed by every host thread straight into pinned
+	// staging ([words | woff | boff]), so the upload runs at the link's rate
+	std::vector<uint64_t> woff((size_t) n + 1), boff((size_t) n + 1);
+	uint64_t nw = 0, nb = 0;
+	for (int64_t i = 0; i < n; i++) {
+		woff[(size_t) i] = nw, boff[(size_t) i] = nb;
+		nw += (uint64_t) (len[i] + 31) / 32, nb += (uint64_t) len[i];
+	}
+	woff[(size_t) n] = nw, boff[(size_t) n] = nb;
+	const size_t words = (size_t) nw + 2; // two zero words of tail padding
+	HIPCHK(c, c->stage.ensure(words + 2 * ((size_t) n + 1)));
+	uint64_t *hw = c->stage.p, *hwoff = hw + words, *hboff = hwoff + (n + 1);
+	pack_reads_into(seq, off, len, n, woff.data(), hw, words);
+	std::memcpy(hwoff, woff.data(), ((size_t) n + 1) * 8);
+	std::memcpy(hboff, boff.data(), ((size_t) n + 1) * 8);
 	// grow-only buffers: a steady stream of batches allocates nothing after the largest one
-	HIPCHK(c, d.words.ensure(words.size()));
+	HIPCHK(c, d.words.ensure(words));
 	HIPCHK(c, d.woff.ensure((size_t) n + 1));
 	HIPCHK(c, d.boff.ensure((size_t) n + 1));
 	HIPCHK(c, d.len.ensure((size_t) n + 1));
 	HIPCHK(c, d.uid.ensure((size_t) n + 1));
 	HIPCHK(c, d.good.ensure((size_t) n + 1));
-	HIPCHK(c, hipMemcpyAsync(d.words.p, words.data(), words.size() * 8, hipMemcpyHostToDevice, st));
+	HIPCHK(c, hipMemcpyAsync(d.words.p, hw, words * 8, hipMemcpyHostToDevice, st));
 	if (n) {
-		HIPCHK(c, hipMemcpyAsync(d.woff.p, woff.data(), (size_t) n * 8, hipMemcpyHostToDevice, st));
-		HIPCHK(c, hipMemcpyAsync(d.boff.p, boff.data(), (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.woff.p, hwoff, (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.boff.p, hboff, (size_t) n * 8, hipMemcpyHostToDevice, st));
 		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, st));
 		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, st));
 		HIPCHK(c, hipMemcpyAsync(d.good.p, good.data(), (size_t) n, hipMemcpyHostToDevice, st));
@@ -1446,6 +1459,7 @@ void hero_destroy(hero_ctx *c) {
 	c->Rd.release(), c->six.release();
 	c->ba_pairs.release(), c->ba_seq.release(), c->ba_types.release(), c->ba_off.release();
 	c->ba_band.release(), c->ba_n.release(), c->ba_counts.release(), c->ba_scratch.release();
+	c->stage.release();
 	c->w.release();
 	if (c->w2_ready) c->w2.release();
 	delete c;
@@ -1460,30 +1474,50 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	c->q_off.assign(off, off + n);
 	c->q_len.assign(len, len + n);
 	c->q_uid.assign(uid, uid + n);
-	int64_t bases = 0;
-	for (int64_t i = 0; i < n; i++) bases = std::max<int64_t>(bases, off[i] + len[i]);
-	c->q_seq.assign(seq, seq + bases);
 	c->q_good.resize((size_t) n);
 	c->q_active_as_subject.resize((size_t) n);
 	c->q_not_good.clear();
 	c->q_contiguous = true;
-	int64_t expect = 0;
+	int64_t expect = 0, qb = 0;
 	for (int64_t i = 0; i < n; i++) {
-		c->q_good[(size_t) i] = query_good(seq + off[i], len[i], c->o.min_overlap);
-		c->q_active_as_subject[(size_t) i] = subject_active(seq + off[i], len[i], c->o.min_overlap, c->o.k);
-		if (!c->q_good[(size_t) i]) c->q_not_good.push_back(i);
 		if (off[i] != expect) c->q_contiguous = false;
 		expect = off[i] + len[i];
+		qb += len[i];
 	}
+	const int64_t min_ov = c->o.min_overlap;
+	const int k = c->o.k;
+#pragma omp parallel for schedule(static, 4096)
+	for (int64_t i = 0; i < n; i++) {
+		c->q_good[(size_t) i] = query_good(seq + off[i], len[i], min_ov);
+		c->q_active_as_subject[(size_t) i] = subject_active(seq + off[i], len[i], min_ov, k);
+	}
+	// reads that are not corrected go out as they came in: keep their bytes (all bytes when the
+	// layout has gaps, which the output keeps too)
+	c->q_seq.clear();
+	c->q_keep_off.clear();
+	// record mode writes the queries' inserted bases: it keeps every byte, like a gapped layout
+	c->q_full_copy = !c->q_contiguous || c->opt.records;
+	if (c->q_full_copy) {
+		int64_t bases = 0;
+		for (int64_t i = 0; i < n; i++) bases = std::max<int64_t>(bases, off[i] + len[i]);
+		c->q_seq.assign(seq, seq + bases);
+	}
+	for (int64_t i = 0; i < n; i++)
+		if (!c->q_good[(size_t) i]) {
+			c->q_not_good.push_back(i);
+			if (!c->q_full_copy) {
+				c->q_keep_off.push_back((int64_t) c->q_seq.size());
+				c->q_seq.insert(c->q_seq.end(), seq + off[i], seq + off[i] + len[i]);
+			}
+		}
 	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good)) return r;
-	HIPCHK(c, c->q_subject_good.exact((size_t) n));
-	if (n) HIPCHK(c, hipMemcpy(c->q_subject_good.p, c->q_active_as_subject.data(), (size_t) n, hipMemcpyHostToDevice));
-	int64_t qb = 0;
-	for (int64_t i = 0; i < n; i++) qb += len[i];
+	HIPCHK(c, c->q_subject_good.ensure((size_t) n + 1));
+	if (n) HIPCHK(c, hipMemcpyAsync(c->q_subject_good.p, c->q_active_as_subject.data(), (size_t) n, hipMemcpyHostToDevice, c->w.st));
 	c->q_bases = qb;
 	// 4 mismatch counters per base, then the matched-run difference array (1 per base)
-	HIPCHK(c, c->tally.exact((size_t) 5 * qb));
-	HIPCHK(c, hipMemset(c->tally.p, 0, (size_t) 20 * qb));
+	if (c->tally.n < (size_t) 5 * qb + 1 || !c->tally.p) HIPCHK(c, c->tally.exact((size_t) 5 * qb + 1)); // no slack: 20 B per base
+	HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * qb, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	c->qix.built = false;
 	c->cons_valid = false;
 	c->st_host.queries = (uint64_t) n;
@@ -1565,8 +1599,11 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 	if (c->q_contiguous) {
 		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
 		HIPCHK(c, hipStreamSynchronize(c->w.st));
-		for (int64_t i : c->q_not_good)
-			std::memcpy(corrected + c->q_off[(size_t) i], c->q_seq.data() + c->q_off[(size_t) i], (size_t) c->q_len[(size_t) i]);
+		for (size_t t = 0; t < c->q_not_good.size(); t++) {
+			const int64_t i = c->q_not_good[t];
+			const char *src = c->q_seq.data() + (c->q_full_copy ? c->q_off[(size_t) i] : c->q_keep_off[t]);
+			std::memcpy(corrected + c->q_off[(size_t) i], src, (size_t) c->q_len[(size_t) i]);
+		}
 	} else {
 		std::vector<char> h((size_t) c->q_bases);
 		HIPCHK(c, hipMemcpyAsync(h.data(), c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));

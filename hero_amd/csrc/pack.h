@@ -54,4 +54,23 @@ inline void pack_reads(const char *seq, const int64_t *off, const int32_t *len, 
 	}
 }
 
+// pack_reads into a caller buffer of `words` u64 (woff from the caller; tail words cleared).
+inline void pack_reads_into(const char *seq, const int64_t *off, const int32_t *len, int64_t n, const uint64_t *woff,
+		uint64_t *out, size_t words) {
+	const size_t used = n ? (size_t) (woff[n - 1] + (uint64_t) ((len[n - 1] + 31) / 32)) : 0;
+	for (size_t i = used; i < words; i++) out[i] = 0;
+#pragma omp parallel for schedule(static, 4096)
+	for (int64_t r = 0; r < n; r++) {
+		const char *s = seq + off[r];
+		uint64_t *w = out + woff[r];
+		const int L = len[r];
+		for (int i = 0; i < L; i += 32) {
+			uint64_t x = 0;
+			const int m = L - i < 32 ? L - i : 32;
+			for (int t = 0; t < m; t++) x |= (uint64_t) ascii_code(s[i + t]) << (62 - 2 * t);
+			w[i >> 5] = x;
+		}
+	}
+}
+
 } // namespace hero

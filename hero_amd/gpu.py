@@ -221,17 +221,22 @@ class Context:
                                              rs.length.ctypes.data, rs.unique_id.ctypes.data, out.ctypes.data))
         return out
 
-    def consensus(self, device_only=False):
-        """Corrected reads (bytes in the layout of the query seq); device_only keeps them on the GPU."""
+    def consensus(self, device_only=False, out=None):
+        """Corrected reads (bytes in the layout of the query seq, into `out` when given);
+        device_only keeps them on the GPU."""
         if device_only:
             self._check(lib().hero_call_consensus(self.h, None))
             return None
-        out = np.empty_like(self._queries.seq)
+        if out is None:
+            out = np.empty_like(self._queries.seq)
+        assert out.dtype == np.uint8 and out.nbytes >= self._queries.seq.nbytes and out.flags.c_contiguous
         self._check(lib().hero_call_consensus(self.h, out.ctypes.data))
         return out
 
-    def copy_corrected(self):
-        out = np.empty_like(self._queries.seq)
+    def copy_corrected(self, out=None):
+        if out is None:
+            out = np.empty_like(self._queries.seq)
+        assert out.dtype == np.uint8 and out.nbytes >= self._queries.seq.nbytes and out.flags.c_contiguous
         self._check(lib().hero_copy_corrected(self.h, out.ctypes.data))
         return out
 
