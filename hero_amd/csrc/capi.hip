@@ -19,6 +19,7 @@
 #include "dp_pk16.h"
 #include "kernels.h"
 #include "pack.h"
+#include "prime_table.h"
 #include "pipeline_core.h"
 
 using namespace hero;
@@ -274,6 +275,11 @@ struct hero_ctx {
 	IndexBufs six;
 	// tallies: 4 u32 per query base (own-base slot difference coded, see VoteSink)
 	DevBuf<uint32_t> tally;
+	DevBuf<uint32_t> ext; // --merge: kExtPositions x 4 u32 per query read
+	// record mode: the accepted records since the last hero_take_records
+	std::vector<hero_record> rec_out;
+	std::vector<uint32_t> rec_out_ops;
+	int64_t rec_subjects = 0; // subjects aligned since the last hero_take_records
 	// record mode: .aln text of the subjects aligned since the last hero_take_alignments
 	std::string aln;
 	// the context's own pipeline (stream, events, chunk buffers)
@@ -363,6 +369,7 @@ Opts make_opts(const hero_options &p) {
 	o.mismatch_rate = p.mismatch_rate, o.indel_rate = p.indel_rate, o.consensus_freq = p.consensus_freq;
 	o.dx = p.dx, o.dy = p.dy, o.ex = p.ex, o.sx = p.sx, o.miss = p.miss, o.hit = p.hit, o.unobserved = p.unobserved;
 	o.mapping = p.mapping;
+	o.long_format = p.long_format, o.merge = p.merge;
 	return o;
 }
 
@@ -809,7 +816,8 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	if (c->tune) return HERO_OK; // autotuning times the DP only: no records, no votes
 	HIPCHK(c, launch_pass_b(S, Q, c->o, cd, stats, w->st));
 	if (vote_mode && c->o.mapping) HIPCHK(c, launch_map_select(S, cd, w->st)); // SubjectReadAligner.cpp:558-560
-	if (vote_mode) HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, diff, stats, w->st));
+	if (vote_mode)
+		HIPCHK(c, launch_votes(S, Q, c->o, cd, (uint32_t) nvotes, q_boff, tally, diff, c->o.merge ? c->ext.p : nullptr, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[4].a, w->st));
 	HIPCHK(c, hipStreamSynchronize(w->st));
 	w->acc->ms_seed += elapsed(w, 0, 1);
@@ -865,6 +873,7 @@ int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64
 			if (h[i].qid != kNoRecord) order[cur[(size_t) h[i].sub]++] = i;
 	}
 	std::vector<std::string> text((size_t) nsub);
+	std::vector<std::vector<uint32_t>> kept_of((size_t) nsub);
 #pragma omp parallel for schedule(dynamic, 256)
 	for (int32_t j = 0; j < nsub; j++) {
 		std::string &t = text[(size_t) j];
@@ -872,8 +881,14 @@ int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64
 		t += '>';
 		t += std::to_string((long long) s_uid[s0 + j]);
 		t += '\n';
-		// addAlignment: keep one record per query id; lower rate sum wins, then the longer
+		// addAlignment: keep one record per query id; lower rate sum wins, then the longer. Under -v
+		// every accepted record is kept in processing order (src/SubjectRead.cpp:26-29).
 		std::vector<uint32_t> kept;
+		if (c->o.long_format) {
+			for (uint32_t x = first[(size_t) j]; x < first[(size_t) j + 1]; x++) kept.push_back(order[x]);
+			kept_of[(size_t) j] = kept;
+			continue;
+		}
 		uint64_t largest = 0;
 		auto id_of = [&](uint32_t i) { return (uint64_t) c->q_uid[h[i].qid]; };
 		auto keep_better = [&](uint32_t &slot, uint32_t r) {
@@ -906,6 +921,7 @@ int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64
 				kept.push_back(r);
 			}
 		}
+		kept_of[(size_t) j] = kept;
 		for (uint32_t r : kept) {
 			const RecHead &x = h[r];
 			const int n = (int) opn[r];
@@ -939,7 +955,20 @@ int format_records(hero_ctx *c, Worker *w, int64_t s0, int32_t nsub, const int64
 		}
 		t += '\n';
 	}
-	for (const std::string &t : text) c->aln += t;
+	if (!c->o.long_format)
+		for (const std::string &t : text) c->aln += t;
+	// the same records as data (hero_take_records)
+	for (int32_t j = 0; j < nsub; j++) {
+		for (uint32_t r : kept_of[(size_t) j]) {
+			hero_record rec;
+			std::memset(&rec, 0, sizeof rec);
+			rec.subject = c->rec_subjects + j, rec.query = h[r].qid, rec.strand = h[r].strand, rec.position = pos[r];
+			rec.op_off = (int64_t) c->rec_out_ops.size(), rec.op_n = (int32_t) opn[r];
+			c->rec_out_ops.insert(c->rec_out_ops.end(), ops.begin() + opoff[r], ops.begin() + opoff[r] + opn[r]);
+			c->rec_out.push_back(rec);
+		}
+	}
+	c->rec_subjects += nsub;
 	return HERO_OK;
 }
 
@@ -1406,6 +1435,8 @@ int steps_to_cigars(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const c
 
 extern "C" {
 
+uint64_t hero_prime_larger_than(uint64_t n) { return ref_prime_larger_than(n); }
+
 void hero_default_options(hero_options *p) {
 	std::memset(p, 0, sizeof *p);
 	p->k = 39, p->aligner = 1, p->call_consensus = 0, p->hash_factor = 32, p->min_overlap = 40;
@@ -1422,6 +1453,8 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 	if (opt->records && opt->call_consensus) return HERO_ERR_ARG; // the reference writes no .aln under -c
 	if (opt->min_overlap < 0 || opt->min_overlap > 65535) return HERO_ERR_ARG;
 	if (opt->hash_factor < 1) return HERO_ERR_ARG;
+	if (opt->long_format && opt->call_consensus) return HERO_ERR_ARG; // -v is built for record mode (.aln)
+	if (opt->merge && (!opt->call_consensus || opt->mapping)) return HERO_ERR_ARG; // --merge acts on -c votes
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt->device || opt->device < 0) return HERO_ERR_HIP;
 	if (hipSetDevice(opt->device) != hipSuccess) return HERO_ERR_HIP;
@@ -1455,7 +1488,7 @@ void hero_destroy(hero_ctx *c) {
 	(void) hipSetDevice(c->opt.device);
 	(void) hipStreamSynchronize(c->w.st);
 	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
-	c->qix.release(), c->tally.release(), c->cons_out.release();
+	c->qix.release(), c->tally.release(), c->ext.release(), c->cons_out.release();
 	c->Rd.release(), c->six.release();
 	c->ba_pairs.release(), c->ba_seq.release(), c->ba_types.release(), c->ba_off.release();
 	c->ba_band.release(), c->ba_n.release(), c->ba_counts.release(), c->ba_scratch.release();
@@ -1517,6 +1550,10 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	// 4 mismatch counters per base, then the matched-run difference array (1 per base)
 	if (c->tally.n < (size_t) 5 * qb + 1 || !c->tally.p) HIPCHK(c, c->tally.exact((size_t) 5 * qb + 1)); // no slack: 20 B per base
 	HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * qb, c->w.st));
+	if (c->opt.merge) {
+		HIPCHK(c, c->ext.ensure((size_t) kExtPositions * 4 * n + 1));
+		HIPCHK(c, hipMemsetAsync(c->ext.p, 0, (size_t) kExtPositions * 16 * n, c->w.st));
+	}
 	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	c->qix.built = false;
 	c->cons_valid = false;
@@ -1529,9 +1566,10 @@ int hero_build_index(hero_ctx *c) {
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	// a new index starts a new correction run: clear the vote tallies
 	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * c->q_bases, c->w.st));
+	if (c->opt.merge && c->Qd.n) HIPCHK(c, hipMemsetAsync(c->ext.p, 0, (size_t) kExtPositions * 16 * c->Qd.n, c->w.st));
 	// --mapping sizes the table by the bases' k-mers (src/HashTableLongKey.cpp:52-60)
 	const uint64_t units = c->opt.mapping ? (uint64_t) c->q_bases / (uint64_t) c->opt.k : (uint64_t) c->Qd.n;
-	const uint64_t P = next_prime_above(units * (uint64_t) c->opt.hash_factor);
+	const uint64_t P = ref_prime_larger_than(units * (uint64_t) c->opt.hash_factor); // :61
 	c->cons_valid = false;
 	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
 	c->st_host.table_size = c->qix.P;
@@ -1568,6 +1606,28 @@ int hero_take_alignments(hero_ctx *c, char *out, int64_t cap, int64_t *len) {
 	if (cap < *len) return fail(c, HERO_ERR_LIMIT, "alignment text is %lld bytes, buffer %lld", (long long) *len, (long long) cap);
 	std::memcpy(out, c->aln.data(), c->aln.size());
 	std::string().swap(c->aln);
+	return HERO_OK;
+}
+
+int hero_take_records(hero_ctx *c, hero_record *recs, int64_t rec_cap, int64_t *n_recs, uint32_t *ops, int64_t op_cap,
+		int64_t *n_ops) {
+	if (!c || !n_recs || !n_ops) return HERO_ERR_ARG;
+	*n_recs = (int64_t) c->rec_out.size(), *n_ops = (int64_t) c->rec_out_ops.size();
+	if (!recs || !ops) return HERO_OK;
+	if (rec_cap < *n_recs || op_cap < *n_ops) return fail(c, HERO_ERR_LIMIT, "record buffers too small");
+	if (*n_recs) std::memcpy(recs, c->rec_out.data(), (size_t) *n_recs * sizeof(hero_record));
+	if (*n_ops) std::memcpy(ops, c->rec_out_ops.data(), (size_t) *n_ops * 4);
+	c->rec_out.clear(), c->rec_out_ops.clear();
+	c->rec_subjects = 0;
+	return HERO_OK;
+}
+
+int hero_get_extension_tally(hero_ctx *c, uint32_t *out) {
+	if (!c || !out) return HERO_ERR_ARG;
+	if (!c->opt.merge) return fail(c, HERO_ERR_STATE, "the extension tally exists with merge = 1");
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (c->Qd.n)
+		HIPCHK(c, hipMemcpy(out, c->ext.p, (size_t) kExtPositions * 16 * c->Qd.n, hipMemcpyDeviceToHost));
 	return HERO_OK;
 }
 
@@ -1946,6 +2006,7 @@ extern "C" int hero_build_subject_index(hero_ctx *c) {
 extern "C" int hero_correct_batch(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
 		const int64_t *uid, char *corrected) {
 	if (!c || (n > 0 && !corrected)) return HERO_ERR_ARG;
+	if (c->o.merge) return fail(c, HERO_ERR_ARG, "serving does not take --merge");
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	return correct_batch_on(c, &c->w, n, seq, off, len, uid, corrected);
 }
@@ -2079,6 +2140,7 @@ extern "C" int hero_server_start(hero_ctx *c, const hero_server_options *opt, he
 	*out = nullptr;
 	if (!c->six.built) return fail(c, HERO_ERR_STATE, "hero_build_subject_index must run before serving");
 	if (!c->o.call_consensus) return fail(c, HERO_ERR_ARG, "serving corrects reads: call_consensus must be 1");
+	if (c->o.merge) return fail(c, HERO_ERR_ARG, "serving does not take --merge");
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	hero_server *sv = new hero_server();
 	sv->c = c;

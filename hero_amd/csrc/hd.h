@@ -25,6 +25,8 @@ struct Opts {
 	int64_t min_overlap;
 	double mismatch_rate, indel_rate, consensus_freq;
 	int32_t dx, dy, ex, sx, miss, hit, unobserved, _pad2;
+	int32_t long_format; // -v (Config::bSpeicialVersion): no self-hit skip (src/SubjectReadAligner.cpp:464-465)
+	int32_t merge;       // --merge: subject bases past a record's end vote into the query's extension tally
 };
 
 // 2-bit codes of the reference: (c >> 1) & 3 → A0 C1 T2 G3 (src/QueryRead.cpp:124-125).

@@ -190,8 +190,9 @@ hipError_t launch_pass_a2(const Packed &S, const Packed &Q, const Opts &o, const
 // (at c.sub_pair_off[subject] + rank); launch_votes then casts their votes.
 hipError_t launch_pass_b(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, unsigned long long *stats,
 		hipStream_t st);
+// ext (--merge, else null): kExtPositions x 4 u32 per query read, the extension tally
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st);
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, uint32_t *ext, unsigned long long *stats, hipStream_t st);
 // --mapping: per subject of the chunk, keep the votes of the best record covering it (see the kernel)
 hipError_t launch_map_select(const Packed &S, const ChunkDev &c, hipStream_t st);
 // --mapping output: hist[v] += bases of the nq reads with coverage v (bins 0 .. cap-1, the last one

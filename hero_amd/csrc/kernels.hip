@@ -260,7 +260,7 @@ __device__ __forceinline__ uint32_t bucket_seeds(const Packed &S, const Packed &
 	for (uint32_t e = start; e < start + count; e++) {
 		const uint64_t data = ix.entries[e];
 		const uint64_t qid = data & kIdMask;
-		if (entry_tag(data) == tag && Q.uid[qid] == uid) continue;
+		if (!o.long_format && entry_tag(data) == tag && Q.uid[qid] == uid) continue;
 		int pq = entry_pos(data), ps = w;
 		if (!o.call_consensus && strand == 1) ps = Ls - (ps + o.k), pq = Q.len[qid] - (pq + o.k);
 		emit(seed_pack(qid, pq, ps));
@@ -583,7 +583,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 // Votes of the kept records (SubjectRead::updateQueryCoverage, src/SubjectRead.cpp:208-330): one
 // lane per record; the record is replayed from its chain and DP results.
 __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkDev c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, uint32_t *ext, unsigned long long *stats) {
 	const uint32_t i = blockIdx.x * 64u + threadIdx.x;
 	if (i >= nvotes) return;
 	const VoteItem v = c.votes[i];
@@ -593,12 +593,17 @@ __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkD
 	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	const uint64_t b = q_boff[v.q];
 	VoteSink sink{tally + 4 * b, diff + b, pr, &stats[kStOob]};
-	vote_record(c.prog + v.script, R, v.task0, sink);
+	if (!ext) {
+		vote_record(c.prog + v.script, R, v.task0, sink);
+		return;
+	}
+	// --merge: the same votes, then the subject bases after the record's end into the extension tally
+	vote_record_merging(c.prog + v.script, R, v.task0, sink, ext + (size_t) kExtPositions * 4 * v.q);
 }
 hipError_t launch_votes(const Packed &S, const Packed &Q, const Opts &o, const ChunkDev &c, uint32_t nvotes,
-		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats, hipStream_t st) {
+		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, uint32_t *ext, unsigned long long *stats, hipStream_t st) {
 	if (nvotes == 0) return hipSuccess;
-	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, diff, stats);
+	k_votes<<<(nvotes + 63) / 64, 64, 0, st>>>(S, Q, o, c, nvotes, q_boff, tally, diff, ext, stats);
 	return hipGetLastError();
 }
 

@@ -397,6 +397,47 @@ HD void vote_record(const uint32_t *script, const DpResults &R, uint32_t task0, 
 	replay_script(script, task0, vv);
 }
 
+// --merge (SubjectRead::updateQueryCoverageConsiderMerging, src/SubjectRead.cpp:332-461): the
+// record's votes, and when its CIGAR has more than one op (AlignmentRecord::addCigar merging, as the
+// record writer counts them) the subject bases from the record's end, up to kExtPositions of them and
+// in the record's subject frame, each vote into row (index, base) of the query's extension tally
+// (QueryRead::updateExtraCoverageMatrix, src/QueryRead.cpp:353-371).
+constexpr int kExtPositions = 20; // QueryRead::iLength4ExtraMatrix
+struct MergeVoteV {
+	VoteV vote;
+	int nops;      // CIGAR ops after merging equal adjacent types
+	int cur_t;     // type of the open op ('M', 'I', 'D' or the query base of a mismatch)
+	bool pending;
+	HD void push(int t) {
+		if (!pending || cur_t != t) nops++;
+		pending = true, cur_t = t;
+	}
+	HD void op(int c, int t) {
+		if (t == kOpM) push('M');
+		else if (t == kOpI) push('I');
+		else if (t == kOpD) push('D');
+		else
+			for (int x = 0; x < c; x++) push(code_ascii(vote.sink.pr.qbase(vote.pq + x)));
+		vote.op(c, t);
+	}
+	HD void start(int pos) { vote.start(pos); }
+	HD void add(int c, int t) { op(c, t); }
+	HD void run(uint32_t task, bool swapped) {
+		feed_runs(vote.R, task, swapped, [&](int cnt, int t) { op(cnt, t); });
+	}
+	HD void finish() { vote.finish(); }
+};
+
+HD void vote_record_merging(const uint32_t *script, const DpResults &R, uint32_t task0, const VoteSink &sink,
+		uint32_t *ext) {
+	MergeVoteV mv{VoteV{R, sink, 0, 0}, 0, 0, false};
+	replay_script(script, task0, mv);
+	if (mv.nops <= 1) return; // a one-op record stops before the extension (src/SubjectRead.cpp:416-418)
+	const PairReads &pr = sink.pr;
+	for (int x = 0, ps = mv.vote.ps; x < kExtPositions && ps >= 0 && ps < pr.Ls; x++, ps++)
+		sink.add_u32(&ext[4 * x + pr.sbase(ps)], 1u);
+}
+
 // ---- records (without -c) -------------------------------------------------------------------
 // The record's CIGAR as AlignmentRecord::addCigar builds it (src/AlignmentRecord.cpp:35-54):
 // adjacent ops of equal type merge (zero counts included); a mismatch step is typed by the

@@ -26,7 +26,8 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int32), ("dp_form", ctypes.c_int32), ("chunk_subjects", ctypes.c_int64),
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
         ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
-        ("mapping", ctypes.c_int32), ("band_align_form", ctypes.c_int32),
+        ("mapping", ctypes.c_int32), ("band_align_form", ctypes.c_int32), ("long_format", ctypes.c_int32),
+        ("merge", ctypes.c_int32),
     ]
 
 
@@ -109,6 +110,10 @@ SIGNATURES = {
     "hero_band_align_batch": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_int64]),
     "hero_band_align_batch_noalloc": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P,
                                                      _P, _P, _P, _P]),
+    "hero_prime_larger_than": (ctypes.c_uint64, [ctypes.c_uint64]),
+    "hero_get_extension_tally": (ctypes.c_int, [_P, _P]),
+    "hero_take_records": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _P, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_int64)]),
     "hero_hash_kmers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_uint64, _P]),
     "hero_lookup": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_int64]),
     "hero_autotune": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(TuneReport)]),

@@ -63,6 +63,14 @@ typedef struct {
 	int32_t band_align_form; /* hero_band_align_batch: 0 the plain form (one wavefront per call); for tests
 	                            of the pipeline's kernels on given calls (A/C/G/T, bands <= 32, < 512 rows):
 	                            1 batched general rows, 2 batched tagged affine rows, 3 batched packed 16-bit */
+	int32_t long_format;      /* Config::bSpeicialVersion (-v), record mode only: seeds on a query with the
+	                             subject's uniqueId are kept (src/SubjectReadAligner.cpp:464-465) and every
+	                             accepted record is kept, none replaced (src/SubjectRead.cpp:26-29); read them
+	                             with hero_take_records */
+	int32_t merge;            /* Config::bMerging (--merge), with call_consensus: each kept record whose CIGAR has
+	                             more than one op also votes the (up to) 20 subject bases after its end into its
+	                             query's extension tally (SubjectRead::updateQueryCoverageConsiderMerging,
+	                             src/SubjectRead.cpp:332-461); read it with hero_get_extension_tally */
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
@@ -112,6 +120,24 @@ int hero_align_self(hero_ctx *ctx);
  * (without a terminating zero) and released. */
 int hero_take_alignments(hero_ctx *ctx, char *out, int64_t cap, int64_t *len);
 
+/* Record mode: the accepted alignment records of the subjects aligned since the last call, as data
+ * (SubjectRead::vAlignment after addAlignment, src/SubjectRead.cpp:25-79; under long_format every
+ * accepted record, in processing order). Records are grouped by subject in stream order; subject
+ * counts subjects since the last call. Each record's CIGAR is ops[op_off .. op_off + op_n): (count << 8)
+ * | type with type 'M', 'I', 'D' or the query base of a mismatch run (AlignmentRecord::addCigar,
+ * src/AlignmentRecord.cpp:35-48). *n_recs / *n_ops receive the sizes; when recs and ops are not
+ * null and the caps suffice, the records are copied and released. */
+typedef struct {
+	int64_t subject;  /* subject index since the last call */
+	int64_t query;    /* innerId of the query read */
+	int32_t strand;   /* AlignmentRecord::iOrientation: 0 forward, 1 reverse complement */
+	int32_t position; /* iPositionOfQueryOnSubject */
+	int64_t op_off;
+	int32_t op_n, _pad;
+} hero_record;
+int hero_take_records(hero_ctx *ctx, hero_record *recs, int64_t rec_cap, int64_t *n_recs, uint32_t *ops, int64_t op_cap,
+		int64_t *n_ops);
+
 /* QueryDataset::callConsensus (src/QueryDataset.cpp:393-412, QueryRead.cpp:148-180): writes the
  * corrected reads in the layout of hero_set_queries' `seq` (reads that are not good are copied
  * unchanged, as the reference writes them). With corrected == NULL the corrected reads stay in
@@ -122,6 +148,11 @@ int hero_copy_corrected(hero_ctx *ctx, char *corrected);
 /* The vote matrices (QueryRead::aCoverageMatrix, 4 u32 per base in A,C,T,G order) as they stand
  * after the subject passes, before consensus adds the self vote. */
 int hero_get_tally(hero_ctx *ctx, uint32_t *tally);
+
+/* --merge: QueryRead::aExtraCoverageMatrix of every query read (src/QueryRead.cpp:100-115,353-371),
+ * 20 positions x 4 u32 in A,C,T,G order, position 0 = the first subject base after a record's end;
+ * out holds 80 x the query count. */
+int hero_get_extension_tally(hero_ctx *ctx, uint32_t *out);
 
 /* --mapping output (QueryDataset::getCoverageAndWriteResults, src/QueryDataset.cpp:560-600): counts[v]
  * = bases of the query reads whose coverage (QueryRead::getCoverageAt: the sum of the base's four
@@ -161,6 +192,11 @@ int hero_band_align_batch(hero_ctx *ctx, int32_t n, const hero_band_pair *pairs,
 int hero_band_align_batch_noalloc(hero_ctx *ctx, int32_t n, const hero_band_pair *pairs, const char *seq,
 		int32_t max_len, int32_t max_band, int32_t *band_out, const int64_t *cigar_off, int32_t *cigar_n,
 		int32_t *cigar_counts, char *cigar_types, void *stream);
+
+/* HashTableLongKey::getPrimeLargerThanNumber (src/HashTableLongKey.cpp:101-166): the smallest prime
+ * of the reference's table-size list above n, else n + 1. hero_build_index sizes the query index with
+ * it (factor x reads, or factor x bases / k under --mapping, :52-62). Host only. */
+uint64_t hero_prime_larger_than(uint64_t n);
 
 /* HashTableLongKey::hashFunction (src/HashTableLongKey.cpp:188-206) on the device for n k-mers
  * of length k packed back to back in `kmers` (A/C/G/T). */

@@ -14,6 +14,11 @@ Run `make -C oracle ref` first. The fixtures are data only (inputs and expected 
                        per-read digests of the corrected FASTA and the .aln output
   mapping.json         reference program runs with -c --mapping: the coverage histogram text
                        (`make_golden.py mapping` writes this file alone)
+  prime_cases.jsonl    getPrimeLargerThanNumber(n) around each entry of its list and at random n
+                       (`make_golden.py primes` writes this file alone)
+  formats.json         reference program runs on the file inputs of inputs.py (FASTQ, wrapped FASTA,
+                       several files, mates, names): digests of the .fasta / .info / _merged /
+                       _unmerged / long-format .aln files it writes (`make_golden.py formats`)
 """
 import hashlib
 import json
@@ -230,7 +235,57 @@ def mapping_cases():
     return out
 
 
+def prime_cases(rng, n_random=300):
+    """getPrimeLargerThanNumber around every entry of its list and at random sizes."""
+    import re
+    src = open("/root/reference/src/HashTableLongKey.cpp").read()
+    a = src.index("getPrimeLargerThanNumber(UINT64 number)")
+    body = src[src.index("{", src.index("=", a)):src.index("};", a)]
+    listed = [int(x) for x in re.findall(r"\d+", body)]
+    ns = {0, 1, 2, listed[-1] + 5, 2 ** 62}
+    for p in listed:
+        ns.update((p - 1, p, p + 1))
+    for _ in range(n_random):
+        ns.add(rng.randrange(0, 2 ** rng.randrange(1, 52)))
+    ns = sorted(ns)
+    out = probe([f"P {n}" for n in ns])
+    return [dict(n=n, P=int(r)) for n, r in zip(ns, out)]
+
+
+def format_cases():
+    """The reference program on the file-format inputs of tests/golden/inputs.py: sha256 and size of
+    every output file it writes."""
+    sys.path.insert(0, HERE)
+    import inputs
+    out = []
+    for name, case in inputs.CASES.items():
+        d = tempfile.mkdtemp(prefix="golden_fmt_", dir="/tmp")
+        q, s = inputs.case_files(name, d)
+        prefix = os.path.join(d, "out")
+        cmd = [HERO_REF, "-q", ",".join(q), "-s", ",".join(s), "-o", prefix, "-t", "8"] + case["flags"]
+        p = subprocess.run(cmd, capture_output=True, text=True, check=True)
+        res = dict(name=name, flags=case["flags"], files={})
+        for suf in case["outputs"]:
+            data = open(prefix + suf, "rb").read()
+            res["files"][suf] = dict(sha256=hashlib.sha256(data).hexdigest(), bytes=len(data))
+        merged = [ln for ln in p.stdout.splitlines() if ln.startswith("Merged Reads")]
+        if merged:
+            res["merged_reads"] = int(merged[0].split()[-1])
+        out.append(res)
+        print(name, {k: v["bytes"] for k, v in res["files"].items()}, res.get("merged_reads"), flush=True)
+    return out
+
+
 def main():
+    if sys.argv[1:] == ["formats"]:
+        with open(os.path.join(HERE, "formats.json"), "w") as f:
+            json.dump(format_cases(), f, indent=1)
+        return
+    if sys.argv[1:] == ["primes"]:
+        with open(os.path.join(HERE, "prime_cases.jsonl"), "w") as f:
+            for c in prime_cases(random.Random(20261204)):
+                f.write(json.dumps(c, separators=(",", ":")) + "\n")
+        return
     if sys.argv[1:] == ["mapping"]:
         with open(os.path.join(HERE, "mapping.json"), "w") as f:
             json.dump(mapping_cases(), f, separators=(",", ":"))

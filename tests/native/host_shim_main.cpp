@@ -9,6 +9,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "../../hero_amd/host/hero_host.h"
 
@@ -85,6 +86,85 @@ static int run_band(int argc, char **argv) {
 	return 0;
 }
 
+// run <reference flags>: main.cpp's flow (main.cpp:26-106) over the host classes, with the flags of
+// the reference's command line that reach the hot path (Config::setConfig, Config.cpp:118-243):
+// -q -s (comma-separated files) -o -k -m -i -l -a -t -z -y -f -c [freq] -v --mapping
+// --singleReads --covinfo --covfilter --merge.
+static int run_flags(int argc, char **argv) {
+	Config cfg;
+	std::vector<std::string> qfiles;
+	auto split = [](const std::string &s) {
+		std::vector<std::string> out;
+		std::stringstream ss(s);
+		std::string item;
+		while (std::getline(ss, item, ',')) out.push_back(item);
+		return out;
+	};
+	for (int i = 2; i < argc; i++) {
+		const std::string a = argv[i];
+		auto val = [&]() -> std::string { return i + 1 < argc ? argv[++i] : ""; };
+		if (a == "-q") qfiles = split(val());
+		else if (a == "-s") cfg.vSubjectFiles = split(val());
+		else if (a == "-o") cfg.outputfilename = val();
+		else if (a == "-k") cfg.setKmer(std::atoi(val().c_str()));
+		else if (a == "-m") cfg.setMaxMismatchRatePercent(std::atof(val().c_str()));
+		else if (a == "-i") cfg.setMaxIndelRatePercent(std::atof(val().c_str()));
+		else if (a == "-l") cfg.setMinimumOverlap(std::atoll(val().c_str()));
+		else if (a == "-a") cfg.setAffine(val() == "a");
+		else if (a == "-t") cfg.numberOfThreads = (uint16_t) std::atoi(val().c_str());
+		else if (a == "-z") cfg.streamChunkSize = (uint64_t) std::atoll(val().c_str());
+		else if (a == "-y") cfg.queryChunkSize = (uint64_t) std::atoll(val().c_str());
+		else if (a == "-f") cfg.setHashTableFactor(std::atoi(val().c_str()));
+		else if (a == "-v") cfg.setSpecialVersion(true);
+		else if (a == "--mapping") cfg.setMapping(true);
+		else if (a == "--singleReads") cfg.bMergedReads = true;
+		else if (a == "--covinfo") cfg.bCoverageStatistic = true;
+		else if (a == "--covfilter") cfg.iCoverageDepthFilter = (uint32_t) std::atoi(val().c_str());
+		else if (a == "--merge") cfg.setMerging(true);
+		else if (a == "-c") {
+			cfg.setCallConsensus(true);
+			if (i + 1 < argc && argv[i + 1][0] != '-') cfg.setConsensusFrequency(std::atof(argv[++i]));
+		} else {
+			std::fprintf(stderr, "unknown flag %s\n", a.c_str());
+			return 2;
+		}
+	}
+	Device dev(cfg);
+	QueryDataset q = qfiles.size() == 1 ? QueryDataset(dev, qfiles[0]) : QueryDataset(dev, qfiles);
+	if (!q.buildDataset()) return 3;
+	HashTableLongKey table(&q);
+	table.createHashTables();
+	table.insertQueryDataset(&q);
+	SubjectReadAligner aligner(&table);
+	aligner.start();
+	if (cfg.opt.mapping) {
+		q.getCoverageAndWriteResults(cfg.outputfilename);
+		return 0;
+	}
+	if (cfg.opt.call_consensus) {
+		q.callConsensus();
+		if (cfg.bMerging) q.mergePairedReads();
+		q.writeQueryOut(cfg.outputfilename);
+	}
+	return 0;
+}
+
+// parse <out> <file>[,<file>...]: the query loader alone (no device): every record as
+// ">atoi(name)\nsequence\n", as the reference writes reads it leaves unchanged (--singleReads)
+static int run_parse(int argc, char **argv) {
+	if (argc < 4) return 2;
+	std::vector<std::string> files;
+	std::stringstream ss(argv[3]);
+	std::string item;
+	while (std::getline(ss, item, ',')) files.push_back(item);
+	ReadFiles rf(files);
+	if (!rf.ok()) return 3;
+	std::ofstream out(argv[2], std::ios_base::binary | std::ios_base::trunc);
+	std::string name, s;
+	while (rf.next(name, s)) out << '>' << std::atoll(name.c_str()) << '\n' << s << '\n';
+	return 0;
+}
+
 int main(int argc, char **argv) {
 	if (argc < 2) return 2;
 	const std::string mode = argv[1];
@@ -93,6 +173,8 @@ int main(int argc, char **argv) {
 		if (mode == "aln") return run_path(false, false, argc, argv);
 		if (mode == "mapping") return run_path(true, true, argc, argv);
 		if (mode == "band") return run_band(argc, argv);
+		if (mode == "run") return run_flags(argc, argv);
+		if (mode == "parse") return run_parse(argc, argv);
 	} catch (const std::exception &e) {
 		std::fprintf(stderr, "%s\n", e.what());
 		return 1;

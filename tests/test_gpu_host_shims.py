@@ -61,3 +61,26 @@ def test_band_aligner_protocol(aligner, tmp_path):
     for c, line in zip(cases, got):
         band, cigar = (line.split(" ") + [""])[:2]
         assert (int(band), cigar) == (c["out_band"], c["cigar"]), c
+
+
+def _formats():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "formats.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _formats(), ids=lambda c: c["name"])
+def test_file_formats_through_host_classes(case, tmp_path):
+    """main.cpp's flow over the host classes on FASTQ / wrapped FASTA / multi-file inputs: paired
+    and single-read FASTA, .info (--covinfo), --merge's _merged / _unmerged files and the -v
+    long-format .aln equal the reference's bytes."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import inputs
+    q, s = inputs.case_files(case["name"], str(tmp_path))
+    prefix = str(tmp_path / "out")
+    cmd = [DRIVER, "run", "-q", ",".join(q), "-s", ",".join(s), "-o", prefix, "-t", "8"] + case["flags"]
+    subprocess.run(cmd, check=True, timeout=300)
+    for suf, want in case["files"].items():
+        data = open(prefix + suf, "rb").read()
+        assert (len(data), sha256(data)) == (want["bytes"], want["sha256"]), suf

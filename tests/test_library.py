@@ -49,3 +49,13 @@ def test_default_options_and_null_context_errors():
     lib = ctypes.CDLL(LIB)
     lib.hero_last_error.restype = ctypes.c_char_p
     assert lib.hero_build_index(None) == -1
+
+
+def test_table_size_list_matches_reference():
+    """hero_prime_larger_than against getPrimeLargerThanNumber (src/HashTableLongKey.cpp:101-166) at
+    every list entry and its neighbours, random sizes and sizes beyond the list (n + 1)."""
+    from helpers import load_jsonl
+    from hero_amd import gpu
+    f = gpu.lib().hero_prime_larger_than
+    bad = [(c["n"], c["P"], f(c["n"])) for c in load_jsonl("prime_cases.jsonl") if f(c["n"]) != c["P"]]
+    assert not bad, bad[:5]
