@@ -19,6 +19,8 @@ Run `make -C oracle ref` first. The fixtures are data only (inputs and expected 
   formats.json         reference program runs on the file inputs of inputs.py (FASTQ, wrapped FASTA,
                        several files, mates, names): digests of the .fasta / .info / _merged /
                        _unmerged / long-format .aln files it writes (`make_golden.py formats`)
+  tables.json          the reference's index size and collision counts on three read sets
+                       (`make_golden.py tables`)
 """
 import hashlib
 import json
@@ -276,7 +278,46 @@ def format_cases():
     return out
 
 
+TABLES = [
+    dict(name="c1_k31", synth=dict(seed=101, genome_len=20000, len_min=101, coverage=30.0, sub_5p=0.01), k=31,
+         flags=[]),
+    dict(name="c3_k25_f8", synth=dict(seed=303, genome_len=15000, len_min=75, len_max=250, coverage=40.0,
+                                      sub_5p=0.01), k=25, flags=["-f", "8"]),
+    dict(name="repeats_k39_mapping", synth=dict(seed=713, genome_len=15100, len_min=150, coverage=30.0,
+                                                sub_5p=0.02, repeat_frac=0.2, repeat_seg_len=1500,
+                                                repeat_copies_per_family=4, repeat_div=0.01),
+         k=39, flags=["--mapping"]),
+]
+
+
+def table_cases():
+    """The reference's query index on three read sets: its size and the collision counts it prints
+    (HashTableLongKey.cpp:296-297,352)."""
+    import re
+    from hero_amd import synth
+    out = []
+    for case in TABLES:
+        rs = synth.generate(**case["synth"])
+        d = tempfile.mkdtemp(prefix="golden_tab_", dir="/tmp")
+        fa = os.path.join(d, "reads.fasta")
+        rs.write_fasta(fa)
+        cmd = [HERO_REF, "-q", fa, "-s", fa, "-o", os.path.join(d, "out"), "-t", "8", "-k", str(case["k"]), "-c",
+               "--singleReads"] + case["flags"]
+        p = subprocess.run(cmd, capture_output=True, text=True, check=True)
+        grab = lambda pat: int(re.search(pat, p.stdout).group(1))
+        res = dict(case, n_reads=rs.n, size=grab(r"Hash Table size set to: (\d+)"),
+                   collisions=grab(r"maximum collision number is: (\d+)"),
+                   max_single=grab(r"maximum single read collision number is: (\d+)"))
+        out.append(res)
+        print(res["name"], res["size"], res["collisions"], res["max_single"], flush=True)
+    return out
+
+
 def main():
+    if sys.argv[1:] == ["tables"]:
+        with open(os.path.join(HERE, "tables.json"), "w") as f:
+            json.dump(table_cases(), f, indent=1)
+        return
     if sys.argv[1:] == ["formats"]:
         with open(os.path.join(HERE, "formats.json"), "w") as f:
             json.dump(format_cases(), f, indent=1)

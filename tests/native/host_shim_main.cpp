@@ -165,6 +165,38 @@ static int run_parse(int argc, char **argv) {
 	return 0;
 }
 
+// table <reads.fasta> <k> <factor> [mapping]: the HashTableMethod seam over the device index:
+// table size, the reference's collision counts, empty slots, and the bucket of every key of read 0
+static int run_table(int argc, char **argv) {
+	if (argc < 5) return 2;
+	Config cfg;
+	cfg.setKmer(std::atoi(argv[3]));
+	cfg.setHashTableFactor(std::atoi(argv[4]));
+	cfg.setCallConsensus(true);
+	if (argc > 5) cfg.setMapping(true);
+	Device dev(cfg);
+	QueryDataset q(dev, std::string(argv[2]));
+	if (!q.buildDataset()) return 3;
+	HashTableLongKey t(&q);
+	HashTableMethod *m = &t;
+	m->createHashTables();
+	m->insertQueryDataset(&q);
+	uint64_t empty = 0;
+	for (uint64_t i = 0; i < m->hashTableSize; i++) empty += m->isEmptyAt((int64_t) i) ? 1 : 0;
+	std::printf("size %llu collisions %lld max_single %llu empty %llu bits %lld\n", (unsigned long long) m->hashTableSize,
+			(long long) m->numberOfHashCollision, (unsigned long long) m->maxSingleHashCollision,
+			(unsigned long long) empty, (long long) m->iBitOfReadId);
+	const std::string r0 = *q.getReadFromID(0);
+	for (int64_t p = 0; p + m->hashKeyLength <= (int64_t) r0.size(); p += m->hashKeyLength) {
+		std::vector<int64_t> *l = m->getListOfReads(r0, p, m->hashKeyLength);
+		std::printf("key %lld:", (long long) p);
+		if (l) for (int64_t e : *l) std::printf(" %lld", (long long) e);
+		// the same pointer again: the table keeps its lists
+		std::printf(" same %d\n", (int) (l == m->getListOfReads(r0, p, m->hashKeyLength)));
+	}
+	return 0;
+}
+
 int main(int argc, char **argv) {
 	if (argc < 2) return 2;
 	const std::string mode = argv[1];
@@ -175,6 +207,7 @@ int main(int argc, char **argv) {
 		if (mode == "band") return run_band(argc, argv);
 		if (mode == "run") return run_flags(argc, argv);
 		if (mode == "parse") return run_parse(argc, argv);
+		if (mode == "table") return run_table(argc, argv);
 	} catch (const std::exception &e) {
 		std::fprintf(stderr, "%s\n", e.what());
 		return 1;

@@ -84,3 +84,35 @@ def test_file_formats_through_host_classes(case, tmp_path):
     for suf, want in case["files"].items():
         data = open(prefix + suf, "rb").read()
         assert (len(data), sha256(data)) == (want["bytes"], want["sha256"]), suf
+
+
+def _tables():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "tables.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _tables(), ids=lambda c: c["name"])
+def test_hash_table_method_seam_matches_reference(case, tmp_path):
+    """HashTableLongKey through its HashTableMethod base: the table size from the reference's prime
+    list, the reference's collision counts (its own printout), iBitOfReadId, and getListOfReads
+    returning the table's own list (the same pointer twice) holding read 0's occurrence."""
+    rs = synth.generate(**case["synth"])
+    fa = str(tmp_path / "reads.fasta")
+    rs.write_fasta(fa)
+    mapping = "--mapping" in case["flags"]
+    factor = case["flags"][case["flags"].index("-f") + 1] if "-f" in case["flags"] else "32"
+    cmd = [DRIVER, "table", fa, str(case["k"]), factor] + (["mapping"] if mapping else [])
+    out = subprocess.run(cmd, check=True, timeout=120, capture_output=True, text=True).stdout.splitlines()
+    head = out[0].split()
+    got = dict(zip(head[0::2], (int(x) for x in head[1::2])))
+    assert (got["size"], got["collisions"], got["max_single"]) == (case["size"], case["collisions"], case["max_single"])
+    bits = 35 if mapping else 40
+    assert got["bits"] == bits
+    k = case["k"]
+    for line in out[1:]:
+        pos = int(line.split()[1].rstrip(":"))
+        entries = [int(x) for x in line.split(":")[1].split()[:-2]]
+        assert line.endswith("same 1")
+        assert ((pos << bits) | 0) in entries, line
+        assert all(rs.read(e & ((1 << bits) - 1))[e >> bits:(e >> bits) + k] == rs.read(0)[pos:pos + k] for e in entries)
