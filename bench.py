@@ -54,8 +54,14 @@ C4_READS = dict(seed=404, genome_len=20_000_000, len_min=150, n_reads=C4_RESIDEN
 C2_REF_OPTS = dict(call_consensus=1, k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0267, min_overlap=40)
 C4_OPTS = dict(C2_OPTS, max_candidates=0)  # a per-subject candidate cap would depend on the batch mix
 C4_RATES = (5, 20, 50, 100, 200)
-C3_READS = dict(seed=303, genome_len=1_000_000, len_min=75, len_max=250, coverage=40.0, sub_5p=0.01)
-C3_OPTS = dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40)
+# mixed lengths (the autotuner's workload): 10 Mb genome, 75-250 bp at 40x, band 2*round(0.025*overlap)
+C3_READS = synth.WORKLOADS["c3"]
+C3_LEGS = [("k25_global", dict(call_consensus=1, k=25, aligner=1, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40)),
+           ("k41_affine", dict(call_consensus=1, k=41, aligner=0, mismatch_rate=0.04, indel_rate=0.025, min_overlap=40))]
+# repeat-heavy share: 50 Mb genome with 10 % in 5 kb repeat families, 150 bp at 100x (33.3 M reads),
+# 2 % substitutions, 0.2 % indels, affine band 10, no candidate cap
+C5_READS = synth.WORKLOADS["c5"]
+C5_OPTS = dict(call_consensus=1, k=39, aligner=0, mismatch_rate=0.04, indel_rate=0.0333, min_overlap=40, band_override=10)
 
 
 def log(*a):
@@ -370,6 +376,43 @@ def copy_calibration(nbytes=1 << 30):
     del x, y
 
 
+def hbm_used():
+    """Device bytes in use (total - free), from the HIP runtime the library links."""
+    import ctypes
+    gpu.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total))
+    return total.value - free.value
+
+
+def c5_run(steps):
+    """The repeat-heavy share at its real shape: every read indexed and streamed as a subject, one
+    untimed step, then `steps` timed steps; reads/s, GCUPS, the stage split and peak HBM."""
+    t0 = time.perf_counter()
+    rs = synth.generate(**C5_READS)
+    log(f"c5: {rs.n} reads generated in {time.perf_counter() - t0:.1f} s")
+    base = hbm_used()
+    with gpu.Context(**C5_OPTS) as ctx:
+        t0 = time.perf_counter()
+        ctx.set_queries(rs)
+        log(f"c5: reads packed and resident in {time.perf_counter() - t0:.1f} s")
+        run_steps(ctx, 1)
+        peak = hbm_used() - base
+        log(f"c5: untimed step done, {peak / 2**30:.1f} GiB of HBM in use")
+        ms, st = run_steps(ctx, steps)
+    line = dict(metric=METRIC, workload="c5: 50 Mb genome with 10% in 5 kb repeat families at 1% divergence, 150 bp, "
+                                        "100x, 2% substitutions, 0.2% indels, k=39, affine -3/-1, band 10, no cap",
+                reads=rs.n, steps=steps, ms_per_step=round(ms, 3), reads_per_s=round(rs.n / (ms / 1000.0), 1),
+                gcups=round(gcups_of(st), 2) if steps else None,
+                stages_ms={k: round(st[k] / steps, 3) for k in ("ms_index", "ms_seed", "ms_plan", "ms_dp",
+                                                               "ms_score_vote", "ms_consensus")},
+                counts={k: st[k] // steps for k in ("seeds", "dp_tasks", "dp_calls_exec", "dp_cells_exec",
+                                                    "records_kept", "reads_changed")},
+                hbm_gib=round(peak / 2**30, 2), note="stage times summed over the chunks of a step; two chunks run at once")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +426,8 @@ def main():
                     help="default DP schedules instead of the autotuner's (profiles of exactly the step's kernels)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (the reference point for HBM byte counters)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5"],
+                    help="c2: the headline step (default); c5: the repeat-heavy share at its real shape")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path; a subject set "
@@ -390,6 +435,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        c5_run(args.steps)
+        return
     if args.copy_calibration:
         copy_calibration()
     t0 = time.perf_counter()
@@ -447,13 +495,18 @@ def main():
         gcups["plain"] = round(gcups_of(stp), 2)
         gcups["batched_over_plain"] = round(gcups_batched / gcups["plain"], 2) if gcups["plain"] else None
         rs3 = synth.generate(**C3_READS)
-        tune_c3 = []
-        s3b, s3p = one_form(rs3, C3_OPTS, 0, tune_c3), one_form(rs3, C3_OPTS, 1)
-        gcups["mixed_length"] = dict(workload=f"1 Mb genome, 75-250 bp reads at 40x ({rs3.n} reads), k=25, global, "
-                                              "band 2*round(0.025*overlap)",
-                                     batched=round(gcups_of(s3b), 2), plain=round(gcups_of(s3p), 2),
-                                     batched_over_plain=round(gcups_of(s3b) / max(gcups_of(s3p), 1e-9), 2),
-                                     autotune=tune_c3)
+        mixed = dict(workload=f"10 Mb genome, 75-250 bp reads at 40x ({rs3.n} reads), band 2*round(0.025*overlap)")
+        for leg, opts in C3_LEGS:
+            tune_c3 = []
+            s3b, s3p = one_form(rs3, opts, 0, tune_c3), one_form(rs3, opts, 1)
+            gb, gp = gcups_of(s3b), gcups_of(s3p)
+            mixed[leg] = dict(batched=round(gb, 2), plain=round(gp, 2), batched_over_plain=round(gb / max(gp, 1e-9), 2),
+                              dp_cells=s3b["dp_cells_exec"], ms_dp_batched=round(s3b["ms_dp"], 3),
+                              ms_dp_plain=round(s3p["ms_dp"], 3),
+                              autotune=[{k: v for k, v in t.items() if k != "candidates"} for t in tune_c3])
+            log(f"mixed lengths {leg}:", json.dumps(mixed[leg]))
+        del rs3
+        gcups["mixed_length"] = mixed
         log("gcups:", json.dumps(gcups))
 
     mapping = None
