@@ -328,9 +328,28 @@ def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_r
                                p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
                                vs_offline=round(rps / offline, 4), merged_steps=merged)
         log("serving fixed 4k:", json.dumps(out["fixed_4k"]))
+        # the largest Poisson rate of 4096-read batches whose p99 stays under 50 ms: rates rise by 12 %
+        # from half the offline rate until a point fails; each point runs >= 200 batches (>= 1 s)
+        r = 0.5 * offline / 4096
+        points, best = [], None
+        while len(points) < 16:
+            nb = max(200, int(r * 1.0))
+            arrivals = np.cumsum(rng.exponential(1.0 / r, nb))
+            batches = [batch(4096, j * 4096) for j in range(nb)]
+            lat, rps, merged = _drive(srv, batches, arrivals)
+            pt = dict(rate_batches_per_s=round(r, 1), batches=nb, p50_ms=round(1e3 * _percentile(lat, 50), 3),
+                      p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
+                      vs_offline=round(rps / offline, 4), merged_steps=merged)
+            points.append(pt)
+            log("serving 4k search:", json.dumps(pt))
+            if pt["p99_ms"] >= 50.0:
+                break
+            best = pt
+            r *= 1.12
+        out["max_rate_4k_p99_under_50ms"] = dict(best=best, points=points)
         sweep = []
         for r in sweep_rates:
-            nb = max(10, int(r * seed_duration))
+            nb = max(200, int(r * seed_duration))
             arrivals = np.cumsum(rng.exponential(1.0 / r, nb))
             sizes = np.exp(rng.uniform(np.log(1024), np.log(65536), nb)).astype(int)
             starts = np.cumsum(np.concatenate([[0], sizes[:-1]]))

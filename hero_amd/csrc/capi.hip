@@ -8,6 +8,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <cstdio>
@@ -147,7 +148,7 @@ struct Timer {
 struct IndexBufs {
 	DevBuf<Slot> slots;
 	DevBuf<uint64_t> entries;
-	uint64_t P = 0, total_keys = 0;
+	uint64_t P = 0, total_keys = 0, distinct_keys = 0;
 	uint32_t max_probe = 0;
 	bool built = false;
 	DevIndex view(int k) const {
@@ -155,7 +156,7 @@ struct IndexBufs {
 	}
 	void release() {
 		slots.release(), entries.release();
-		P = total_keys = 0, max_probe = 0, built = false;
+		P = total_keys = distinct_keys = 0, max_probe = 0, built = false;
 	}
 };
 
@@ -201,6 +202,40 @@ struct Worker {
 	DevBuf<uint32_t> b_tally, b_cnt, b_off, b_gflag, b_pflag, b_gx, b_px, b_glist, b_plist;
 	DevBuf<uint64_t> b_kref, b_keys, b_vals, b_keys2, b_vals2;
 	DevBuf<uint8_t> b_sort_tmp;
+	// frees the grow-only work buffers (stream, events, counters and the DP graph stay)
+	void release_buffers() {
+		if (st) (void) hipStreamSynchronize(st);
+		if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+		if (graph) (void) hipGraphDestroy(graph);
+		graph_exec = nullptr, graph = nullptr, graph_key.clear();
+		seg_n.release(), seg_off.release(), scan_tmp.release(), lane_n.release(), seeds.release(), win_info.release();
+		cnext.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
+		g_tasks.release(), g_task_off.release(), g_words.release(), g_word_off.release();
+		g_prog.release(), g_prog_off.release(), prog.release();
+		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), wave_scratch.release(), wave_over.release();
+		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
+		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
+		recs.release(), rec_pos.release(), sums.release(), sub_pairs.release(), sub_pair_off.release(), votes.release();
+		bq.release(), b_cons.release(), b_raw.release(), b_koff.release();
+		b_tally.release(), b_cnt.release(), b_off.release();
+		b_gflag.release(), b_pflag.release(), b_gx.release(), b_px.release(), b_glist.release(), b_plist.release();
+		b_kref.release(), b_keys.release(), b_vals.release(), b_keys2.release(), b_vals2.release(), b_sort_tmp.release();
+	}
+	// device bytes the worker's grow-only buffers hold
+	size_t device_bytes() const {
+		size_t b = 0;
+		auto add = [&](const auto &buf) { b += buf.n * sizeof(*buf.p); };
+		add(seg_n), add(seg_off), add(scan_tmp), add(lane_n), add(seeds), add(win_info), add(cnext), add(cfirst);
+		add(clast), add(clen), add(gflag), add(g_tasks), add(g_task_off), add(g_words), add(g_word_off), add(g_prog);
+		add(g_prog_off), add(prog), add(tasks), add(words), add(nsteps), add(dp_scratch), add(wave_scratch), add(wave_over);
+		add(bucket_hist), add(bucket_off), add(order), add(g_nch), add(g_best), add(g_recs), add(g_rec_off);
+		add(rec_opn), add(rec_opoff), add(rec_ops), add(recs), add(rec_pos), add(sub_pairs), add(sub_pair_off);
+		add(votes), add(sums), add(bq.words), add(bq.woff), add(bq.boff), add(bq.len), add(bq.uid), add(bq.good);
+		add(b_cons), add(b_raw), add(b_koff), add(b_tally), add(b_cnt), add(b_off), add(b_gflag), add(b_pflag);
+		add(b_gx), add(b_px), add(b_glist), add(b_plist), add(b_kref), add(b_keys), add(b_vals), add(b_keys2);
+		add(b_vals2), add(b_sort_tmp);
+		return b;
+	}
 	hipError_t init() {
 		hipError_t e = hipStreamCreate(&st);
 		for (auto &t : ev)
@@ -1197,10 +1232,12 @@ int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int
 }
 
 // Seed index over the good reads of R: keys at 0, k, 2k, ... and L - k (the reference's query
-// index), or at every position (all_positions: the serving index over resident subjects).
-// table_size 0 = the smallest prime above 2 * keys.
+// index), or at every position (all_positions: the serving index over resident subjects). The
+// device table is sized to the keys, not to the reference's size (slot placement never reaches an
+// output); ref_size, when not 0, is the reference's table size, whose distinct keys must fit
+// (HashTableLongKey.cpp:320-326 exits otherwise).
 int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good, const std::vector<int32_t> &len,
-		bool all_positions, uint64_t table_size, IndexBufs &ix) {
+		bool all_positions, uint64_t ref_size, IndexBufs &ix) {
 	const int k = c->o.k;
 	const int64_t n = R.n;
 	std::vector<uint64_t> koff((size_t) n + 1, 0);
@@ -1212,10 +1249,10 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 	}
 	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu index keys exceed 2^32-1", (unsigned long long) nocc);
 	ix.release();
-	ix.P = table_size ? table_size : next_prime_above(2 * nocc + 1);
+	// about 8 slots per key: half the windows probed miss, and a miss walks to the next empty slot;
+	// at a load of 0.4 those walks cost more than the larger table's cache misses (measured on C2)
+	ix.P = next_prime_above(std::max<uint64_t>((all_positions ? 2 : 8) * nocc + 1, 1021));
 	if (ix.P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) ix.P);
-	if (nocc >= ix.P) return fail(c, HERO_ERR_REF_ABORT, "hash table too small (%llu keys, %llu slots); the reference exits here",
-			(unsigned long long) nocc, (unsigned long long) ix.P);
 	hipEvent_t e0 = c->w.ev[5].a, e1 = c->w.ev[6].a;
 	HIPCHK(c, hipEventRecord(e0, c->w.st));
 	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
@@ -1245,7 +1282,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(owner.exact(ix.P));
 		IXCHK(maxp.exact(1));
 		IXCHK(errf.exact(1));
-		IXCHK(cursor.exact(1));
+		IXCHK(cursor.exact(2));
 		IXCHK(ix.slots.exact(ix.P));
 		IXCHK(ix.entries.exact(nocc));
 		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
@@ -1253,7 +1290,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(hipMemsetAsync(ix.slots.p, 0, ix.P * sizeof(Slot), c->w.st));
 		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->w.st));
 		IXCHK(hipMemsetAsync(errf.p, 0, 4, c->w.st));
-		IXCHK(hipMemsetAsync(cursor.p, 0, 8, c->w.st));
+		IXCHK(hipMemsetAsync(cursor.p, 0, 16, c->w.st));
 		const Packed Rp = view(R, R.good.p);
 		IXCHK(launch_index_gen(Rp, dkoff.p, k, all_positions, occ_hi.p, occ_lo.p, occ_entry.p, occ_rank.p, c->w.st));
 		IXCHK(launch_index_claim(occ_hi.p, occ_lo.p, nocc, ix.P, k, owner.p, occ_slot.p, maxp.p, errf.p, c->w.st));
@@ -1263,13 +1300,19 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(launch_index_sort_buckets(ix.slots.p, ix.P, ix.entries.p, c->w.st));
 		IXCHK(hipEventRecord(e1, c->w.st));
 		int herr = 0;
-		unsigned long long used = 0;
+		unsigned long long used = 0, distinct = 0;
 		IXCHK(hipMemcpyAsync(&ix.max_probe, maxp.p, 4, hipMemcpyDeviceToHost, c->w.st));
 		IXCHK(hipMemcpyAsync(&herr, errf.p, 4, hipMemcpyDeviceToHost, c->w.st));
 		IXCHK(hipMemcpyAsync(&used, cursor.p, 8, hipMemcpyDeviceToHost, c->w.st));
+		IXCHK(hipMemcpyAsync(&distinct, cursor.p + 1, 8, hipMemcpyDeviceToHost, c->w.st));
 		IXCHK(hipStreamSynchronize(c->w.st));
 		if (herr) {
 			rc = fail(c, HERO_ERR_REF_ABORT, "hash table full");
+			break;
+		}
+		if (ref_size && distinct > ref_size) {
+			rc = fail(c, HERO_ERR_REF_ABORT, "hash table too small (%llu distinct keys, %llu slots); the reference exits here",
+					distinct, (unsigned long long) ref_size);
 			break;
 		}
 		if (used != nocc) {
@@ -1279,6 +1322,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		// distinct keys = claimed slots
 		std::vector<uint32_t> dummy;
 		ix.total_keys = nocc;
+		ix.distinct_keys = distinct;
 		float ms = 0;
 		(void) hipEventElapsedTime(&ms, e0, e1);
 		c->st_host.ms_index += ms;
@@ -1572,7 +1616,8 @@ int hero_build_index(hero_ctx *c) {
 	const uint64_t P = ref_prime_larger_than(units * (uint64_t) c->opt.hash_factor); // :61
 	c->cons_valid = false;
 	if (int r = build_index(c, c->Qd, c->q_good, c->q_len, false, P, c->qix)) return r;
-	c->st_host.table_size = c->qix.P;
+	c->st_host.table_size = P; // the reference's size; the device table is sized to the keys
+	c->st_host.distinct_keys = c->qix.distinct_keys;
 	c->st_host.total_keys = c->qix.total_keys;
 	c->st_host.max_probe = c->qix.max_probe;
 	return HERO_OK;
@@ -1765,8 +1810,10 @@ int hero_reset_stats(hero_ctx *c) {
 	c->w.graph_builds = 0, c->w.graph_launches = 0, c->w2.graph_builds = 0, c->w2.graph_launches = 0;
 	std::memset(&c->st_w2, 0, sizeof c->st_w2);
 	const uint64_t P = c->st_host.table_size, tk = c->st_host.total_keys, mp = c->st_host.max_probe, q = c->st_host.queries;
+	const uint64_t dk = c->st_host.distinct_keys;
 	std::memset(&c->st_host, 0, sizeof c->st_host);
 	c->st_host.table_size = P, c->st_host.total_keys = tk, c->st_host.max_probe = mp, c->st_host.queries = q;
+	c->st_host.distinct_keys = dk;
 	return HERO_OK;
 }
 
@@ -2045,10 +2092,16 @@ struct hero_server {
 	std::mutex mu;
 	std::condition_variable cv_work, cv_done;
 	std::deque<int64_t> pending;
-	std::deque<Request> reqs; // by ticket; a deque keeps references valid while others are appended
+	// by ticket until hero_server_wait returns it (std::map keeps references valid across inserts)
+	std::map<int64_t, Request> reqs;
+	int64_t next_ticket = 0;
 	std::atomic<int64_t> steps{0};
 	int64_t inflight_bytes = 0;
+	// device bytes a step needs per query base: kServeBytesPerBase until a step has run, then the
+	// largest worker footprint over the bases of the largest step it ran (buffers are grow-only)
 	int64_t bytes_per_base = 0;
+	std::vector<int64_t> max_step_bases;
+	bool learned = false;
 	bool stop = false;
 	std::string err;
 };
@@ -2060,6 +2113,32 @@ namespace {
 // the serving workload); the admission budget is charged this much per admitted base.
 constexpr int64_t kServeBytesPerBase = 2048;
 
+// One step over the requests `ids` on worker w: their reads merged, corrected, copied back.
+int serve_step(hero_server *sv, Worker *w, const std::vector<int64_t> &ids, std::vector<char> &seq,
+		std::vector<char> &out, std::vector<int64_t> &off, std::vector<int64_t> &uid, std::vector<int32_t> &len) {
+	hero_ctx *c = sv->c;
+	seq.clear(), off.clear(), uid.clear(), len.clear();
+	for (int64_t id : ids) {
+		const Request &r = sv->reqs.at(id);
+		for (int64_t i = 0; i < r.n; i++) {
+			off.push_back((int64_t) seq.size());
+			len.push_back(r.len[i]);
+			uid.push_back(r.uid[i]);
+			seq.insert(seq.end(), r.seq + r.off[i], r.seq + r.off[i] + r.len[i]);
+		}
+	}
+	out.resize(seq.size());
+	const int rc = correct_batch_on(c, w, (int64_t) len.size(), seq.data(), off.data(), len.data(), uid.data(), out.data());
+	if (rc != HERO_OK) return rc;
+	size_t i0 = 0;
+	for (int64_t id : ids) {
+		const Request &r = sv->reqs.at(id);
+		for (int64_t i = 0; i < r.n; i++) std::memcpy(r.out + r.off[i], out.data() + off[i0 + (size_t) i], (size_t) r.len[i]);
+		i0 += (size_t) r.n;
+	}
+	return HERO_OK;
+}
+
 void serve_loop(hero_server *sv, int wi) {
 	hero_ctx *c = sv->c;
 	Worker *w = &sv->workers[(size_t) wi];
@@ -2070,21 +2149,22 @@ void serve_loop(hero_server *sv, int wi) {
 	const int64_t share = sv->opt.hbm_budget / std::max(1, sv->opt.workers);
 	for (;;) {
 		std::vector<int64_t> take;
-		int64_t fp = 0;
+		int64_t fp = 0, bases = 0;
 		{
 			std::unique_lock<std::mutex> lk(sv->mu);
 			sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
 			if (sv->pending.empty()) return; // stopping with nothing left
 			int64_t reads = 0;
 			while (!sv->pending.empty()) {
-				Request &r = sv->reqs[(size_t) sv->pending.front()];
+				Request &r = sv->reqs.at(sv->pending.front());
+				const int64_t rfp = r.bases * sv->bytes_per_base;
 				const bool first = take.empty();
-				if (!first && (reads + r.n > sv->opt.max_step_reads || fp + r.footprint > share)) break;
-				if (!first && sv->inflight_bytes + fp + r.footprint > sv->opt.hbm_budget) break;
-				if (first && sv->inflight_bytes + r.footprint > sv->opt.hbm_budget && sv->inflight_bytes > 0) break;
+				if (!first && (reads + r.n > sv->opt.max_step_reads || fp + rfp > share)) break;
+				if (!first && sv->inflight_bytes + fp + rfp > sv->opt.hbm_budget) break;
+				if (first && sv->inflight_bytes + rfp > sv->opt.hbm_budget && sv->inflight_bytes > 0) break;
 				take.push_back(sv->pending.front());
 				sv->pending.pop_front();
-				reads += r.n, fp += r.footprint;
+				reads += r.n, fp += rfp, bases += r.bases;
 			}
 			if (take.empty()) {
 				// memory is held by other steps: wait for one to finish
@@ -2093,37 +2173,41 @@ void serve_loop(hero_server *sv, int wi) {
 			}
 			sv->inflight_bytes += fp;
 			const double t = now_s();
-			for (int64_t id : take) sv->reqs[(size_t) id].info.t_admit = t;
+			for (int64_t id : take) sv->reqs.at(id).info.t_admit = t;
 		}
-		// one step over the merged batch
-		seq.clear(), off.clear(), uid.clear(), len.clear();
-		for (int64_t id : take) {
-			const Request &r = sv->reqs[(size_t) id];
-			for (int64_t i = 0; i < r.n; i++) {
-				off.push_back((int64_t) seq.size());
-				len.push_back(r.len[i]);
-				uid.push_back(r.uid[i]);
-				seq.insert(seq.end(), r.seq + r.off[i], r.seq + r.off[i] + r.len[i]);
-			}
-		}
-		out.resize(seq.size());
 		const int64_t step_id = sv->steps.fetch_add(1);
-		const int rc = correct_batch_on(c, w, (int64_t) len.size(), seq.data(), off.data(), len.data(), uid.data(),
-				out.data());
-		size_t i0 = 0;
-		for (int64_t id : take) {
-			Request &r = sv->reqs[(size_t) id];
-			if (rc == HERO_OK)
-				for (int64_t i = 0; i < r.n; i++) std::memcpy(r.out + r.off[i], out.data() + off[i0 + (size_t) i], (size_t) r.len[i]);
-			i0 += (size_t) r.n;
+		std::vector<int> status(take.size(), HERO_OK);
+		int rc = serve_step(sv, w, take, seq, out, off, uid, len);
+		if (rc == HERO_ERR_NOMEM && take.size() > 1) {
+			// the merged step did not fit: free this worker's buffers, then its batches one by one
+			w->release_buffers();
+			for (size_t t = 0; t < take.size(); t++) status[t] = serve_step(sv, w, {take[t]}, seq, out, off, uid, len);
+			rc = HERO_OK;
+		} else {
+			for (int &x : status) x = rc;
+		}
+		if (rc == HERO_OK) {
+			// learn the footprint: the worker's buffers over the bases of the largest step it ran
+			std::lock_guard<std::mutex> lk(sv->mu);
+			int64_t &mb = sv->max_step_bases[(size_t) wi];
+			mb = std::max(mb, bases);
+			if (mb > 0) {
+				const int64_t per = (int64_t) w->device_bytes() / mb + 1;
+				sv->bytes_per_base = sv->learned ? std::max(sv->bytes_per_base, per) : per;
+				sv->learned = true;
+			}
 		}
 		{
 			std::lock_guard<std::mutex> lk(sv->mu);
-			if (rc != HERO_OK && sv->err.empty()) sv->err = c->err;
+			for (size_t t = 0; t < take.size(); t++)
+				if (status[t] != HERO_OK && sv->err.empty()) {
+					std::lock_guard<std::mutex> ek(c->err_mu);
+					sv->err = c->err;
+				}
 			const double t = now_s();
-			for (int64_t id : take) {
-				Request &r = sv->reqs[(size_t) id];
-				r.info.status = rc, r.info.t_done = t, r.info.step_id = step_id;
+			for (size_t x = 0; x < take.size(); x++) {
+				Request &r = sv->reqs.at(take[x]);
+				r.info.status = status[x], r.info.t_done = t, r.info.step_id = step_id;
 				r.info.step_reads = (int32_t) len.size(), r.info.step_batches = (int32_t) take.size();
 				r.done = true;
 			}
@@ -2153,6 +2237,7 @@ extern "C" int hero_server_start(hero_ctx *c, const hero_server_options *opt, he
 		sv->opt.hbm_budget = (int64_t) (fr / 2);
 	}
 	sv->bytes_per_base = kServeBytesPerBase;
+	sv->max_step_bases.assign((size_t) sv->opt.workers, 0);
 	sv->workers.resize((size_t) sv->opt.workers);
 	sv->acc.assign((size_t) sv->opt.workers, hero_stats{});
 	for (int i = 0; i < sv->opt.workers; i++) {
@@ -2174,14 +2259,14 @@ extern "C" int hero_server_submit(hero_server *sv, int64_t n, const char *seq, c
 	Request r;
 	r.n = n, r.seq = seq, r.off = off, r.len = len, r.uid = uid, r.out = corrected;
 	for (int64_t i = 0; i < n; i++) r.bases += len[i];
-	r.footprint = r.bases * sv->bytes_per_base;
-	if (r.footprint > sv->opt.hbm_budget) return HERO_ERR_NOMEM; // would never be admitted
 	r.info.t_submit = now_s();
 	{
 		std::lock_guard<std::mutex> lk(sv->mu);
 		if (sv->stop) return HERO_ERR_STATE;
-		*ticket = (int64_t) sv->reqs.size();
-		sv->reqs.push_back(r);
+		r.footprint = r.bases * sv->bytes_per_base;
+		if (r.footprint > sv->opt.hbm_budget) return HERO_ERR_NOMEM; // would never be admitted
+		*ticket = sv->next_ticket++;
+		sv->reqs.emplace(*ticket, r);
 		sv->pending.push_back(*ticket);
 	}
 	sv->cv_work.notify_one();
@@ -2191,11 +2276,12 @@ extern "C" int hero_server_submit(hero_server *sv, int64_t n, const char *seq, c
 extern "C" int hero_server_wait(hero_server *sv, int64_t ticket, hero_request_info *info) {
 	if (!sv) return HERO_ERR_ARG;
 	std::unique_lock<std::mutex> lk(sv->mu);
-	if (ticket < 0 || ticket >= (int64_t) sv->reqs.size()) return HERO_ERR_ARG;
-	sv->cv_done.wait(lk, [&] { return sv->reqs[(size_t) ticket].done; });
-	const Request &r = sv->reqs[(size_t) ticket];
-	if (info) *info = r.info;
-	return r.info.status;
+	if (sv->reqs.find(ticket) == sv->reqs.end()) return HERO_ERR_ARG; // unknown, or already waited for
+	sv->cv_done.wait(lk, [&] { return sv->reqs.at(ticket).done; });
+	const hero_request_info ri = sv->reqs.at(ticket).info;
+	sv->reqs.erase(ticket); // a ticket is returned once; its slot is freed
+	if (info) *info = ri;
+	return ri.status;
 }
 
 extern "C" int hero_server_stop(hero_server *sv, hero_stats *merged) {
@@ -2217,7 +2303,10 @@ extern "C" int hero_server_stop(hero_server *sv, hero_stats *merged) {
 	}
 	for (auto &w : sv->workers) w.release();
 	const int rc = sv->err.empty() ? HERO_OK : HERO_ERR_HIP;
-	if (!sv->err.empty()) sv->c->err = sv->err;
+	if (!sv->err.empty()) {
+		std::lock_guard<std::mutex> ek(sv->c->err_mu);
+		sv->c->err = sv->err;
+	}
 	delete sv;
 	return rc;
 }

@@ -143,16 +143,24 @@ hipError_t launch_index_count(const uint32_t *occ_slot, uint64_t nocc, Slot *slo
 // Bucket ranges: one cursor atomic per block of kAllocPerThread * 256 slots (the slots of a block
 // get consecutive ranges; where buckets lie in `entries` is invisible to lookups).
 constexpr int kAllocPerThread = 16;
+// cursor[0]: the next free entry; cursor[1] += the reference's distinct keys (each half of a
+// canonical slot that holds occurrences is one of the reference's keys).
 __global__ __launch_bounds__(256) void k_index_alloc(const uint32_t *owner, uint64_t P, const uint64_t *occ_hi,
 		const uint64_t *occ_lo, int k, Slot *slots, unsigned long long *cursor) {
 	__shared__ uint32_t wave_tot[4];
 	__shared__ uint32_t block_base;
 	const uint64_t base = (uint64_t) blockIdx.x * 256 * kAllocPerThread + threadIdx.x;
-	uint32_t cnt = 0;
+	uint32_t cnt = 0, keys = 0;
 	for (int i = 0; i < kAllocPerThread; i++) {
 		const uint64_t s = base + (uint64_t) i * 256;
-		if (s < P && owner[s]) cnt += slots[s].nfwd + slots[s].nrc;
+		if (s < P && owner[s]) {
+			const uint32_t nf = slots[s].nfwd, nr = slots[s].nrc;
+			cnt += nf + nr;
+			keys += (nf ? 1u : 0u) + (nr ? 1u : 0u);
+		}
 	}
+	keys = wave_sum(keys);
+	if (lane_id() == 0 && keys) atomicAdd(&cursor[1], (unsigned long long) keys);
 	const uint32_t pre = wave_excl_scan(cnt);
 	const int wv = threadIdx.x >> 6;
 	if (lane_id() == 63) wave_tot[wv] = pre + cnt;

@@ -113,3 +113,33 @@ def test_lookup_buckets_match_oracle(k):
                 pos.append(L - k)
             exp += [(p << 40) | q for p in pos if s[p:p + k] == w]
         assert [int(x) for x in g] == exp
+
+
+def test_distinct_keys_and_the_reference_table_limit():
+    """The device table is sized to the keys; the reference's size only decides whether it would
+    exit: more distinct keys (a k-mer and its reverse complement count apart) than its table holds
+    (HashTableLongKey.cpp:320-326) gives HERO_ERR_REF_ABORT."""
+    rs = synth.generate(seed=17, genome_len=40000, len_min=1400, len_max=1600, coverage=4.0, sub_5p=0.01)
+    k = 31
+    keys = set()
+    for i in range(rs.n):
+        s = rs.read(i)
+        if len(s) <= 40 or any(c not in "ACGT" for c in s) or max(s.count(c) for c in "ACGT") >= int(len(s) * 0.8):
+            continue
+        pos = list(range(0, len(s) - k + 1, k))
+        if pos[-1] + k < len(s):
+            pos.append(len(s) - k)
+        keys.update(s[p:p + k] for p in pos)
+    with gpu.Context(call_consensus=1, k=k, hash_factor=128) as ctx:
+        ctx.set_queries(rs)
+        ctx.build_index()
+        st = ctx.stats()
+    assert st["distinct_keys"] == len(keys)
+    assert st["table_size"] == gpu.lib().hero_prime_larger_than(128 * rs.n)
+    small = gpu.lib().hero_prime_larger_than(1 * rs.n)
+    assert len(keys) > small  # the reference's table of -f 1 cannot hold them
+    with gpu.Context(call_consensus=1, k=k, hash_factor=1) as ctx:
+        ctx.set_queries(rs)
+        with pytest.raises(gpu.HeroError) as e:
+            ctx.build_index()
+        assert e.value.code == -6

@@ -128,3 +128,20 @@ def test_serving_requires_correction_mode():
         ctx.build_subject_index()
         with pytest.raises(gpu.HeroError):
             ctx.correct_batch(rs.subset(fresh[:10]))
+
+
+def test_request_slot_is_freed_after_wait(served):
+    """A ticket is returned once: its slot is released at hero_server_wait, a second wait on it is
+    refused, and later tickets keep working."""
+    import ctypes
+    rs, resident, fresh, again, ctx = served
+    with gpu.Server(ctx, workers=1) as srv:
+        t = srv.submit(rs.subset(fresh[:5]))
+        out, info = srv.wait(t)
+        assert info["status"] == 0
+        rc = gpu.lib().hero_server_wait(srv.h, t, ctypes.byref(gpu.RequestInfo()))
+        assert rc == -1  # HERO_ERR_ARG: already returned
+        t2 = srv.submit(rs.subset(fresh[5:9]))
+        assert t2 != t
+        out2, info2 = srv.wait(t2)
+        assert info2["status"] == 0 and np.array_equal(out2, ctx.correct_batch(rs.subset(fresh[5:9])))
