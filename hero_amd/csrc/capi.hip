@@ -2114,12 +2114,14 @@ namespace {
 constexpr int64_t kServeBytesPerBase = 2048;
 
 // One step over the requests `ids` on worker w: their reads merged, corrected, copied back.
-int serve_step(hero_server *sv, Worker *w, const std::vector<int64_t> &ids, std::vector<char> &seq,
+// The requests are the worker's: taken under the lock as stable map nodes, erased only after they
+// are done, so no lookup in the shared map happens here.
+int serve_step(hero_server *sv, Worker *w, const std::vector<Request *> &ids, std::vector<char> &seq,
 		std::vector<char> &out, std::vector<int64_t> &off, std::vector<int64_t> &uid, std::vector<int32_t> &len) {
 	hero_ctx *c = sv->c;
 	seq.clear(), off.clear(), uid.clear(), len.clear();
-	for (int64_t id : ids) {
-		const Request &r = sv->reqs.at(id);
+	for (const Request *rp : ids) {
+		const Request &r = *rp;
 		for (int64_t i = 0; i < r.n; i++) {
 			off.push_back((int64_t) seq.size());
 			len.push_back(r.len[i]);
@@ -2131,8 +2133,8 @@ int serve_step(hero_server *sv, Worker *w, const std::vector<int64_t> &ids, std:
 	const int rc = correct_batch_on(c, w, (int64_t) len.size(), seq.data(), off.data(), len.data(), uid.data(), out.data());
 	if (rc != HERO_OK) return rc;
 	size_t i0 = 0;
-	for (int64_t id : ids) {
-		const Request &r = sv->reqs.at(id);
+	for (const Request *rp : ids) {
+		const Request &r = *rp;
 		for (int64_t i = 0; i < r.n; i++) std::memcpy(r.out + r.off[i], out.data() + off[i0 + (size_t) i], (size_t) r.len[i]);
 		i0 += (size_t) r.n;
 	}
@@ -2148,7 +2150,7 @@ void serve_loop(hero_server *sv, int wi) {
 	std::vector<int32_t> len;
 	const int64_t share = sv->opt.hbm_budget / std::max(1, sv->opt.workers);
 	for (;;) {
-		std::vector<int64_t> take;
+		std::vector<Request *> take;
 		int64_t fp = 0, bases = 0;
 		{
 			std::unique_lock<std::mutex> lk(sv->mu);
@@ -2162,7 +2164,7 @@ void serve_loop(hero_server *sv, int wi) {
 				if (!first && (reads + r.n > sv->opt.max_step_reads || fp + rfp > share)) break;
 				if (!first && sv->inflight_bytes + fp + rfp > sv->opt.hbm_budget) break;
 				if (first && sv->inflight_bytes + rfp > sv->opt.hbm_budget && sv->inflight_bytes > 0) break;
-				take.push_back(sv->pending.front());
+				take.push_back(&r);
 				sv->pending.pop_front();
 				reads += r.n, fp += rfp, bases += r.bases;
 			}
@@ -2173,7 +2175,7 @@ void serve_loop(hero_server *sv, int wi) {
 			}
 			sv->inflight_bytes += fp;
 			const double t = now_s();
-			for (int64_t id : take) sv->reqs.at(id).info.t_admit = t;
+			for (Request *rp : take) rp->info.t_admit = t;
 		}
 		const int64_t step_id = sv->steps.fetch_add(1);
 		std::vector<int> status(take.size(), HERO_OK);
@@ -2206,7 +2208,7 @@ void serve_loop(hero_server *sv, int wi) {
 				}
 			const double t = now_s();
 			for (size_t x = 0; x < take.size(); x++) {
-				Request &r = sv->reqs.at(take[x]);
+				Request &r = *take[x];
 				r.info.status = status[x], r.info.t_done = t, r.info.step_id = step_id;
 				r.info.step_reads = (int32_t) len.size(), r.info.step_batches = (int32_t) take.size();
 				r.done = true;
