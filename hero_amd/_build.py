@@ -2,7 +2,10 @@
 
   hero_amd/_lib/libhero_gpu.so    HIP kernels + C ABI (include/hero_gpu.h), gfx950
   hero_amd/_lib/libhero_synth.so  synthetic read generator (host C++)
-  hero_amd/_lib/hero_cli          command-line driver with the reference's flags
+  hero_amd/_lib/libhero_host.so   C++ host classes with the reference's class names (over libhero_gpu)
+  tests/native/host_shim_main     test driver of the host classes (the reference's flags in `run` mode)
+  tests/native/libhero_hostsim.so CPU restatement of the device cores, for CPU tests
+  oracle/_ref/                    the reference program and probe, where its sources exist
 """
 import os
 import shutil

@@ -1747,6 +1747,32 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 	return HERO_OK;
 }
 
+int hero_set_tally(hero_ctx *c, const uint32_t *tally) {
+	if (!c || !tally) return HERO_ERR_ARG;
+	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (!c->q_bases) return HERO_OK;
+	// The device form: the four counts as given, and a difference array whose running sum is each
+	// base's total count, so that consensus decodes the own base's count to exactly its given value.
+	std::vector<uint32_t> h((size_t) 5 * c->q_bases);
+	uint32_t *h4 = h.data(), *hd = h.data() + 4 * c->q_bases;
+	uint64_t b = 0;
+	uint32_t prev = 0;
+	for (int64_t i = 0; i < c->Qd.n; i++) {
+		const int L = c->q_len[(size_t) i];
+		const uint32_t *src = tally + 4 * c->q_off[(size_t) i];
+		for (int p = 0; p < L; p++, b++) {
+			uint32_t tot = 0;
+			for (int j = 0; j < 4; j++) h4[4 * b + j] = src[4 * p + j], tot += src[4 * p + j];
+			hd[b] = tot - prev; // mod 2^32, as the votes themselves
+			prev = tot;
+		}
+	}
+	HIPCHK(c, hipMemcpyAsync(c->tally.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->w.st));
+	HIPCHK(c, hipStreamSynchronize(c->w.st));
+	c->cons_valid = false;
+	return HERO_OK;
+}
+
 extern "C" int hero_coverage_histogram(hero_ctx *c, uint64_t *counts, int32_t cap, int32_t *max_coverage) {
 	if (!c || !counts || cap < 1 || !max_coverage) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));

@@ -104,6 +104,7 @@ SIGNATURES = {
     "hero_copy_corrected": (ctypes.c_int, [_P, _P]),
     "hero_take_alignments": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "hero_get_tally": (ctypes.c_int, [_P, _P]),
+    "hero_set_tally": (ctypes.c_int, [_P, _P]),
     "hero_coverage_histogram": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "hero_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "hero_reset_stats": (ctypes.c_int, [_P]),
@@ -257,6 +258,12 @@ class Context:
         t = np.zeros(4 * len(self._queries.seq), dtype=np.uint32)
         self._check(lib().hero_get_tally(self.h, t.ctypes.data))
         return t
+
+    def set_tally(self, t):
+        """Replaces the vote matrices (4 u32 per base, A,C,T,G) via hero_set_tally."""
+        t = np.ascontiguousarray(t, dtype=np.uint32)
+        assert t.size == 4 * len(self._queries.seq)
+        self._check(lib().hero_set_tally(self.h, t.ctypes.data))
 
     def coverage_histogram(self, cap=1 << 16):
         """--mapping output: bases by coverage (counts[v], v = 0 .. max) via hero_coverage_histogram."""

@@ -149,6 +149,12 @@ int hero_copy_corrected(hero_ctx *ctx, char *corrected);
  * after the subject passes, before consensus adds the self vote. */
 int hero_get_tally(hero_ctx *ctx, uint32_t *tally);
 
+/* Writes the vote matrices (QueryRead::aCoverageMatrix, src/QueryRead.h:19, the tally that
+ * QueryRead::callConsensus reads at src/QueryRead.cpp:148-180): 4 u32 per base in A,C,T,G order at
+ * the queries' offsets, the layout hero_get_tally returns. It replaces the votes of the subject
+ * passes; hero_call_consensus then decides every base from it (self vote included). */
+int hero_set_tally(hero_ctx *ctx, const uint32_t *tally);
+
 /* --merge: QueryRead::aExtraCoverageMatrix of every query read (src/QueryRead.cpp:100-115,353-371),
  * 20 positions x 4 u32 in A,C,T,G order, position 0 = the first subject base after a record's end;
  * out holds 80 x the query count. */

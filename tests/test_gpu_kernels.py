@@ -143,3 +143,29 @@ def test_distinct_keys_and_the_reference_table_limit():
         with pytest.raises(gpu.HeroError) as e:
             ctx.build_index()
         assert e.value.code == -6
+
+
+@pytest.mark.gpu
+def test_consensus_kernel_known_answers():
+    """k_consensus on the reference's callConsensus answers (QueryRead.cpp:148-180): every tally of
+    consensus_cases.jsonl, written with hero_set_tally. Each case read follows a 40-base ACGT prefix
+    with an empty tally, so every read passes the query filter (QueryDataset.cpp:60-75) and is
+    corrected; the prefix keeps its bases (the self vote alone)."""
+    cases = load_jsonl("consensus_cases.jsonl")
+    pad = "ACGT" * 10
+    for thr in sorted({c["thr"] for c in cases}):
+        group = [c for c in cases if c["thr"] == thr]
+        rs = synth.ReadSet.from_strings([pad + c["seq"] for c in group])
+        t = np.zeros(4 * len(rs.seq), dtype=np.uint32)
+        for i, c in enumerate(group):
+            o = int(rs.offset[i]) + len(pad)
+            t[4 * o:4 * (o + len(c["seq"]))] = c["tally"]
+        with gpu.Context(call_consensus=1, k=11, min_overlap=0, consensus_freq=thr) as ctx:
+            ctx.set_queries(rs)
+            ctx.set_tally(t)
+            assert np.array_equal(ctx.tally(), t)
+            out = ctx.consensus()
+        for i, c in enumerate(group):
+            o = int(rs.offset[i])
+            got = bytes(out[o:o + len(pad) + len(c["seq"])]).decode()
+            assert got == pad + c["out"], (thr, c)
