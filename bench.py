@@ -381,6 +381,36 @@ def prepare_serving():
     return rs, pool_idx, ctx, t2 - t1
 
 
+def profile_pass(mode):
+    """One pass of a single form, so that a profiler's per-kernel averages cover launches of one kind."""
+    if mode == "serve":
+        rs, pool_idx, ctx, _ = prepare_serving()
+        with gpu.Server(ctx, workers=2, max_step_reads=32768) as srv:
+            batches = [rs.subset(pool_idx[(j * 4096 + np.arange(4096)) % pool_idx.size]) for j in range(400)]
+            t0 = time.perf_counter()
+            for t in [srv.submit(b) for b in batches]:
+                srv.wait(t)
+            dt = time.perf_counter() - t0
+        ctx.close()
+        log(f"profile serve: {len(batches)} batches of 4096 reads in {dt * 1e3:.1f} ms "
+            f"({len(batches) * 4096 / dt:.0f} reads/s)")
+        return
+    rs = synth.generate(**C2_READS)
+    if mode == "serial":
+        st = one_form(rs, C2_OPTS, 0, autotune=False)
+        log(f"profile serial (two identical passes, the second timed): index {st['ms_index']:.1f}, seeds "
+            f"{st['ms_seed']:.1f}, plan {st['ms_plan']:.1f}, dp {st['ms_dp']:.1f}, score+votes "
+            f"{st['ms_score_vote']:.1f}, consensus {st['ms_consensus']:.1f} ms; per pass: dp_tasks {st['dp_tasks']}, "
+            f"dp_cells_all {st['dp_cells_all']}, dp_cells_exec {st['dp_cells_exec']}, lookups {st['lookups']}, "
+            f"seeds {st['seeds']}")
+        return
+    with gpu.Context(**C2_OPTS) as ctx:
+        ctx.set_queries(rs)
+        run_steps(ctx, 1)
+        ms, st = run_steps(ctx, 1)
+    log(f"profile step (a warmup and a timed step): {ms:.1f} ms, {rs.n / (ms / 1e3):.0f} reads/s")
+
+
 def copy_calibration(nbytes=1 << 30):
     """A plain device copy of nbytes (reads nbytes, writes nbytes): under a counter profiler its
     FETCH_SIZE / WRITE_SIZE give the scale between counter units and bytes."""
@@ -447,6 +477,10 @@ def main():
                     help="also run a 1 GiB device copy (the reference point for HBM byte counters)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c5"],
                     help="c2: the headline step (default); c5: the repeat-heavy share at its real shape")
+    ap.add_argument("--profile", choices=["step", "serial", "serve"],
+                    help="one pass for a profiler, no JSON line: step = a warmup and a timed C2 step (two chunk "
+                         "workers); serial = the same with one chunk in flight; serve = 4,096-read batches "
+                         "through the request loop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path; a subject set "
@@ -456,6 +490,9 @@ def main():
 
     if args.workload == "c5":
         c5_run(args.steps)
+        return
+    if args.profile:
+        profile_pass(args.profile)
         return
     if args.copy_calibration:
         copy_calibration()

@@ -1756,10 +1756,10 @@ int hero_set_tally(hero_ctx *c, const uint32_t *tally) {
 	std::vector<uint32_t> h((size_t) 5 * c->q_bases);
 	uint32_t *h4 = h.data(), *hd = h.data() + 4 * c->q_bases;
 	uint64_t b = 0;
-	uint32_t prev = 0;
 	for (int64_t i = 0; i < c->Qd.n; i++) {
 		const int L = c->q_len[(size_t) i];
 		const uint32_t *src = tally + 4 * c->q_off[(size_t) i];
+		uint32_t prev = 0; // each read's running sum starts at its first base
 		for (int p = 0; p < L; p++, b++) {
 			uint32_t tot = 0;
 			for (int j = 0; j < 4; j++) h4[4 * b + j] = src[4 * p + j], tot += src[4 * p + j];

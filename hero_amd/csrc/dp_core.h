@@ -10,6 +10,7 @@
 // above -MAXDUL), so every comparison the reference makes has the same outcome.
 #pragma once
 #include <type_traits>
+#include <utility>
 
 #include "hd.h"
 
@@ -20,6 +21,12 @@ constexpr int kCodeN = 4;    // 'N' (BandAligner::CWILD) in the standalone align
 constexpr int kCodeNone = 7; // beyond the end of a sequence (never equal to a base)
 
 HD int imax_(int a, int b) { return a > b ? a : b; }
+
+// f(integral_constant<int, K>) for each K of the sequence, in order
+template <class F, int... K>
+HD void for_each_index(F &f, std::integer_sequence<int, K...>) {
+	(f(std::integral_constant<int, K>()), ...);
+}
 
 // Wave-uniform row decisions of the per-lane DP (one task per lane): true when the predicate
 // holds in any / every active lane. A single task (host code) is its own wave.
@@ -381,8 +388,63 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 		if (MODE & kCQ) Gq = gq, XEq = xeq;
 		if (kLast < 16) pw[NW - 1] >>= 2 * (16 - kLast);
 	};
+	// Head rows 1..BMAX when every call of the wavefront has band BMAX and no aligned column lenQ
+	// among them (lenQ > 2 BMAX where aligned left): row I's column 0 is band cell BMAX - I for all
+	// lanes, a compile-time index, so the row starts there with the column-0 values and takes the
+	// interior rule for the cells after it. The cells before it lie left of column 0: no cell of
+	// the matrix reads them and the traceback never visits them, so they are not computed (their
+	// pointer bits stay 0; the shift-in below still puts every computed cell at bit 2 (d % 16)).
+	auto row_head = [&](int sc, uint32_t *pw, auto itag) {
+		constexpr int I = decltype(itag)::value;
+		constexpr int D0 = BMAX - I;
+		const int xv = right_al ? I * o.sx : o.dx + (I - 1) * o.ex;
+		const int yv = right_al ? o.dy : xv;
+		const int off = step * I;
+		const int h0 = 8 * xv + 2 - off, x0 = 8 * xv + 3 - off, y0 = 8 * yv - off, g0 = 8 * xv - off + C;
+		const uint64_t mbits = qw.differ_bits(sc);
+		const uint32_t mlo = (uint32_t) mbits, mhi = (uint32_t) (mbits >> 32);
+		int gl, yel;
+		{ // column 0 (src/BandAligner.cpp:150-163): a vertical gap from the origin, pointing up
+			const int mx = max3_(h0, x0, y0);
+			pw[D0 >> 4] = (pw[D0 >> 4] >> 2) | (3u << 30);
+			MK[D0] = (mx & ~3) | 2;
+			U[D0] = imax_(g0, x0 + E) | 3;
+			gl = g0, yel = y0 + E;
+		}
+#pragma unroll
+		for (int d = D0 + 1; d < W; d++) {
+			const int m = (int) bit_u32(d < 16 ? mlo : mhi, 2 * (d & 15));
+			const int h = mad01(m, DM, MK[d]);
+			const int x = U[d + 1];
+			const int y = imax_(gl, yel);
+			const int g = h + m + (C - 2);
+			const int mx = max3_(h, x, y);
+			pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) mx << 30);
+			MK[d] = (mx & ~3) | 2;
+			U[d] = imax_(g, x + E) | 3;
+			gl = g, yel = y + E;
+		}
+		if (kLast < 16) pw[NW - 1] >>= 2 * (16 - kLast);
+	};
+	auto head = [&](auto itag) {
+		constexpr int I = decltype(itag)::value + 1;
+		if (I <= lenS) {
+			const int sc = S.next();
+			uint32_t pw[NW];
+#pragma unroll
+			for (int w = 0; w < NW; w++) pw[w] = 0;
+			row_head(sc, pw, std::integral_constant<int, I>());
+			ptrs.put_row(I, pw);
+			qw.slide(Q.next()); // I - b + W - 1 = I + BMAX >= 0
+		}
+	};
+	int i0 = 1;
+	if (mode_or == 0 && wave_all(b == BMAX && !(left_al && lenQ <= 2 * BMAX))) {
+		for_each_index(head, std::make_integer_sequence<int, BMAX>());
+		i0 = BMAX + 1;
+	}
 	const int cut = mode_or | (wave_all(b == BMAX) ? 0 : kCut);
-	for (int i = 1; i <= lenS; i++) {
+	for (int i = i0; i <= lenS; i++) {
 		const int sc = S.next();
 		uint32_t pw[NW];
 #pragma unroll
@@ -391,6 +453,7 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 		switch (mode) {
 		case 0: row(i, sc, pw, std::integral_constant<int, 0>()); break;
 		case kCut: row(i, sc, pw, std::integral_constant<int, kCut>()); break;
+		case kCQ: row(i, sc, pw, std::integral_constant<int, kCQ>()); break;
 		default: row(i, sc, pw, std::integral_constant<int, kC0 | kCQ | kCut>()); break;
 		}
 		ptrs.put_row(i, pw);

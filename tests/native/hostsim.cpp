@@ -391,6 +391,7 @@ static int tagged_check_one(const Opts &o, uint64_t &rng, int max_len, int mode_
 	T.lenQ = T.lenS - rnd((uint64_t) (T.lenS < BMAX ? T.lenS + 1 : BMAX + 1));
 	if (rnd(5) == 0) T.lenQ = rnd((uint64_t) T.lenS + 1);
 	T.b = T.lenS - T.lenQ + rnd(3);
+	if (rnd(2)) T.b = BMAX; // the full band class: the head rows of the wavefront path
 	if (T.b > BMAX) T.b = BMAX, T.lenQ = T.lenS - BMAX > 0 ? T.lenS - BMAX : 0;
 	if (T.b < T.lenS - T.lenQ) T.lenQ = T.lenS - T.b;
 	const int f = rnd(3);
