@@ -72,16 +72,24 @@ uint64_t next_prime_above(uint64_t n) {
 	return p;
 }
 
+// Device and pinned allocations, frees and DP graph captures of the whole process take turns.
+// Two chunk or serving workers run on their own host threads and streams: a free (which waits for
+// the device) or an allocation on one of them must not fall inside the other's stream capture,
+// whose work is recorded, not run, until the capture ends.
+std::recursive_mutex g_dev_mu;
+
 template <class T>
 struct DevBuf {
 	T *p = nullptr;
 	size_t n = 0;
 	void release() {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		if (p) (void) hipFree(p);
 		p = nullptr, n = 0;
 	}
 	hipError_t ensure(size_t want) {
 		if (want <= n && p) return hipSuccess;
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		release();
 		size_t alloc = want + want / 4 + 64;
 		hipError_t e = hipMalloc((void **) &p, alloc * sizeof(T));
@@ -93,6 +101,7 @@ struct DevBuf {
 		return hipSuccess;
 	}
 	hipError_t exact(size_t want) {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		release();
 		hipError_t e = hipMalloc((void **) &p, (want ? want : 1) * sizeof(T));
 		if (e != hipSuccess) {
@@ -110,11 +119,13 @@ struct HostBuf {
 	T *p = nullptr;
 	size_t n = 0;
 	void release() {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		if (p) (void) hipHostFree(p);
 		p = nullptr, n = 0;
 	}
 	hipError_t ensure(size_t want) {
 		if (want <= n && p) return hipSuccess;
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		release();
 		const size_t alloc = want + want / 4 + 64;
 		hipError_t e = hipHostMalloc((void **) &p, alloc * sizeof(T), hipHostMallocDefault);
@@ -204,6 +215,7 @@ struct Worker {
 	DevBuf<uint8_t> b_sort_tmp;
 	// frees the grow-only work buffers (stream, events, counters and the DP graph stay)
 	void release_buffers() {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		if (st) (void) hipStreamSynchronize(st);
 		if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
 		if (graph) (void) hipGraphDestroy(graph);
@@ -237,6 +249,7 @@ struct Worker {
 		return b;
 	}
 	hipError_t init() {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		hipError_t e = hipStreamCreate(&st);
 		for (auto &t : ev)
 			if (e == hipSuccess) e = hipEventCreate(&t.a);
@@ -249,6 +262,7 @@ struct Worker {
 		return e;
 	}
 	void release() {
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu);
 		if (st) (void) hipStreamSynchronize(st);
 		seg_n.release(), seg_off.release(), scan_tmp.release(), lane_n.release(), seeds.release(), win_info.release();
 		cnext.release(), cfirst.release(), clast.release(), clen.release(), gflag.release();
@@ -702,7 +716,11 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		key.push_back(v);
 	for (int cls = 0; cls < kNumBandClasses; cls++) key.push_back((uintptr_t) (per_lane[cls] * 100000 + waves[cls]));
 	if (!w->graph_exec || key != w->graph_key) {
-		if (w->graph_exec) (void) hipGraphExecDestroy(w->graph_exec);
+		std::lock_guard<std::recursive_mutex> lk(g_dev_mu); // no allocation or free elsewhere meanwhile
+		if (w->graph_exec) {
+			HIPCHK(c, hipStreamSynchronize(w->st)); // the old graph has finished before it goes
+			(void) hipGraphExecDestroy(w->graph_exec);
+		}
 		if (w->graph) (void) hipGraphDestroy(w->graph);
 		w->graph_exec = nullptr, w->graph = nullptr, w->graph_key.clear();
 		uint32_t *ctl = w->ctl.p;
