@@ -550,6 +550,14 @@ def main():
         stp = one_form(rs, C2_OPTS, 1)
         gcups["plain"] = round(gcups_of(stp), 2)
         gcups["batched_over_plain"] = round(gcups_batched / gcups["plain"], 2) if gcups["plain"] else None
+        # the plain form's own roofline: one wavefront per call, a lane per pair of band diagonals, so
+        # at band 8 about 9 of 64 lanes hold a cell on each anti-diagonal step
+        plain_tops = gcups["plain"] * AFFINE_OPS_PER_CELL / 1e3
+        lane_bound = (C2_OPTS["band_override"] + 1) / 64
+        gcups["plain_roofline"] = dict(bound="valu", achieved=round(plain_tops, 3), peak=round(VALU_PEAK_TOPS, 2),
+                                       unit="Tops/s (int32 VALU)", frac=round(plain_tops / VALU_PEAK_TOPS, 4),
+                                       lane_bound=round(lane_bound, 4),
+                                       frac_of_lane_bound=round(plain_tops / VALU_PEAK_TOPS / lane_bound, 4))
         rs3 = synth.generate(**C3_READS)
         mixed = dict(workload=f"10 Mb genome, 75-250 bp reads at 40x ({rs3.n} reads), band 2*round(0.025*overlap)")
         for leg, opts in C3_LEGS:

@@ -340,8 +340,9 @@ struct hero_ctx {
 	hero_stats st_w2;
 	std::mutex err_mu; // fail() may run on a chunk worker's thread
 	// autotuned DP schedule (hero_autotune): tasks per lane and persistent waves per CU per band class
-	int tuned_per_lane[kNumBandClasses] = {1, 1, 1, 1, 1, 1};
-	int tuned_waves_per_cu[kNumBandClasses] = {0, 0, 0, 0, 0, 0};
+	int tuned_per_lane[kNumBandClasses] = {1, 1, 1, 1, 1, 1, 1, 1};
+	int tuned_waves_per_cu[kNumBandClasses] = {0, 0, 0, 0, 0, 0, 0, 0};
+	static_assert(kNumBandClasses == 8 && kNumBandClasses <= HERO_TUNE_MAX_CLASSES, "per-class tables");
 	hero_tune_report *tune = nullptr; // set while hero_autotune runs
 	// BandAligner drop-in (hero_band_align_batch): grow-only device buffers
 	DevBuf<hero_band_pair> ba_pairs;
@@ -502,6 +503,8 @@ int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, 
 	return HERO_OK;
 }
 
+int worker_stat(hero_ctx *c, Worker *w, int slot, uint64_t &out);
+
 // Autotuner step for one band class: runs the class's tasks under every candidate schedule (one
 // or two tasks per lane; persistent waves per CU from 1 to the occupancy limit), each timed with
 // HIP events after an untimed run, and keeps the fastest. Every candidate computes the same
@@ -511,6 +514,8 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	hero_tune_class &tc = rep.cls[cls];
 	tc.band = bm, tc.tasks = (int64_t) Lc.ntasks;
 	float best = -1.0f;
+	uint64_t cells0 = 0, cells1 = 0;
+	if (int r = worker_stat(c, w, kStCellsAll, cells0)) return r;
 	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand && pk16_class_fits(c->o, rows) ? 2 : 1); per_lane++) {
 		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane, rows, Lc.tagged != 0);
 		for (int wpc = 1; wpc <= occ; wpc *= 2) {
@@ -528,6 +533,10 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 				HIPCHK(c, hipEventRecord(w->ev[3].a, w->st));
 				HIPCHK(c, hipStreamSynchronize(w->st));
 				HIPCHK(c, hipEventElapsedTime(&ms, w->ev[9].a, w->ev[3].a));
+			}
+			if (tc.ncand == 0) { // band cells of the class: the first candidate's two runs
+				if (int r = worker_stat(c, w, kStCellsAll, cells1)) return r;
+				tc.cells = (int64_t) ((cells1 - cells0) / 2);
 			}
 			if (per_lane == 1 && wpc_eff == dp_batched_waves_per_cu(bm, aff, 1, rows, Lc.tagged != 0)) tc.ms_default = ms;
 			if (tc.ncand < (int) (sizeof tc.cand / sizeof tc.cand[0]))
@@ -568,6 +577,16 @@ hipError_t launch_wave_stage(const DpLaunch &Lp, const WaveSizing &z, uint32_t *
 	for (int l = 0; l < kWaveLaunches; l++)
 		if (hipError_t e = launch_dp_wave_tasks(Lp, z, l, ctl_wave, over, st); e != hipSuccess) return e;
 	return hipSuccess;
+}
+
+// One device counter of a worker, summed over its stripes (waits for the worker's stream).
+int worker_stat(hero_ctx *c, Worker *w, int slot, uint64_t &out) {
+	std::vector<unsigned long long> stripes((size_t) kStatStripes * kStatStride);
+	HIPCHK(c, hipMemcpyAsync(stripes.data(), w->dstats.p, stripes.size() * 8, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	out = 0;
+	for (int i = 0; i < kStatStripes; i++) out += stripes[(size_t) i * kStatStride + slot];
+	return HERO_OK;
 }
 
 // Tasks the DP skipped since the last check (beyond kWaveMaxBand or the read-length limits).
@@ -1398,7 +1417,7 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 	const int limit = form == 1 ? kBandClass[kNumBandClasses - 1] : (form == 2 ? kTaggedMaxBand : kPk16MaxBand);
 	if (cls < 0 || kBandClass[cls] > limit) return fail(c, HERO_ERR_ARG, "band %d beyond the batched form's classes", bmax);
 	if (rows > 512) return fail(c, HERO_ERR_ARG, "the batched forms take at most 511 rows");
-	if (form == 2 && !dp_tagged_form(c->o, rows)) return fail(c, HERO_ERR_ARG, "the tagged rows need affine costs with dx == dy");
+	if (form == 2 && !dp_tagged_form(c->o, rows)) return fail(c, HERO_ERR_ARG, "the tagged affine rows need dx == dy");
 	if (form == 3 && !pk16_class_fits(c->o, rows)) return fail(c, HERO_ERR_ARG, "the calls exceed the 16-bit range");
 	std::vector<uint64_t> pw, pwoff;
 	pack_reads(seq, roff.data(), rlen.data(), 2 * (int64_t) n, pw, pwoff);

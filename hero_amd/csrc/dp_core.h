@@ -269,7 +269,7 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 // G and y + E of the cell to the left run along the row. The aligned column lenQ's rule needs G
 // and XE of the cell above separately: each lane keeps them for its next row's column lenQ.
 // Sentinel cells hold kSentT; the caller keeps every real value within the scaled range
-// (dp_tagged_fits). Pointer codes: 3 up, 2 match, 1 mismatch, 0 left (dp_traceback<true>).
+// (dp_tagged_fits). Pointer codes: 3 up, 2 match, 1 mismatch, 0 left (kPtrTaggedAffine).
 constexpr int kSentT = -(1 << 29);
 constexpr int kTagRange = 1 << 27;
 HD int max3_(int a, int b, int c) { return imax_(imax_(a, b), c); }
@@ -461,11 +461,111 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 	}
 }
 
+// Tagged global forward pass: the results and pointers of dp_forward<BMAX, false> (the global
+// aligner, src/BandAligner.cpp:63-111) for 2-bit codes, in fewer operations per cell.
+//
+// A cell's value h is kept as V = 8 h - 4 hit (i + j) + t, the tag t in the two low bits ranking the
+// moves as the reference breaks ties (the diagonal when dg >= lf and dg >= up, then the left move
+// when lf >= up): 3 diagonal match, 2 diagonal mismatch, 1 left, 0 up. A diagonal move adds
+// 8 (s - hit), nothing on a match; an up move adds 8 g - 4 hit, a left move 8 dy - 4 hit, both
+// multiples of 4, so the tag of a candidate is set by one or. The max3 of the three candidates is
+// the cell's value and, in its low bits, its pointer. Each row carries two values per band cell to
+// the next: MK, the cell's value with the match tag (the next row's diagonal candidate on a match;
+// a mismatch adds 8 (miss - hit) - 1), and UP, its value plus the vertical move (the up candidate
+// of the cell below). The aligned column lenQ's vertical move costs sx instead of dx: its up
+// candidate adds 8 (sx - dx). Values outside the matrix are never read (as in dp_forward).
+// Pointer codes: 0 up, 1 left, 2 mismatch, 3 match (kPtrTaggedGlobal).
+template <int BMAX, class SS, class QS, class PS>
+HD void dp_forward_tagged_global(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool right_al, SS &S, QS &Q,
+		PS &ptrs, int mode_or = 0) {
+	static_assert(SS::kTwoBit && QS::kTwoBit, "the tagged form reads 2-bit codes");
+	constexpr int W = 2 * BMAX + 1;
+	static_assert(W <= 32, "the packed query window holds 32 cells");
+	constexpr int NW = (W + 15) / 16;
+	constexpr int kLast = W - 16 * (NW - 1);
+	const int step = 4 * o.hit;
+	const int CU = 8 * o.dx - step;          // up move (dx)
+	const int CL1 = 8 * o.dy - step + 1;     // left move (dy), tag 1
+	const int DM = 8 * (o.miss - o.hit) - 1; // diagonal on a mismatch: match tag 3 -> 2
+	const int CQ = 8 * (o.sx - o.dx);        // the aligned column lenQ's vertical move costs sx
+	const int g0 = (right_al || (lenQ == 0 && left_al)) ? o.sx : o.dx; // column-0 vertical cost
+	int MK[W + 1], UP[W + 1];
+	QWin<W, true> qw;
+	{ // row 0 (src/BandAligner.cpp:63-111 at i = 0): h = j dy, pointers 1 (left) for j >= 1
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+#pragma unroll
+		for (int d = 0; d <= W; d++) {
+			int h, x, y;
+			const int j = d - b;
+			const bool in = row0_cell<false>(o, j, b, lenQ, h, x, y) && d < W;
+			const int v = 8 * h - step * j; // untagged
+			MK[d] = in ? v | 3 : kSentT;
+			UP[d] = in ? v + CU : kSentT;
+			if (d < W && in && j >= 1) pw[d >> 4] |= 1u << (2 * (d & 15));
+		}
+		ptrs.put_row(0, pw);
+	}
+#pragma unroll
+	for (int d = 0; d < W; d++) qw.set(d, d < b ? kCodeNone : Q.next()); // row 1: Q[d - b]
+	constexpr int kC0 = 1, kCQ = 2, kCut = 4;
+	const int d0base = b, dQbase = lenQ + b;
+	const int edge_d = 2 * b;
+	auto row = [&](int i, int sc, uint32_t *pw, auto mode_tag) {
+		constexpr int MODE = decltype(mode_tag)::value;
+		const int d0 = d0base - i, dQ = left_al ? dQbase - i : -1;
+		const int v0 = 8 * i * g0 - step * i; // column 0 (a vertical gap from the origin), pointing up
+		const uint64_t mbits = qw.differ_bits(sc);
+		const uint32_t mlo = (uint32_t) mbits, mhi = (uint32_t) (mbits >> 32);
+		int hl = kSentT; // this row's cell d - 1, untagged
+#pragma unroll
+		for (int d = 0; d < W; d++) {
+			const int m = (int) bit_u32(d < 16 ? mlo : mhi, 2 * (d & 15));
+			const int dg = mad01(m, DM, MK[d]);
+			int up = UP[d + 1];
+			if (MODE & kCQ) up = d == dQ ? up + CQ : up;
+			const int lf = hl + CL1;
+			int mx = max3_(dg, lf, up);
+			if (MODE & kC0) mx = d == d0 ? v0 : mx;
+			if (MODE & kCut) mx = d <= edge_d ? mx : kSentT;
+			pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) mx << 30);
+			const int v = mx & ~3;
+			MK[d] = v | 3;
+			UP[d] = v + CU;
+			hl = v;
+		}
+		if (kLast < 16) pw[NW - 1] >>= 2 * (16 - kLast);
+	};
+	const int cut = mode_or | (wave_all(b == BMAX) ? 0 : kCut);
+	for (int i = 1; i <= lenS; i++) {
+		const int sc = S.next();
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+		const int mode = cut | (wave_any(i <= b) ? kC0 : 0) | (wave_any(left_al && lenQ - i <= b) ? kCQ : 0);
+		switch (mode) {
+		case 0: row(i, sc, pw, std::integral_constant<int, 0>()); break;
+		case kCut: row(i, sc, pw, std::integral_constant<int, kCut>()); break;
+		case kCQ: row(i, sc, pw, std::integral_constant<int, kCQ>()); break;
+		default: row(i, sc, pw, std::integral_constant<int, kC0 | kCQ | kCut>()); break;
+		}
+		ptrs.put_row(i, pw);
+		qw.slide((i - b + W - 1 >= 0) ? Q.next() : kCodeNone);
+	}
+}
+
+// Pointer codes of the forward passes as maps from a stored 2-bit code t to the plain code
+// p = (MAP >> 2 t) & 3 (0 up, 1 left, 2 match, 3 mismatch).
+constexpr uint32_t kPtrPlain = 0xE4u;         // dp_forward: stored as is
+constexpr uint32_t kPtrTaggedAffine = 0x2Du;  // dp_forward_tagged: 0 left, 1 mismatch, 2 match, 3 up
+constexpr uint32_t kPtrTaggedGlobal = 0xB4u;  // dp_forward_tagged_global: 0 up, 1 left, 2 mismatch, 3 match
+
 // Traceback (src/BandAligner.cpp:114-125,227-238): from (lenS, lenQ) follow the pointers to the
 // origin. Steps come last-to-first through emit(step): kStepMatch / kStepMismatch for a diagonal,
 // kStepLeft (j-1), kStepUp (i-1). Returns the step count, or -1 if the path leaves the band
-// (never for pointers written by dp_forward). TAGGED: the pointers of dp_forward_tagged.
-template <bool TAGGED = false, class PS, class EM>
+// (never for pointers written by the forward passes). MAP: the forward pass's pointer codes.
+template <uint32_t MAP = kPtrPlain, class PS, class EM>
 HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 	int i = lenS, j = lenQ, n = 0;
 	while (!(i == 0 && j == 0)) {
@@ -473,8 +573,7 @@ HD int dp_traceback(int lenS, int lenQ, int b, const PS &ptrs, EM &emit) {
 		if (i < 0 || j < 0 || d < 0 || d > 2 * b) return -1; // left the band: corrupt pointers
 		const uint32_t word = ptrs.get(i, d >> 4);
 		const int t = (int) ((word >> (2 * (d & 15))) & 3u);
-		// tagged codes 0 left, 1 mismatch, 2 match, 3 up -> 1, 3, 2, 0
-		const int p = TAGGED ? (int) ((0x2Du >> (2 * t)) & 3u) : t;
+		const int p = (int) ((MAP >> (2 * t)) & 3u);
 		if (p >= 2) {
 			emit(p == 2 ? kStepMatch : kStepMismatch);
 			i--, j--;

@@ -221,8 +221,11 @@ inline __host__ __device__ size_t pk_lds_bytes(int rows_max) { return (size_t) 6
 // Band classes of the batched form; tasks above the last class (or with rows beyond the pointer
 // scratch) go to the plain form. Buckets are (class, 1023 - min(lenS, 1023)): one class at a time,
 // longest rows first, so the 64 lanes of a wavefront run tasks of similar length.
-constexpr int kNumBandClasses = 6;
-constexpr int kBandClass[kNumBandClasses] = {4, 8, 12, 16, 24, 32};
+// the reference's bands are even (2 round(rate * overlap), then widened to the length gap): one
+// class per even band from 4 to 12 (band 2 calls are short and few: they share class 4), then 16,
+// 24, 32
+constexpr int kNumBandClasses = 8;
+constexpr int kBandClass[kNumBandClasses] = {4, 6, 8, 10, 12, 16, 24, 32};
 constexpr int kLenBins = 1024; // per band class: 512 row counts x (aligned-left or not)
 constexpr int kNumBuckets = (kNumBandClasses + 1) * kLenBins;
 // ntasks_dev (optional) supplies the task count on the device; the grid then covers max_tasks
@@ -232,9 +235,12 @@ hipError_t launch_task_scatter(const DpTask *tasks, uint32_t ntasks, const uint3
 		uint32_t *cursor, uint32_t *order, hipStream_t st);
 // Graph form of the DP stage: class task ranges from the bucket offsets (ctl[1 + c], c = 0 ..
 // kNumBandClasses, then ctl[2 + kNumBandClasses] = ntasks = ctl[0]) and zeroed class counters
-// (ctl[16 + c]).
-constexpr int kCtlRange = 1, kCtlCounter = 16, kCtlWords = 32;
-constexpr int kCtlWave = 24; // the plain form's kWaveCtlWords control words (zeroed with the counters)
+// (ctl[16 + c]); the plain form's control words follow the counters.
+constexpr int kCtlRange = 1, kCtlCounter = 16;
+constexpr int kCtlWave = 32; // the plain form's kWaveCtlWords control words (zeroed with the counters)
+constexpr int kCtlWords = 40;
+static_assert(kCtlRange + kNumBandClasses + 2 <= kCtlCounter, "class ranges overlap the class counters");
+static_assert(kCtlCounter + kNumBandClasses <= kCtlWave, "class counters overlap the plain form's words");
 hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 // Tallies: 4 counters per base (mismatch votes by base code) at 4 * q_boff[q], and the matched-run
@@ -305,6 +311,7 @@ size_t wave_call_ptr_words(int lenS, int b);
 // work counters and the overflow list's range; launch 0 runs the given tasks and lists those wider
 // than its range in `over` (capacity: the task count) for launches 1 and 2.
 constexpr int kWaveCtlCounter = 0, kWaveCtlOver = 4, kWaveCtlWords = 8;
+static_assert(kCtlWave + kWaveCtlWords <= kCtlWords, "the plain form's words lie inside the control block");
 hipError_t launch_dp_wave_tasks(const DpLaunch &L, const WaveSizing &z, int launch, uint32_t *ctl_wave, uint32_t *over,
 		hipStream_t st);
 

@@ -1320,7 +1320,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 
 // Occupancy the register allocator plans the one-task-per-lane DP for (waves per SIMD): the most
 // that holds a band row in registers without spilling.
 constexpr int dp_waves_per_simd(int bmax) {
-	return bmax <= 4 ? 4 : bmax <= 8 ? 3 : bmax <= 12 ? 2 : 1;
+	return bmax <= 2 ? 5 : bmax <= 4 ? 4 : bmax <= 8 ? 3 : bmax <= 12 ? 2 : 1;
 }
 
 // Batched form: one task per lane, persistent waves pulling 64 tasks at a time. TAG: affine rows
@@ -1328,7 +1328,7 @@ constexpr int dp_waves_per_simd(int bmax) {
 template <int BMAX, bool AFF, bool TAG = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per_simd(BMAX), 8))) void k_dp_batched(
 		DpLaunch L0) {
-	static_assert(!TAG || (AFF && BMAX <= kTaggedMaxBand), "tagged rows: affine, band classes up to 12");
+	static_assert(!TAG || BMAX <= kTaggedMaxBand, "tagged rows: band classes up to 12");
 	DpLaunch L = L0;
 	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
@@ -1350,11 +1350,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per
 			task_streams(task, L.S, L.Q, as, aq, lenS, lenQ);
 			if (task.band <= BMAX && lenS < L.rows_max) {
 				const bool la = (task.flags & kTaskAlignLeft) != 0, ra = (task.flags & kTaskAlignRight) != 0;
-				if constexpr (TAG) dp_forward_tagged<BMAX>(L.o, lenS, lenQ, task.band, la, ra, as, aq, ptrs);
+				if constexpr (TAG && AFF) dp_forward_tagged<BMAX>(L.o, lenS, lenQ, task.band, la, ra, as, aq, ptrs);
+				else if constexpr (TAG) dp_forward_tagged_global<BMAX>(L.o, lenS, lenQ, task.band, la, ra, as, aq, ptrs);
 				else dp_forward<BMAX, AFF>(L.o, lenS, lenQ, task.band, la, ra, as, aq, ptrs);
 				const int cap = task.s_len + task.q_len;
 				StepWriter wr{L.words + task.out_off, cap, -1, 0};
-				const int ns = dp_traceback<TAG>(lenS, lenQ, task.band, ptrs, wr);
+				const int ns = dp_traceback<!TAG ? kPtrPlain : AFF ? kPtrTaggedAffine : kPtrTaggedGlobal>(lenS, lenQ,
+						task.band, ptrs, wr);
 				wr.flush();
 				L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
 				L.sums[t] = wr.sb.finish();
@@ -1367,8 +1369,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
+// the tagged rows: the global aligner, or the affine one with equal gap opens
 bool dp_tagged_form(const Opts &o, int rows_max) {
-	return o.aligner == 0 && o.dx == o.dy && dp_tagged_fits(o, rows_max);
+	return (o.aligner == 1 || o.dx == o.dy) && dp_tagged_fits(o, rows_max);
 }
 
 int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool tagged) {
@@ -1379,17 +1382,18 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool
 		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, true>, 64, 0)          \
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false>, 64, 0);
 #define HERO_OCCT(BM)                                                                                           \
-	if (bmax == BM && per_lane == 1 && aff && tagged)                                                           \
-		e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, true, true>, 64, 0);
+	if (bmax == BM && per_lane == 1 && tagged)                                                                  \
+		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, true, true>, 64, 0)    \
+		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false, true>, 64, 0);
 #define HERO_OCC2(BM)                                                                                           \
 	if (bmax == BM && per_lane == 2)                                                                            \
 		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64,           \
 		                  pk_lds_bytes(rows_max))                                                          \
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64,          \
 		                  pk_lds_bytes(rows_max));
-	HERO_OCC(4) HERO_OCC(8) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
-	HERO_OCCT(4) HERO_OCCT(8) HERO_OCCT(12)
-	HERO_OCC2(4) HERO_OCC2(8) HERO_OCC2(12)
+	HERO_OCC(4) HERO_OCC(6) HERO_OCC(8) HERO_OCC(10) HERO_OCC(12) HERO_OCC(16) HERO_OCC(24) HERO_OCC(32)
+	HERO_OCCT(4) HERO_OCCT(6) HERO_OCCT(8) HERO_OCCT(10) HERO_OCCT(12)
+	HERO_OCC2(4) HERO_OCC2(6) HERO_OCC2(8) HERO_OCC2(10) HERO_OCC2(12)
 #undef HERO_OCC
 #undef HERO_OCCT
 #undef HERO_OCC2
@@ -1497,8 +1501,9 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 		return hipGetLastError();                                                                  \
 	}
 #define HERO_DPBT(BM)                                                                              \
-	if (bmax == BM && per_lane == 1 && aff && L.tagged) {                                          \
-		k_dp_batched<BM, true, true><<<nwaves, 64, 0, st>>>(L);                                      \
+	if (bmax == BM && per_lane == 1 && L.tagged) {                                                 \
+		if (aff) k_dp_batched<BM, true, true><<<nwaves, 64, 0, st>>>(L);                             \
+		else k_dp_batched<BM, false, true><<<nwaves, 64, 0, st>>>(L);                                \
 		return hipGetLastError();                                                                  \
 	}
 #define HERO_DPB2(BM)                                                                              \
@@ -1507,9 +1512,9 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 		else k_dp_batched_pk<BM, false><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);            \
 		return hipGetLastError();                                                                  \
 	}
-	HERO_DPBT(4) HERO_DPBT(8) HERO_DPBT(12)
-	HERO_DPB(4) HERO_DPB(8) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
-	HERO_DPB2(4) HERO_DPB2(8) HERO_DPB2(12)
+	HERO_DPBT(4) HERO_DPBT(6) HERO_DPBT(8) HERO_DPBT(10) HERO_DPBT(12)
+	HERO_DPB(4) HERO_DPB(6) HERO_DPB(8) HERO_DPB(10) HERO_DPB(12) HERO_DPB(16) HERO_DPB(24) HERO_DPB(32)
+	HERO_DPB2(4) HERO_DPB2(6) HERO_DPB2(8) HERO_DPB2(10) HERO_DPB2(12)
 #undef HERO_DPBT
 #undef HERO_DPB
 #undef HERO_DPB2

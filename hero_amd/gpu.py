@@ -54,13 +54,13 @@ class TuneCandidate(ctypes.Structure):
 
 class TuneClass(ctypes.Structure):
     _fields_ = [("band", ctypes.c_int32), ("per_lane", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
-                ("ncand", ctypes.c_int32), ("tasks", ctypes.c_int64), ("ms", ctypes.c_float),
+                ("ncand", ctypes.c_int32), ("tasks", ctypes.c_int64), ("cells", ctypes.c_int64), ("ms", ctypes.c_float),
                 ("ms_default", ctypes.c_float), ("cand", TuneCandidate * 16)]
 
 
 class TuneReport(ctypes.Structure):
     _fields_ = [("classes", ctypes.c_int32), ("_pad", ctypes.c_int32), ("subjects", ctypes.c_int64),
-                ("cls", TuneClass * 6)]
+                ("cls", TuneClass * 12)]  # HERO_TUNE_MAX_CLASSES
 
     def as_list(self):
         out = []
@@ -68,8 +68,9 @@ class TuneReport(ctypes.Structure):
             c = self.cls[i]
             if c.tasks == 0:
                 continue
-            out.append(dict(band=c.band, tasks=c.tasks, per_lane=c.per_lane, waves_per_cu=c.waves_per_cu,
+            out.append(dict(band=c.band, tasks=c.tasks, cells=c.cells, per_lane=c.per_lane, waves_per_cu=c.waves_per_cu,
                             ms=round(c.ms, 4), ms_default=round(c.ms_default, 4),
+                            gcups=round(c.cells / (c.ms * 1e6), 2) if c.ms > 0 else None,
                             candidates=[dict(per_lane=c.cand[j].per_lane, waves_per_cu=c.cand[j].waves_per_cu,
                                              ms=round(c.cand[j].ms, 4)) for j in range(c.ncand)]))
         return out

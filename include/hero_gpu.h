@@ -225,14 +225,15 @@ typedef struct {
 } hero_tune_candidate;
 typedef struct {
 	int32_t band, per_lane, waves_per_cu, ncand;
-	int64_t tasks;
+	int64_t tasks, cells; /* the class's DP calls and their band cells */
 	float ms, ms_default; /* chosen candidate / one task per lane at the occupancy limit */
 	hero_tune_candidate cand[16];
 } hero_tune_class;
+#define HERO_TUNE_MAX_CLASSES 12 /* band classes of the batched form: 4, 6, 8, 10, 12, 16, 24, 32 */
 typedef struct {
 	int32_t classes, _pad;
 	int64_t subjects;
-	hero_tune_class cls[6];
+	hero_tune_class cls[HERO_TUNE_MAX_CLASSES];
 } hero_tune_report;
 int hero_autotune(hero_ctx *ctx, int64_t max_subjects, hero_tune_report *report);
 

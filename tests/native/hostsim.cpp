@@ -379,7 +379,7 @@ static int pk16_check_one(const Opts &o, uint64_t &rng, int max_len) {
 // Tagged affine DP against dp_forward on random tasks (as pk16_check_one): the tracebacks must
 // agree. mode_or forces row variants on every row (a lane among other tasks). Returns the number
 // of differing tasks; -1 when the options do not admit the tagged form.
-template <int BMAX>
+template <int BMAX, bool AFF>
 static int tagged_check_one(const Opts &o, uint64_t &rng, int max_len, int mode_or) {
 	auto rnd = [&](uint64_t n) {
 		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
@@ -403,25 +403,27 @@ static int tagged_check_one(const Opts &o, uint64_t &rng, int max_len, int mode_
 	RowStore rt, rr;
 	rt.nw = rr.nw = nw;
 	CodeStream S1{&s, 0}, Q1{&q, 0}, S2{&s, 0}, Q2{&q, 0};
-	dp_forward_tagged<BMAX>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S1, Q1, rt, mode_or);
-	dp_forward<BMAX, true>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S2, Q2, rr);
+	if constexpr (AFF) dp_forward_tagged<BMAX>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S1, Q1, rt, mode_or);
+	else dp_forward_tagged_global<BMAX>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S1, Q1, rt, mode_or);
+	dp_forward<BMAX, AFF>(o, T.lenS, T.lenQ, T.b, T.left_al, T.right_al, S2, Q2, rr);
 	std::vector<int> x, y;
 	auto emx = [&](int st) { x.push_back(st); };
 	auto emy = [&](int st) { y.push_back(st); };
-	const int nx = dp_traceback<true>(T.lenS, T.lenQ, T.b, rt, emx);
+	const int nx = dp_traceback<AFF ? kPtrTaggedAffine : kPtrTaggedGlobal>(T.lenS, T.lenQ, T.b, rt, emx);
 	const int ny = dp_traceback(T.lenS, T.lenQ, T.b, rr, emy);
 	return (nx != ny || x != y) ? 1 : 0;
 }
 
 extern "C" int hostsim_tagged_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len, int mode_or) {
 	const Opts o = to_opts(cfg);
-	if (o.aligner != 0 || o.dx != o.dy || !dp_tagged_fits(o, max_len + 1)) return -1;
+	const bool aff = o.aligner == 0;
+	if ((aff && o.dx != o.dy) || !dp_tagged_fits(o, max_len + 1)) return -1;
 	uint64_t rng = seed | 1;
 	int bad = 0;
 	for (int t = 0; t < n; t++) {
 #define TG_CASE(BM) \
-	if (bmax == BM) bad += tagged_check_one<BM>(o, rng, max_len, mode_or);
-		TG_CASE(4) TG_CASE(8) TG_CASE(12)
+	if (bmax == BM) bad += aff ? tagged_check_one<BM, true>(o, rng, max_len, mode_or) : tagged_check_one<BM, false>(o, rng, max_len, mode_or);
+		TG_CASE(2) TG_CASE(4) TG_CASE(6) TG_CASE(8) TG_CASE(10) TG_CASE(12)
 #undef TG_CASE
 	}
 	return bad;

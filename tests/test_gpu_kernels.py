@@ -21,14 +21,16 @@ def test_band_align_batch_known_answers(aligner):
 FORMS = [(1, 32), (2, 12), (3, 12)]  # band_align_form, widest band class it takes
 
 
+@pytest.mark.parametrize("aligner", [0, 1], ids=["affine", "global"])
 @pytest.mark.parametrize("form,bmax", FORMS, ids=["batched_general", "batched_tagged", "batched_pk16"])
-def test_timed_dp_kernels_on_known_answers(form, bmax):
-    """The pipeline's batched DP kernels (the ones the correction step times) on every affine
-    reference DP case they can take: A/C/G/T only, widened band within the kernel's classes."""
-    cases = [c for c in load_jsonl("dp_cases.jsonl") if c["aligner"] == 0 and "N" not in c["S"] + c["Q"]
+def test_timed_dp_kernels_on_known_answers(form, bmax, aligner):
+    """The pipeline's batched DP kernels (the ones the correction step times) on every reference
+    DP case of the aligner they can take: A/C/G/T only, widened band within the kernel's classes
+    (band classes 4, 6, 8, 10, 12, 16, 24, 32)."""
+    cases = [c for c in load_jsonl("dp_cases.jsonl") if c["aligner"] == aligner and "N" not in c["S"] + c["Q"]
              and max(c["band"], abs(len(c["S"]) - len(c["Q"]))) <= bmax]
     assert len(cases) > 1000
-    with gpu.Context(aligner=0, band_align_form=form) as ctx:
+    with gpu.Context(aligner=aligner, band_align_form=form) as ctx:
         res = ctx.band_align([(c["S"], c["Q"], c["band"], c["flag"]) for c in cases])
     bad = [(c, r) for c, r in zip(cases, res) if r != (c["cigar"], c["out_band"])]
     assert not bad, bad[:3]

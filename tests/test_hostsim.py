@@ -72,7 +72,7 @@ def test_packed_two_task_dp_equals_one_task_dp(aligner, bmax):
         assert H.pk16_check(cfg, 400, 12345 + bmax + max_len, bmax, max_len) == 0, (aligner, bmax, max_len)
 
 
-@pytest.mark.parametrize("bmax", [4, 8, 12])
+@pytest.mark.parametrize("bmax", [2, 4, 6, 8, 10, 12])
 @pytest.mark.parametrize(
     "costs",
     [
@@ -91,4 +91,26 @@ def test_tagged_affine_dp_equals_plain_dp(costs, bmax):
     for max_len in (3, 20, 140, 500):
         for mode_or in (0, 4, 7):
             seed = 777 + bmax + max_len + 13 * mode_or
+            assert H.tagged_check(cfg, 300, seed, bmax, max_len, mode_or) == 0, (costs, bmax, max_len, mode_or)
+
+
+@pytest.mark.parametrize("bmax", [2, 4, 6, 8, 10, 12])
+@pytest.mark.parametrize(
+    "costs",
+    [
+        dict(),
+        dict(dx=-2, dy=-3, sx=0, miss=-2, hit=1),
+        dict(dx=-3, dy=-1, sx=-1, miss=-1, hit=0),
+        dict(dx=-4, dy=-4, sx=-2, miss=-5, hit=3),
+    ],
+    ids=["default", "dx_ne_dy", "hit0", "hit3"],
+)
+def test_tagged_global_dp_equals_plain_dp(costs, bmax):
+    """The tagged global DP (value x 8, offset 4 hit (i + j), move tags 3 match / 2 mismatch / 1 left /
+    0 up) traces the same paths as the plain 32-bit global DP, with each task's own row variants and
+    with every row forced through the cut / all-edge variants."""
+    cfg = O.make_cfg(aligner=1, **costs)
+    for max_len in (3, 20, 140, 500):
+        for mode_or in (0, 4, 7):
+            seed = 919 + bmax + max_len + 13 * mode_or
             assert H.tagged_check(cfg, 300, seed, bmax, max_len, mode_or) == 0, (costs, bmax, max_len, mode_or)
