@@ -516,7 +516,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 	float best = -1.0f;
 	uint64_t cells0 = 0, cells1 = 0;
 	if (int r = worker_stat(c, w, kStCellsAll, cells0)) return r;
-	for (int per_lane = 1; per_lane <= (bm <= kPk16MaxBand && pk16_class_fits(c->o, rows) ? 2 : 1); per_lane++) {
+	for (int per_lane = 1; per_lane <= (pk_form_fits(c->o, rows, bm, Lc.tagged != 0) ? 2 : 1); per_lane++) {
 		const int occ = dp_batched_waves_per_cu(bm, aff, per_lane, rows, Lc.tagged != 0);
 		for (int wpc = 1; wpc <= occ; wpc *= 2) {
 			const int wpc_eff = wpc * 2 > occ ? occ : wpc; // the last candidate is the occupancy limit
@@ -648,7 +648,7 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			// tasks per lane: 2 = packed 16-bit (band classes up to kPk16MaxBand), 1 = 32-bit; the
 			// option, else the tuned choice for this class, else 1
 			const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
-			const int per_lane = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
+			const int per_lane = (want == 2 && pk_form_fits(c->o, rows, bm, L.tagged != 0)) ? 2 : 1;
 			const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
 			                                               : dp_batched_waves_per_cu(bm, aff, per_lane, rows, L.tagged != 0);
 			int waves = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
@@ -708,7 +708,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	for (int cls = 0; cls < kNumBandClasses; cls++) {
 		const int bm = kBandClass[cls], nw = (2 * bm + 1 + 15) / 16;
 		const int want = c->opt.dp_per_lane > 0 ? c->opt.dp_per_lane : c->tuned_per_lane[cls];
-		per_lane[cls] = (want == 2 && bm <= kPk16MaxBand && pk16_class_fits(c->o, rows)) ? 2 : 1;
+		per_lane[cls] = (want == 2 && pk_form_fits(c->o, rows, bm, tagged)) ? 2 : 1;
 		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
 		                                               : dp_batched_waves_per_cu(bm, aff, per_lane[cls], rows, tagged);
 		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
@@ -1375,15 +1375,16 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 
 namespace {
 
-// Test route of hero_band_align_batch (hero_options.band_align_form 1-3): the calls become DP
+// Test route of hero_band_align_batch (hero_options.band_align_form 1-4): the calls become DP
 // tasks over a packed read set (each call's two sides as two reads) and run through one batched
-// kernel of the correction pipeline: 1 one task per lane in the general rows, 2 the tagged affine
-// rows, 3 two tasks per lane in packed 16-bit. Returns per call its edit words (at woff[i]) and
-// step count.
+// kernel of the correction pipeline: 1 one task per lane in the general rows, 2 the tagged rows,
+// 3 two tasks per lane in packed 16-bit, 4 two tasks per lane in the tagged packed 16-bit rows.
+// Returns per call its edit words (at woff[i]) and step count.
 int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, const char *seq,
 		std::vector<int64_t> &woff, std::vector<uint32_t> &words, std::vector<uint32_t> &nsteps) {
 	const int form = c->opt.band_align_form;
-	if (form < 1 || form > 3) return fail(c, HERO_ERR_ARG, "band_align_form %d (0 wave, 1 general, 2 tagged, 3 packed)", form);
+	if (form < 1 || form > 4)
+		return fail(c, HERO_ERR_ARG, "band_align_form %d (0 wave, 1 general, 2 tagged, 3 packed, 4 tagged packed)", form);
 	std::vector<int64_t> roff((size_t) 2 * n);
 	std::vector<int32_t> rlen((size_t) 2 * n);
 	std::vector<DpTask> tasks((size_t) n);
@@ -1419,6 +1420,8 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 	if (rows > 512) return fail(c, HERO_ERR_ARG, "the batched forms take at most 511 rows");
 	if (form == 2 && !dp_tagged_form(c->o, rows)) return fail(c, HERO_ERR_ARG, "the tagged affine rows need dx == dy");
 	if (form == 3 && !pk16_class_fits(c->o, rows)) return fail(c, HERO_ERR_ARG, "the calls exceed the 16-bit range");
+	if (form == 4 && (c->o.aligner != 0 || !pk_form_fits(c->o, rows, kBandClass[cls], true)))
+		return fail(c, HERO_ERR_ARG, "the tagged packed rows take affine calls within their 16-bit range");
 	std::vector<uint64_t> pw, pwoff;
 	pack_reads(seq, roff.data(), rlen.data(), 2 * (int64_t) n, pw, pwoff);
 	DevBuf<uint64_t> dw, dwoff;
@@ -1426,7 +1429,7 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 	DevBuf<DpTask> dt;
 	DevBuf<uint32_t> dwords, dns, dscr;
 	DevBuf<StepSummary> dsums;
-	const int bm = kBandClass[cls], per_lane = form == 3 ? 2 : 1, nwd = (2 * bm + 1 + 15) / 16;
+	const int bm = kBandClass[cls], per_lane = form >= 3 ? 2 : 1, nwd = (2 * bm + 1 + 15) / 16;
 	const int waves = (int) std::max<int64_t>(1, std::min<int64_t>((int64_t) c->cu_count * 2, (n + 64 * per_lane - 1) / (64 * per_lane)));
 	hipStream_t st = c->w.st;
 	HIPCHK(c, dw.exact(pw.size()));
@@ -1449,7 +1452,7 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 	std::memset(&L, 0, sizeof L);
 	L.tasks = dt.p, L.ntasks = (uint32_t) n, L.S = R, L.Q = R, L.o = c->o;
 	L.words = dwords.p, L.nsteps = dns.p, L.sums = dsums.p, L.scratch = dscr.p, L.rows_max = rows;
-	L.tagged = form == 2 ? 1 : 0, L.batch_counter = c->w.counter.p, L.stats = c->w.dstats.p;
+	L.tagged = form == 2 || form == 4 ? 1 : 0, L.batch_counter = c->w.counter.p, L.stats = c->w.dstats.p;
 	HIPCHK(c, launch_dp_batched(L, bm, per_lane, waves, st));
 	words.resize((size_t) nw + 1);
 	nsteps.resize((size_t) n);

@@ -232,4 +232,245 @@ HD void dp_forward_pk16(const Opts &o, const Pk16Task &A, const Pk16Task &B, SS 
 	}
 }
 
+// ---- tagged rows, two tasks per lane ---------------------------------------------------------
+// dp_forward_tagged's recurrence (dp_core.h) in packed 16-bit halves: task A in the low half, B in
+// the high half, one packed add / max / mad / bit operation per cell for both tasks.
+//
+// Range. The scaled value of a cell is V = 8 v + t - 4 hit (i + j). With hit >= 0, miss <= hit and
+// every gap cost (dx == dy, ex, sx) <= 0, a cell's score is at most hit * min(i, j), so V <= 3.
+// Every state of an in-band cell of the matrix is at least the score of a simple path (down its
+// diagonal from row 0 or column 0, whose closed forms are >= -s (b + 1), s the largest |cost|)
+// plus one gap move: v >= min(miss, 0) (rows + 1) - s (BMAX + 3) for rows >= max(i, j). So
+// V >= L(rows) = 8 (min(miss, 0) (rows + 1) - s (BMAX + 3)) - 8 hit (rows + BMAX) - 4. Values
+// derived from the sentinel kSent16T never rise above kSent16T + 3 (every constant added is <= 0)
+// and never fall below kSent16T - D, D = |C| + |E| + |DM| + 8: a cut cell's y is the sentinel and
+// its max3 takes it, so its values restart near the sentinel every row instead of drifting. With
+// L(rows) > kSent16T + 3 and kSent16T - D > -32768 every comparison that involves a cell of the
+// matrix has the outcome it has in 32 bits. Cells outside the matrix may wrap: nothing reads them.
+constexpr int kSent16T = -24576;
+
+HD bool pk16t_fits(const Opts &o, int rows, int bmax) {
+	if (o.hit < 0 || o.miss > o.hit || o.dx != o.dy || o.dx > 0 || o.ex > 0 || o.sx > 0) return false;
+	int s = o.hit;
+	const int c[5] = {o.dx, o.ex, o.sx, o.miss, o.dy};
+	for (int k = 0; k < 5; k++) s = imax_(s, c[k] < 0 ? -c[k] : c[k]);
+	const int64_t miss0 = o.miss < 0 ? o.miss : 0;
+	const int64_t L = 8 * (miss0 * (rows + 1) - (int64_t) s * (bmax + 3)) - 8 * (int64_t) o.hit * (rows + bmax) - 4;
+	const int64_t D = (int64_t) 8 * -o.dx + 4 * o.hit + 8 * -o.ex + 4 * o.hit + 8 * (o.hit - o.miss) + 1 + 8;
+	return L > kSent16T + 3 && kSent16T - D > -32768;
+}
+// every call of a band class fits: both sides are shorter than rows_max
+HD bool pk16t_class_fits(const Opts &o, int rows_max, int bmax) { return pk16t_fits(o, rows_max, bmax); }
+
+// Bit planes of a task's query window: bit d of lo / hi = the low / high bit of cell d's code.
+template <int W>
+struct QPlanes {
+	static_assert(W <= 32, "bit planes hold 32 cells");
+	uint32_t lo = 0, hi = 0;
+	HD void set(int d, int c) { lo |= (uint32_t) (c & 1) << d, hi |= (uint32_t) ((c >> 1) & 1) << d; }
+	HD void slide(int next) {
+		lo = (lo >> 1) | ((uint32_t) (next & 1) << (W - 1));
+		hi = (hi >> 1) | ((uint32_t) ((next >> 1) & 1) << (W - 1));
+	}
+	// bit d set where cell d's code differs from sc
+	HD uint32_t differ(int sc) const {
+		return (lo ^ (0u - (uint32_t) (sc & 1))) | (hi ^ (0u - (uint32_t) ((sc >> 1) & 1)));
+	}
+};
+
+// c + m * a in each half, m in {0, 1}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ I16x2 pk_mad01(I16x2 m, I16x2 a, I16x2 c) { return pk_w(pk_v(m) * pk_v(a) + pk_v(c)); }
+#else
+HD I16x2 pk_mad01(I16x2 m, I16x2 a, I16x2 c) { return pk2(pk_lo(c) + pk_lo(m) * pk_lo(a), pk_hi(c) + pk_hi(m) * pk_hi(a)); }
+#endif
+HD I16x2 pk_max3(I16x2 a, I16x2 b, I16x2 c) { return pk_max(pk_max(a, b), c); }
+HD I16x2 pk_or(I16x2 a, uint32_t m) { return I16x2{a.v | m}; }
+
+// Forward pass of tasks A and B in tagged packed rows; the pointers each task receives are
+// dp_forward_tagged's (kPtrTaggedAffine). mode_or: row variants taken on every row.
+template <int BMAX, class SS, class QS, class PA, class PB>
+HD void dp_forward_pk16_tagged(const Opts &o, const Pk16Task &A, const Pk16Task &B, SS &SA, QS &QA, SS &SB, QS &QB,
+		PA &pa, PB &pb, int mode_or = 0) {
+	static_assert(SS::kTwoBit && QS::kTwoBit, "the tagged form reads 2-bit codes");
+	constexpr int W = 2 * BMAX + 1;
+	static_assert(W <= 32, "bit planes hold 32 cells");
+	constexpr int NW = (W + 15) / 16;
+	constexpr int kLast = W - 16 * (NW - 1);
+	const int step = 4 * o.hit;
+	const int C = 8 * o.dx - step, E = 8 * o.ex - step, DM = 8 * (o.miss - o.hit) - 1;
+	const int CQx = 8 * (o.sx - o.dx), CQe = 8 * (o.sx - o.ex);
+	const I16x2 S2 = pk2(kSent16T, kSent16T);
+	const I16x2 Cm2 = pk2(C - 2, C - 2), E2 = pk2(E, E), DM2 = pk2(DM, DM);
+	const I16x2 CQx2 = pk2(CQx, CQx), CQe2 = pk2(CQe, CQe);
+	I16x2 MK[W + 1], U[W + 1];
+	I16x2 Gq = S2, XEq = S2; // G and XE of the cell above the next row's column lenQ, per task
+	QPlanes<W> wa, wb;
+	{ // row 0 (src/BandAligner.cpp:140-171): pointers 0 (left) everywhere
+		uint32_t pw[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pw[w] = 0;
+		int gqa = kSent16T, xqa = kSent16T, gqb = kSent16T, xqb = kSent16T;
+#pragma unroll
+		for (int d = 0; d <= W; d++) {
+			int mk[2], u[2];
+			const Pk16Task *T[2] = {&A, &B};
+			for (int k = 0; k < 2; k++) {
+				int h, x, y;
+				const int j = d - T[k]->b;
+				const bool in = row0_cell<true>(o, j, T[k]->b, T[k]->lenQ, h, x, y) && d < W;
+				const int off = step * j;
+				const int g = in ? 8 * h - off + C : kSent16T, xe = in ? 8 * x + 3 - off + E : kSent16T;
+				mk[k] = in ? 8 * max3_(h, x, y) - off + 2 : kSent16T;
+				u[k] = in ? imax_(g, xe) | 3 : kSent16T;
+				if (T[k]->left_al && d == T[k]->lenQ + T[k]->b) (k ? gqb : gqa) = g, (k ? xqb : xqa) = xe;
+			}
+			MK[d] = pk2(mk[0], mk[1]);
+			U[d] = pk2(u[0], u[1]);
+		}
+		Gq = pk2(gqa, gqb), XEq = pk2(xqa, xqb);
+		pa.put_row(0, pw);
+		pb.put_row(0, pw);
+	}
+#pragma unroll
+	for (int d = 0; d < W; d++) {
+		wa.set(d, d < A.b ? kCodeNone : QA.next());
+		wb.set(d, d < B.b ? kCodeNone : QB.next());
+	}
+	constexpr int kC0 = 1, kCQ = 2, kCut = 4;
+	using M = uint32_t;
+	const M band_a = bits_range<M>(0, 2 * A.b), band_b = bits_range<M>(0, 2 * B.b);
+	auto row = [&](int i, auto mode_tag) {
+		constexpr int MODE = decltype(mode_tag)::value;
+		// mismatch bits of both tasks: cells 0..15 in lo16 (A at bit d, B at bit 16 + d), 16.. in hi16
+		const uint32_t da = wa.differ(SA.next()), db = wb.differ(SB.next());
+		const uint32_t lo16 = (da & 0xFFFFu) | (db << 16), hi16 = (da >> 16) | (db & 0xFFFF0000u);
+		M c0a = 0, c0b = 0, cqa = 0, cqb = 0;
+		I16x2 h0 = S2, x0 = S2, y0 = S2, g0 = S2;
+		if (MODE & kC0) {
+			c0a = bit_at<M>(A.b - i, W), c0b = bit_at<M>(B.b - i, W);
+			const int off = step * i;
+			const int xa = A.right_al ? i * o.sx : o.dx + (i - 1) * o.ex;
+			const int xb = B.right_al ? i * o.sx : o.dx + (i - 1) * o.ex;
+			const int ya = A.right_al ? o.dy : xa, yb = B.right_al ? o.dy : xb;
+			h0 = pk2(8 * xa + 2 - off, 8 * xb + 2 - off), x0 = pk2(8 * xa + 3 - off, 8 * xb + 3 - off);
+			y0 = pk2(8 * ya - off, 8 * yb - off), g0 = pk2(8 * xa - off + C, 8 * xb - off + C);
+		}
+		if (MODE & kCQ) {
+			cqa = A.left_al ? bit_at<M>(A.lenQ - i + A.b, W) : 0;
+			cqb = B.left_al ? bit_at<M>(B.lenQ - i + B.b, W) : 0;
+		}
+		uint32_t pwa[NW], pwb[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pwa[w] = 0, pwb[w] = 0;
+		I16x2 gl = S2, yel = S2, gq = Gq, xeq = XEq;
+#pragma unroll
+		for (int d = 0; d < W; d++) {
+			const I16x2 m{((d < 16 ? lo16 : hi16) >> (d & 15)) & 0x00010001u};
+			I16x2 h = pk_mad01(m, DM2, MK[d]);
+			I16x2 x = U[d + 1];
+			I16x2 y = pk_max(gl, yel);
+			if (MODE & kCQ) {
+				const uint32_t cq = pk_bitmask(cqa, cqb, d);
+				x = pk_sel(cq, pk_or(pk_max(pk_add(Gq, CQx2), pk_add(XEq, CQe2)), 0x00030003u), x);
+			}
+			if (MODE & kCut) y = pk_sel(pk_bitmask(band_a, band_b, d), y, S2);
+			I16x2 g = pk_add(pk_add(h, m), Cm2);
+			uint32_t tagv;
+			I16x2 mx;
+			if (MODE & kC0) {
+				const uint32_t c0 = pk_bitmask(c0a, c0b, d);
+				h = pk_sel(c0, h0, h), x = pk_sel(c0, x0, x), y = pk_sel(c0, y0, y), g = pk_sel(c0, g0, g);
+				mx = pk_max3(h, x, y);
+				tagv = mx.v | (c0 & 0x00030003u); // column 0 points up
+			} else {
+				mx = pk_max3(h, x, y);
+				tagv = mx.v;
+			}
+			pwa[d >> 4] = (pwa[d >> 4] >> 2) | (tagv << 30);
+			pwb[d >> 4] = (pwb[d >> 4] >> 2) | ((tagv >> 16) << 30);
+			MK[d] = I16x2{(mx.v & 0xFFFCFFFCu) | 0x00020002u};
+			const I16x2 xe = pk_add(x, E2);
+			U[d] = pk_or(pk_max(g, xe), 0x00030003u);
+			if (MODE & kCQ) {
+				const uint32_t cq = pk_bitmask(cqa, cqb, d);
+				gq = pk_sel(cq, g, gq), xeq = pk_sel(cq, xe, xeq);
+			}
+			gl = g, yel = pk_add(y, E2);
+		}
+		if (MODE & kCQ) Gq = gq, XEq = xeq;
+		if (kLast < 16) pwa[NW - 1] >>= 2 * (16 - kLast), pwb[NW - 1] >>= 2 * (16 - kLast);
+		pa.put_row(i, pwa);
+		pb.put_row(i, pwb);
+		wa.slide((i - A.b + W - 1 >= 0) ? QA.next() : kCodeNone);
+		wb.slide((i - B.b + W - 1 >= 0) ? QB.next() : kCodeNone);
+	};
+	const int rows = imax_(A.lenS, B.lenS);
+	// Head rows 1..BMAX when both tasks of every lane have band BMAX and no aligned column lenQ in
+	// them: row I's column 0 is band cell BMAX - I in both halves, so the row starts there with
+	// the column-0 values and takes the interior rule after it (as dp_forward_tagged's head rows).
+	auto head = [&](auto itag) {
+		constexpr int I = decltype(itag)::value + 1;
+		constexpr int D0 = BMAX - I;
+		if (I > rows) return;
+		const uint32_t da = wa.differ(SA.next()), db = wb.differ(SB.next());
+		const uint32_t lo16 = (da & 0xFFFFu) | (db << 16), hi16 = (da >> 16) | (db & 0xFFFF0000u);
+		const int off = step * I;
+		const int xa = A.right_al ? I * o.sx : o.dx + (I - 1) * o.ex;
+		const int xb = B.right_al ? I * o.sx : o.dx + (I - 1) * o.ex;
+		const int ya = A.right_al ? o.dy : xa, yb = B.right_al ? o.dy : xb;
+		uint32_t pwa[NW], pwb[NW];
+#pragma unroll
+		for (int w = 0; w < NW; w++) pwa[w] = 0, pwb[w] = 0;
+		I16x2 gl, yel;
+		{ // column 0: a vertical gap from the origin, pointing up
+			const I16x2 h0 = pk2(8 * xa + 2 - off, 8 * xb + 2 - off), x0 = pk2(8 * xa + 3 - off, 8 * xb + 3 - off);
+			const I16x2 y0 = pk2(8 * ya - off, 8 * yb - off), g0 = pk2(8 * xa - off + C, 8 * xb - off + C);
+			const I16x2 mx = pk_max3(h0, x0, y0);
+			pwa[D0 >> 4] = (pwa[D0 >> 4] >> 2) | (3u << 30);
+			pwb[D0 >> 4] = (pwb[D0 >> 4] >> 2) | (3u << 30);
+			MK[D0] = I16x2{(mx.v & 0xFFFCFFFCu) | 0x00020002u};
+			U[D0] = pk_or(pk_max(g0, pk_add(x0, E2)), 0x00030003u);
+			gl = g0, yel = pk_add(y0, E2);
+		}
+#pragma unroll
+		for (int d = D0 + 1; d < W; d++) {
+			const I16x2 m{((d < 16 ? lo16 : hi16) >> (d & 15)) & 0x00010001u};
+			const I16x2 h = pk_mad01(m, DM2, MK[d]);
+			const I16x2 x = U[d + 1];
+			const I16x2 y = pk_max(gl, yel);
+			const I16x2 g = pk_add(pk_add(h, m), Cm2);
+			const I16x2 mx = pk_max3(h, x, y);
+			pwa[d >> 4] = (pwa[d >> 4] >> 2) | (mx.v << 30);
+			pwb[d >> 4] = (pwb[d >> 4] >> 2) | ((mx.v >> 16) << 30);
+			MK[d] = I16x2{(mx.v & 0xFFFCFFFCu) | 0x00020002u};
+			U[d] = pk_or(pk_max(g, pk_add(x, E2)), 0x00030003u);
+			gl = g, yel = pk_add(y, E2);
+		}
+		if (kLast < 16) pwa[NW - 1] >>= 2 * (16 - kLast), pwb[NW - 1] >>= 2 * (16 - kLast);
+		pa.put_row(I, pwa);
+		pb.put_row(I, pwb);
+		wa.slide(QA.next()); // I - b + W - 1 = I + BMAX >= 0
+		wb.slide(QB.next());
+	};
+	int i0 = 1;
+	if (mode_or == 0 && wave_all(A.b == BMAX && B.b == BMAX && !(A.left_al && A.lenQ <= 2 * BMAX) &&
+	                             !(B.left_al && B.lenQ <= 2 * BMAX))) {
+		for_each_index(head, std::make_integer_sequence<int, BMAX>());
+		i0 = BMAX + 1;
+	}
+	const int cut = mode_or | (wave_any(A.b < BMAX || B.b < BMAX) ? kCut : 0);
+	for (int i = i0; i <= rows; i++) {
+		const bool c0 = i <= A.b || i <= B.b;
+		const bool cq = (A.left_al && A.lenQ - i <= A.b) || (B.left_al && B.lenQ - i <= B.b);
+		const int mode = cut | (wave_any(c0) ? kC0 : 0) | (wave_any(cq) ? kCQ : 0);
+		switch (mode) {
+		case 0: row(i, std::integral_constant<int, 0>()); break;
+		case kCut: row(i, std::integral_constant<int, kCut>()); break;
+		case kCQ: row(i, std::integral_constant<int, kCQ>()); break;
+		default: row(i, std::integral_constant<int, kC0 | kCQ | kCut>()); break;
+		}
+	}
+}
+
 } // namespace hero

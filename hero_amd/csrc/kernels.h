@@ -210,9 +210,13 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane, int rows_max, bool tagged);
 constexpr int kPk16MaxBand = 12;   // widest band class of the two-tasks-per-lane form (windows of 32 cells)
 constexpr int kTaggedMaxBand = 12; // widest band class of the tagged affine rows (windows of 32 cells)
-// Whether one-task-per-lane affine rows may run in the tagged form: gap opens of equal cost both
-// ways and every scaled value of a task of up to rows_max rows within range (dp_core.h).
+// Whether one-task-per-lane rows may run in the tagged form: the global aligner, or the affine one
+// with gap opens of equal cost both ways; every scaled value of a task of up to rows_max rows
+// within range (dp_core.h).
 bool dp_tagged_form(const Opts &o, int rows_max);
+// Whether a band class may run two tasks per lane in 16-bit halves (tagged: the tagged packed
+// rows of affine launches with L.tagged set; their range test is dp_pk16.h's pk16t_fits).
+bool pk_form_fits(const Opts &o, int rows_max, int bmax, bool tagged);
 // The packed form stages each lane's four segments (< rows_max codes each) in LDS: words per
 // segment (odd, so the lanes' segments start in different banks) and bytes per wavefront.
 inline __host__ __device__ int pk_lds_words(int rows_max) { return ((rows_max + 15) / 16) | 1; }

@@ -1216,12 +1216,12 @@ struct SlotPtrs {
 };
 
 // Traceback of a forward-filled task into its edit words, step count and summary.
-template <class PS>
+template <uint32_t MAP, class PS>
 __device__ __forceinline__ uint64_t finish_task(const DpLaunch &L, uint32_t t, const DpTask &task, int lenS, int lenQ,
 		const PS &ptrs) {
 	const int cap = task.s_len + task.q_len;
 	StepWriter wr{L.words + task.out_off, cap, -1, 0};
-	const int ns = dp_traceback(lenS, lenQ, task.band, ptrs, wr);
+	const int ns = dp_traceback<MAP>(lenS, lenQ, task.band, ptrs, wr);
 	wr.flush();
 	L.nsteps[t] = ns < 0 ? 0u : (uint32_t) ns;
 	L.sums[t] = wr.sb.finish();
@@ -1256,9 +1256,10 @@ __device__ __forceinline__ LdsStream stage_codes(PackedStream s, int len, uint32
 // tasks 2l and 2l+1 of the batch (neighbours in the length-bucketed order) and runs them together
 // in packed 16-bit arithmetic (dp_pk16.h), its four segments staged in LDS. The host selects this
 // form only when every task of the class is within the 16-bit bound.
-template <int BMAX, bool AFF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 3 : 1, 8))) void k_dp_batched_pk(
+template <int BMAX, bool AFF, bool TAG = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TAG ? 2 : BMAX <= 8 ? 3 : 1, 8))) void k_dp_batched_pk(
 		DpLaunch L0) {
+	static_assert(!TAG || AFF, "the packed tagged rows are affine");
 	DpLaunch L = L0;
 	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
@@ -1298,7 +1299,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 
 		}
 		// the host selects this form only when every task of the class is within the 16-bit bound
 		// (pk16_class_fits); a pair that is not (never expected) is counted as skipped
-		const bool fit = oka && (!has_b || okb) && pk16_fits(L.o, lsa, lqa) && (!has_b || pk16_fits(L.o, lsb, lqb));
+		const bool fit = oka && (!has_b || okb) &&
+				(TAG ? pk16t_fits(L.o, lsa > lsb ? lsa : lsb, BMAX)
+				     : pk16_fits(L.o, lsa, lqa) && (!has_b || pk16_fits(L.o, lsb, lqb)));
 		if (!fit) {
 			stat_add(L.stats, kStDpSkipped, has_b ? 2ull : 1ull);
 			continue;
@@ -1309,9 +1312,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BMAX <= 8 ? 
 				has_b && (B.flags & kTaskAlignRight) != 0};
 		LdsStream ls_sa = stage_codes(sa, lsa, mine), ls_qa = stage_codes(qa, lqa, mine + wps);
 		LdsStream ls_sb = stage_codes(sb, lsb, mine + 2 * wps), ls_qb = stage_codes(qb, lqb, mine + 3 * wps);
-		dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, ls_sa, ls_qa, ls_sb, ls_qb, pa, pb);
-		cells += finish_task(L, ta, A, lsa, lqa, pa);
-		if (has_b) cells += finish_task(L, tb, B, lsb, lqb, pb);
+		constexpr uint32_t kMap = TAG ? kPtrTaggedAffine : kPtrPlain;
+		if constexpr (TAG) dp_forward_pk16_tagged<BMAX>(L.o, PA, PB, ls_sa, ls_qa, ls_sb, ls_qb, pa, pb);
+		else dp_forward_pk16<BMAX, AFF>(L.o, PA, PB, ls_sa, ls_qa, ls_sb, ls_qb, pa, pb);
+		cells += finish_task<kMap>(L, ta, A, lsa, lqa, pa);
+		if (has_b) cells += finish_task<kMap>(L, tb, B, lsb, lqb, pb);
 	}
 	cells = wave_sum64(cells);
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
@@ -1369,6 +1374,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per
 	if (lane == 0) stat_add(L.stats, kStCellsAll, (unsigned long long) cells);
 }
 
+// two tasks per lane in 16-bit halves: the tagged packed rows for tagged affine launches, the
+// general packed rows otherwise; both within their 16-bit range for every call of the class
+bool pk_form_fits(const Opts &o, int rows_max, int bmax, bool tagged) {
+	if (bmax > kPk16MaxBand) return false;
+	return tagged && o.aligner == 0 ? pk16t_class_fits(o, rows_max, bmax) : pk16_class_fits(o, rows_max);
+}
+
 // the tagged rows: the global aligner, or the affine one with equal gap opens
 bool dp_tagged_form(const Opts &o, int rows_max) {
 	return (o.aligner == 1 || o.dx == o.dy) && dp_tagged_fits(o, rows_max);
@@ -1387,7 +1399,9 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched<BM, false, true>, 64, 0);
 #define HERO_OCC2(BM)                                                                                           \
 	if (bmax == BM && per_lane == 2)                                                                            \
-		e = aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64,           \
+		e = aff && tagged ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true, true>,  \
+		                  64, pk_lds_bytes(rows_max))                                                      \
+		  : aff ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, true>, 64,             \
 		                  pk_lds_bytes(rows_max))                                                          \
 		        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_dp_batched_pk<BM, false>, 64,          \
 		                  pk_lds_bytes(rows_max));
@@ -1508,7 +1522,8 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 	}
 #define HERO_DPB2(BM)                                                                              \
 	if (bmax == BM && per_lane == 2) {                                                             \
-		if (aff) k_dp_batched_pk<BM, true><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);         \
+		if (aff && L.tagged) k_dp_batched_pk<BM, true, true><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L); \
+		else if (aff) k_dp_batched_pk<BM, true><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);    \
 		else k_dp_batched_pk<BM, false><<<nwaves, 64, pk_lds_bytes(L.rows_max), st>>>(L);            \
 		return hipGetLastError();                                                                  \
 	}

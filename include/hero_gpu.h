@@ -62,7 +62,8 @@ typedef struct {
 	                           hero_coverage_histogram instead of calling consensus */
 	int32_t band_align_form; /* hero_band_align_batch: 0 the plain form (one wavefront per call); for tests
 	                            of the pipeline's kernels on given calls (A/C/G/T, bands <= 32, < 512 rows):
-	                            1 batched general rows, 2 batched tagged affine rows, 3 batched packed 16-bit */
+	                            1 batched general rows, 2 batched tagged rows, 3 batched packed 16-bit,
+	                            4 batched tagged packed 16-bit (affine) */
 	int32_t long_format;      /* Config::bSpeicialVersion (-v), record mode only: seeds on a query with the
 	                             subject's uniqueId are kept (src/SubjectReadAligner.cpp:464-465) and every
 	                             accepted record is kept, none replaced (src/SubjectRead.cpp:26-29); read them

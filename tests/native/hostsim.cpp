@@ -376,6 +376,69 @@ static int pk16_check_one(const Opts &o, uint64_t &rng, int max_len) {
 	return bad;
 }
 
+// Tagged packed two-task DP against dp_forward on random task pairs: each task's traceback must
+// agree. mode_or forces row variants on every row. Pairs outside the 16-bit range are not drawn
+// (the caller's options and max_len decide); returns the number of differing tasks.
+template <int BMAX>
+static int pk16t_check_one(const Opts &o, uint64_t &rng, int max_len, int mode_or) {
+	auto rnd = [&](uint64_t n) {
+		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+		return (int) (rng % n);
+	};
+	std::vector<int> sa, qa, sb, qb;
+	Pk16Task A, B;
+	auto mk = [&](Pk16Task &T, std::vector<int> &S, std::vector<int> &Q) {
+		T.lenS = rnd((uint64_t) max_len + 1);
+		T.lenQ = T.lenS - rnd((uint64_t) (T.lenS < BMAX ? T.lenS + 1 : BMAX + 1));
+		if (rnd(5) == 0) T.lenQ = rnd((uint64_t) T.lenS + 1);
+		T.b = T.lenS - T.lenQ + rnd(3);
+		if (rnd(2)) T.b = BMAX;
+		if (T.b > BMAX) T.b = BMAX, T.lenQ = T.lenS - BMAX > 0 ? T.lenS - BMAX : 0;
+		if (T.b < T.lenS - T.lenQ) T.lenQ = T.lenS - T.b;
+		const int f = rnd(3);
+		T.left_al = f == 1, T.right_al = f == 2;
+		S.resize((size_t) T.lenS), Q.resize((size_t) T.lenQ);
+		for (auto &c : S) c = rnd(4);
+		for (size_t j = 0; j < Q.size(); j++) Q[j] = (j < S.size() && rnd(10)) ? S[j] : rnd(4);
+	};
+	mk(A, sa, qa);
+	mk(B, sb, qb);
+	const int nw = (2 * BMAX + 1 + 15) / 16;
+	RowStore pa, pb, ra, rb;
+	pa.nw = pb.nw = ra.nw = rb.nw = nw;
+	CodeStream SA{&sa, 0}, QA{&qa, 0}, SB{&sb, 0}, QB{&qb, 0};
+	dp_forward_pk16_tagged<BMAX>(o, A, B, SA, QA, SB, QB, pa, pb, mode_or);
+	CodeStream S1{&sa, 0}, Q1{&qa, 0}, S2{&sb, 0}, Q2{&qb, 0};
+	dp_forward<BMAX, true>(o, A.lenS, A.lenQ, A.b, A.left_al, A.right_al, S1, Q1, ra);
+	dp_forward<BMAX, true>(o, B.lenS, B.lenQ, B.b, B.left_al, B.right_al, S2, Q2, rb);
+	std::vector<int> x, y;
+	auto trace = [&](const RowStore &r, const Pk16Task &T, std::vector<int> &steps, auto map) {
+		steps.clear();
+		auto emit = [&](int st) { steps.push_back(st); };
+		return dp_traceback<decltype(map)::value>(T.lenS, T.lenQ, T.b, r, emit);
+	};
+	using TG = std::integral_constant<uint32_t, kPtrTaggedAffine>;
+	using PL = std::integral_constant<uint32_t, kPtrPlain>;
+	int bad = 0;
+	if (trace(pa, A, x, TG()) != trace(ra, A, y, PL()) || x != y) bad++;
+	if (trace(pb, B, x, TG()) != trace(rb, B, y, PL()) || x != y) bad++;
+	return bad;
+}
+
+extern "C" int hostsim_pk16t_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len, int mode_or) {
+	const Opts o = to_opts(cfg);
+	if (o.aligner != 0 || !pk16t_fits(o, max_len, bmax)) return -1;
+	uint64_t rng = seed | 1;
+	int bad = 0;
+	for (int t = 0; t < n; t++) {
+#define PT_CASE(BM) \
+	if (bmax == BM) bad += pk16t_check_one<BM>(o, rng, max_len, mode_or);
+		PT_CASE(4) PT_CASE(6) PT_CASE(8) PT_CASE(10) PT_CASE(12)
+#undef PT_CASE
+	}
+	return bad;
+}
+
 // Tagged affine DP against dp_forward on random tasks (as pk16_check_one): the tracebacks must
 // agree. mode_or forces row variants on every row (a lane among other tasks). Returns the number
 // of differing tasks; -1 when the options do not admit the tagged form.

@@ -64,3 +64,12 @@ def tagged_check(cfg, n, seed, bmax, max_len, mode_or=0):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     return f(ctypes.addressof(cfg), n, seed, bmax, max_len, mode_or)
+
+
+def pk16t_check(cfg, n, seed, bmax, max_len, mode_or=0):
+    """Tasks whose tagged packed two-task DP traceback differs from the one-task DP's (0 = all
+    equal; -1 = options or lengths outside the tagged 16-bit form). mode_or forces row variants."""
+    f = lib().hostsim_pk16t_check
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return f(ctypes.addressof(cfg), n, seed, bmax, max_len, mode_or)

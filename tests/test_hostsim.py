@@ -114,3 +114,26 @@ def test_tagged_global_dp_equals_plain_dp(costs, bmax):
         for mode_or in (0, 4, 7):
             seed = 919 + bmax + max_len + 13 * mode_or
             assert H.tagged_check(cfg, 300, seed, bmax, max_len, mode_or) == 0, (costs, bmax, max_len, mode_or)
+
+
+@pytest.mark.parametrize("bmax", [4, 6, 8, 10, 12])
+@pytest.mark.parametrize(
+    "costs",
+    [
+        dict(dx=-4, dy=-4, ex=-1, sx=0, miss=-4, hit=1),
+        dict(dx=-7, dy=-7, ex=-1, sx=0, miss=-4, hit=1),
+        dict(dx=-7, dy=-7, ex=-2, sx=-1, miss=-3, hit=2),
+        dict(dx=-3, dy=-3, ex=-3, sx=0, miss=-1, hit=0),
+    ],
+    ids=["default", "c2", "hit2", "hit0"],
+)
+def test_tagged_pk16_dp_equals_plain_dp(costs, bmax):
+    """The tagged rows in packed 16-bit halves, two tasks per lane, trace the same paths as the
+    plain 32-bit DP for every task of the pair, up to the longest calls the 16-bit range admits,
+    with each pair's own row variants and with every row forced through the cut / all-edge ones."""
+    cfg = O.make_cfg(aligner=0, **costs)
+    for max_len in (3, 20, 140, 400):
+        for mode_or in (0, 4, 7):
+            seed = 313 + bmax + max_len + 13 * mode_or
+            r = H.pk16t_check(cfg, 300, seed, bmax, max_len, mode_or)
+            assert r == 0, (costs, bmax, max_len, mode_or, r)
