@@ -2139,6 +2139,7 @@ struct Request {
 	char *out = nullptr;
 	int64_t bases = 0, footprint = 0;
 	hero_request_info info{};
+	std::string err; // the step's error text when status != HERO_OK
 	bool done = false;
 };
 
@@ -2169,7 +2170,6 @@ struct hero_server {
 	std::vector<int64_t> max_step_bases;
 	bool learned = false;
 	bool stop = false;
-	std::string err;
 };
 
 namespace {
@@ -2267,14 +2267,16 @@ void serve_loop(hero_server *sv, int wi) {
 		}
 		{
 			std::lock_guard<std::mutex> lk(sv->mu);
-			for (size_t t = 0; t < take.size(); t++)
-				if (status[t] != HERO_OK && sv->err.empty()) {
+			std::string msg;
+			for (size_t t = 0; t < take.size() && msg.empty(); t++)
+				if (status[t] != HERO_OK) {
 					std::lock_guard<std::mutex> ek(c->err_mu);
-					sv->err = c->err;
+					msg = c->err;
 				}
 			const double t = now_s();
 			for (size_t x = 0; x < take.size(); x++) {
 				Request &r = *take[x];
+				if (status[x] != HERO_OK) r.err = msg;
 				r.info.status = status[x], r.info.t_done = t, r.info.step_id = step_id;
 				r.info.step_reads = (int32_t) len.size(), r.info.step_batches = (int32_t) take.size();
 				r.done = true;
@@ -2347,6 +2349,10 @@ extern "C" int hero_server_wait(hero_server *sv, int64_t ticket, hero_request_in
 	if (sv->reqs.find(ticket) == sv->reqs.end()) return HERO_ERR_ARG; // unknown, or already waited for
 	sv->cv_done.wait(lk, [&] { return sv->reqs.at(ticket).done; });
 	const hero_request_info ri = sv->reqs.at(ticket).info;
+	if (ri.status != HERO_OK) { // the batch's own error: its status here, its text in hero_last_error
+		std::lock_guard<std::mutex> ek(sv->c->err_mu);
+		sv->c->err = sv->reqs.at(ticket).err;
+	}
 	sv->reqs.erase(ticket); // a ticket is returned once; its slot is freed
 	if (info) *info = ri;
 	return ri.status;
@@ -2370,13 +2376,8 @@ extern "C" int hero_server_stop(hero_server *sv, hero_stats *merged) {
 		merged->ms_total = merged->ms_seed + merged->ms_plan + merged->ms_dp + merged->ms_score_vote + merged->ms_consensus;
 	}
 	for (auto &w : sv->workers) w.release();
-	const int rc = sv->err.empty() ? HERO_OK : HERO_ERR_HIP;
-	if (!sv->err.empty()) {
-		std::lock_guard<std::mutex> ek(sv->c->err_mu);
-		sv->c->err = sv->err;
-	}
-	delete sv;
-	return rc;
+	delete sv; // a failed batch reported its error through hero_server_wait
+	return HERO_OK;
 }
 
 extern "C" int hero_autotune(hero_ctx *c, int64_t max_subjects, hero_tune_report *report) {

@@ -277,8 +277,11 @@ int hero_server_start(hero_ctx *ctx, const hero_server_options *opt, hero_server
 /* HERO_ERR_NOMEM: the batch alone exceeds hbm_budget (never admissible). */
 int hero_server_submit(hero_server *srv, int64_t n, const char *seq, const int64_t *offset, const int32_t *length,
 		const int64_t *unique_id, char *corrected, int64_t *ticket);
+/* Returns the batch's status (also in info->status); a failed batch's error text is then in
+ * hero_last_error. A failed batch leaves the server and its workers serving the next ones. */
 int hero_server_wait(hero_server *srv, int64_t ticket, hero_request_info *info);
-/* Drains the queue, stops the workers; `merged` (optional) receives their summed device timings. */
+/* Drains the queue, stops the workers; `merged` (optional) receives their summed device timings.
+ * Batch errors are reported by hero_server_wait, not here. */
 int hero_server_stop(hero_server *srv, hero_stats *merged);
 
 #ifdef __cplusplus

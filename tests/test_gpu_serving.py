@@ -145,3 +145,18 @@ def test_request_slot_is_freed_after_wait(served):
         assert t2 != t
         out2, info2 = srv.wait(t2)
         assert info2["status"] == 0 and np.array_equal(out2, ctx.correct_batch(rs.subset(fresh[5:9])))
+
+
+def test_request_loop_recovers_after_a_refused_batch(served):
+    """A batch holding a read beyond the 16,383-base limit is refused (at submit, or with an error
+    status from its step); the same server and worker then correct the next batch as if alone."""
+    rs, resident, fresh, again, ctx = served
+    long_read = synth.ReadSet.from_strings(["ACGT" * 4200])  # 16,800 bases
+    with gpu.Server(ctx, workers=1) as srv:
+        with pytest.raises(gpu.HeroError, match="16800"):
+            srv.wait(srv.submit(long_read))
+        batch = rs.subset(fresh[:300])
+        t2 = srv.submit(batch)
+        out2, info2 = srv.wait(t2)
+        assert info2["status"] == 0
+        assert np.array_equal(out2, ctx.correct_batch(batch))
