@@ -429,6 +429,63 @@ Packed view(const ReadsDev &r, const uint8_t *good) {
 	return P;
 }
 
+// Host copy by every host thread (the staging copies below are memory-bound on one thread).
+void par_memcpy(char *dst, const char *src, size_t n) {
+	constexpr size_t kPiece = (size_t) 256 << 10;
+	const int64_t pieces = (int64_t) ((n + kPiece - 1) / kPiece);
+#pragma omp parallel for schedule(static) if (pieces > 1)
+	for (int64_t p = 0; p < pieces; p++) {
+		const size_t o = (size_t) p * kPiece;
+		std::memcpy(dst + o, src + o, std::min(kPiece, n - o));
+	}
+}
+
+// Caller (pageable) memory <-> device through pinned staging in two halves of kStageChunk bytes:
+// the link moves one half while the host threads fill or drain the other, which runs near the
+// link's rate where the runtime's own pageable path does not. Both return once the last byte has
+// landed (the staging is free again); the stream's earlier work finishes first.
+constexpr size_t kStageChunk = (size_t) 8 << 20;
+constexpr size_t kStageMin = (size_t) 1 << 20; // smaller copies go straight through the runtime
+template <class T> int staged_d2h(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, char *dst, const char *src, size_t n) {
+	if (n < kStageMin) {
+		if (n) HIPCHK(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, st));
+		HIPCHK(c, hipStreamSynchronize(st));
+		return HERO_OK;
+	}
+	HIPCHK(c, stg.ensure((2 * kStageChunk + sizeof(T) - 1) / sizeof(T)));
+	char *half[2] = {(char *) stg.p, (char *) stg.p + kStageChunk};
+	const size_t chunks = (n + kStageChunk - 1) / kStageChunk;
+	HIPCHK(c, hipMemcpyAsync(half[0], src, std::min(kStageChunk, n), hipMemcpyDeviceToHost, st));
+	for (size_t i = 0; i < chunks; i++) {
+		HIPCHK(c, hipStreamSynchronize(st)); // chunk i is in half[i % 2]
+		if (i + 1 < chunks) {
+			const size_t o = (i + 1) * kStageChunk;
+			HIPCHK(c, hipMemcpyAsync(half[(i + 1) % 2], src + o, std::min(kStageChunk, n - o), hipMemcpyDeviceToHost, st));
+		}
+		const size_t o = i * kStageChunk;
+		par_memcpy(dst + o, half[i % 2], std::min(kStageChunk, n - o));
+	}
+	return HERO_OK;
+}
+template <class T> int staged_h2d(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, char *dst, const char *src, size_t n) {
+	if (n < kStageMin) {
+		if (n) HIPCHK(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipStreamSynchronize(st));
+		return HERO_OK;
+	}
+	HIPCHK(c, stg.ensure((2 * kStageChunk + sizeof(T) - 1) / sizeof(T)));
+	char *half[2] = {(char *) stg.p, (char *) stg.p + kStageChunk};
+	const size_t chunks = (n + kStageChunk - 1) / kStageChunk;
+	for (size_t i = 0; i < chunks; i++) {
+		const size_t o = i * kStageChunk, m = std::min(kStageChunk, n - o);
+		par_memcpy(half[i % 2], src + o, m); // chunk i - 1 is on the link meanwhile
+		HIPCHK(c, hipStreamSynchronize(st)); // chunk i - 1 has landed: its half is free next
+		HIPCHK(c, hipMemcpyAsync(dst + o, half[i % 2], m, hipMemcpyHostToDevice, st));
+	}
+	HIPCHK(c, hipStreamSynchronize(st));
+	return HERO_OK;
+}
+
 int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
 		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good);
 int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
@@ -437,21 +494,28 @@ int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int
 }
 int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
 		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good) {
-	// word and base offsets, then the 2-bit words packed by every host thread straight into pinned
-	// staging ([words | woff | boff]), so the upload runs at the link's rate
-	std::vector<uint64_t> woff((size_t) n + 1), boff((size_t) n + 1);
-	uint64_t nw = 0, nb = 0;
-	for (int64_t i = 0; i < n; i++) {
-		woff[(size_t) i] = nw, boff[(size_t) i] = nb;
-		nw += (uint64_t) (len[i] + 31) / 32, nb += (uint64_t) len[i];
-	}
-	woff[(size_t) n] = nw, boff[(size_t) n] = nb;
+	// everything goes through pinned staging, so the upload runs at the link's rate:
+	// [words | woff | boff | uid | len | good], the 2-bit words packed by every host thread
+	uint64_t nw = 0;
+	for (int64_t i = 0; i < n; i++) nw += (uint64_t) (len[i] + 31) / 32;
 	const size_t words = (size_t) nw + 2; // two zero words of tail padding
-	HIPCHK(c, c->stage.ensure(words + 2 * ((size_t) n + 1)));
-	uint64_t *hw = c->stage.p, *hwoff = hw + words, *hboff = hwoff + (n + 1);
-	pack_reads_into(seq, off, len, n, woff.data(), hw, words);
-	std::memcpy(hwoff, woff.data(), ((size_t) n + 1) * 8);
-	std::memcpy(hboff, boff.data(), ((size_t) n + 1) * 8);
+	const size_t n1 = (size_t) n + 1, len_words = (n1 * 4 + 7) / 8, good_words = (n1 + 7) / 8;
+	HIPCHK(c, c->stage.ensure(words + 3 * n1 + len_words + good_words));
+	uint64_t *hw = c->stage.p, *hwoff = hw + words, *hboff = hwoff + n1, *huid = hboff + n1;
+	int32_t *hlen = (int32_t *) (huid + n1);
+	uint8_t *hgood = (uint8_t *) (huid + n1 + len_words);
+	uint64_t w_at = 0, b_at = 0;
+	for (int64_t i = 0; i < n; i++) {
+		hwoff[i] = w_at, hboff[i] = b_at;
+		w_at += (uint64_t) (len[i] + 31) / 32, b_at += (uint64_t) len[i];
+	}
+	hwoff[n] = w_at, hboff[n] = b_at;
+	pack_reads_into(seq, off, len, n, hwoff, hw, words);
+	if (n) {
+		std::memcpy(huid, uid, (size_t) n * 8);
+		std::memcpy(hlen, len, (size_t) n * 4);
+		std::memcpy(hgood, good.data(), (size_t) n);
+	}
 	// grow-only buffers: a steady stream of batches allocates nothing after the largest one
 	HIPCHK(c, d.words.ensure(words));
 	HIPCHK(c, d.woff.ensure((size_t) n + 1));
@@ -463,9 +527,9 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 	if (n) {
 		HIPCHK(c, hipMemcpyAsync(d.woff.p, hwoff, (size_t) n * 8, hipMemcpyHostToDevice, st));
 		HIPCHK(c, hipMemcpyAsync(d.boff.p, hboff, (size_t) n * 8, hipMemcpyHostToDevice, st));
-		HIPCHK(c, hipMemcpyAsync(d.len.p, len, (size_t) n * 4, hipMemcpyHostToDevice, st));
-		HIPCHK(c, hipMemcpyAsync(d.uid.p, uid, (size_t) n * 8, hipMemcpyHostToDevice, st));
-		HIPCHK(c, hipMemcpyAsync(d.good.p, good.data(), (size_t) n, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.len.p, hlen, (size_t) n * 4, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.uid.p, huid, (size_t) n * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(c, hipMemcpyAsync(d.good.p, hgood, (size_t) n, hipMemcpyHostToDevice, st));
 	}
 	HIPCHK(c, hipStreamSynchronize(st));
 	d.n = n;
@@ -1096,7 +1160,11 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, w->b_kref.ensure((size_t) nkeys + 1));
 	HIPCHK(c, w->b_cnt.ensure((size_t) nkeys + 1));
 	HIPCHK(c, w->b_off.ensure((size_t) nkeys + 1));
-	if (bases) HIPCHK(c, hipMemcpyAsync(w->b_raw.p, src, (size_t) bases, hipMemcpyHostToDevice, w->st));
+	if (bases && contiguous) {
+		if (int r = staged_h2d(c, w->st, w->h_bytes, (char *) w->b_raw.p, src, (size_t) bases)) return r;
+	} else if (bases) {
+		HIPCHK(c, hipMemcpyAsync(w->b_raw.p, src, (size_t) bases, hipMemcpyHostToDevice, w->st));
+	}
 	if (n) {
 		HIPCHK(c, hipMemcpyAsync(d.boff.p, hb, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
 		HIPCHK(c, hipMemcpyAsync(d.woff.p, hw, (size_t) n * 8, hipMemcpyHostToDevice, w->st));
@@ -1207,10 +1275,11 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, w->b_tally.p + 4 * bases, 0, n, w->b_cons.p, nullptr,
 			w->dstats.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
-	if (bases) {
-		// back in the caller's layout: one copy when its reads are back to back
-		char *dst = contiguous ? corrected + off[0] : w->h_bytes.p;
-		HIPCHK(c, hipMemcpyAsync(dst, w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
+	// back in the caller's layout: one staged copy when its reads are back to back
+	if (bases && contiguous) {
+		if (int r = staged_d2h(c, w->st, w->h_bytes, corrected + off[0], (const char *) w->b_cons.p, (size_t) bases)) return r;
+	} else if (bases) {
+		HIPCHK(c, hipMemcpyAsync(w->h_bytes.p, w->b_cons.p, (size_t) bases, hipMemcpyDeviceToHost, w->st));
 	}
 	HIPCHK(c, hipStreamSynchronize(w->st));
 	if (!contiguous)
@@ -1604,10 +1673,8 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	const int64_t min_ov = c->o.min_overlap;
 	const int k = c->o.k;
 #pragma omp parallel for schedule(static, 4096)
-	for (int64_t i = 0; i < n; i++) {
-		c->q_good[(size_t) i] = query_good(seq + off[i], len[i], min_ov);
-		c->q_active_as_subject[(size_t) i] = subject_active(seq + off[i], len[i], min_ov, k);
-	}
+	for (int64_t i = 0; i < n; i++)
+		read_flags(seq + off[i], len[i], min_ov, k, c->q_good[(size_t) i], c->q_active_as_subject[(size_t) i]);
 	// reads that are not corrected go out as they came in: keep their bytes (all bytes when the
 	// layout has gaps, which the output keeps too)
 	c->q_seq.clear();
@@ -1742,8 +1809,7 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 	// The device holds corrected bases at packed base offsets; the caller's layout usually is that
 	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
 	if (c->q_contiguous) {
-		HIPCHK(c, hipMemcpyAsync(corrected, c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
-		HIPCHK(c, hipStreamSynchronize(c->w.st));
+		if (int r = staged_d2h(c, c->w.st, c->stage, corrected, (const char *) c->cons_out.p, (size_t) c->q_bases)) return r;
 		for (size_t t = 0; t < c->q_not_good.size(); t++) {
 			const int64_t i = c->q_not_good[t];
 			const char *src = c->q_seq.data() + (c->q_full_copy ? c->q_off[(size_t) i] : c->q_keep_off[t]);
@@ -1751,8 +1817,7 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 		}
 	} else {
 		std::vector<char> h((size_t) c->q_bases);
-		HIPCHK(c, hipMemcpyAsync(h.data(), c->cons_out.p, (size_t) c->q_bases, hipMemcpyDeviceToHost, c->w.st));
-		HIPCHK(c, hipStreamSynchronize(c->w.st));
+		if (int r = staged_d2h(c, c->w.st, c->stage, h.data(), (const char *) c->cons_out.p, h.size())) return r;
 		std::memcpy(corrected, c->q_seq.data(), c->q_seq.size());
 		uint64_t b = 0;
 		for (int64_t i = 0; i < n; i++) {

@@ -88,6 +88,14 @@ static int subject_is_good(const oracle_cfg *c, const char *s, int64_t n) {
 	return (uint64_t) n < mo || quality_filter(s, n);
 }
 
+/* Both dataset predicates of every read of a set. */
+void oracle_read_flags(const oracle_cfg *cfg, const oracle_reads *r, uint8_t *query_good, uint8_t *subject_good) {
+	for (int64_t i = 0; i < r->n; i++) {
+		query_good[i] = (uint8_t) query_is_good(cfg, r->seq + r->offset[i], r->length[i]);
+		subject_good[i] = (uint8_t) subject_is_good(cfg, r->seq + r->offset[i], r->length[i]);
+	}
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Hash function and table size                                                              */
 /* ------------------------------------------------------------------------------------------ */

@@ -61,6 +61,11 @@ typedef struct {
 /* Default reference configuration (Config.cpp defaults + BandAligner::setCost). */
 void oracle_default_cfg(oracle_cfg *cfg);
 
+/* Dataset predicates of every read: query_good = len > (u16) minOverlap && qualityFilter
+ * (src/QueryDataset.cpp:60-75,226); subject_good = len < (u16) minOverlap || qualityFilter
+ * (src/SubjectDataset.cpp:157-162). */
+void oracle_read_flags(const oracle_cfg *cfg, const oracle_reads *reads, uint8_t *query_good, uint8_t *subject_good);
+
 /* HashTableLongKey::hashFunction (src/HashTableLongKey.cpp:188-206). */
 uint64_t oracle_hash_kmer(const char *s, int64_t k, uint64_t table_size);
 

@@ -69,6 +69,7 @@ def lib():
         L.oracle_seeds.restype = ctypes.c_int64
         L.oracle_seeds.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(Reads), ctypes.c_char_p, ctypes.c_int32,
                                    ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
+        L.oracle_read_flags.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(Reads), ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
@@ -88,6 +89,14 @@ def _reads(rs):
     r = Reads(rs.n, rs.seq.ctypes.data, rs.offset.ctypes.data, rs.length.ctypes.data, rs.unique_id.ctypes.data)
     r._keep = rs
     return r
+
+
+def read_flags(cfg, rs):
+    """Per-read (query predicate, subject predicate) as two uint8 arrays."""
+    q = np.zeros(rs.n, dtype=np.uint8)
+    s = np.zeros(rs.n, dtype=np.uint8)
+    lib().oracle_read_flags(ctypes.byref(cfg), _reads(rs), q.ctypes.data, s.ctypes.data)
+    return q, s
 
 
 def hash_kmer(kmer, P):

@@ -319,6 +319,21 @@ extern "C" int hostsim_band_align(const hs_cfg *cfg, const char *subj, int ls, c
 	return 0;
 }
 
+// The product's host-side read preparation: both predicates per read (read_flags, one scan) and
+// the 2-bit packing (pack_reads, 2 zero words of tail padding), so tests can hold them against the
+// oracle's predicates and a plain packing.
+extern "C" void hostsim_read_flags(const hs_reads *r, int64_t min_overlap, int k, uint8_t *good, uint8_t *active) {
+	for (int64_t i = 0; i < r->n; i++) read_flags(r->seq + r->offset[i], r->length[i], min_overlap, k, good[i], active[i]);
+}
+extern "C" int64_t hostsim_pack(const hs_reads *r, uint64_t *out, int64_t cap, uint64_t *woff_out) {
+	std::vector<uint64_t> words, woff;
+	pack_reads(r->seq, r->offset, r->length, r->n, words, woff);
+	if ((int64_t) words.size() > cap) return -(int64_t) words.size();
+	std::memcpy(out, words.data(), words.size() * 8);
+	std::memcpy(woff_out, woff.data(), woff.size() * 8);
+	return (int64_t) words.size();
+}
+
 extern "C" uint64_t hostsim_band_cells(int lenS, int lenQ, int b) { return band_cells(lenS, lenQ, b); }
 
 // Packed two-task DP against the one-task DP on random task pairs: the tracebacks of both tasks

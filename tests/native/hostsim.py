@@ -21,8 +21,33 @@ def lib():
         L.hostsim_band_align.argtypes = [ctypes.POINTER(Cfg), ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]
+        L.hostsim_read_flags.argtypes = [ctypes.POINTER(Reads), ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        L.hostsim_pack.restype = ctypes.c_int64
+        L.hostsim_pack.argtypes = [ctypes.POINTER(Reads), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+def read_flags(reads, min_overlap, k):
+    """The product's (query good, active as subject) flags per read."""
+    from oracle.oracle import _reads
+    good = np.zeros(reads.n, dtype=np.uint8)
+    active = np.zeros(reads.n, dtype=np.uint8)
+    lib().hostsim_read_flags(_reads(reads), min_overlap, k, good.ctypes.data, active.ctypes.data)
+    return good, active
+
+
+def pack(reads):
+    """The product's 2-bit packing: (words, per-read word offsets)."""
+    from oracle.oracle import _reads
+    cap = int(((reads.length.astype(np.int64) + 31) // 32).sum()) + 2
+    words = np.zeros(cap, dtype=np.uint64)
+    woff = np.zeros(reads.n, dtype=np.uint64)
+    got = lib().hostsim_pack(_reads(reads), words.ctypes.data, cap, woff.ctypes.data)
+    if got != cap:
+        raise RuntimeError(f"hostsim_pack wrote {got} words, expected {cap}")
+    return words, woff
 
 
 def run(cfg, queries, subjects=None):

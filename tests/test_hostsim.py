@@ -137,3 +137,78 @@ def test_tagged_pk16_dp_equals_plain_dp(costs, bmax):
             seed = 313 + bmax + max_len + 13 * mode_or
             r = H.pk16t_check(cfg, 300, seed, bmax, max_len, mode_or)
             assert r == 0, (costs, bmax, max_len, mode_or, r)
+
+
+def _byte_reads(reads):
+    """ReadSet from a list of bytes objects (any byte values)."""
+    lens = np.array([len(r) for r in reads], dtype=np.int32)
+    offs = np.zeros(len(reads), dtype=np.int64)
+    offs[1:] = np.cumsum(lens[:-1].astype(np.int64))
+    seq = np.frombuffer(b"".join(reads), dtype=np.uint8).copy()
+    return synth.ReadSet(seq, offs, lens, np.arange(len(reads), dtype=np.int64))
+
+
+def _predicate_reads(seed=5):
+    rng = np.random.default_rng(seed)
+    reads = []
+    for n in list(range(0, 48)) + [254, 255, 256, 510, 511, 900, 2047]:
+        for _ in range(3):
+            reads.append(rng.choice(list(b"ACGT"), n).astype(np.uint8).tobytes())
+        if n:
+            # one foreign byte anywhere (N, lower case, DEL, high bytes)
+            r = bytearray(rng.choice(list(b"ACGT"), n).astype(np.uint8).tobytes())
+            r[int(rng.integers(n))] = int(rng.choice([ord("N"), ord("a"), ord("t"), 0x7F, 0x80, 0xFF, 0]))
+            reads.append(bytes(r))
+            # one letter right at, just under and just over the 80% threshold
+            thr = int(n * 0.8)
+            for cnt in (thr - 1, thr, thr + 1):
+                if 0 <= cnt <= n:
+                    other = [c for c in b"ACGT" if c != b"ACGT"[n % 4]]
+                    body = [b"ACGT"[n % 4]] * cnt + list(rng.choice(other, n - cnt))
+                    reads.append(bytes(rng.permutation(np.array(body, dtype=np.uint8))))
+    return _byte_reads(reads)
+
+
+def test_oracle_read_predicates_known_answers():
+    # QueryDataset::qualityFilter (src/QueryDataset.cpp:60-75) fixes these by hand: letters other
+    # than upper-case A/C/G/T fail; a letter at floor(0.8 n) or more fails; the length tests
+    # compare with minOverlap cast to 16 bits (src/QueryDataset.cpp:226, SubjectDataset.cpp:157-162).
+    reads = [b"ACGT" * 10, b"AAAAAAAACG", b"AAAAAAACGT", b"ACGTN" * 8, b"acgt" * 10, b"", b"ACGT" * 9 + b"ACG"]
+    rs = _byte_reads(reads)
+    q, s = O.read_flags(O.make_cfg(min_overlap=39), rs)
+    assert q.tolist() == [1, 0, 0, 0, 0, 0, 0]
+    assert s.tolist() == [1, 1, 1, 0, 0, 1, 1]
+    q, s = O.read_flags(O.make_cfg(min_overlap=0), rs)
+    assert q.tolist() == [1, 0, 1, 0, 0, 0, 1]
+    assert s.tolist() == [1, 0, 1, 0, 0, 0, 1]
+    q, s = O.read_flags(O.make_cfg(min_overlap=65536 + 9), rs)  # compared as 9
+    assert q.tolist() == [1, 0, 1, 0, 0, 0, 1]
+    assert s.tolist() == [1, 0, 1, 0, 0, 1, 1]
+
+
+@pytest.mark.parametrize("min_overlap,k", [(40, 39), (0, 1), (12, 30), (65536 + 20, 25)])
+def test_read_flags_match_oracle_predicates(min_overlap, k):
+    rs = _predicate_reads()
+    q, s = O.read_flags(O.make_cfg(min_overlap=min_overlap), rs)
+    good, active = H.read_flags(rs, min_overlap, k)
+    n = rs.length.astype(np.int64)
+    assert np.array_equal(good, q)
+    # subjects too short for the overlap test or for one k-mer need no device work
+    assert np.array_equal(active, s & (n >= min_overlap) & (n >= k))
+
+
+def test_pack_matches_plain_packing():
+    rs = _predicate_reads(seed=6)
+    words, woff = H.pack(rs)
+    codes = ((rs.seq >> 1) & 3).astype(np.uint64)
+    expect = np.zeros_like(words)
+    w = 0
+    for i in range(rs.n):
+        o, L = int(rs.offset[i]), int(rs.length[i])
+        assert int(woff[i]) == w
+        for j in range(0, L, 32):
+            c = codes[o + j:o + min(L, j + 32)]
+            expect[w] = np.bitwise_or.reduce(c << (62 - 2 * np.arange(len(c), dtype=np.uint64)))
+            w += 1
+    assert w + 2 == len(words)
+    assert np.array_equal(words, expect)
