@@ -47,7 +47,7 @@ def build_synth():
 
 def build_gpu(jobs=4):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = ["kernels.hip", "dp_wave.hip", "capi.hip", "serve_sort.hip"]
+    srcs = ["kernels.hip", "dp_wave.hip", "capi.hip", "serve_sort.hip", "locality.hip"]
     objs = []
     procs = []
     for s in srcs:

@@ -312,6 +312,15 @@ struct hero_ctx {
 	bool cons_valid = false; // cons_out holds the consensus of the current tallies
 	ReadsDev Qd;
 	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
+	// locality order (vote modes unless file_order): the query set as subjects in minimizer order
+	// (flags: the subject predicate), each query's tally offset in that order, and the ordered copy
+	// of a separate subject set
+	bool q_ordered = false;
+	ReadsDev Qo, So;
+	DevBuf<uint64_t> t_off;
+	DevBuf<uint8_t> order_tmp;
+	const uint64_t *tally_off() const { return q_ordered ? t_off.p : Qd.boff.p; }
+	bool locality_order() const { return o.call_consensus && !opt.records && !opt.file_order; }
 	// subjects (separate stream)
 	ReadsDev Sd;
 	// seed index over the query reads (the reference's orientation)
@@ -533,6 +542,28 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 	}
 	HIPCHK(c, hipStreamSynchronize(st));
 	d.n = n;
+	return HERO_OK;
+}
+
+// The locality-ordered copy `out` of read set R (launch_order): flags from `flag`, keys over the
+// reads `active` marks; t_off, when given, receives each read's tally offset in that order.
+int order_reads(hero_ctx *c, hipStream_t st, const ReadsDev &R, const uint8_t *active, const uint8_t *flag,
+		const std::vector<int32_t> &len, ReadsDev &out, uint64_t *t_off) {
+	const int64_t n = R.n;
+	uint64_t nw = 0;
+	for (int64_t i = 0; i < n; i++) nw += (uint64_t) (len[(size_t) i] + 31) / 32;
+	HIPCHK(c, out.words.ensure((size_t) nw + 2));
+	HIPCHK(c, out.woff.ensure((size_t) n + 1));
+	HIPCHK(c, out.len.ensure((size_t) n + 1));
+	HIPCHK(c, out.uid.ensure((size_t) n + 1));
+	HIPCHK(c, out.good.ensure((size_t) n + 1));
+	HIPCHK(c, c->order_tmp.ensure(order_tmp_bytes(n)));
+	OrderOut o{out.words.p, out.woff.p, out.len.p, out.uid.p, out.good.p, t_off, nullptr};
+	const Packed P{R.words.p, R.woff.p, R.len.p, R.uid.p, flag, n};
+	HIPCHK(c, launch_order(P, active, flag, o, c->order_tmp.p, c->order_tmp.n, st));
+	HIPCHK(c, hipMemsetAsync(out.words.p + nw, 0, 16, st)); // the two zero words of tail padding
+	HIPCHK(c, hipStreamSynchronize(st));
+	out.n = n;
 	return HERO_OK;
 }
 
@@ -868,7 +899,7 @@ int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub,
 	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
 	uint64_t nseeds = 0;
 	if (int r = wscan(c, w, cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
-	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->Qd.boff.p, c->tally.p, c->tally.p + 4 * c->q_bases,
+	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->tally_off(), c->tally.p, c->tally.p + 4 * c->q_bases,
 			s_uid, rec_mode);
 }
 
@@ -1272,7 +1303,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	// corrected bytes: the batch's own bytes, overwritten by the consensus of its good reads
 	HIPCHK(c, hipEventRecord(w->ev[5].a, w->st));
 	if (bases) HIPCHK(c, hipMemcpyAsync(w->b_cons.p, w->b_raw.p, (size_t) bases, hipMemcpyDeviceToDevice, w->st));
-	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->b_tally.p, w->b_tally.p + 4 * bases, 0, n, w->b_cons.p, nullptr,
+	HIPCHK(c, launch_consensus(Q, c->o, w->bq.boff.p, w->bq.boff.p, w->b_tally.p, w->b_tally.p + 4 * bases, 0, n, w->b_cons.p, nullptr,
 			w->dstats.p, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[6].a, w->st));
 	// back in the caller's layout: one staged copy when its reads are back to back
@@ -1697,6 +1728,13 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good)) return r;
 	HIPCHK(c, c->q_subject_good.ensure((size_t) n + 1));
 	if (n) HIPCHK(c, hipMemcpyAsync(c->q_subject_good.p, c->q_active_as_subject.data(), (size_t) n, hipMemcpyHostToDevice, c->w.st));
+	c->q_ordered = false;
+	if (c->locality_order() && n > 0) {
+		HIPCHK(c, c->t_off.ensure((size_t) n + 1));
+		if (int r = order_reads(c, c->w.st, c->Qd, c->q_subject_good.p, c->q_subject_good.p, c->q_len, c->Qo, c->t_off.p))
+			return r;
+		c->q_ordered = true;
+	}
 	c->q_bases = qb;
 	// 4 mismatch counters per base, then the matched-run difference array (1 per base)
 	if (c->tally.n < (size_t) 5 * qb + 1 || !c->tally.p) HIPCHK(c, c->tally.exact((size_t) 5 * qb + 1)); // no slack: 20 B per base
@@ -1740,6 +1778,12 @@ int hero_align_subjects(hero_ctx *c, int64_t n, const char *seq, const int64_t *
 	int ml = 0;
 	for (int64_t i = 0; i < n; i++) ml = std::max(ml, len[i]);
 	for (int32_t l : c->q_len) ml = std::max(ml, l);
+	if (c->locality_order() && n > 0) {
+		if (int r = order_reads(c, c->w.st, c->Sd, c->Sd.good.p, c->Sd.good.p, std::vector<int32_t>(len, len + n), c->So,
+				nullptr))
+			return r;
+		return align_packed(c, view(c->So, c->So.good.p), n, ml, nullptr);
+	}
 	return align_packed(c, view(c->Sd, c->Sd.good.p), n, ml, uid);
 }
 
@@ -1748,6 +1792,7 @@ int hero_align_self(hero_ctx *c) {
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	int ml = 0;
 	for (int32_t l : c->q_len) ml = std::max(ml, l);
+	if (c->q_ordered) return align_packed(c, view(c->Qo, c->Qo.good.p), c->Qo.n, ml, nullptr);
 	return align_packed(c, view(c->Qd, c->q_subject_good.p), c->Qd.n, ml, c->q_uid.data());
 }
 
@@ -1789,7 +1834,7 @@ int hero_call_consensus(hero_ctx *c, char *corrected) {
 	const int64_t n = c->Qd.n;
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
 	HIPCHK(c, hipEventRecord(c->w.ev[5].a, c->w.st));
-	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p,
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally_off(), c->tally.p,
 			c->tally.p + 4 * c->q_bases, 0, n, c->cons_out.p, nullptr,
 			c->w.dstats.p, c->w.st));
 	HIPCHK(c, hipEventRecord(c->w.ev[6].a, c->w.st));
@@ -1836,7 +1881,7 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 	DevBuf<uint32_t> dec;
 	HIPCHK(c, codes.exact((size_t) c->q_bases));
 	HIPCHK(c, dec.exact((size_t) 4 * c->q_bases));
-	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally.p,
+	HIPCHK(c, launch_consensus(view(c->Qd, c->Qd.good.p), c->o, c->Qd.boff.p, c->tally_off(), c->tally.p,
 			c->tally.p + 4 * c->q_bases, 0, c->Qd.n, codes.p, dec.p,
 			nullptr, c->w.st));
 	std::vector<uint32_t> h((size_t) 4 * c->q_bases);
@@ -1860,11 +1905,14 @@ int hero_set_tally(hero_ctx *c, const uint32_t *tally) {
 	// base's total count, so that consensus decodes the own base's count to exactly its given value.
 	std::vector<uint32_t> h((size_t) 5 * c->q_bases);
 	uint32_t *h4 = h.data(), *hd = h.data() + 4 * c->q_bases;
-	uint64_t b = 0;
+	// each read's counters sit at its tally offset (its place in the locality order, or its base offset)
+	std::vector<uint64_t> toff((size_t) c->Qd.n);
+	if (c->Qd.n) HIPCHK(c, hipMemcpy(toff.data(), c->tally_off(), toff.size() * 8, hipMemcpyDeviceToHost));
 	for (int64_t i = 0; i < c->Qd.n; i++) {
 		const int L = c->q_len[(size_t) i];
 		const uint32_t *src = tally + 4 * c->q_off[(size_t) i];
 		uint32_t prev = 0; // each read's running sum starts at its first base
+		uint64_t b = toff[(size_t) i];
 		for (int p = 0; p < L; p++, b++) {
 			uint32_t tot = 0;
 			for (int j = 0; j < 4; j++) h4[4 * b + j] = src[4 * p + j], tot += src[4 * p + j];
@@ -1887,7 +1935,7 @@ extern "C" int hero_coverage_histogram(hero_ctx *c, uint64_t *counts, int32_t ca
 	HIPCHK(c, mx.exact(1));
 	HIPCHK(c, hipMemsetAsync(hist.p, 0, (size_t) cap * 8, c->w.st));
 	HIPCHK(c, hipMemsetAsync(mx.p, 0, 4, c->w.st));
-	HIPCHK(c, launch_coverage_hist(c->Qd.len.p, c->Qd.boff.p, c->tally.p + 4 * c->q_bases, c->Qd.n, (uint32_t) cap,
+	HIPCHK(c, launch_coverage_hist(c->Qd.len.p, c->tally_off(), c->tally.p + 4 * c->q_bases, c->Qd.n, (uint32_t) cap,
 			hist.p, mx.p, c->w.st));
 	uint32_t m = 0;
 	HIPCHK(c, hipMemcpyAsync(counts, hist.p, (size_t) cap * 8, hipMemcpyDeviceToHost, c->w.st));

@@ -247,11 +247,12 @@ static_assert(kCtlRange + kNumBandClasses + 2 <= kCtlCounter, "class ranges over
 static_assert(kCtlCounter + kNumBandClasses <= kCtlWave, "class counters overlap the plain form's words");
 hipError_t launch_class_ranges(const uint32_t *bucket_off, uint32_t *ctl, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
-// Tallies: 4 counters per base (mismatch votes by base code) at 4 * q_boff[q], and the matched-run
-// difference array, 1 counter per base, at q_boff[q].
-hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally,
-		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats,
-		hipStream_t st);
+// Tallies: 4 counters per base (mismatch votes by base code) at 4 * t_off[q], and the matched-run
+// difference array, 1 counter per base, at t_off[q]; the read's outputs (codes_out, decoded_out)
+// at its base offset q_boff[q].
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint64_t *t_off,
+		const uint32_t *tally, const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
+		unsigned long long *stats, hipStream_t st);
 
 // Serving batch intake: packs raw bytes into 2-bit words, evaluates the query predicate into good
 // and writes key references (query << 14 | position) at koff for reads longer than min_ov.
@@ -287,6 +288,32 @@ hipError_t launch_serve_b(const Packed &S, const Packed &Q, const Opts &o, const
 // Radix sort of (key, value) u64 pairs on the low key_bits bits (serve_sort.hip).
 hipError_t sort_pairs_u64(const uint64_t *keys_in, uint64_t *keys_out, const uint64_t *vals_in, uint64_t *vals_out,
 		uint32_t n, int key_bits, void *tmp, size_t &tmp_bytes, hipStream_t st);
+
+// Locality order of a read set (locality.hip): reads sorted by their minimizer (the smallest mix
+// of a canonical 16-base window; inactive reads last), written as an ordered copy: read r of the
+// copy is read perm[r] of the set, its flag taken from `flag`. With t_off given, t_off[i] is the
+// base offset of read i's place in the order (the tally layout); with perm given, the order.
+struct OrderOut {
+	uint64_t *words, *woff; // the copy's packed words (capacity: the set's words) and word offsets
+	int32_t *len;
+	int64_t *uid;
+	uint8_t *good;
+	uint64_t *t_off; // may be null
+	uint64_t *perm;  // may be null
+};
+size_t order_tmp_bytes(int64_t n);
+hipError_t launch_order(const Packed &R, const uint8_t *active, const uint8_t *flag, OrderOut &out, void *tmp,
+		size_t tmp_bytes, hipStream_t st);
+
+#if defined(__HIPCC__)
+// Item of workgroup b of n: workgroups go to the 8 XCDs in turn (b % 8), so XCD x is given the
+// x-th contiguous eighth of [0, n), in order; neighbouring items (subjects in locality order) then
+// share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
+	const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, t = b >> 3;
+	return x * q + (x < r ? x : r) + t;
+}
+#endif
 
 // Exclusive scan of n u32 values (n < 2^31); *total_dev receives the u64 sum.
 hipError_t scan_u32(const uint32_t *in, uint32_t *out, int64_t n, unsigned long long *total_dev, uint32_t *tmp,

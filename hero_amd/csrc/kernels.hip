@@ -291,7 +291,7 @@ __device__ __forceinline__ uint32_t window_seeds(const Packed &S, const Packed &
 // lane's seed count, so pass A1 emits the seeds without hashing or probing again.
 __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
-	const int sl = blockIdx.x;
+	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) {
@@ -407,7 +407,7 @@ __device__ __forceinline__ Group chained_group(const ChunkDev &c, uint32_t base,
 __global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
 	__shared__ uint64_t lds[kLdsSeeds];
-	const int sl = blockIdx.x;
+	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) {
@@ -501,7 +501,7 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 
 // Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
 __global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
-	const int sl = blockIdx.x;
+	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
@@ -592,7 +592,7 @@ __device__ __forceinline__ void pass_b_pair(const Packed &S, const Packed &Q, co
 // lane per record; the record is replayed from its chain and DP results.
 __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkDev c, uint32_t nvotes,
 		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, uint32_t *ext, unsigned long long *stats) {
-	const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+	const uint32_t i = xcd_item(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
 	if (i >= nvotes) return;
 	const VoteItem v = c.votes[i];
 	if (v.q == kNoRecord) return;
@@ -644,7 +644,7 @@ struct MapStatsV {
 	__device__ void finish() {}
 };
 __global__ __launch_bounds__(64) void k_map_select(Packed S, ChunkDev c) {
-	const int sl = blockIdx.x;
+	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int lane = lane_id();
 	const uint32_t b0 = c.sub_pair_off[sl], n = c.sub_pairs[sl];
 	if (n == 0) return;
@@ -888,7 +888,7 @@ __global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, Chunk
 	__shared__ int32_t match0[kPassBGroups];
 	__shared__ uint32_t rev_only[kPassBGroups];
 	__shared__ uint8_t used1[kPassBGroups];
-	const int sl = blockIdx.x;
+	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
@@ -1544,14 +1544,14 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 // difference array is decoded by a wave-wide inclusive scan (wrapping u32 sums) with a carry
 // between rounds and added to the own base's counter, then every position decides independently
 // (consensus_base).
-__global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint32_t *tally,
-		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
+__global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint64_t *q_boff, const uint64_t *t_off,
+		const uint32_t *tally, const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
 		unsigned long long *stats) {
 	const int64_t i = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
 	const int lane = lane_id();
 	if (i >= nq) return;
 	const int64_t q = q0 + i;
-	const uint64_t b = q_boff[q];
+	const uint64_t b = q_boff[q], tb = t_off[q];
 	const int L = Q.len[q];
 	const bool good = Q.good[q];
 	const uint64_t *w = Q.words + Q.woff[q];
@@ -1561,9 +1561,9 @@ __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint6
 		uint32_t c[4] = {0, 0, 0, 0}, x = 0;
 		int own = 0;
 		if (p < L) {
-			const uint4 t = *(const uint4 *) (tally + 4 * (b + p));
+			const uint4 t = *(const uint4 *) (tally + 4 * (tb + p));
 			c[0] = t.x, c[1] = t.y, c[2] = t.z, c[3] = t.w;
-			x = diff[b + p];
+			x = diff[tb + p];
 			own = base_at(w, p);
 		}
 		const uint32_t run = carry + wave_excl_scan(x) + x;
@@ -1585,12 +1585,12 @@ __global__ __launch_bounds__(256) void k_consensus(Packed Q, Opts o, const uint6
 	const uint64_t m = __ballot(changed != 0);
 	if (stats && good && lane == 0 && m) stat_add(stats, kStReadsChanged, 1ull);
 }
-hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint32_t *tally,
-		const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out, unsigned long long *stats,
-		hipStream_t st) {
+hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint64_t *t_off,
+		const uint32_t *tally, const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
+		unsigned long long *stats, hipStream_t st) {
 	if (nq == 0) return hipSuccess;
-	k_consensus<<<(unsigned) ((nq + 3) / 4), 256, 0, st>>>(Q, o, q_boff, tally, diff, q0, nq, codes_out, decoded_out,
-			stats);
+	k_consensus<<<(unsigned) ((nq + 3) / 4), 256, 0, st>>>(Q, o, q_boff, t_off, tally, diff, q0, nq, codes_out,
+			decoded_out, stats);
 	return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ class Options(ctypes.Structure):
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
         ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
         ("mapping", ctypes.c_int32), ("band_align_form", ctypes.c_int32), ("long_format", ctypes.c_int32),
-        ("merge", ctypes.c_int32),
+        ("merge", ctypes.c_int32), ("file_order", ctypes.c_int32),
     ]
 
 
@@ -258,6 +258,12 @@ class Context:
     def tally(self):
         t = np.zeros(4 * len(self._queries.seq), dtype=np.uint32)
         self._check(lib().hero_get_tally(self.h, t.ctypes.data))
+        return t
+
+    def extension_tally(self):
+        """--merge: 20 x 4 u32 per query read (QueryRead::aExtraCoverageMatrix) via hero_get_extension_tally."""
+        t = np.zeros(20 * 4 * len(self._queries.length), dtype=np.uint32)
+        self._check(lib().hero_get_extension_tally(self.h, t.ctypes.data))
         return t
 
     def set_tally(self, t):

@@ -72,6 +72,9 @@ typedef struct {
 	                             more than one op also votes the (up to) 20 subject bases after its end into its
 	                             query's extension tally (SubjectRead::updateQueryCoverageConsiderMerging,
 	                             src/SubjectRead.cpp:332-461); read it with hero_get_extension_tally */
+	int32_t file_order;       /* 1: subjects run in file order and the tallies lie in read order; 0 (vote modes):
+	                             locality order, subjects run and tallies lie in the order of each read's
+	                             minimizer, so overlapping reads share cache lines; results are identical */
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */
