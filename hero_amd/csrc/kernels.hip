@@ -379,7 +379,11 @@ struct GlobArr {
 	__device__ uint64_t &operator[](int i) const { return p[i]; }
 };
 
-constexpr int kLdsSeeds = 1024;
+// Seeds of a subject strand sorted in LDS (more take the global-memory path). 4 KB per wavefront,
+// and passes A1 / A2 / B are register-allocated for 8 / 7 / 8 waves per SIMD: these kernels wait on
+// chains of dependent loads, so more waves in flight is what speeds them up (C2: a strand has 26
+// seeds on average).
+constexpr int kLdsSeeds = 512;
 
 __device__ __forceinline__ PairReads make_pair(const Packed &S, const Packed &Q, const Opts &o, int64_t s, int Ls,
 		int64_t q, int strand) {
@@ -404,7 +408,7 @@ __device__ __forceinline__ Group chained_group(const ChunkDev &c, uint32_t base,
 
 // Pass A1: per subject strand, emit the seeds, sort them, chain every kept query group and count
 // the DP tasks / edit words its surviving chains need.
-__global__ __launch_bounds__(64) void k_pass_a1(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pass_a1(Packed S, Packed Q, DevIndex ix, Opts o, ChunkDev c,
 		unsigned long long *stats) {
 	__shared__ uint64_t lds[kLdsSeeds];
 	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
@@ -500,7 +504,7 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 }
 
 // Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
-__global__ __launch_bounds__(64) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
 	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
@@ -879,9 +883,9 @@ __device__ __forceinline__ uint32_t collect_groups(const uint8_t *gflag, uint32_
 // Pass B: one wavefront per subject. The kept groups of both strands are listed in LDS with their
 // query ids and paired by query (binary search over the reverse list); lanes then take the pairs
 // densely. More kept groups than the lists hold: the segments are walked directly instead.
-constexpr uint32_t kPassBGroups = 256;
+constexpr uint32_t kPassBGroups = 128;
 template <bool RECORDS>
-__global__ __launch_bounds__(64) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, unsigned long long *stats) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_pass_b(Packed S, Packed Q, Opts o, ChunkDev c, unsigned long long *stats) {
 	__shared__ uint32_t vcount;
 	__shared__ uint32_t list0[kPassBGroups + 1], list1[kPassBGroups + 1];
 	__shared__ uint32_t q0s[kPassBGroups], q1s[kPassBGroups];
