@@ -319,14 +319,19 @@ def serving_latency(seed_duration=2.0, fixed_duration=4.0, workers=2, max_step_r
         nb = max(20, int(rate * fixed_duration))
         arrivals = np.cumsum(rng.exponential(1.0 / rate, nb))
         batches = [batch(4096, j * 4096) for j in range(nb)]
-        # untimed: full-size steps on every worker grow their buffers once
+        # untimed: full-size steps on every worker grow their buffers once, then 2 s of the same
+        # stream: the first seconds of a server still grow buffers (each move re-captures a worker's
+        # DP graph), so a queue builds up then; the timed stream measures the steady state
         warm = [srv.submit(batches[j % nb]) for j in range(4 * workers * max(1, max_step_reads // 4096))]
         for t in warm:
             srv.wait(t)
+        nw = max(20, int(rate * 2.0))
+        warm_lat, _, _ = _drive(srv, [batch(4096, j * 4096) for j in range(nw)], np.cumsum(rng.exponential(1.0 / rate, nw)))
         lat, rps, merged = _drive(srv, batches, arrivals)
         out["fixed_4k"] = dict(rate_batches_per_s=round(rate, 2), batches=nb, p50_ms=round(1e3 * _percentile(lat, 50), 3),
                                p99_ms=round(1e3 * _percentile(lat, 99), 3), reads_per_s=round(rps, 1),
-                               vs_offline=round(rps / offline, 4), merged_steps=merged)
+                               vs_offline=round(rps / offline, 4), merged_steps=merged,
+                               warmup_stream=dict(batches=nw, p99_ms=round(1e3 * _percentile(warm_lat, 99), 3)))
         log("serving fixed 4k:", json.dumps(out["fixed_4k"]))
         # the largest Poisson rate of 4096-read batches whose p99 stays under 50 ms: rates rise by 12 %
         # from half the offline rate until a point fails; each point runs >= 200 batches (>= 1 s)

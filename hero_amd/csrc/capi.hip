@@ -144,11 +144,13 @@ struct ReadsDev {
 	DevBuf<int64_t> uid;
 	DevBuf<uint8_t> good;
 	int64_t n = 0;
+	uint64_t gen = 0; // which upload the contents come from (0: none)
 	void release() {
 		words.release(), woff.release(), boff.release(), len.release(), uid.release(), good.release();
-		n = 0;
+		n = 0, gen = 0;
 	}
 };
+std::atomic<uint64_t> g_upload_gen{0};
 
 struct Timer {
 	hipEvent_t a = nullptr, b = nullptr;
@@ -162,11 +164,25 @@ struct IndexBufs {
 	uint64_t P = 0, total_keys = 0, distinct_keys = 0;
 	uint32_t max_probe = 0;
 	bool built = false;
+	// build scratch, kept between builds of the query index (grow-only): key offsets of the read
+	// set they were computed for (src_gen), occurrences, slot owners and counters
+	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
+	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
+	DevBuf<int> errf;
+	DevBuf<unsigned long long> cursor;
+	uint64_t src_gen = 0, src_nocc = 0;
+	int64_t src_n = -1;
 	DevIndex view(int k) const {
 		return DevIndex{slots.p, entries.p, P, max_probe, k, make_fastmod(P ? P : 1)};
 	}
+	void release_scratch() {
+		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
+		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
+		src_gen = 0, src_nocc = 0, src_n = -1;
+	}
 	void release() {
 		slots.release(), entries.release();
+		release_scratch();
 		P = total_keys = distinct_keys = 0, max_probe = 0, built = false;
 	}
 };
@@ -542,6 +558,7 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 	}
 	HIPCHK(c, hipStreamSynchronize(st));
 	d.n = n;
+	d.gen = ++g_upload_gen;
 	return HERO_OK;
 }
 
@@ -1377,28 +1394,38 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		bool all_positions, uint64_t ref_size, IndexBufs &ix) {
 	const int k = c->o.k;
 	const int64_t n = R.n;
-	std::vector<uint64_t> koff((size_t) n + 1, 0);
+	// The query index (not all_positions) keeps its table and build scratch between builds, and its
+	// key offsets while the read set is the same upload: a step that rebuilds it allocates nothing.
+	// The serving index (every position of the resident set) is built once and frees its scratch.
+	const bool keep = !all_positions;
+	const bool same_set = keep && R.gen != 0 && ix.src_gen == R.gen && ix.src_n == n;
+	std::vector<uint64_t> koff;
 	uint64_t nocc = 0;
-	for (int64_t i = 0; i < n; i++) {
-		koff[(size_t) i] = nocc;
-		const int L = len[(size_t) i];
-		if (good[(size_t) i]) nocc += (uint64_t) (all_positions ? std::max(0, L - k + 1) : num_keys(L, k));
+	if (same_set) {
+		nocc = ix.src_nocc;
+	} else {
+		koff.assign((size_t) n + 1, 0);
+		for (int64_t i = 0; i < n; i++) {
+			koff[(size_t) i] = nocc;
+			const int L = len[(size_t) i];
+			if (good[(size_t) i]) nocc += (uint64_t) (all_positions ? std::max(0, L - k + 1) : num_keys(L, k));
+		}
 	}
 	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu index keys exceed 2^32-1", (unsigned long long) nocc);
-	ix.release();
+	if (!keep) ix.release();
+	ix.built = false;
 	// about 8 slots per key: half the windows probed miss, and a miss walks to the next empty slot;
 	// at a load of 0.4 those walks cost more than the larger table's cache misses (measured on C2)
 	ix.P = next_prime_above(std::max<uint64_t>((all_positions ? 2 : 8) * nocc + 1, 1021));
 	if (ix.P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) ix.P);
 	hipEvent_t e0 = c->w.ev[5].a, e1 = c->w.ev[6].a;
 	HIPCHK(c, hipEventRecord(e0, c->w.st));
-	DevBuf<uint64_t> dkoff, occ_hi, occ_lo, occ_entry;
-	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
-	DevBuf<int> errf;
-	DevBuf<unsigned long long> cursor;
+	DevBuf<uint64_t> &dkoff = ix.dkoff, &occ_hi = ix.occ_hi, &occ_lo = ix.occ_lo, &occ_entry = ix.occ_entry;
+	DevBuf<uint32_t> &occ_slot = ix.occ_slot, &occ_rank = ix.occ_rank, &owner = ix.owner, &maxp = ix.maxp;
+	DevBuf<int> &errf = ix.errf;
+	DevBuf<unsigned long long> &cursor = ix.cursor;
 	auto cleanup = [&]() {
-		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
-		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
+		if (!keep) ix.release_scratch();
 	};
 	int rc = HERO_OK;
 	do {
@@ -1410,19 +1437,22 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 			break;                                                                                                  \
 		}                                                                                                           \
 	}
-		IXCHK(dkoff.exact((size_t) n + 1));
-		IXCHK(occ_hi.exact(nocc));
-		IXCHK(occ_lo.exact(nocc));
-		IXCHK(occ_entry.exact(nocc));
-		IXCHK(occ_slot.exact(nocc));
-		IXCHK(occ_rank.exact(nocc));
-		IXCHK(owner.exact(ix.P));
-		IXCHK(maxp.exact(1));
-		IXCHK(errf.exact(1));
-		IXCHK(cursor.exact(2));
-		IXCHK(ix.slots.exact(ix.P));
-		IXCHK(ix.entries.exact(nocc));
-		IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
+		IXCHK(dkoff.ensure((size_t) n + 1));
+		IXCHK(occ_hi.ensure(nocc));
+		IXCHK(occ_lo.ensure(nocc));
+		IXCHK(occ_entry.ensure(nocc));
+		IXCHK(occ_slot.ensure(nocc));
+		IXCHK(occ_rank.ensure(nocc));
+		IXCHK(owner.ensure(ix.P));
+		IXCHK(maxp.ensure(1));
+		IXCHK(errf.ensure(1));
+		IXCHK(cursor.ensure(2));
+		IXCHK(ix.slots.ensure(ix.P));
+		IXCHK(ix.entries.ensure(nocc));
+		if (!same_set) {
+			IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
+			ix.src_gen = 0; // set once the build has succeeded
+		}
 		IXCHK(hipMemsetAsync(owner.p, 0, ix.P * 4, c->w.st));
 		IXCHK(hipMemsetAsync(ix.slots.p, 0, ix.P * sizeof(Slot), c->w.st));
 		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->w.st));
@@ -1457,7 +1487,6 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 			break;
 		}
 		// distinct keys = claimed slots
-		std::vector<uint32_t> dummy;
 		ix.total_keys = nocc;
 		ix.distinct_keys = distinct;
 		float ms = 0;
@@ -1467,6 +1496,7 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 	} while (0);
 	cleanup();
 	if (rc != HERO_OK) return rc;
+	if (keep) ix.src_gen = R.gen, ix.src_n = n, ix.src_nocc = nocc;
 	ix.built = true;
 	return HERO_OK;
 }

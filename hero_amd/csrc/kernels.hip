@@ -101,24 +101,32 @@ hipError_t launch_index_gen(const Packed &Q, const uint64_t *koff, int k, bool a
 __global__ void k_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
 		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err) {
 	const uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (o >= nocc) return;
-	const uint64_t hi = occ_hi[o], lo = occ_lo[o];
-	uint64_t h = hash_kmer(k, hi, lo, make_fastmod(P));
-	for (uint64_t probe = 0; probe < P; probe++) {
-		const uint32_t cur = atomicCAS(&owner[h], 0u, (uint32_t) (o + 1));
-		bool done = cur == 0;
-		if (!done) {
-			const uint64_t other = cur - 1;
-			done = occ_hi[other] == hi && (k <= 32 || occ_lo[other] == lo);
+	uint32_t longest = 0; // probes past the home slot (most claims take none)
+	if (o < nocc) {
+		const uint64_t hi = occ_hi[o], lo = occ_lo[o];
+		uint64_t h = hash_kmer(k, hi, lo, make_fastmod(P));
+		bool placed = false;
+		for (uint64_t probe = 0; probe < P; probe++) {
+			const uint32_t cur = atomicCAS(&owner[h], 0u, (uint32_t) (o + 1));
+			bool done = cur == 0;
+			if (!done) {
+				const uint64_t other = cur - 1;
+				done = occ_hi[other] == hi && (k <= 32 || occ_lo[other] == lo);
+			}
+			if (done) {
+				occ_slot[o] = (uint32_t) h;
+				longest = (uint32_t) probe;
+				placed = true;
+				break;
+			}
+			h = (h + 1 == P) ? 0 : h + 1;
 		}
-		if (done) {
-			occ_slot[o] = (uint32_t) h;
-			atomicMax(max_probe, (uint32_t) probe);
-			return;
-		}
-		h = (h + 1 == P) ? 0 : h + 1;
+		if (!placed) *err = 1; // table full ("The size of hash table is too small.")
 	}
-	*err = 1; // table full ("The size of hash table is too small.")
+	// one atomic per wavefront, and none when no lane probed past its home slot: every thread's
+	// atomic on the one address serialized the kernel
+	longest = wave_max_u32(longest);
+	if (lane_id() == 0 && longest) atomicMax(max_probe, longest);
 }
 hipError_t launch_index_claim(const uint64_t *occ_hi, const uint64_t *occ_lo, uint64_t nocc, uint64_t P, int k,
 		uint32_t *owner, uint32_t *occ_slot, uint32_t *max_probe, int *err, hipStream_t st) {
