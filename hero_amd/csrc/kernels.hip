@@ -463,12 +463,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 			bitonic_sort(a, (int) n);
 		}
 		__syncthreads();
+		// the sorted seeds as this pass reads them: from LDS when they were sorted there (the global
+		// copy is for the later passes)
+		const uint64_t *sd = in_lds ? (const uint64_t *) lds : c.seeds + off;
 		// groups: a seed starts a group when its query differs from the previous seed's
 		uint32_t groups_before = 0;
 		for (uint32_t t0 = 0; t0 < n; t0 += 64) {
 			const uint32_t t = t0 + lane;
 			bool start = false;
-			if (t < n) start = t == 0 || seed_qid(c.seeds[off + t]) != seed_qid(c.seeds[off + t - 1]);
+			if (t < n) start = t == 0 || seed_qid(sd[t]) != seed_qid(sd[t - 1]);
 			const uint64_t m = __ballot(start);
 			const uint32_t rank = groups_before + (uint32_t) __popcll(m & lanes_below_mask());
 			groups_before += (uint32_t) __popcll(m);
@@ -478,10 +481,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 				const bool kept = o.max_candidates <= 0 || rank < (uint32_t) o.max_candidates;
 				flag = kept ? 3 : 1;
 				if (kept) {
-					const int64_t q = seed_qid(c.seeds[off + t]);
+					const int64_t q = seed_qid(sd[t]);
 					uint32_t e = t + 1;
-					while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
+					while (e < n && seed_qid(sd[e]) == q) e++;
 					Group g = make_group(c, off + t, (int) (e - t));
+					g.seeds = sd + t;
 					chain(o, g);
 					c.g_nch[off + t] = g.nch;
 					c.g_best[off + t] = g.best;
@@ -500,6 +504,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 			}
 		}
 		kept_groups += o.max_candidates > 0 ? min(groups_before, (uint32_t) o.max_candidates) : groups_before;
+		__syncthreads(); // the next strand's seeds go to the same LDS
 	}
 	if (c.sub_pairs && lane == 0) c.sub_pairs[sl] = kept_groups;
 	if (lane == 0) stat_add(stats, kStSeeds, (unsigned long long) (c.seg_n[2 * sl] + c.seg_n[2 * sl + 1]));
