@@ -191,13 +191,22 @@ def cpu_threads():
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with its family / model / stepping numbers (a
+    virtualized host may report only the brand, e.g. "AMD EPYC")."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                    info[k] = v
+            if not line.strip() and info:
+                break
     except OSError:
         pass
-    return "unknown"
+    name = info.get("model name", "unknown")
+    nums = ", ".join(f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info)
+    return f"{name} ({info.get('vendor_id', '?')}, {nums})" if nums else name
 
 
 def reference_baseline(rs, sample, threads):
@@ -496,11 +505,11 @@ def main():
     if args.workload == "c5":
         c5_run(args.steps)
         return
+    if args.copy_calibration:
+        copy_calibration()
     if args.profile:
         profile_pass(args.profile)
         return
-    if args.copy_calibration:
-        copy_calibration()
     t0 = time.perf_counter()
     rs = synth.generate(**C2_READS)
     log(f"workload: {rs.n} reads x 150 bp generated in {time.perf_counter() - t0:.1f} s")
