@@ -325,6 +325,7 @@ struct hero_ctx {
 	bool q_contiguous = true; // caller offsets are the back-to-back prefix sums of the lengths
 	int64_t q_bases = 0;
 	DevBuf<uint8_t> cons_out;
+	DevBuf<uint32_t> cons_packed; // cons_out as 2-bit codes for the copy to the host
 	bool cons_valid = false; // cons_out holds the consensus of the current tallies
 	ReadsDev Qd;
 	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
@@ -489,6 +490,46 @@ template <class T> int staged_d2h(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, 
 		}
 		const size_t o = i * kStageChunk;
 		par_memcpy(dst + o, half[i % 2], std::min(kStageChunk, n - o));
+	}
+	return HERO_OK;
+}
+// Corrected bases packed by launch_pack_bases (2-bit codes, four per byte) -> ASCII at dst, through
+// the pinned staging halves: the link moves a quarter of the bytes and the host threads unpack one
+// half while the other is on the link.
+template <class T> int staged_d2h_bases(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, char *dst, const uint32_t *packed,
+		size_t nbases) {
+	static const struct Lut {
+		uint32_t v[256];
+		Lut() {
+			const char base[4] = {'A', 'C', 'T', 'G'}; // the 2-bit codes' bases
+			for (int b = 0; b < 256; b++) {
+				uint32_t w = 0;
+				for (int i = 0; i < 4; i++) w |= (uint32_t) (uint8_t) base[(b >> (2 * i)) & 3] << (8 * i);
+				v[b] = w;
+			}
+		}
+	} lut;
+	const size_t nbytes = (nbases + 3) / 4;
+	if (nbytes == 0) return HERO_OK;
+	HIPCHK(c, stg.ensure((2 * kStageChunk + sizeof(T) - 1) / sizeof(T)));
+	const uint8_t *src = (const uint8_t *) packed;
+	uint8_t *half[2] = {(uint8_t *) stg.p, (uint8_t *) stg.p + kStageChunk};
+	const size_t chunks = (nbytes + kStageChunk - 1) / kStageChunk;
+	HIPCHK(c, hipMemcpyAsync(half[0], src, std::min(kStageChunk, nbytes), hipMemcpyDeviceToHost, st));
+	for (size_t i = 0; i < chunks; i++) {
+		HIPCHK(c, hipStreamSynchronize(st)); // chunk i is in half[i % 2]
+		if (i + 1 < chunks) {
+			const size_t o = (i + 1) * kStageChunk;
+			HIPCHK(c, hipMemcpyAsync(half[(i + 1) % 2], src + o, std::min(kStageChunk, nbytes - o), hipMemcpyDeviceToHost, st));
+		}
+		const size_t o = i * kStageChunk, m = std::min(kStageChunk, nbytes - o);
+		const uint8_t *h = half[i % 2];
+		char *d = dst + 4 * o;
+		const size_t bases = std::min(4 * m, nbases - 4 * o);
+		const int64_t full = (int64_t) (bases / 4);
+#pragma omp parallel for schedule(static) if (full > 65536)
+		for (int64_t b = 0; b < full; b++) std::memcpy(d + 4 * b, &lut.v[h[b]], 4);
+		for (size_t r = (size_t) full * 4; r < bases; r++) d[r] = (char) ((lut.v[h[r / 4]] >> (8 * (r % 4))) & 0xFF);
 	}
 	return HERO_OK;
 }
@@ -1702,7 +1743,7 @@ void hero_destroy(hero_ctx *c) {
 	(void) hipSetDevice(c->opt.device);
 	(void) hipStreamSynchronize(c->w.st);
 	c->Qd.release(), c->Sd.release(), c->q_subject_good.release();
-	c->qix.release(), c->tally.release(), c->ext.release(), c->cons_out.release();
+	c->qix.release(), c->tally.release(), c->ext.release(), c->cons_out.release(), c->cons_packed.release();
 	c->Rd.release(), c->six.release();
 	c->ba_pairs.release(), c->ba_seq.release(), c->ba_types.release(), c->ba_off.release();
 	c->ba_band.release(), c->ba_n.release(), c->ba_counts.release(), c->ba_scratch.release();
@@ -1883,8 +1924,11 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 	const int64_t n = c->Qd.n;
 	// The device holds corrected bases at packed base offsets; the caller's layout usually is that
 	// same back-to-back layout, which takes one copy. Reads that are not good keep their bytes.
+	// Corrected bases of good reads are A/C/G/T: they cross the link as 2-bit codes.
+	HIPCHK(c, c->cons_packed.ensure((size_t) (c->q_bases + 15) / 16 + 1));
+	HIPCHK(c, launch_pack_bases(c->cons_out.p, (uint64_t) c->q_bases, c->cons_packed.p, c->w.st));
 	if (c->q_contiguous) {
-		if (int r = staged_d2h(c, c->w.st, c->stage, corrected, (const char *) c->cons_out.p, (size_t) c->q_bases)) return r;
+		if (int r = staged_d2h_bases(c, c->w.st, c->stage, corrected, c->cons_packed.p, (size_t) c->q_bases)) return r;
 		for (size_t t = 0; t < c->q_not_good.size(); t++) {
 			const int64_t i = c->q_not_good[t];
 			const char *src = c->q_seq.data() + (c->q_full_copy ? c->q_off[(size_t) i] : c->q_keep_off[t]);
@@ -1892,7 +1936,7 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 		}
 	} else {
 		std::vector<char> h((size_t) c->q_bases);
-		if (int r = staged_d2h(c, c->w.st, c->stage, h.data(), (const char *) c->cons_out.p, h.size())) return r;
+		if (int r = staged_d2h_bases(c, c->w.st, c->stage, h.data(), c->cons_packed.p, h.size())) return r;
 		std::memcpy(corrected, c->q_seq.data(), c->q_seq.size());
 		uint64_t b = 0;
 		for (int64_t i = 0; i < n; i++) {

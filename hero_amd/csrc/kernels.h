@@ -250,6 +250,8 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t st);
 // Tallies: 4 counters per base (mismatch votes by base code) at 4 * t_off[q], and the matched-run
 // difference array, 1 counter per base, at t_off[q]; the read's outputs (codes_out, decoded_out)
 // at its base offset q_boff[q].
+// ASCII A/C/G/T bases -> 2-bit codes, 16 per u32 ((nb + 15) / 16 words), for the copy to the host.
+hipError_t launch_pack_bases(const uint8_t *ascii, uint64_t nb, uint32_t *packed, hipStream_t st);
 hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_boff, const uint64_t *t_off,
 		const uint32_t *tally, const uint32_t *diff, int64_t q0, int64_t nq, uint8_t *codes_out, uint32_t *decoded_out,
 		unsigned long long *stats, hipStream_t st);

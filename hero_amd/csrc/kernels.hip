@@ -1611,6 +1611,36 @@ hipError_t launch_consensus(const Packed &Q, const Opts &o, const uint64_t *q_bo
 	return hipGetLastError();
 }
 
+// Corrected bases (ASCII A/C/G/T) as 2-bit codes, 16 per u32: base i at bits 2 (i % 16) of word
+// i / 16, i.e. at bits 2 (i % 4) of byte i / 4 (the code is (c >> 1) & 3, as the read packing's).
+// The host copies a quarter of the bytes and unpacks them; bytes of reads that were not corrected
+// pack to some code and the host puts their own bytes back.
+__global__ __launch_bounds__(256) void k_pack_bases(const uint8_t *ascii, uint64_t nb, uint32_t *packed) {
+	const uint64_t t = (uint64_t) blockIdx.x * 256 + threadIdx.x;
+	const uint64_t b0 = t * 16;
+	if (b0 >= nb) return;
+	uint32_t w = 0;
+	if (b0 + 16 <= nb) {
+		const uint4 v = *(const uint4 *) (ascii + b0);
+		const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			const uint32_t y = (x[k] >> 1) & 0x03030303u; // one code in the low bits of each byte
+			const uint32_t byte = (y & 3u) | ((y >> 6) & 0xCu) | ((y >> 12) & 0x30u) | ((y >> 18) & 0xC0u);
+			w |= byte << (8 * k);
+		}
+	} else {
+		for (uint64_t i = b0; i < nb; i++) w |= (((uint32_t) ascii[i] >> 1) & 3u) << (2 * (i - b0));
+	}
+	packed[t] = w;
+}
+hipError_t launch_pack_bases(const uint8_t *ascii, uint64_t nb, uint32_t *packed, hipStream_t st) {
+	const uint64_t words = (nb + 15) / 16;
+	if (words == 0) return hipSuccess;
+	k_pack_bases<<<(unsigned) ((words + 255) / 256), 256, 0, st>>>(ascii, nb, packed);
+	return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------------------
 // Exclusive scan of u32 (block-local scan, recursive scan of block sums, add-back)
 // ---------------------------------------------------------------------------------------------

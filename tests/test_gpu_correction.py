@@ -136,6 +136,39 @@ def test_locality_order_equals_file_order(mode):
         assert a["stats"][key] == b["stats"][key], key
 
 
+def test_gapped_query_layout_gives_the_same_reads():
+    """Queries whose bytes lie with gaps between reads (caller offsets that are not back to back)
+    come back corrected at their own offsets, equal to the back-to-back layout and to the oracle;
+    the gap bytes between reads are the caller's. Lengths 61-197 put the base count off any multiple of 4."""
+    rs = synth.generate(seed=303, genome_len=12000, len_min=61, len_max=197, coverage=30.0, sub_5p=0.01)
+    opts = dict(call_consensus=1, k=25, aligner=0, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40)
+    gap = 7
+    offs = np.zeros(rs.n, dtype=np.int64)
+    parts = []
+    at = 0
+    for i in range(rs.n):
+        o, L = int(rs.offset[i]), int(rs.length[i])
+        parts.append(rs.seq[o:o + L])
+        parts.append(np.frombuffer(b"#" * gap, dtype=np.uint8))
+        offs[i] = at
+        at += L + gap
+    gapped = synth.ReadSet(np.concatenate(parts), offs, rs.length.copy(), rs.unique_id.copy())
+    outs = []
+    for r in (rs, gapped):
+        with gpu.Context(**opts) as ctx:
+            ctx.set_queries(r)
+            ctx.build_index()
+            ctx.align_self()
+            outs.append(ctx.consensus())
+    ro = O.run(O.make_cfg(**opts), rs, threads=8, want_tally=True)
+    assert np.array_equal(outs[0], ro["corrected"])
+    for i in range(rs.n):
+        a, L = int(rs.offset[i]), int(rs.length[i])
+        assert np.array_equal(outs[1][offs[i]:offs[i] + L], outs[0][a:a + L]), i
+        if i + 1 < rs.n:  # the gaps between reads; nothing past the last read is written
+            assert outs[1][offs[i] + L:offs[i] + L + gap].tobytes() == b"#" * gap, i
+
+
 def test_edge_inputs():
     # N-containing, low-complexity, shorter-than-overlap and shorter-than-k reads are filtered exactly
     # as the reference's predicates do; uncorrected reads come back unchanged.
