@@ -63,6 +63,22 @@ __device__ __forceinline__ bool ix_probe(const DevIndex &ix, uint64_t chi, uint6
 	}
 	return false;
 }
+// ix_probe with the home slot h already loaded (first): the probe continues past it only on a
+// collision.
+__device__ __forceinline__ bool ix_probe_from(const DevIndex &ix, uint64_t chi, uint64_t clo, uint64_t h,
+		const Slot &first, Slot &out) {
+	Slot s = first;
+	for (uint32_t pr = 0; pr <= ix.max_probe; pr++) {
+		if (pr) s = ix.slots[h];
+		if (s.count == 0) return false;
+		if (s.key == chi && (ix.k <= 32 || s.key_lo == clo)) {
+			out = s;
+			return true;
+		}
+		h = (h + 1 == ix.P) ? 0 : h + 1;
+	}
+	return false;
+}
 // The half of a canonical bucket holding the occurrences of the k-mer whose canonical-form test
 // gave fwd (kmer_canonical).
 __device__ __forceinline__ void ix_half(const Slot &s, bool fwd, uint32_t &start, uint32_t &count) {

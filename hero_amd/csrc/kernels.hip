@@ -313,23 +313,41 @@ __global__ __launch_bounds__(64) void k_seed_count(Packed S, Packed Q, DevIndex 
 	// Window w of the forward strand and window nwin - 1 - w of the reverse strand are the two
 	// orientations of one k-mer: one probe of its canonical form finds both windows' buckets (the
 	// reverse-strand window is the lane's too; pass A1 follows the same order).
+	// A lane takes windows lane, lane + 64, lane + 128, ... two at a time: both keys are hashed and
+	// both home slots loaded before either is used, so their loads overlap (the kernel waits on
+	// these chains of dependent loads far more than on bandwidth).
 	uint32_t cnt[2] = {0, 0};
-	for (int w = lane; w < nwin; w += 64) {
-		uint64_t hi, lo, rhi, rlo;
-		kmer_fwd(ws, w, o.k, hi, lo);
-		kmer_revcomp(o.k, hi, lo, rhi, rlo); // the reverse strand's window nwin - 1 - w
-		const bool fwd = hi < rhi || (hi == rhi && lo <= rlo);
-		const bool fwd_r = !fwd || (hi == rhi && lo == rlo); // the reverse window's test
-		Slot sl_ix;
-		const bool found = ix_probe(ix, fwd ? hi : rhi, fwd ? lo : rlo, sl_ix);
+	for (int w0 = lane; w0 < nwin; w0 += 128) {
+		const int nw = w0 + 64 < nwin ? 2 : 1;
+		uint64_t ch[2], cl[2], h[2];
+		bool fwd[2], fwd_r[2];
 #pragma unroll
-		for (int strand = 0; strand < 2; strand++) {
-			const int wi = strand ? nwin - 1 - w : w;
-			uint32_t start = 0, count = 0;
-			if (found) ix_half(sl_ix, strand ? fwd_r : fwd, start, count);
-			if (c.win_info)
-				c.win_info[(int64_t) (2 * sl + strand) * c.win_stride + wi] = count ? ((uint64_t) start << 32) | count : 0;
-			if (count) cnt[strand] += bucket_seeds(S, Q, ix, o, Ls, uid, wi, strand, start, count, [](uint64_t) {});
+		for (int u = 0; u < 2; u++) {
+			const int w = w0 + 64 * (u < nw ? u : 0);
+			uint64_t hi, lo, rhi, rlo;
+			kmer_fwd(ws, w, o.k, hi, lo);
+			kmer_revcomp(o.k, hi, lo, rhi, rlo); // the reverse strand's window nwin - 1 - w
+			fwd[u] = hi < rhi || (hi == rhi && lo <= rlo);
+			fwd_r[u] = !fwd[u] || (hi == rhi && lo == rlo); // the reverse window's test
+			ch[u] = fwd[u] ? hi : rhi, cl[u] = fwd[u] ? lo : rlo;
+			h[u] = hash_kmer(ix.k, ch[u], cl[u], ix.fm);
+		}
+		const Slot home[2] = {ix.slots[h[0]], ix.slots[h[1]]};
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			if (u >= nw) break;
+			const int w = w0 + 64 * u;
+			Slot sl_ix;
+			const bool found = ix_probe_from(ix, ch[u], cl[u], h[u], home[u], sl_ix);
+#pragma unroll
+			for (int strand = 0; strand < 2; strand++) {
+				const int wi = strand ? nwin - 1 - w : w;
+				uint32_t start = 0, count = 0;
+				if (found) ix_half(sl_ix, strand ? fwd_r[u] : fwd[u], start, count);
+				if (c.win_info)
+					c.win_info[(int64_t) (2 * sl + strand) * c.win_stride + wi] = count ? ((uint64_t) start << 32) | count : 0;
+				if (count) cnt[strand] += bucket_seeds(S, Q, ix, o, Ls, uid, wi, strand, start, count, [](uint64_t) {});
+			}
 		}
 	}
 #pragma unroll
