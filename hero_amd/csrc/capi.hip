@@ -170,6 +170,7 @@ struct IndexBufs {
 	DevBuf<uint32_t> occ_slot, occ_rank, owner, maxp;
 	DevBuf<int> errf;
 	DevBuf<unsigned long long> cursor;
+	DevBuf<uint8_t> ktmp; // key-offset scan scratch
 	uint64_t src_gen = 0, src_nocc = 0;
 	int64_t src_n = -1;
 	DevIndex view(int k) const {
@@ -178,6 +179,7 @@ struct IndexBufs {
 	void release_scratch() {
 		dkoff.release(), occ_hi.release(), occ_lo.release(), occ_entry.release();
 		occ_slot.release(), occ_rank.release(), owner.release(), maxp.release(), errf.release(), cursor.release();
+		ktmp.release();
 		src_gen = 0, src_nocc = 0, src_n = -1;
 	}
 	void release() {
@@ -327,6 +329,7 @@ struct hero_ctx {
 	DevBuf<uint8_t> cons_out;
 	DevBuf<uint32_t> cons_packed; // cons_out as 2-bit codes for the copy to the host
 	bool cons_valid = false; // cons_out holds the consensus of the current tallies
+	bool votes_clear = false; // the tallies (and the extension tally) are zero or hold this query set's votes
 	ReadsDev Qd;
 	DevBuf<uint8_t> q_subject_good; // subject predicate over the query set (self-correction)
 	// locality order (vote modes unless file_order): the query set as subjects in minimizer order
@@ -1440,20 +1443,21 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 	// The serving index (every position of the resident set) is built once and frees its scratch.
 	const bool keep = !all_positions;
 	const bool same_set = keep && R.gen != 0 && ix.src_gen == R.gen && ix.src_n == n;
-	std::vector<uint64_t> koff;
+	(void) good, (void) len; // the device copies of the flags and lengths (R.good, R.len) are used
+	if (!keep) ix.release();
 	uint64_t nocc = 0;
 	if (same_set) {
 		nocc = ix.src_nocc;
 	} else {
-		koff.assign((size_t) n + 1, 0);
-		for (int64_t i = 0; i < n; i++) {
-			koff[(size_t) i] = nocc;
-			const int L = len[(size_t) i];
-			if (good[(size_t) i]) nocc += (uint64_t) (all_positions ? std::max(0, L - k + 1) : num_keys(L, k));
-		}
+		// each read's key offset, computed on the device (the good flags and lengths are there)
+		ix.src_gen = 0; // set once the build has succeeded
+		HIPCHK(c, ix.dkoff.ensure((size_t) n + 1));
+		HIPCHK(c, ix.ktmp.ensure(key_offsets_tmp_bytes(n)));
+		HIPCHK(c, launch_key_offsets(view(R, R.good.p), k, all_positions, ix.dkoff.p, ix.ktmp.p, ix.ktmp.n, c->w.st));
+		HIPCHK(c, hipMemcpyAsync(&nocc, ix.dkoff.p + n, 8, hipMemcpyDeviceToHost, c->w.st));
+		HIPCHK(c, hipStreamSynchronize(c->w.st));
 	}
 	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu index keys exceed 2^32-1", (unsigned long long) nocc);
-	if (!keep) ix.release();
 	ix.built = false;
 	// about 8 slots per key: half the windows probed miss, and a miss walks to the next empty slot;
 	// at a load of 0.4 those walks cost more than the larger table's cache misses (measured on C2)
@@ -1490,10 +1494,6 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 		IXCHK(cursor.ensure(2));
 		IXCHK(ix.slots.ensure(ix.P));
 		IXCHK(ix.entries.ensure(nocc));
-		if (!same_set) {
-			IXCHK(hipMemcpyAsync(dkoff.p, koff.data(), ((size_t) n + 1) * 8, hipMemcpyHostToDevice, c->w.st));
-			ix.src_gen = 0; // set once the build has succeeded
-		}
 		IXCHK(hipMemsetAsync(owner.p, 0, ix.P * 4, c->w.st));
 		IXCHK(hipMemsetAsync(ix.slots.p, 0, ix.P * sizeof(Slot), c->w.st));
 		IXCHK(hipMemsetAsync(maxp.p, 0, 4, c->w.st));
@@ -1809,11 +1809,8 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	c->q_bases = qb;
 	// 4 mismatch counters per base, then the matched-run difference array (1 per base)
 	if (c->tally.n < (size_t) 5 * qb + 1 || !c->tally.p) HIPCHK(c, c->tally.exact((size_t) 5 * qb + 1)); // no slack: 20 B per base
-	HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * qb, c->w.st));
-	if (c->opt.merge) {
-		HIPCHK(c, c->ext.ensure((size_t) kExtPositions * 4 * n + 1));
-		HIPCHK(c, hipMemsetAsync(c->ext.p, 0, (size_t) kExtPositions * 16 * n, c->w.st));
-	}
+	if (c->opt.merge) HIPCHK(c, c->ext.ensure((size_t) kExtPositions * 4 * n + 1));
+	c->votes_clear = false; // cleared by hero_build_index (every run builds one), or by the first reader
 	HIPCHK(c, hipStreamSynchronize(c->w.st));
 	c->qix.built = false;
 	c->cons_valid = false;
@@ -1821,12 +1818,21 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	return HERO_OK;
 }
 
+// Zeroes the vote tallies and the --merge extension tally of the current query set.
+static int clear_votes(hero_ctx *c) {
+	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * c->q_bases, c->w.st));
+	if (c->opt.merge && c->Qd.n) HIPCHK(c, hipMemsetAsync(c->ext.p, 0, (size_t) kExtPositions * 16 * c->Qd.n, c->w.st));
+	c->votes_clear = true;
+	return HERO_OK;
+}
+// Readers of the tallies after hero_set_queries without a build since: they read zeros.
+static int clear_votes_once(hero_ctx *c) { return c->votes_clear ? HERO_OK : clear_votes(c); }
+
 int hero_build_index(hero_ctx *c) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
 	// a new index starts a new correction run: clear the vote tallies
-	if (c->q_bases) HIPCHK(c, hipMemsetAsync(c->tally.p, 0, (size_t) 20 * c->q_bases, c->w.st));
-	if (c->opt.merge && c->Qd.n) HIPCHK(c, hipMemsetAsync(c->ext.p, 0, (size_t) kExtPositions * 16 * c->Qd.n, c->w.st));
+	if (int r = clear_votes(c)) return r;
 	// --mapping sizes the table by the bases' k-mers (src/HashTableLongKey.cpp:52-60)
 	const uint64_t units = c->opt.mapping ? (uint64_t) c->q_bases / (uint64_t) c->opt.k : (uint64_t) c->Qd.n;
 	const uint64_t P = ref_prime_larger_than(units * (uint64_t) c->opt.hash_factor); // :61
@@ -1894,6 +1900,7 @@ int hero_get_extension_tally(hero_ctx *c, uint32_t *out) {
 	if (!c || !out) return HERO_ERR_ARG;
 	if (!c->opt.merge) return fail(c, HERO_ERR_STATE, "the extension tally exists with merge = 1");
 	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = clear_votes_once(c)) return r;
 	if (c->Qd.n)
 		HIPCHK(c, hipMemcpy(out, c->ext.p, (size_t) kExtPositions * 16 * c->Qd.n, hipMemcpyDeviceToHost));
 	return HERO_OK;
@@ -1902,6 +1909,7 @@ int hero_get_extension_tally(hero_ctx *c, uint32_t *out) {
 int hero_call_consensus(hero_ctx *c, char *corrected) {
 	if (!c) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = clear_votes_once(c)) return r;
 	const int64_t n = c->Qd.n;
 	HIPCHK(c, c->cons_out.ensure((size_t) c->q_bases + 1));
 	HIPCHK(c, hipEventRecord(c->w.ev[5].a, c->w.st));
@@ -1951,6 +1959,7 @@ int hero_copy_corrected(hero_ctx *c, char *corrected) {
 int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 	if (!c || !tally) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = clear_votes_once(c)) return r;
 	DevBuf<uint8_t> codes;
 	DevBuf<uint32_t> dec;
 	HIPCHK(c, codes.exact((size_t) c->q_bases));
@@ -1974,6 +1983,7 @@ int hero_get_tally(hero_ctx *c, uint32_t *tally) {
 int hero_set_tally(hero_ctx *c, const uint32_t *tally) {
 	if (!c || !tally) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = clear_votes_once(c)) return r; // the extension tally; the tallies are written below
 	if (!c->q_bases) return HERO_OK;
 	// The device form: the four counts as given, and a difference array whose running sum is each
 	// base's total count, so that consensus decodes the own base's count to exactly its given value.
@@ -2003,6 +2013,7 @@ int hero_set_tally(hero_ctx *c, const uint32_t *tally) {
 extern "C" int hero_coverage_histogram(hero_ctx *c, uint64_t *counts, int32_t cap, int32_t *max_coverage) {
 	if (!c || !counts || cap < 1 || !max_coverage) return HERO_ERR_ARG;
 	HIPCHK(c, hipSetDevice(c->opt.device));
+	if (int r = clear_votes_once(c)) return r;
 	DevBuf<unsigned long long> hist;
 	DevBuf<uint32_t> mx;
 	HIPCHK(c, hist.exact((size_t) cap));

@@ -320,6 +320,11 @@ struct OrderOut {
 	uint64_t *perm;  // may be null
 };
 size_t order_tmp_bytes(int64_t n);
+// Index key offsets of a read set on the device: koff[i] = keys of the good reads before i (the
+// reference's key positions, or every position), koff[n] = their total.
+size_t key_offsets_tmp_bytes(int64_t n);
+hipError_t launch_key_offsets(const Packed &R, int k, bool all_positions, uint64_t *koff, void *tmp, size_t tmp_bytes,
+		hipStream_t st);
 hipError_t launch_order(const Packed &R, const uint8_t *active, const uint8_t *flag, OrderOut &out, void *tmp,
 		size_t tmp_bytes, hipStream_t st);
 

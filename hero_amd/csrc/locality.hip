@@ -133,7 +133,33 @@ hipError_t scan64_rec(const TI *in, uint64_t *out, int64_t n, uint64_t *tmp, hip
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
 
+// Index keys per read: the reference's key positions (0, k, 2k, ... and L - k) or every position.
+__global__ __launch_bounds__(256) void k_key_counts(Packed R, int k, int all_positions, uint32_t *cnt) {
+	const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+	if (i >= R.n) return;
+	const int L = R.len[i];
+	cnt[i] = R.good[i] ? (uint32_t) (all_positions ? (L >= k ? L - k + 1 : 0) : num_keys(L, k)) : 0u;
+}
+
 } // namespace
+
+size_t key_offsets_tmp_bytes(int64_t n) { return align256(((size_t) n + 1) * 4) + align256(scan64_tmp_words(n + 1) * 8); }
+
+// koff[i] = the index keys of reads before i (exclusive scan), koff[n] = all of them.
+hipError_t launch_key_offsets(const Packed &R, int k, bool all_positions, uint64_t *koff, void *tmp, size_t tmp_bytes,
+		hipStream_t st) {
+	const int64_t n = R.n;
+	if (tmp_bytes < key_offsets_tmp_bytes(n)) return hipErrorInvalidValue;
+	uint32_t *cnt = (uint32_t *) tmp;
+	uint64_t *scan_tmp = (uint64_t *) ((uint8_t *) tmp + align256(((size_t) n + 1) * 4));
+	hipError_t e = hipMemsetAsync(cnt + n, 0, 4, st); // the scan's last element is the total
+	if (e != hipSuccess) return e;
+	if (n > 0) {
+		k_key_counts<<<(unsigned) ((n + 255) / 256), 256, 0, st>>>(R, k, all_positions ? 1 : 0, cnt);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	return scan64_rec<uint32_t>(cnt, koff, n + 1, scan_tmp, st);
+}
 
 size_t order_tmp_bytes(int64_t n) {
 	size_t sort_bytes = 0;
