@@ -1459,9 +1459,10 @@ int build_index(hero_ctx *c, const ReadsDev &R, const std::vector<uint8_t> &good
 	}
 	if (nocc >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "%llu index keys exceed 2^32-1", (unsigned long long) nocc);
 	ix.built = false;
-	// about 8 slots per key: half the windows probed miss, and a miss walks to the next empty slot;
-	// at a load of 0.4 those walks cost more than the larger table's cache misses (measured on C2)
-	ix.P = next_prime_above(std::max<uint64_t>((all_positions ? 2 : 8) * nocc + 1, 1021));
+	// about 4 slots per key: half the windows probed miss, and a miss walks to the next empty slot.
+	// With subjects in locality order, neighbouring probes share cache lines, and 4 and 8 slots per
+	// key time the same on the 1.7 M and 33 M read sets; 4 halves the table and its clearing.
+	ix.P = next_prime_above(std::max<uint64_t>((all_positions ? 2 : 4) * nocc + 1, 1021));
 	if (ix.P >= 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "table size %llu exceeds 2^32-1 slots", (unsigned long long) ix.P);
 	hipEvent_t e0 = c->w.ev[5].a, e1 = c->w.ev[6].a;
 	HIPCHK(c, hipEventRecord(e0, c->w.st));
