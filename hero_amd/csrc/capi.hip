@@ -681,6 +681,7 @@ int tune_class(hero_ctx *c, Worker *w, const DpLaunch &Lc, int cls, int bm, bool
 			HIPCHK(c, w->dp_scratch.ensure((size_t) waves * rows * nw * 64 * per_lane));
 			DpLaunch L2 = Lc;
 			L2.scratch = w->dp_scratch.p;
+			L2.ptr_lds_bytes = per_lane == 1 ? dp_batched_lds_bytes(bm, aff, Lc.tagged != 0, wpc_eff) : 0u;
 			float ms = 0;
 			for (int rep_i = 0; rep_i < 2; rep_i++) {
 				HIPCHK(c, hipMemsetAsync(w->counter.p, 0, 4, w->st));
@@ -813,6 +814,9 @@ int run_dp(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const Chunk
 			DpLaunch Lc = L;
 			Lc.order = w->order.p + b0, Lc.ntasks = b1 - b0;
 			Lc.scratch = w->dp_scratch.p, Lc.rows_max = rows;
+			// the LDS share of the waves per CU the launch asks for (an explicit wave count included)
+			const int wpc_launch = c->opt.dp_waves > 0 ? (c->opt.dp_waves + c->cu_count - 1) / c->cu_count : wpc;
+			Lc.ptr_lds_bytes = per_lane == 1 ? dp_batched_lds_bytes(bm, aff, L.tagged != 0, wpc_launch) : 0u;
 			if (c->tune) {
 				// autotuning: time every (tasks per lane, waves per CU) candidate on this class's tasks
 				if (int r = tune_class(c, w, Lc, cls, bm, aff, rows, nw)) return r;
@@ -859,6 +863,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 	HIPCHK(c, w->order.ensure(cap));
 	HIPCHK(c, w->scan_tmp.ensure(scan_tmp_words(kNumBuckets)));
 	int per_lane[kNumBandClasses], waves[kNumBandClasses];
+	uint32_t lds[kNumBandClasses];
 	size_t scratch = 0;
 	const bool tagged = !c->opt.dp_untagged && dp_tagged_form(c->o, rows);
 	for (int cls = 0; cls < kNumBandClasses; cls++) {
@@ -868,6 +873,8 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		const int wpc = c->tuned_waves_per_cu[cls] > 0 ? c->tuned_waves_per_cu[cls]
 		                                               : dp_batched_waves_per_cu(bm, aff, per_lane[cls], rows, tagged);
 		waves[cls] = c->opt.dp_waves > 0 ? c->opt.dp_waves : c->cu_count * wpc;
+		const int wpc_launch = c->opt.dp_waves > 0 ? (c->opt.dp_waves + c->cu_count - 1) / c->cu_count : wpc;
+		lds[cls] = per_lane[cls] == 1 ? dp_batched_lds_bytes(bm, aff, tagged, wpc_launch) : 0u;
 		scratch = std::max(scratch, (size_t) waves[cls] * rows * nw * 64 * per_lane[cls]);
 	}
 	HIPCHK(c, w->dp_scratch.ensure(scratch));
@@ -889,7 +896,10 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 				 (uintptr_t) w->bucket_hist.p, (uintptr_t) w->bucket_off.p, (uintptr_t) w->scan_tmp.p,
 				 (uintptr_t) w->scan_tmp.n, (uintptr_t) w->dtotal.p, (uintptr_t) w->ctl.p, (uintptr_t) plain_waves})
 		key.push_back(v);
-	for (int cls = 0; cls < kNumBandClasses; cls++) key.push_back((uintptr_t) (per_lane[cls] * 100000 + waves[cls]));
+	for (int cls = 0; cls < kNumBandClasses; cls++) {
+		key.push_back((uintptr_t) (per_lane[cls] * 100000 + waves[cls]));
+		key.push_back((uintptr_t) lds[cls]);
+	}
 	if (!w->graph_exec || key != w->graph_key) {
 		std::lock_guard<std::recursive_mutex> lk(g_dev_mu); // no allocation or free elsewhere meanwhile
 		if (w->graph_exec) {
@@ -909,7 +919,7 @@ int run_dp_graph(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, const
 		for (int cls = 0; cls < kNumBandClasses && e == hipSuccess; cls++) {
 			DpLaunch Lc = L;
 			Lc.range = ctl + kCtlRange + cls, Lc.batch_counter = ctl + kCtlCounter + cls;
-			Lc.scratch = w->dp_scratch.p;
+			Lc.scratch = w->dp_scratch.p, Lc.ptr_lds_bytes = lds[cls];
 			e = launch_dp_batched(Lc, kBandClass[cls], per_lane[cls], waves[cls], w->st);
 		}
 		if (e == hipSuccess) {
@@ -1625,6 +1635,10 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 	L.tasks = dt.p, L.ntasks = (uint32_t) n, L.S = R, L.Q = R, L.o = c->o;
 	L.words = dwords.p, L.nsteps = dns.p, L.sums = dsums.p, L.scratch = dscr.p, L.rows_max = rows;
 	L.tagged = form == 2 || form == 4 ? 1 : 0, L.batch_counter = c->w.counter.p, L.stats = c->w.dstats.p;
+	// the near pointer rows of the pipeline's default occupancy, so longer calls cross into scratch
+	if (per_lane == 1)
+		L.ptr_lds_bytes = dp_batched_lds_bytes(bm, c->o.aligner == 0, L.tagged != 0,
+				dp_batched_waves_per_cu(bm, c->o.aligner == 0, 1, rows, L.tagged != 0));
 	HIPCHK(c, launch_dp_batched(L, bm, per_lane, waves, st));
 	words.resize((size_t) nw + 1);
 	nsteps.resize((size_t) n);

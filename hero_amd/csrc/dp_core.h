@@ -555,6 +555,53 @@ HD void dp_forward_tagged_global(const Opts &o, int lenS, int lenQ, int b, bool 
 	}
 }
 
+// Pointer rows of one lane among LANES, split between two stores: rows i < near_rows in `near`
+// (the wavefront's LDS on the device) and the others in `far` (global scratch: word w of row i at
+// ((i * NW) + w) * LANES + lane). Both are lane-interleaved, so a wavefront's row store is one
+// coalesced write and lanes reading different rows never share an LDS bank. When a row's last word
+// holds a single cell (2 BMAX + 1 = 16 (NW - 1) + 1: band classes 8 and 16) the near store keeps
+// only the NW - 1 full words of a row and puts that cell's 2-bit code into a side word shared by
+// 16 consecutive rows, so it holds almost twice the rows. Rows are put in order from row 0 (as the
+// forward passes do): a side word restarts at every 16th row.
+template <int NW, bool SIDE, int LANES>
+struct SplitRowPtrs {
+	static_assert(!SIDE || NW > 1, "a side word takes the single cell of a row's last word");
+	static constexpr int kNear = SIDE ? NW - 1 : NW; // words per row in the near store
+	uint32_t *near, *far;
+	int lane, near_rows;
+	uint32_t side;
+	// the rows whose near-store words (and side words) fit in `words` words per lane
+	static HD int rows_for(int words) {
+		if (words <= 0) return 0;
+		int r = SIDE ? (16 * words) / (16 * kNear + 1) : words / kNear;
+		while (r > 0 && r * kNear + (SIDE ? (r + 15) / 16 : 0) > words) r--;
+		return r;
+	}
+	HD void put_row(int i, const uint32_t *pw) {
+		if (i < near_rows) {
+#pragma unroll
+			for (int w = 0; w < kNear; w++) near[(i * kNear + w) * LANES + lane] = pw[w];
+			if constexpr (SIDE) {
+				const uint32_t c = (pw[NW - 1] & 3u) << (2 * (i & 15));
+				side = (i & 15) ? (side | c) : c;
+				near[(near_rows * kNear + (i >> 4)) * LANES + lane] = side;
+			}
+		} else {
+#pragma unroll
+			for (int w = 0; w < NW; w++) far[((size_t) i * NW + w) * LANES + lane] = pw[w];
+		}
+	}
+	// word w of row i; of a side word, the row's code in the low two bits (the traceback reads the
+	// last word's single cell at bit 0)
+	HD uint32_t get(int i, int w) const {
+		if (i < near_rows) {
+			if (SIDE && w == NW - 1) return near[(near_rows * kNear + (i >> 4)) * LANES + lane] >> (2 * (i & 15));
+			return near[(i * kNear + w) * LANES + lane];
+		}
+		return far[((size_t) i * NW + w) * LANES + lane];
+	}
+};
+
 // Pointer codes of the forward passes as maps from a stored 2-bit code t to the plain code
 // p = (MAP >> 2 t) & 3 (0 up, 1 left, 2 match, 3 mismatch).
 constexpr uint32_t kPtrPlain = 0xE4u;         // dp_forward: stored as is

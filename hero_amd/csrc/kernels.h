@@ -178,6 +178,9 @@ struct DpLaunch {
 	uint32_t *scratch;
 	int rows_max;
 	int tagged;                 // one task per lane, affine: rows in the tagged form (dp_tagged_form)
+	// one task per lane: the wavefront's LDS bytes (the launch's dynamic LDS) for its first pointer
+	// rows; the rest go to scratch (0: every row in scratch)
+	uint32_t ptr_lds_bytes;
 	uint32_t *batch_counter;
 	unsigned long long *stats;
 	// graph launches: the task range [range[0], range[1]) of the order array, read on the device
@@ -224,6 +227,9 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 // occupancy of one batched instantiation (rows_max sizes the packed form's LDS; tagged: the
 // one-task-per-lane affine rows in the tagged form)
 int dp_batched_waves_per_cu(int bmax, bool affine, int per_lane, int rows_max, bool tagged);
+// LDS bytes for the near pointer rows of a one-task-per-lane launch at waves_per_cu waves per CU:
+// the CU's LDS shared out over those waves, no more than still lets them all be resident
+uint32_t dp_batched_lds_bytes(int bmax, bool affine, bool tagged, int waves_per_cu);
 constexpr int kPk16MaxBand = 12;   // widest band class of the two-tasks-per-lane form (windows of 32 cells)
 constexpr int kTaggedMaxBand = 12; // widest band class of the tagged affine rows (windows of 32 cells)
 // Whether one-task-per-lane rows may run in the tagged form: the global aligner, or the affine one

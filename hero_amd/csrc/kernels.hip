@@ -1224,21 +1224,8 @@ __device__ __forceinline__ void task_streams(const DpTask &t, const Packed &S, c
 	lenQ = sw ? t.s_len : t.q_len;
 }
 
-// Pointer rows of the lanes of one wavefront, interleaved so a row store is one coalesced
-// 256-byte write: word w of row i of lane l at ((i * NW) + w) * 64 + l.
-template <int NW>
-struct LanePtrs {
-	uint32_t *base;
-	int lane;
-	__device__ __forceinline__ void put_row(int i, const uint32_t *pw) {
-#pragma unroll
-		for (int w = 0; w < NW; w++) base[((size_t) i * NW + w) * 64 + lane] = pw[w];
-	}
-	__device__ __forceinline__ uint32_t get(int i, int w) const { return base[((size_t) i * NW + w) * 64 + lane]; }
-};
-
 // Pointer rows of one of SLOTS tasks per wavefront (two per lane in the packed form), interleaved
-// like LanePtrs: word w of row i of slot s at ((i * NW) + w) * SLOTS + s.
+// so a row store is one coalesced write: word w of row i of slot s at ((i * NW) + w) * SLOTS + s.
 template <int NW, int SLOTS>
 struct SlotPtrs {
 	uint32_t *base;
@@ -1372,8 +1359,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(dp_waves_per
 	DpLaunch L = L0;
 	if (L.range) L.order += L.range[0], L.ntasks = L.range[1] - L.range[0];
 	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	constexpr bool kSide = NW > 1 && (2 * BMAX + 1) % 16 == 1;
+	using Ptrs = SplitRowPtrs<NW, kSide, 64>;
 	const int lane = lane_id();
-	LanePtrs<NW> ptrs{L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 64, lane};
+	extern __shared__ uint32_t ptr_lds[];
+	Ptrs ptrs{ptr_lds, L.scratch + (size_t) blockIdx.x * L.rows_max * NW * 64, lane,
+			Ptrs::rows_for((int) (L.ptr_lds_bytes / 256u)), 0u};
 	uint64_t cells = 0;
 	for (;;) {
 		uint32_t batch = 0;
@@ -1447,6 +1438,35 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool
 #undef HERO_OCCT
 #undef HERO_OCC2
 	return (e == hipSuccess && blocks > 0) ? blocks : 4;
+}
+
+uint32_t dp_batched_lds_bytes(int bmax, bool aff, bool tagged, int waves_per_cu) {
+	int dev = 0, per_cu = 0, per_block = 0;
+	if (waves_per_cu < 1 || hipGetDevice(&dev) != hipSuccess ||
+			hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+			hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+		return 0;
+	const void *fn = nullptr;
+#define HERO_LDSF(BM)                                                                                           \
+	if (bmax == BM) fn = tagged ? (aff ? (const void *) k_dp_batched<BM, true, true> : (const void *) k_dp_batched<BM, false, true>) \
+	                            : (aff ? (const void *) k_dp_batched<BM, true> : (const void *) k_dp_batched<BM, false>);
+	HERO_LDSF(4) HERO_LDSF(6) HERO_LDSF(8) HERO_LDSF(10) HERO_LDSF(12)
+#undef HERO_LDSF
+	if (!tagged) {
+		if (bmax == 16) fn = aff ? (const void *) k_dp_batched<16, true> : (const void *) k_dp_batched<16, false>;
+		if (bmax == 24) fn = aff ? (const void *) k_dp_batched<24, true> : (const void *) k_dp_batched<24, false>;
+		if (bmax == 32) fn = aff ? (const void *) k_dp_batched<32, true> : (const void *) k_dp_batched<32, false>;
+	}
+	if (!fn) return 0;
+	int without = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&without, fn, 64, 0) != hipSuccess || without < 1) return 0;
+	const int want = waves_per_cu < without ? waves_per_cu : without;
+	uint32_t bytes = (uint32_t) std::min(per_cu / want, per_block) & ~511u;
+	for (; bytes > 0; bytes -= 512) { // LDS allocation granules may round a share up
+		int blocks = 0;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 64, bytes) == hipSuccess && blocks >= want) break;
+	}
+	return bytes;
 }
 
 __device__ __forceinline__ int task_bucket(const DpTask &t, int rows_max) {
@@ -1545,14 +1565,14 @@ hipError_t launch_dp_batched(const DpLaunch &L, int bmax, int per_lane, int nwav
 	const bool aff = L.o.aligner == 0;
 #define HERO_DPB(BM)                                                                               \
 	if (bmax == BM && per_lane == 1) {                                                             \
-		if (aff) k_dp_batched<BM, true><<<nwaves, 64, 0, st>>>(L);                                   \
-		else k_dp_batched<BM, false><<<nwaves, 64, 0, st>>>(L);                                      \
+		if (aff) k_dp_batched<BM, true><<<nwaves, 64, L.ptr_lds_bytes, st>>>(L);                     \
+		else k_dp_batched<BM, false><<<nwaves, 64, L.ptr_lds_bytes, st>>>(L);                        \
 		return hipGetLastError();                                                                  \
 	}
 #define HERO_DPBT(BM)                                                                              \
 	if (bmax == BM && per_lane == 1 && L.tagged) {                                                 \
-		if (aff) k_dp_batched<BM, true, true><<<nwaves, 64, 0, st>>>(L);                             \
-		else k_dp_batched<BM, false, true><<<nwaves, 64, 0, st>>>(L);                                \
+		if (aff) k_dp_batched<BM, true, true><<<nwaves, 64, L.ptr_lds_bytes, st>>>(L);               \
+		else k_dp_batched<BM, false, true><<<nwaves, 64, L.ptr_lds_bytes, st>>>(L);                  \
 		return hipGetLastError();                                                                  \
 	}
 #define HERO_DPB2(BM)                                                                              \

@@ -507,6 +507,165 @@ extern "C" int hostsim_tagged_check(const hs_cfg *cfg, int n, uint64_t seed, int
 	return bad;
 }
 
+// The split pointer store (near rows with side words, far rows beyond) against RowStore: LANES
+// lanes each run a random task's forward pass into one shared near and far area, then every lane
+// traces back; each traceback must equal the one over RowStore. near_words: near-store words per
+// lane (so near_rows = rows_for(near_words), from 0 rows up to every row). Returns the number of
+// differing tasks.
+template <int BMAX, bool AFF>
+static int split_check_one(const Opts &o, uint64_t &rng, int max_len, int near_words) {
+	constexpr int LANES = 5;
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	constexpr bool SIDE = NW > 1 && (2 * BMAX + 1) % 16 == 1;
+	using SP = SplitRowPtrs<NW, SIDE, LANES>;
+	auto rnd = [&](uint64_t n) {
+		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+		return (int) (rng % n);
+	};
+	// the kernels' forms: 1 tagged affine rows (equal gap opens), 2 tagged global rows, 0 the general rows
+	const int form = BMAX > 12 ? 0 : AFF ? (o.dx == o.dy ? 1 : 0) : 2;
+	const int near_rows = SP::rows_for(near_words);
+	std::vector<uint32_t> near((size_t) (near_words > 0 ? near_words : 1) * LANES, 0xA5A5A5A5u);
+	std::vector<uint32_t> far((size_t) (max_len + 1) * NW * LANES, 0x5A5A5A5Au);
+	Pk16Task T[LANES];
+	std::vector<int> s[LANES], q[LANES];
+	RowStore ref[LANES];
+	SP sp[LANES];
+	for (int l = 0; l < LANES; l++) {
+		Pk16Task &t = T[l];
+		t.lenS = rnd((uint64_t) max_len + 1);
+		t.lenQ = t.lenS - rnd((uint64_t) (t.lenS < BMAX ? t.lenS + 1 : BMAX + 1));
+		if (rnd(5) == 0) t.lenQ = rnd((uint64_t) t.lenS + 1);
+		t.b = t.lenS - t.lenQ + rnd(3);
+		if (rnd(2)) t.b = BMAX;
+		if (t.b > BMAX) t.b = BMAX, t.lenQ = t.lenS - BMAX > 0 ? t.lenS - BMAX : 0;
+		if (t.b < t.lenS - t.lenQ) t.lenQ = t.lenS - t.b;
+		const int f = rnd(3);
+		t.left_al = f == 1, t.right_al = f == 2;
+		s[l].resize((size_t) t.lenS), q[l].resize((size_t) t.lenQ);
+		for (auto &c : s[l]) c = rnd(4);
+		for (size_t j = 0; j < q[l].size(); j++) q[l][j] = (j < s[l].size() && rnd(10)) ? s[l][j] : rnd(4);
+		ref[l].nw = NW;
+		sp[l] = SP{near.data(), far.data(), l, near_rows, 0u};
+		CodeStream S1{&s[l], 0}, Q1{&q[l], 0}, S2{&s[l], 0}, Q2{&q[l], 0};
+		if constexpr (BMAX <= 12) {
+			if (form == 1) {
+				dp_forward_tagged<BMAX>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S1, Q1, sp[l]);
+				dp_forward_tagged<BMAX>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S2, Q2, ref[l]);
+			} else if (form == 2) {
+				dp_forward_tagged_global<BMAX>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S1, Q1, sp[l]);
+				dp_forward_tagged_global<BMAX>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S2, Q2, ref[l]);
+			}
+		}
+		if (form == 0) {
+			dp_forward<BMAX, AFF>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S1, Q1, sp[l]);
+			dp_forward<BMAX, AFF>(o, t.lenS, t.lenQ, t.b, t.left_al, t.right_al, S2, Q2, ref[l]);
+		}
+	}
+	int bad = 0;
+	for (int l = 0; l < LANES; l++) {
+		std::vector<int> x, y;
+		auto emx = [&](int st) { x.push_back(st); };
+		auto emy = [&](int st) { y.push_back(st); };
+		auto trace = [&](const auto &ps, auto &em) {
+			const Pk16Task &t = T[l];
+			return form == 1 ? dp_traceback<kPtrTaggedAffine>(t.lenS, t.lenQ, t.b, ps, em)
+			     : form == 2 ? dp_traceback<kPtrTaggedGlobal>(t.lenS, t.lenQ, t.b, ps, em)
+			                 : dp_traceback(t.lenS, t.lenQ, t.b, ps, em);
+		};
+		const int nx = trace(sp[l], emx), ny = trace(ref[l], emy);
+		if (nx != ny || x != y || nx < 0) bad++;
+	}
+	return bad;
+}
+
+extern "C" int hostsim_split_ptrs_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len, int near_words) {
+	const Opts o = to_opts(cfg);
+	uint64_t rng = seed | 1;
+	int bad = 0;
+	for (int t = 0; t < n; t++) {
+#define SP_CASE(BM) \
+	if (bmax == BM) bad += o.aligner == 0 ? split_check_one<BM, true>(o, rng, max_len, near_words) : split_check_one<BM, false>(o, rng, max_len, near_words);
+		SP_CASE(4) SP_CASE(6) SP_CASE(8) SP_CASE(10) SP_CASE(12) SP_CASE(16) SP_CASE(24) SP_CASE(32)
+#undef SP_CASE
+	}
+	return bad;
+}
+
+// The split pointer store as storage: LANES lanes put rows 0..len of random words (a last word of
+// one cell holds a 2-bit code, as the forward passes leave it), all into one near and far area,
+// then every (lane, row, word) is read back. Returns the number of words read back wrong.
+template <int BMAX>
+static int split_roundtrip_one(uint64_t &rng, int max_len, int near_words) {
+	constexpr int LANES = 7;
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	constexpr bool SIDE = NW > 1 && (2 * BMAX + 1) % 16 == 1;
+	using SP = SplitRowPtrs<NW, SIDE, LANES>;
+	auto rnd = [&](uint64_t n) {
+		rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+		return (uint32_t) (rng % n);
+	};
+	const int near_rows = SP::rows_for(near_words);
+	// the near area with a canary after it: no put may reach past near_words words per lane
+	const size_t near_n = (size_t) (near_words > 0 ? near_words : 0) * LANES, canary = 4 * LANES;
+	std::vector<uint32_t> near(near_n + canary, 0xA5A5A5A5u);
+	std::vector<uint32_t> far((size_t) (max_len + 1) * NW * LANES, 0x5A5A5A5Au);
+	std::vector<uint32_t> put[LANES];
+	int len[LANES];
+	for (int l = 0; l < LANES; l++) {
+		len[l] = (int) rnd((uint64_t) max_len + 1);
+		SP sp{near.data(), far.data(), l, near_rows, 0u};
+		for (int i = 0; i <= len[l]; i++) {
+			uint32_t pw[NW];
+			for (int w = 0; w < NW; w++) pw[w] = (SIDE && w == NW - 1) ? rnd(4) : (uint32_t) (rng = rng * 6364136223846793005ull + 1442695040888963407ull) ^ (uint32_t) (rng >> 32);
+			sp.put_row(i, pw);
+			put[l].insert(put[l].end(), pw, pw + NW);
+		}
+	}
+	int bad = 0;
+	for (size_t k = near_n; k < near.size(); k++) bad += near[k] != 0xA5A5A5A5u;
+	for (int l = 0; l < LANES; l++) {
+		const SP sp{near.data(), far.data(), l, near_rows, 0u};
+		for (int i = 0; i <= len[l]; i++)
+			for (int w = 0; w < NW; w++) {
+				const uint32_t want = put[l][(size_t) i * NW + w], got = sp.get(i, w);
+				if (SIDE && w == NW - 1 ? (got & 3u) != want : got != want) bad++;
+			}
+	}
+	return bad;
+}
+
+// rows_for(words) is the most rows whose near words fit: r rows fit, r + 1 do not
+template <int BMAX>
+static int split_rows_for_check(int max_words) {
+	constexpr int NW = (2 * BMAX + 1 + 15) / 16;
+	constexpr bool SIDE = NW > 1 && (2 * BMAX + 1) % 16 == 1;
+	using SP = SplitRowPtrs<NW, SIDE, 1>;
+	auto need = [](int r) { return r * SP::kNear + (SIDE ? (r + 15) / 16 : 0); };
+	int bad = 0;
+	for (int words = 0; words <= max_words; words++) {
+		const int r = SP::rows_for(words);
+		bad += r < 0 || need(r) > words || need(r + 1) <= words;
+	}
+	return bad;
+}
+
+extern "C" int hostsim_split_ptrs_roundtrip(int n, uint64_t seed, int bmax, int max_len, int near_words) {
+	uint64_t rng = seed | 1;
+	int bad = 0;
+#define SF_CASE(BM) \
+	if (bmax == BM) bad += split_rows_for_check<BM>(4096);
+	SF_CASE(4) SF_CASE(6) SF_CASE(8) SF_CASE(10) SF_CASE(12) SF_CASE(16) SF_CASE(24) SF_CASE(32)
+#undef SF_CASE
+	for (int t = 0; t < n; t++) {
+#define SR_CASE(BM) \
+	if (bmax == BM) bad += split_roundtrip_one<BM>(rng, max_len, near_words);
+		SR_CASE(4) SR_CASE(6) SR_CASE(8) SR_CASE(10) SR_CASE(12) SR_CASE(16) SR_CASE(24) SR_CASE(32)
+#undef SR_CASE
+	}
+	return bad;
+}
+
 extern "C" int hostsim_pk16_check(const hs_cfg *cfg, int n, uint64_t seed, int bmax, int max_len) {
 	const Opts o = to_opts(cfg);
 	uint64_t rng = seed | 1;

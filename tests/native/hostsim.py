@@ -98,3 +98,22 @@ def pk16t_check(cfg, n, seed, bmax, max_len, mode_or=0):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     return f(ctypes.addressof(cfg), n, seed, bmax, max_len, mode_or)
+
+
+def split_ptrs_check(cfg, n, seed, bmax, max_len, near_words):
+    """Tasks whose traceback over the split pointer store (near rows in near_words words per lane,
+    side words for a last word of one cell, the rest far) differs from the plain row store's
+    (0 = all equal); five lanes share the near and far areas."""
+    f = lib().hostsim_split_ptrs_check
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return f(ctypes.addressof(cfg), n, seed, bmax, max_len, near_words)
+
+
+def split_ptrs_roundtrip(n, seed, bmax, max_len, near_words):
+    """Words read back wrong from the split pointer store after seven lanes put random rows into
+    one near and far area (0 = every word as put)."""
+    f = lib().hostsim_split_ptrs_roundtrip
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return f(n, seed, bmax, max_len, near_words)

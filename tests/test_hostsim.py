@@ -116,6 +116,36 @@ def test_tagged_global_dp_equals_plain_dp(costs, bmax):
             assert H.tagged_check(cfg, 300, seed, bmax, max_len, mode_or) == 0, (costs, bmax, max_len, mode_or)
 
 
+@pytest.mark.parametrize("bmax", [4, 6, 8, 10, 12, 16, 24, 32])
+def test_split_pointer_store_reads_back_every_word(bmax):
+    """The DP kernels' split pointer store (the first rows in LDS, a side word per 16 rows for a
+    last word of one cell, the other rows in scratch): seven lanes put random rows into one area,
+    then every word reads back as put, nothing lands past a lane's near words, and rows_for gives
+    the most rows that fit (near budgets from none to more than every row)."""
+    for max_len in (0, 5, 20, 70, 140):
+        for near_words in (0, 1, 2, 3, 17, 48, 52, 300):
+            assert H.split_ptrs_roundtrip(20, 3 + bmax + max_len + near_words, bmax, max_len, near_words) == 0, (
+                max_len, near_words)
+
+
+@pytest.mark.parametrize("bmax", [4, 6, 8, 10, 12, 16, 24, 32])
+@pytest.mark.parametrize(
+    "aligner,costs",
+    [(0, dict(dx=-7, dy=-7, ex=-1, sx=0, miss=-4, hit=1)), (0, dict(dx=-3, dy=-5, ex=-1, sx=0, miss=-4, hit=1)),
+     (1, dict())],
+    ids=["tagged_affine", "affine", "global"],
+)
+def test_split_pointer_store_gives_the_same_tracebacks(aligner, costs, bmax):
+    """Forward passes of five lanes into one split store, then their tracebacks: each equals the
+    traceback over a plain row store (the kernels' forms: tagged affine, general affine, tagged
+    global rows; near budgets with the boundary inside and outside the tasks)."""
+    cfg = O.make_cfg(aligner=aligner, **costs)
+    for max_len in (3, 20, 70, 140):
+        for near_words in (0, 1, 3, 17, 48, 52, 300):
+            assert H.split_ptrs_check(cfg, 40, 99 + 7 * bmax + max_len + near_words, bmax, max_len, near_words) == 0, (
+                max_len, near_words)
+
+
 @pytest.mark.parametrize("bmax", [4, 6, 8, 10, 12])
 @pytest.mark.parametrize(
     "costs",
