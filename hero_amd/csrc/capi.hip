@@ -567,6 +567,8 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 	// [words | woff | boff | uid | len | good], the 2-bit words packed by every host thread
 	uint64_t nw = 0;
 	for (int64_t i = 0; i < n; i++) nw += (uint64_t) (len[i] + 31) / 32;
+	// DP tasks locate their segments by u32 word indices (DpTask::s_word, q_word)
+	if (nw + 2 > 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "read set of %llu packed words exceeds 2^32 - 1", (unsigned long long) nw);
 	const size_t words = (size_t) nw + 2; // two zero words of tail padding
 	const size_t n1 = (size_t) n + 1, len_words = (n1 * 4 + 7) / 8, good_words = (n1 + 7) / 8;
 	HIPCHK(c, c->stage.ensure(words + 3 * n1 + len_words + good_words));
@@ -1585,8 +1587,7 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 		const int lenS = sw ? p.q_len : p.s_len, lenQ = sw ? p.s_len : p.q_len;
 		DpTask &t = tasks[(size_t) i];
 		std::memset(&t, 0, sizeof t);
-		t.s_read = (uint32_t) (2 * i), t.q_read = (uint32_t) (2 * i + 1);
-		t.s0 = 0, t.s_len = (uint16_t) p.s_len, t.q0 = 0, t.q_len = (uint16_t) p.q_len;
+		t.s_len = (uint16_t) p.s_len, t.q_len = (uint16_t) p.q_len; // words and bits: after packing
 		t.band = (uint16_t) std::max(p.band, lenS - lenQ);
 		t.flags = (uint8_t) ((p.flag == 1 ? kTaskAlignLeft : p.flag == 2 ? kTaskAlignRight : 0) | (sw ? kTaskSwapped : 0));
 		t.out_off = (uint32_t) nw;
@@ -1606,6 +1607,12 @@ int band_align_via_batched(hero_ctx *c, int32_t n, const hero_band_pair *pairs, 
 		return fail(c, HERO_ERR_ARG, "the tagged packed rows take affine calls within their 16-bit range");
 	std::vector<uint64_t> pw, pwoff;
 	pack_reads(seq, roff.data(), rlen.data(), 2 * (int64_t) n, pw, pwoff);
+	if (pw.size() >= ((uint64_t) 1 << 32)) return fail(c, HERO_ERR_LIMIT, "the pairs' packed codes exceed 2^32 words");
+	for (int i = 0; i < n; i++) { // both sides in their forward frames from their first base
+		DpTask &t = tasks[(size_t) i];
+		segment_start(pwoff[(size_t) 2 * i], t.s_len, 0, t.s_len, false, t.s_word, t.s_bit);
+		segment_start(pwoff[(size_t) 2 * i + 1], t.q_len, 0, t.q_len, false, t.q_word, t.q_bit);
+	}
 	DevBuf<uint64_t> dw, dwoff;
 	DevBuf<int32_t> dlen;
 	DevBuf<DpTask> dt;

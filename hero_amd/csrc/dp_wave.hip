@@ -211,18 +211,12 @@ __global__ __launch_bounds__(64) void k_dp_wave_tasks(DpLaunch L0, WaveSizing z,
 		}
 		// the aligner's subject and query codes in strand frame (reverse complement: code ^ 2)
 		{
-			const uint64_t *ws = L.S.words + L.S.woff[task.s_read], *wq = L.Q.words + L.Q.woff[task.q_read];
-			const int Ls = L.S.len[task.s_read], Lq = L.Q.len[task.q_read];
 			const bool src = (task.flags & kTaskSubjRC) != 0, qrc = (task.flags & kTaskQryRC) != 0;
 			uint8_t *cs = codes + (sw ? lenS : 0), *cq = codes + (sw ? 0 : lenS);
-			for (int p = lane; p < task.s_len; p += 64) {
-				const int f = task.s0 + p;
-				cs[p] = (uint8_t) (src ? (base_at(ws, Ls - 1 - f) ^ 2) : base_at(ws, f));
-			}
-			for (int p = lane; p < task.q_len; p += 64) {
-				const int f = task.q0 + p;
-				cq[p] = (uint8_t) (qrc ? (base_at(wq, Lq - 1 - f) ^ 2) : base_at(wq, f));
-			}
+			for (int p = lane; p < task.s_len; p += 64)
+				cs[p] = (uint8_t) segment_code(L.S.words, task.s_word, task.s_bit, p, src);
+			for (int p = lane; p < task.q_len; p += 64)
+				cq[p] = (uint8_t) segment_code(L.Q.words, task.q_word, task.q_bit, p, qrc);
 		}
 		__syncthreads();
 		const uint8_t *sc = codes, *qc = codes + lenS;

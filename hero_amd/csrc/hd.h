@@ -179,10 +179,13 @@ enum : uint8_t {
 	kTaskSubjRC = 8,     // record subject side is a reverse complement (reverse strand under -c)
 	kTaskQryRC = 16,     // record query side is a reverse complement (reverse strand without -c)
 };
+// Each side's segment is located by its first code (in the side's own frame) as the packed word
+// that holds it, counted from the start of its read set's words, and its place in that word (code
+// 0 at the word's top bits). An `rc` side reads on towards lower forward positions.
 struct DpTask {
-	uint32_t s_read, q_read; // record subject / query read indices
-	uint16_t s0, s_len;      // record subject segment
-	uint16_t q0, q_len;      // record query segment
+	uint32_t s_word, q_word; // word of the record subject / query segment's first code
+	uint16_t s_bit, s_len;   // its place in the word (0..31); the segment's length
+	uint16_t q_bit, q_len;
 	uint16_t band;           // effective m_band (already widened)
 	uint8_t flags;
 	uint8_t _r;
@@ -191,6 +194,20 @@ struct DpTask {
 	uint32_t _r3;
 };
 static_assert(sizeof(DpTask) == 32, "DpTask layout");
+
+// The first code of the segment [p0, p0 + len) of a read whose words start at word woff (L bases;
+// rc: the segment of its reverse complement, whose first code is forward position L - 1 - p0):
+// its word (from the start of the set's words) and its place in that word.
+HD void segment_start(uint64_t woff, int L, int p0, int len, bool rc, uint32_t &word, uint16_t &bit) {
+	const int f = len <= 0 ? 0 : rc ? L - 1 - p0 : p0;
+	word = (uint32_t) (woff + (uint64_t) (f >> 5));
+	bit = (uint16_t) (f & 31);
+}
+// Code p of a task side (0 <= p < len) from the set's words, in the side's frame.
+HD int segment_code(const uint64_t *words, uint32_t word, int bit, int p, bool rc) {
+	const int64_t f0 = (int64_t) word * 32 + bit;
+	return rc ? (base_at(words, f0 - p) ^ 2) : base_at(words, f0 + p);
+}
 
 HD int edit_words(int lenS, int lenQ) { return (lenS + lenQ + 15) / 16; }
 

@@ -535,12 +535,13 @@ hipError_t launch_pass_a1(const Packed &S, const Packed &Q, const DevIndex &ix, 
 }
 
 // Pass A2: write the DP tasks of every kept group at its scanned task / word offsets.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pass_a2(Packed S, Packed Q, Opts o, ChunkDev c) {
 	const int sl = (int) xcd_item(blockIdx.x, gridDim.x);
 	const int64_t s = c.s_begin + sl;
 	const int lane = lane_id();
 	if (!S.good[s]) return;
 	const int Ls = S.len[s];
+	const uint32_t s_woff = (uint32_t) S.woff[s];
 	for (int strand = 0; strand < 2; strand++) {
 		const uint32_t n = c.seg_n[2 * sl + strand], off = c.seg_off[2 * sl + strand];
 		for (uint32_t t = lane; t < n; t += 64) {
@@ -550,7 +551,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7, 8))) void
 			while (e < n && seed_qid(c.seeds[off + e]) == q) e++;
 			const Group g = chained_group(c, off + t, (int) (e - t));
 			const PairReads pr = make_pair(S, Q, o, s, Ls, q, strand);
-			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t], c.prog + c.g_prog_off[off + t]);
+			plan_emit(o, g, pr, c.tasks, c.g_task_off[off + t], c.g_word_off[off + t], c.prog + c.g_prog_off[off + t], s_woff,
+					(uint32_t) Q.woff[q]);
 		}
 	}
 }
@@ -1103,7 +1105,8 @@ __global__ __launch_bounds__(64) void k_serve_a2(Packed S, Packed Q, Opts o, Chu
 	int64_t s;
 	int strand;
 	const PairReads pr = serve_pair(S, Q, o, v, base, s, strand);
-	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g], c.prog + c.g_prog_off[g]);
+	plan_emit(o, gr, pr, c.tasks, c.g_task_off[g], c.g_word_off[g], c.prog + c.g_prog_off[g], (uint32_t) S.woff[s],
+			(uint32_t) Q.woff[(int64_t) (v.hi[base] & kServeQMask)]);
 }
 __global__ __launch_bounds__(64) void k_serve_b(Packed S, Packed Q, Opts o, ChunkDev c, ServeDev v,
 		const uint64_t *q_boff, uint32_t *tally, uint32_t *diff, unsigned long long *stats) {
@@ -1184,19 +1187,14 @@ struct PackedStream {
 		x = ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1); // back to 2-bit codes
 		return x ^ 0xAAAAAAAAAAAAAAAAull;                                                 // complement: c ^ 2
 	}
-	// the segment [start, start + len) of a read of L bases at w; rc: of its reverse complement
-	__device__ static __forceinline__ PackedStream make(const uint64_t *w, int L, int start, int len, bool rc) {
+	// len codes from code `bit` of the word at q on (rc: the reverse complement, from that code
+	// towards lower positions)
+	__device__ static __forceinline__ PackedStream make(const uint64_t *q, int bit, int len, bool rc) {
 		PackedStream s;
-		s.remain = len, s.step = rc ? -1 : 1, s.p = w, s.cur = 0, s.nleft = 0;
+		s.remain = len, s.step = rc ? -1 : 1, s.p = q, s.cur = 0, s.nleft = 0;
 		if (len <= 0) return s;
-		if (!rc) {
-			const uint64_t *q = w + (start >> 5);
-			s.cur = *q << (2 * (start & 31)), s.nleft = 32 - (start & 31), s.p = q + 1;
-		} else {
-			const int f = L - 1 - start; // forward position of the first code
-			const uint64_t *q = w + (f >> 5);
-			s.cur = rc_word(*q) << (2 * (31 - (f & 31))), s.nleft = (f & 31) + 1, s.p = q - 1; // read only if needed
-		}
+		if (!rc) s.cur = *q << (2 * bit), s.nleft = 32 - bit, s.p = q + 1;
+		else s.cur = rc_word(*q) << (2 * (31 - bit)), s.nleft = bit + 1, s.p = q - 1; // read only if needed
 		return s;
 	}
 	__device__ __forceinline__ int next() {
@@ -1213,10 +1211,8 @@ struct PackedStream {
 
 __device__ __forceinline__ void task_streams(const DpTask &t, const Packed &S, const Packed &Q, PackedStream &as,
 		PackedStream &aq, int &lenS, int &lenQ) {
-	const PackedStream subj = PackedStream::make(S.words + S.woff[t.s_read], S.len[t.s_read], t.s0, t.s_len,
-			(t.flags & kTaskSubjRC) != 0);
-	const PackedStream qry = PackedStream::make(Q.words + Q.woff[t.q_read], Q.len[t.q_read], t.q0, t.q_len,
-			(t.flags & kTaskQryRC) != 0);
+	const PackedStream subj = PackedStream::make(S.words + t.s_word, t.s_bit, t.s_len, (t.flags & kTaskSubjRC) != 0);
+	const PackedStream qry = PackedStream::make(Q.words + t.q_word, t.q_bit, t.q_len, (t.flags & kTaskQryRC) != 0);
 	const bool sw = (t.flags & kTaskSwapped) != 0;
 	as = sw ? qry : subj;
 	aq = sw ? subj : qry;
@@ -1310,7 +1306,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TAG ? 2 : BM
 		const bool has_b = ib < L.ntasks;
 		uint32_t tb = 0;
 		DpTask B;
-		PackedStream sb = PackedStream::make(nullptr, 0, 0, 0, false), qb = sb; // empty unless a task
+		PackedStream sb = PackedStream::make(nullptr, 0, 0, false), qb = sb; // empty unless a task
 		int lsb = 0, lqb = 0;
 		bool okb = false;
 		if (has_b) {

@@ -58,12 +58,14 @@ HD PairReads make_pair_reads(const Packed &S, const Packed &Q, bool call_consens
 }
 
 // ---- plan ---------------------------------------------------------------------------------
-HD DpTask make_task(const PairReads &pr, const DpCall &c, uint32_t out_off) {
+// s_woff / q_woff: the subject and query reads' first words in their sets (a task locates its
+// segments by word indices in them).
+HD DpTask make_task(const PairReads &pr, const DpCall &c, uint32_t out_off, uint32_t s_woff, uint32_t q_woff) {
 	DpTask t;
-	t.s_read = (uint32_t) pr.s_read;
-	t.q_read = (uint32_t) pr.q_read;
-	t.s0 = (uint16_t) c.s0, t.s_len = (uint16_t) (c.s1 - c.s0);
-	t.q0 = (uint16_t) c.q0, t.q_len = (uint16_t) (c.q1 - c.q0);
+	t.s_len = (uint16_t) (c.s1 - c.s0);
+	t.q_len = (uint16_t) (c.q1 - c.q0);
+	segment_start(s_woff, pr.Ls, c.s0, t.s_len, pr.subj_rc, t.s_word, t.s_bit);
+	segment_start(q_woff, pr.Lq, c.q0, t.q_len, pr.qry_rc, t.q_word, t.q_bit);
 	t.band = (uint16_t) c.band;
 	t.flags = (uint8_t) ((c.flag == 1 ? kTaskAlignLeft : 0) | (c.flag == 2 ? kTaskAlignRight : 0) |
 			(c.swapped ? kTaskSwapped : 0) | (pr.subj_rc ? kTaskSubjRC : 0) | (pr.qry_rc ? kTaskQryRC : 0));
@@ -91,6 +93,7 @@ struct ScriptWriter {
 	uint32_t *out;       // script start; null: count only
 	const PairReads *pr; // with tasks: also write the DP tasks
 	DpTask *tasks;
+	uint32_t s_woff, q_woff; // with tasks: the reads' first words in their sets
 	uint32_t next_task, next_word;
 	int nops, ndp;
 	uint32_t max_mis, max_indel, pos;
@@ -103,7 +106,7 @@ struct ScriptWriter {
 	HD void dp(const DpCall &c) {
 		const uint64_t cells = band_cells(c.lenS, c.lenQ, c.band);
 		put((kScDp << 30) | (c.swapped ? (1u << 29) : 0u) | (uint32_t) (cells < (1u << 29) ? cells : (1u << 29) - 1));
-		if (tasks) tasks[next_task] = make_task(*pr, c, next_word);
+		if (tasks) tasks[next_task] = make_task(*pr, c, next_word, s_woff, q_woff);
 		next_task++;
 		next_word += (uint32_t) edit_words(c.lenS, c.lenQ);
 		ndp++;
@@ -135,23 +138,23 @@ HD void plan_count(const Opts &o, const Group &g, const PairReads &pr, int &ntas
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
 		ChainKeys<SeedAt, I32Arr> keys(sa, nx, g.cfirst[c], g.clast[c]);
-		ScriptWriter w{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0};
+		ScriptWriter w{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 		walk_record(o, pr.Ls, pr.Lq, keys, w);
 		ntasks += w.ndp, nwords += (int) w.next_word, nprog += kScriptHead + w.nops, nscripts++;
 	}
 }
 
 // Writes the tasks (at tasks[task0...], edit words from word0) and the scripts (at prog) of all
-// surviving chains, in chain order.
+// surviving chains, in chain order. s_woff / q_woff: the reads' first words in their sets.
 HD void plan_emit(const Opts &o, const Group &g, const PairReads &pr, DpTask *tasks, uint32_t task0, uint32_t word0,
-		uint32_t *prog) {
+		uint32_t *prog, uint32_t s_woff, uint32_t q_woff) {
 	SeedAt sa{g.seeds};
 	I32Arr nx{g.cnext};
 	uint32_t next_task = task0, next_word = word0, p = 1, n = 0;
 	for (int c = 0; c < g.nch; c++) {
 		if (g.clen[c] != g.best) continue;
 		ChainKeys<SeedAt, I32Arr> keys(sa, nx, g.cfirst[c], g.clast[c]);
-		ScriptWriter w{prog + p, &pr, tasks, next_task, next_word, 0, 0, 0, 0, 0};
+		ScriptWriter w{prog + p, &pr, tasks, s_woff, q_woff, next_task, next_word, 0, 0, 0, 0, 0};
 		walk_record(o, pr.Ls, pr.Lq, keys, w);
 		w.head();
 		next_task = w.next_task, next_word = w.next_word;

@@ -61,13 +61,14 @@ struct PackedSet {
 	}
 };
 
-struct Stream {
+// A DP task side's codes located as the device reads them (segment_code: word, bit, frame).
+struct SegStream {
 	static constexpr bool kTwoBit = true;
-	const Packed *P;
-	int64_t r;
-	int start, len, i;
+	const uint64_t *words;
+	uint32_t word;
+	int bit, len, i;
 	bool rc;
-	int next() { return i < len ? strand_base(*P, r, start + i++, rc) : kCodeNone; }
+	int next() { return i < len ? segment_code(words, word, bit, i++, rc) : kCodeNone; }
 };
 struct RowStore {
 	std::vector<uint32_t> v;
@@ -189,7 +190,7 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 				const uint32_t t0 = (uint32_t) tasks.size(), p0 = (uint32_t) prog.size();
 				tasks.resize(tasks.size() + nt);
 				prog.resize(prog.size() + np);
-				plan_emit(o, g, pr, tasks.data(), t0, words_total, prog.data() + p0);
+				plan_emit(o, g, pr, tasks.data(), t0, words_total, prog.data() + p0, (uint32_t) S.P.woff[s], (uint32_t) Q.P.woff[q]);
 				words_total += (uint32_t) nw;
 				D.groups.push_back(g);
 				D.gtask0.push_back(t0);
@@ -203,10 +204,10 @@ extern "C" int hostsim_run(const hs_cfg *cfg, const hs_reads *qr, const hs_reads
 		for (size_t t = 0; t < tasks.size(); t++) {
 			const DpTask &T = tasks[t];
 			const bool sw = T.flags & kTaskSwapped;
-			Stream subj{&S.P, (int64_t) T.s_read, T.s0, T.s_len, 0, (T.flags & kTaskSubjRC) != 0};
-			Stream qry{&Q.P, (int64_t) T.q_read, T.q0, T.q_len, 0, (T.flags & kTaskQryRC) != 0};
-			Stream &AS = sw ? qry : subj;
-			Stream &AQ = sw ? subj : qry;
+			SegStream subj{S.P.words, T.s_word, T.s_bit, T.s_len, 0, (T.flags & kTaskSubjRC) != 0};
+			SegStream qry{Q.P.words, T.q_word, T.q_bit, T.q_len, 0, (T.flags & kTaskQryRC) != 0};
+			SegStream &AS = sw ? qry : subj;
+			SegStream &AQ = sw ? subj : qry;
 			const int lenS = sw ? T.q_len : T.s_len, lenQ = sw ? T.s_len : T.q_len;
 			st_cells_all += band_cells(lenS, lenQ, T.band);
 			if (dispatch_dp(o, lenS, lenQ, T.band, T.flags & kTaskAlignLeft, T.flags & kTaskAlignRight, AS, AQ, steps) != 0)
