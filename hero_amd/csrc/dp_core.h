@@ -250,21 +250,22 @@ HD void dp_forward(const Opts &o, int lenS, int lenQ, int b, bool left_al, bool 
 // Tagged affine forward pass: the same recurrence, results and pointers as dp_forward<BMAX, true>
 // for gap opens of equal cost (o.dx == o.dy) and 2-bit codes, in fewer operations per cell.
 //
-// A value v of cell (i, j) is kept as 8 v + t - 4 hit (i + j): scaled by 8, a tag t in the two low
-// bits, and an offset that moves by 4 hit per up or left step and by 8 hit per diagonal step, so a
-// diagonal move adds 8 (s - hit): nothing on a match. Values of one cell share the offset, so
-// comparing them compares the values, and equal values compare by tag. The tags rank the three
+// A value v of cell (i, j) is kept as 8 v + t - 8 ex (i + j): scaled by 8, a tag t in the two low
+// bits, and an offset that moves by 8 ex per up or left step and by 16 ex per diagonal step, so a
+// gap extension adds nothing (the two gap chains are plain maxima along the row and down the band)
+// and a diagonal move adds K + 8 (s - hit), K = 8 hit - 16 ex. Values of one cell share the offset,
+// so comparing them compares the values, and equal values compare by tag. The tags rank the three
 // states as the reference breaks ties (src/BandAligner.cpp:176-222: the diagonal needs h > x and
 // h >= y, then the left gap needs y > x): 3 the vertical gap (up), 2 / 1 the diagonal (match /
 // mismatch), 0 the horizontal gap (left). The max3 of a cell's three states is therefore at once
 // the next row's diagonal input and, in its low bits, the cell's pointer, stored as it is.
 //
-// A cell's h plus a gap open, G = 8 h - off + 8 dx - 4 hit (untagged), is its up neighbour's and
-// its left neighbour's gap-open candidate at once (dx == dy; both moves add 4 hit to the offset);
-// the extended vertical gap is XE = x + 8 ex - 4 hit (tag 3 kept), the horizontal one
-// y + 8 ex - 4 hit (tag 0). Per band cell the rows carry two values to the next row:
-//   MK - the diagonal base: 8 max(h, x, y) - off + 2 (the match tag preset), so the next row's h
-//        is MK on a match and MK + 8 (miss - hit) - 1 on a mismatch;
+// A cell's h plus a gap open, G = 8 h - off + 8 (dx - ex) (untagged), is its up neighbour's and
+// its left neighbour's gap-open candidate at once (dx == dy; both moves add 8 ex to the offset);
+// the extended vertical gap is XE = x (tag 3 kept), the horizontal one y (tag 0). Per band cell
+// the rows carry two values to the next row:
+//   MK - the diagonal base: 8 max(h, x, y) - off + 2 + K (the match tag preset), so the next row's
+//        h is MK on a match and MK + 8 (miss - hit) - 1 on a mismatch;
 //   U  - the vertical gap of the cell below: max(G, XE) | 3.
 // G and y + E of the cell to the left run along the row. The aligned column lenQ's rule needs G
 // and XE of the cell above separately: each lane keeps them for its next row's column lenQ.
@@ -281,6 +282,15 @@ HD uint32_t bit_u32(uint32_t w, int p) {
 	return (w >> p) & 1u;
 #endif
 }
+// The mismatch bit of cell d as one extracted value that the cell's two uses share (left to itself,
+// the compiler extracts it once for the product and again, by a shift and an or, for the sum).
+HD int cell_bit(uint32_t lo, uint32_t hi, int d) {
+	int m = (int) bit_u32(d < 16 ? lo : hi, 2 * (d & 15));
+#if defined(__HIP_DEVICE_COMPILE__)
+	asm("" : "+v"(m));
+#endif
+	return m;
+}
 // c + m * a for m in {0, 1} and |a| < 2^23: one v_mad_i32_i24 (written out, because the compiler
 // turns a product with a one-bit value into a bit test and a select: three instructions)
 HD int mad01(int m, int a, int c) {
@@ -293,12 +303,12 @@ HD int mad01(int m, int a, int c) {
 #endif
 }
 // Every scaled value of a task with rows_max rows stays within +-kTagRange: |v| <= s (lenS + lenQ)
-// for the largest step cost s, times 8, plus the offset 4 hit (lenS + lenQ) and the tag.
+// for the largest step cost s, times 8, plus the offset 8 ex (lenS + lenQ) and the tag.
 HD bool dp_tagged_fits(const Opts &o, int rows_max) {
 	int s = o.hit < 0 ? -o.hit : o.hit;
 	const int c[5] = {o.dx, o.ex, o.sx, o.miss, o.dy};
 	for (int k = 0; k < 5; k++) s = imax_(s, c[k] < 0 ? -c[k] : c[k]);
-	return (int64_t) (12 * (int64_t) s + 4) * (int64_t) (2 * rows_max + 2) < (int64_t) kTagRange;
+	return (int64_t) (16 * (int64_t) s + 4) * (int64_t) (2 * rows_max + 2) < (int64_t) kTagRange;
 }
 
 // mode_or: row variants taken in addition to those the task needs (a lane in a wavefront with
@@ -311,9 +321,10 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 	static_assert(W <= 32, "the packed query window holds 32 cells");
 	constexpr int NW = (W + 15) / 16;
 	constexpr int kLast = W - 16 * (NW - 1); // cells in the last pointer word
-	const int step = 4 * o.hit;                 // offset of one up or left move
+	const int step = 8 * o.ex;                  // offset of one up or left move
 	const int C = 8 * o.dx - step;              // G = 8 h - off + C
-	const int E = 8 * o.ex - step;              // XE = x + E, and y + E along the row
+	const int E = 8 * o.ex - step;              // XE = x + E, and y + E along the row: 0
+	const int K = 8 * o.hit - 2 * step;         // a diagonal match
 	const int DM = 8 * (o.miss - o.hit) - 1;    // h on a mismatch: MK + DM (tag 2 -> 1)
 	const int CQx = 8 * (o.sx - o.dx), CQe = 8 * (o.sx - o.ex); // aligned column lenQ: vertical cost sx
 	int MK[W + 1], U[W + 1];
@@ -330,7 +341,7 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 			const bool in = row0_cell<true>(o, j, b, lenQ, h, x, y) && d < W;
 			const int off = step * j;
 			const int g = in ? 8 * h - off + C : kSentT, xe = in ? 8 * x + 3 - off + E : kSentT;
-			MK[d] = in ? 8 * max3_(h, x, y) - off + 2 : kSentT;
+			MK[d] = in ? 8 * max3_(h, x, y) - off + 2 + K : kSentT;
 			U[d] = in ? imax_(g, xe) | 3 : kSentT;
 			if (left_al && d == lenQ + b) Gq = g, XEq = xe; // above row 1's column lenQ
 		}
@@ -361,7 +372,7 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 		int gq = Gq, xeq = XEq;         // of this row's column lenQ, for the next row
 #pragma unroll
 		for (int d = 0; d < W; d++) {
-			const int m = (int) bit_u32(d < 16 ? mlo : mhi, 2 * (d & 15));
+			const int m = cell_bit(mlo, mhi, d);
 			int h = mad01(m, DM, MK[d]);
 			int x = U[d + 1];
 			int y = imax_(gl, yel);
@@ -379,7 +390,7 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 				mx = max3_(h, x, y);
 				pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) mx << 30);
 			}
-			MK[d] = (mx & ~3) | 2;
+			MK[d] = ((mx & ~3) | 2) + K;
 			const int xe = x + E;
 			U[d] = imax_(g, xe) | 3;
 			if (MODE & kCQ) gq = d == dQ ? g : gq, xeq = d == dQ ? xe : xeq;
@@ -407,20 +418,20 @@ HD void dp_forward_tagged(const Opts &o, int lenS, int lenQ, int b, bool left_al
 		{ // column 0 (src/BandAligner.cpp:150-163): a vertical gap from the origin, pointing up
 			const int mx = max3_(h0, x0, y0);
 			pw[D0 >> 4] = (pw[D0 >> 4] >> 2) | (3u << 30);
-			MK[D0] = (mx & ~3) | 2;
+			MK[D0] = ((mx & ~3) | 2) + K;
 			U[D0] = imax_(g0, x0 + E) | 3;
 			gl = g0, yel = y0 + E;
 		}
 #pragma unroll
 		for (int d = D0 + 1; d < W; d++) {
-			const int m = (int) bit_u32(d < 16 ? mlo : mhi, 2 * (d & 15));
+			const int m = cell_bit(mlo, mhi, d);
 			const int h = mad01(m, DM, MK[d]);
 			const int x = U[d + 1];
 			const int y = imax_(gl, yel);
 			const int g = h + m + (C - 2);
 			const int mx = max3_(h, x, y);
 			pw[d >> 4] = (pw[d >> 4] >> 2) | ((uint32_t) mx << 30);
-			MK[d] = (mx & ~3) | 2;
+			MK[d] = ((mx & ~3) | 2) + K;
 			U[d] = imax_(g, x + E) | 3;
 			gl = g, yel = y + E;
 		}
