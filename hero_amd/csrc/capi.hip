@@ -211,6 +211,7 @@ struct Worker {
 	DevBuf<VoteItem> votes;
 	DevBuf<StepSummary> sums;
 	DevBuf<unsigned long long> dstats, dtotal;
+	DevBuf<uint64_t> dtotals; // the multi-array scans' totals
 	DevBuf<uint32_t> counter;
 	// the DP stage captured as a graph (bucketing + every band class + plain form), rebuilt when a
 	// buffer it reads moves or the schedule changes; the task count reaches it through ctl[0]
@@ -273,6 +274,7 @@ struct Worker {
 			if (e == hipSuccess) e = hipEventCreate(&t.a);
 		if (e == hipSuccess) e = dstats.exact((size_t) kStatStripes * kStatStride);
 		if (e == hipSuccess) e = dtotal.exact(1);
+		if (e == hipSuccess) e = dtotals.exact(kScanMulti);
 		if (e == hipSuccess) e = counter.exact(1);
 		if (e == hipSuccess) e = hipMemset(dstats.p, 0, (size_t) kStatStripes * kStatStride * 8);
 		if (e == hipSuccess) e = ctl.exact(kCtlWords);
@@ -289,7 +291,8 @@ struct Worker {
 		tasks.release(), words.release(), nsteps.release(), dp_scratch.release(), wave_scratch.release(), wave_over.release();
 		bucket_hist.release(), bucket_off.release(), order.release(), g_nch.release(), g_best.release();
 		g_recs.release(), g_rec_off.release(), rec_opn.release(), rec_opoff.release(), rec_ops.release();
-		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), counter.release();
+		recs.release(), rec_pos.release(), sums.release(), dstats.release(), dtotal.release(), dtotals.release();
+		counter.release();
 		sub_pairs.release(), sub_pair_off.release(), votes.release();
 		bq.release(), b_cons.release(), b_raw.release(), b_koff.release(), h_meta.release(), h_bytes.release();
 		b_tally.release(), b_cnt.release(), b_off.release();
@@ -663,6 +666,18 @@ int wscan(hero_ctx *c, Worker *w, const uint32_t *in, uint32_t *out, int64_t n, 
 
 int worker_stat(hero_ctx *c, Worker *w, int slot, uint64_t &out);
 
+// Exclusive scans of K arrays of n u32 in one pass on the worker's stream; their 64-bit totals
+// come back to the host with one synchronization.
+int wscan_multi(hero_ctx *c, Worker *w, int K, const uint32_t *const *in, uint32_t *const *out, int64_t n,
+		uint64_t *totals) {
+	const size_t bytes = scan_multi_tmp_bytes(K, n);
+	HIPCHK(c, w->scan_tmp.ensure((bytes + 3) / 4));
+	HIPCHK(c, scan_u32_multi(K, in, out, n, w->dtotals.p, w->scan_tmp.p, w->scan_tmp.n * 4, w->st));
+	HIPCHK(c, hipMemcpyAsync(totals, w->dtotals.p, (size_t) K * 8, hipMemcpyDeviceToHost, w->st));
+	HIPCHK(c, hipStreamSynchronize(w->st));
+	return HERO_OK;
+}
+
 // Autotuner step for one band class: runs the class's tasks under every candidate schedule (one
 // or two tasks per lane; persistent waves per CU from 1 to the occupancy limit), each timed with
 // HIP events after an untimed run, and keeps the fastest. Every candidate computes the same
@@ -1019,13 +1034,16 @@ int run_from_seeds(hero_ctx *c, Worker *w, const Packed &S, const Packed &Q, con
 	}
 	HIPCHK(c, launch_pass_a1(S, Q, ix, c->o, cd, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[1].a, w->st));
-	uint64_t ntasks = 0, nwords = 0;
-	if (int r = wscan(c, w, cd.g_tasks, cd.g_task_off, (int64_t) ns, ntasks)) return r;
-	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ns, nwords)) return r;
+	uint64_t ntasks = 0, nwords = 0, nprog = 0;
+	{
+		const uint32_t *in[3] = {cd.g_tasks, cd.g_words, cd.g_prog};
+		uint32_t *out[3] = {cd.g_task_off, cd.g_word_off, cd.g_prog_off};
+		uint64_t tot[3] = {0, 0, 0};
+		if (int r = wscan_multi(c, w, 3, in, out, (int64_t) ns, tot)) return r;
+		ntasks = tot[0], nwords = tot[1], nprog = tot[2];
+	}
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu DP tasks; lower chunk_subjects", nsub, (unsigned long long) ntasks);
-	uint64_t nprog = 0;
-	if (int r = wscan(c, w, cd.g_prog, cd.g_prog_off, (int64_t) ns, nprog)) return r;
 	if (nprog >= (1ull << 32) - 16) return fail(c, HERO_ERR_LIMIT, "chunk of %d subjects yields %llu script words; lower chunk_subjects", nsub, (unsigned long long) nprog);
 	HIPCHK(c, w->prog.ensure((size_t) nprog + 1));
 	cd.prog = w->prog.p;
@@ -1345,13 +1363,16 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 	unsigned long long *stats = w->dstats.p;
 	HIPCHK(c, launch_serve_a1(S, Q, c->o, cd, v, stats, w->st));
 	HIPCHK(c, hipEventRecord(w->ev[1].a, w->st));
-	uint64_t ntasks = 0, nwords = 0;
-	if (int r = wscan(c, w, cd.g_tasks, cd.g_task_off, (int64_t) ngroups, ntasks)) return r;
-	if (int r = wscan(c, w, cd.g_words, cd.g_word_off, (int64_t) ngroups, nwords)) return r;
+	uint64_t ntasks = 0, nwords = 0, nprog = 0;
+	{
+		const uint32_t *in[3] = {cd.g_tasks, cd.g_words, cd.g_prog};
+		uint32_t *out[3] = {cd.g_task_off, cd.g_word_off, cd.g_prog_off};
+		uint64_t tot[3] = {0, 0, 0};
+		if (int r = wscan_multi(c, w, 3, in, out, (int64_t) ngroups, tot)) return r;
+		ntasks = tot[0], nwords = tot[1], nprog = tot[2];
+	}
 	if (ntasks >= (1ull << 31) || nwords >= (1ull << 32) - 16)
 		return fail(c, HERO_ERR_LIMIT, "batch yields %llu DP tasks", (unsigned long long) ntasks);
-	uint64_t nprog = 0;
-	if (int r = wscan(c, w, cd.g_prog, cd.g_prog_off, (int64_t) ngroups, nprog)) return r;
 	if (nprog >= (1ull << 32) - 16) return fail(c, HERO_ERR_LIMIT, "batch yields %llu script words", (unsigned long long) nprog);
 	HIPCHK(c, w->prog.ensure((size_t) nprog + 1));
 	cd.prog = w->prog.p;

@@ -329,6 +329,15 @@ size_t order_tmp_bytes(int64_t n);
 // Index key offsets of a read set on the device: koff[i] = keys of the good reads before i (the
 // reference's key positions, or every position), koff[n] = their total.
 size_t key_offsets_tmp_bytes(int64_t n);
+// exclusive scan of n u64 (locality.hip); tmp of scan_u64_tmp_bytes(n)
+size_t scan_u64_tmp_bytes(int64_t n);
+hipError_t scan_u64(const uint64_t *in, uint64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t st);
+// Exclusive scans of up to kScanMulti u32 arrays of one length at once (kernels.hip); their exact
+// 64-bit totals go to totals_dev[0 .. K). tmp of scan_multi_tmp_bytes(K, n).
+constexpr int kScanMulti = 4;
+size_t scan_multi_tmp_bytes(int K, int64_t n);
+hipError_t scan_u32_multi(int K, const uint32_t *const *in, uint32_t *const *out, int64_t n, uint64_t *totals_dev,
+		void *tmp, size_t tmp_bytes, hipStream_t st);
 hipError_t launch_key_offsets(const Packed &R, int k, bool all_positions, uint64_t *koff, void *tmp, size_t tmp_bytes,
 		hipStream_t st);
 hipError_t launch_order(const Packed &R, const uint8_t *active, const uint8_t *flag, OrderOut &out, void *tmp,

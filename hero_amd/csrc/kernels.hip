@@ -4,6 +4,8 @@
 // Every kernel that works per subject runs one 64-lane wavefront per subject read.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "dp_pk16.h"
 #include "kernels.h"
 #include "pipeline_core.h"
@@ -1723,6 +1725,94 @@ __global__ __launch_bounds__(256) void k_sum64(const uint32_t *in, int64_t n, un
 		const uint64_t t = part[0] + part[1] + part[2] + part[3];
 		if (t) atomicAdd(total, (unsigned long long) t);
 	}
+}
+
+// Exclusive scans of K arrays of one length in the same launches (blockIdx.y = array): a block
+// scans kScanBlock items of its array and writes its 64-bit sum; the block sums are scanned in 64
+// bits (scan_u64), whose last entry is the array's exact total; a second launch adds each block's
+// offset. The outputs are the scans modulo 2^32 (callers refuse totals of 2^32 or more).
+struct ScanMulti {
+	const uint32_t *in[kScanMulti];
+	uint32_t *out[kScanMulti];
+	uint64_t *bsum[kScanMulti], *boff[kScanMulti];
+};
+__global__ __launch_bounds__(256) void k_scanm_block(ScanMulti m, int64_t n) {
+	const int k = (int) blockIdx.y;
+	const uint32_t *in = m.in[k];
+	uint32_t *out = m.out[k];
+	__shared__ uint64_t sh[kScanThreads];
+	const int64_t base = (int64_t) blockIdx.x * kScanBlock + (int64_t) threadIdx.x * kScanItems;
+	uint32_t v[kScanItems];
+	uint64_t sum = 0;
+#pragma unroll
+	for (int i = 0; i < kScanItems; i++) {
+		v[i] = base + i < n ? in[base + i] : 0u;
+		sum += v[i];
+	}
+	sh[threadIdx.x] = sum;
+	__syncthreads();
+	for (int off = 1; off < kScanThreads; off <<= 1) {
+		const uint64_t t = threadIdx.x >= (unsigned) off ? sh[threadIdx.x - off] : 0ull;
+		__syncthreads();
+		sh[threadIdx.x] += t;
+		__syncthreads();
+	}
+	uint64_t run = sh[threadIdx.x] - sum;
+#pragma unroll
+	for (int i = 0; i < kScanItems; i++) {
+		if (base + i < n) out[base + i] = (uint32_t) run;
+		run += v[i];
+	}
+	if (threadIdx.x == kScanThreads - 1) m.bsum[k][blockIdx.x] = sh[threadIdx.x];
+}
+__global__ __launch_bounds__(256) void k_scanm_add(ScanMulti m, int64_t n) {
+	const int k = (int) blockIdx.y;
+	const int64_t i = (int64_t) blockIdx.x * kScanThreads + threadIdx.x;
+	if (i < n) m.out[k][i] += (uint32_t) m.boff[k][i / kScanBlock];
+}
+// totals[k] = boff[k][nb]; bsum[k][nb] = 0 first (the scan's extra entry that becomes the total)
+__global__ void k_scanm_edge(ScanMulti m, int K, int64_t nb, uint64_t *totals, int zero) {
+	const int k = (int) threadIdx.x;
+	if (k >= K) return;
+	if (zero) m.bsum[k][nb] = 0ull;
+	else totals[k] = m.boff[k][nb];
+}
+
+size_t scan_multi_tmp_bytes(int K, int64_t n) {
+	const size_t nb1 = (size_t) ((n + kScanBlock - 1) / kScanBlock) + 1;
+	return (size_t) K * 2 * ((nb1 * 8 + 255) & ~(size_t) 255) + scan_u64_tmp_bytes((int64_t) nb1);
+}
+
+hipError_t scan_u32_multi(int K, const uint32_t *const *in, uint32_t *const *out, int64_t n, uint64_t *totals_dev,
+		void *tmp, size_t tmp_bytes, hipStream_t st) {
+	if (K < 1 || K > kScanMulti || n < 0 || tmp_bytes < scan_multi_tmp_bytes(K, n)) return hipErrorInvalidValue;
+	const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+	const size_t part = (((size_t) nb + 1) * 8 + 255) & ~(size_t) 255;
+	ScanMulti m;
+	std::memset(&m, 0, sizeof m);
+	uint8_t *t = (uint8_t *) tmp;
+	for (int k = 0; k < K; k++) {
+		m.in[k] = in[k], m.out[k] = out[k];
+		m.bsum[k] = (uint64_t *) (t + (size_t) (2 * k) * part), m.boff[k] = (uint64_t *) (t + (size_t) (2 * k + 1) * part);
+	}
+	void *stmp = t + (size_t) (2 * K) * part;
+	const size_t stmp_bytes = tmp_bytes - (size_t) (2 * K) * part;
+	k_scanm_edge<<<1, 64, 0, st>>>(m, K, nb, totals_dev, 1);
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess && nb > 0) {
+		k_scanm_block<<<dim3((unsigned) nb, (unsigned) K), kScanThreads, 0, st>>>(m, n);
+		e = hipGetLastError();
+	}
+	for (int k = 0; k < K && e == hipSuccess; k++) e = scan_u64(m.bsum[k], m.boff[k], nb + 1, stmp, stmp_bytes, st);
+	if (e == hipSuccess && nb > 1) {
+		k_scanm_add<<<dim3((unsigned) ((n + kScanThreads - 1) / kScanThreads), (unsigned) K), kScanThreads, 0, st>>>(m, n);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) {
+		k_scanm_edge<<<1, 64, 0, st>>>(m, K, nb, totals_dev, 0);
+		e = hipGetLastError();
+	}
+	return e;
 }
 
 size_t scan_tmp_words(int64_t n) {

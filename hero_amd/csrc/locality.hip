@@ -143,6 +143,12 @@ __global__ __launch_bounds__(256) void k_key_counts(Packed R, int k, int all_pos
 
 } // namespace
 
+size_t scan_u64_tmp_bytes(int64_t n) { return scan64_tmp_words(n) * 8; }
+hipError_t scan_u64(const uint64_t *in, uint64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t st) {
+	if (tmp_bytes < scan_u64_tmp_bytes(n)) return hipErrorInvalidValue;
+	return scan64_rec<uint64_t>(in, out, n, (uint64_t *) tmp, st);
+}
+
 size_t key_offsets_tmp_bytes(int64_t n) { return align256(((size_t) n + 1) * 4) + align256(scan64_tmp_words(n + 1) * 8); }
 
 // koff[i] = the index keys of reads before i (exclusive scan), koff[n] = all of them.
