@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "dp_pk16.h"
 #include "kernels.h"
@@ -1410,7 +1412,7 @@ bool dp_tagged_form(const Opts &o, int rows_max) {
 	return (o.aligner == 1 || o.dx == o.dy) && dp_tagged_fits(o, rows_max);
 }
 
-int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool tagged) {
+static int dp_batched_waves_per_cu_query(int bmax, bool aff, int per_lane, int rows_max, bool tagged) {
 	int blocks = 0;
 	hipError_t e = hipErrorInvalidValue;
 #define HERO_OCC(BM)                                                                                            \
@@ -1437,8 +1439,28 @@ int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool
 #undef HERO_OCC2
 	return (e == hipSuccess && blocks > 0) ? blocks : 4;
 }
+// Kept per (device, class, form, tasks per lane, rows): the DP stage of every chunk and serving step
+// asks it for every class.
+int dp_batched_waves_per_cu(int bmax, bool aff, int per_lane, int rows_max, bool tagged) {
+	static std::mutex mu;
+	static std::map<std::pair<uint64_t, int>, int> known;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return 4;
+	const std::pair<uint64_t, int> key{((uint64_t) (uint32_t) dev << 32) | ((uint64_t) (bmax & 0xFFFF) << 16) |
+			((uint64_t) (aff ? 1 : 0) << 15) | ((uint64_t) (tagged ? 1 : 0) << 14) | (uint64_t) (per_lane & 0xFF),
+			per_lane == 2 ? rows_max : 0};
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		const auto it = known.find(key);
+		if (it != known.end()) return it->second;
+	}
+	const int w = dp_batched_waves_per_cu_query(bmax, aff, per_lane, rows_max, tagged);
+	std::lock_guard<std::mutex> lk(mu);
+	known[key] = w;
+	return w;
+}
 
-uint32_t dp_batched_lds_bytes(int bmax, bool aff, bool tagged, int waves_per_cu) {
+static uint32_t dp_batched_lds_bytes_query(int bmax, bool aff, bool tagged, int waves_per_cu) {
 	int dev = 0, per_cu = 0, per_block = 0;
 	if (waves_per_cu < 1 || hipGetDevice(&dev) != hipSuccess ||
 			hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
@@ -1464,6 +1486,26 @@ uint32_t dp_batched_lds_bytes(int bmax, bool aff, bool tagged, int waves_per_cu)
 		int blocks = 0;
 		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 64, bytes) == hipSuccess && blocks >= want) break;
 	}
+	return bytes;
+}
+
+// The answer depends only on the kernel, the device and the waves per CU: every DP stage asks it
+// again (each chunk, each serving step), so it is kept per (device, class, form, waves per CU).
+uint32_t dp_batched_lds_bytes(int bmax, bool aff, bool tagged, int waves_per_cu) {
+	static std::mutex mu;
+	static std::map<uint64_t, uint32_t> known;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return 0;
+	const uint64_t key = ((uint64_t) (uint32_t) dev << 32) | ((uint64_t) (bmax & 0xFFFF) << 16) |
+			((uint64_t) (aff ? 1 : 0) << 15) | ((uint64_t) (tagged ? 1 : 0) << 14) | (uint64_t) (waves_per_cu & 0x3FFF);
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		const auto it = known.find(key);
+		if (it != known.end()) return it->second;
+	}
+	const uint32_t bytes = dp_batched_lds_bytes_query(bmax, aff, tagged, waves_per_cu);
+	std::lock_guard<std::mutex> lk(mu);
+	known[key] = bytes;
 	return bytes;
 }
 
