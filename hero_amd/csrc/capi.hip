@@ -421,7 +421,25 @@ uint8_t *noalloc_scratch_of(int device) {
 
 // Largest per-chunk window-bucket buffer (8 B per subject window and strand); a chunk beyond it
 // probes the index a second time in pass A1 instead.
-constexpr uint64_t kWinInfoBytes = 1ull << 30;
+constexpr uint64_t kWinInfoBytes = 4ull << 30;
+// seeds of one chunk before it is split in two (the seed stage indexes them in 31 bits)
+constexpr uint64_t kChunkSeedsMax = 1ull << 30;
+// Subjects per chunk when the caller leaves it automatic, budgeted at 32 KB per subject within 40 %
+// of the free device memory (the chunk buffers of the 33 M-read repeat set take about 28 KB per
+// subject): the whole set as one chunk when it fits the budget and kChunkMax; otherwise balanced
+// chunks sized for two chunk workers at once, never fewer than kChunkMin subjects. Larger chunks
+// cut the per-chunk launches, scans and kernel tails (1.67 M C2 subjects: 98.3 ms per step in
+// chunks of 262,144, 93.9 ms as one chunk).
+constexpr int64_t kChunkMax = 1 << 21, kChunkMin = 1 << 18;
+int64_t auto_chunk_subjects(int64_t n) {
+	size_t free_b = 0, total_b = 0;
+	double budget = 0;
+	if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = (double) free_b * 0.4 / 32768.0;
+	if (n <= kChunkMax && (double) n <= budget) return std::max<int64_t>(n, 1);
+	const int64_t cap = std::max<int64_t>(std::min<int64_t>(kChunkMax, (int64_t) (budget / 2)), kChunkMin);
+	const int64_t nch = (n + cap - 1) / cap;
+	return (n + nch - 1) / nch;
+}
 
 int fail(hero_ctx *c, int code, const char *fmt, ...) {
 	char buf[512];
@@ -987,6 +1005,13 @@ int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub,
 	HIPCHK(c, hipEventRecord(w->ev[8].a, w->st));
 	uint64_t nseeds = 0;
 	if (int r = wscan(c, w, cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
+	// a chunk whose seeds pass the seed stage's budget runs as two halves, in subject order
+	// (repeat-heavy reads carry thousands of seeds each)
+	if (nseeds > kChunkSeedsMax && nsub > 1) {
+		const int32_t h = nsub / 2;
+		if (int r = run_chunk(c, w, S, s0, h, rows_needed, s_uid)) return r;
+		return run_chunk(c, w, S, s0 + h, nsub - h, rows_needed, s_uid);
+	}
 	return run_from_seeds(c, w, S, Q, ix, cd, nseeds, rows_needed, c->tally_off(), c->tally.p, c->tally.p + 4 * c->q_bases,
 			s_uid, rec_mode);
 }
@@ -1417,7 +1442,7 @@ int correct_batch_on(hero_ctx *c, Worker *w, int64_t n, const char *seq, const i
 int align_packed(hero_ctx *c, const Packed &S, int64_t n, int max_len, const int64_t *s_uid) {
 	if (!c->qix.built) return fail(c, HERO_ERR_STATE, "hero_build_index must run before aligning subjects");
 	c->cons_valid = false;
-	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : 262144;
+	const int64_t chunk = c->opt.chunk_subjects > 0 ? c->opt.chunk_subjects : auto_chunk_subjects(n);
 	const int64_t nchunks = (n + chunk - 1) / chunk;
 	// Chunks in flight at once: votes are integer atomic sums into the shared tallies, so their
 	// order does not matter; records are written in subject order and autotuning times one chunk
