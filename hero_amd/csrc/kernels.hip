@@ -1778,17 +1778,28 @@ struct ScanMulti {
 	uint32_t *out[kScanMulti];
 	uint64_t *bsum[kScanMulti], *boff[kScanMulti];
 };
+// The block's items pass through LDS (a padding word per 16 keeps a thread's 16 consecutive items in
+// distinct banks), so the loads and stores are coalesced rows of the array.
+__device__ __forceinline__ int scanm_pad(int i) { return i + (i >> 4); }
 __global__ __launch_bounds__(256) void k_scanm_block(ScanMulti m, int64_t n) {
 	const int k = (int) blockIdx.y;
 	const uint32_t *in = m.in[k];
 	uint32_t *out = m.out[k];
 	__shared__ uint64_t sh[kScanThreads];
-	const int64_t base = (int64_t) blockIdx.x * kScanBlock + (int64_t) threadIdx.x * kScanItems;
+	__shared__ uint32_t tile[kScanBlock + kScanBlock / 16];
+	const int64_t b0 = (int64_t) blockIdx.x * kScanBlock;
+#pragma unroll
+	for (int j = 0; j < kScanItems; j++) {
+		const int idx = j * kScanThreads + (int) threadIdx.x;
+		tile[scanm_pad(idx)] = b0 + idx < n ? in[b0 + idx] : 0u;
+	}
+	__syncthreads();
+	const int t0 = (int) threadIdx.x * kScanItems;
 	uint32_t v[kScanItems];
 	uint64_t sum = 0;
 #pragma unroll
 	for (int i = 0; i < kScanItems; i++) {
-		v[i] = base + i < n ? in[base + i] : 0u;
+		v[i] = tile[scanm_pad(t0 + i)];
 		sum += v[i];
 	}
 	sh[threadIdx.x] = sum;
@@ -1802,10 +1813,16 @@ __global__ __launch_bounds__(256) void k_scanm_block(ScanMulti m, int64_t n) {
 	uint64_t run = sh[threadIdx.x] - sum;
 #pragma unroll
 	for (int i = 0; i < kScanItems; i++) {
-		if (base + i < n) out[base + i] = (uint32_t) run;
+		tile[scanm_pad(t0 + i)] = (uint32_t) run; // a thread rewrites only its own items
 		run += v[i];
 	}
 	if (threadIdx.x == kScanThreads - 1) m.bsum[k][blockIdx.x] = sh[threadIdx.x];
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < kScanItems; j++) {
+		const int idx = j * kScanThreads + (int) threadIdx.x;
+		if (b0 + idx < n) out[b0 + idx] = tile[scanm_pad(idx)];
+	}
 }
 __global__ __launch_bounds__(256) void k_scanm_add(ScanMulti m, int64_t n) {
 	const int k = (int) blockIdx.y;
