@@ -372,6 +372,65 @@ struct VoteSink {
 	}
 };
 
+// One DP call's steps cast straight into a record's votes: the steps of feed_runs, without merging
+// them into runs first. A mismatch votes per base and a deletion only moves the subject position,
+// so runs of those need no merging; an insertion run stays one hole (its first base and the base
+// after it), as the merged op makes it.
+template <class V>
+HD void vote_steps(const DpResults &R, uint32_t task, bool swapped, V &v) {
+	const DpTask &t = R.tasks[task];
+	const int cap = t.s_len + t.q_len;
+	const int n = (int) R.nsteps[task] <= cap ? (int) R.nsteps[task] : 0;
+	const uint32_t *w = R.words + t.out_off;
+	int e = cap - n;
+	int ins = 0; // bases of the open insertion run, ending at v.pq
+	auto close_ins = [&]() {
+		if (ins) v.sink.hole(v.pq - ins, ins), ins = 0;
+	};
+	auto steps_of = [&](uint32_t raw) { // the steps e .. up to the end of e's word
+		const int stop = (e | 15) + 1 < cap ? (e | 15) + 1 : cap;
+		const int m = stop - e;
+		uint32_t word = raw >> (2 * (e & 15));
+		if (m < 16) word &= (1u << (2 * m)) - 1u;
+		int done = 0;
+		while (word) {
+			const int j = __builtin_ctz(word) >> 1; // matches before the next non-match step
+			if (j) {
+				close_ins();
+				v.pq += j, v.ps += j;
+			}
+			word >>= 2 * j;
+			const int ty = step_type(word & 3u, swapped);
+			if (ty == kOpI) {
+				ins++, v.pq++;
+			} else {
+				close_ins();
+				if (ty == kOpX) v.sink.mismatch(v.pq, v.ps), v.pq++;
+				v.ps++;
+			}
+			word >>= 2;
+			done += j + 1;
+		}
+		if (m > done) {
+			close_ins();
+			v.pq += m - done, v.ps += m - done;
+		}
+		e = stop;
+	};
+	const int w0 = e >> 4, w1 = (cap - 1) >> 4; // words holding steps e .. cap-1
+	if (w1 - w0 < kFeedWords) {
+		uint32_t pre[kFeedWords];
+#pragma unroll
+		for (int k = 0; k < kFeedWords; k++) pre[k] = w0 + k <= w1 ? w[w0 + k] : 0u;
+#pragma unroll
+		for (int k = 0; k < kFeedWords; k++)
+			if (e < cap) steps_of(pre[k]);
+	} else {
+		while (e < cap) steps_of(w[e >> 4]);
+	}
+	close_ins();
+}
+
 struct VoteV {
 	const DpResults &R;
 	const VoteSink &sink;
@@ -391,7 +450,7 @@ struct VoteV {
 	}
 	HD void start(int pos) { ps = pos, pq = 0; }
 	HD void add(int c, int t) { op(c, t); }
-	HD void run(uint32_t task, bool swapped) { feed_runs(R, task, swapped, [&](int cnt, int t) { op(cnt, t); }); }
+	HD void run(uint32_t task, bool swapped) { vote_steps(R, task, swapped, *this); }
 	HD void finish() { sink.span(pq); }
 };
 
