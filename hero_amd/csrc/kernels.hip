@@ -638,7 +638,14 @@ __global__ __launch_bounds__(64) void k_votes(Packed S, Packed Q, Opts o, ChunkD
 	const VoteItem v = c.votes[i];
 	if (v.q == kNoRecord) return;
 	const int64_t s = c.s_begin + v.sub;
-	const PairReads pr = make_pair(S, Q, o, s, S.len[s], (int64_t) v.q, v.strand);
+	// the pair's reads as make_pair gives them, except that the query's bases are located only for
+	// --merge (the plain votes read the subject's bases and the query's length)
+	PairReads pr;
+	pr.s_read = s, pr.q_read = (int64_t) v.q, pr.Ls = S.len[s], pr.Lq = Q.len[v.q];
+	pr.subj_rc = o.call_consensus && v.strand == 1;
+	pr.qry_rc = !o.call_consensus && v.strand == 1;
+	pr.sw = S.words + S.woff[s];
+	pr.qw = ext ? Q.words + Q.woff[v.q] : nullptr;
 	const DpResults R{c.tasks, c.words, c.nsteps, c.sums};
 	const uint64_t b = q_boff[v.q];
 	VoteSink sink{tally + 4 * b, diff + b, pr, &stats[kStOob]};
