@@ -422,7 +422,8 @@ uint8_t *noalloc_scratch_of(int device) {
 // Largest per-chunk window-bucket buffer (8 B per subject window and strand); a chunk beyond it
 // probes the index a second time in pass A1 instead.
 constexpr uint64_t kWinInfoBytes = 4ull << 30;
-// seeds of one chunk before it is split in two (the seed stage indexes them in 31 bits)
+// seeds of one chunk before it is split in two (the seed stage indexes them in 31 bits); the
+// chunk_seeds_max option lowers it
 constexpr uint64_t kChunkSeedsMax = 1ull << 30;
 // Subjects per chunk when the caller leaves it automatic, budgeted at 32 KB per subject within 40 %
 // of the free device memory (the chunk buffers of the 33 M-read repeat set take about 28 KB per
@@ -1007,7 +1008,9 @@ int run_chunk(hero_ctx *c, Worker *w, const Packed &S, int64_t s0, int32_t nsub,
 	if (int r = wscan(c, w, cd.seg_n, cd.seg_off, 2 * (int64_t) nsub, nseeds)) return r;
 	// a chunk whose seeds pass the seed stage's budget runs as two halves, in subject order
 	// (repeat-heavy reads carry thousands of seeds each)
-	if (nseeds > kChunkSeedsMax && nsub > 1) {
+	const uint64_t seeds_max = c->opt.chunk_seeds_max > 0 ? std::min<uint64_t>((uint64_t) c->opt.chunk_seeds_max, kChunkSeedsMax)
+	                                                      : kChunkSeedsMax;
+	if (nseeds > seeds_max && nsub > 1) {
 		const int32_t h = nsub / 2;
 		if (int r = run_chunk(c, w, S, s0, h, rows_needed, s_uid)) return r;
 		return run_chunk(c, w, S, s0 + h, nsub - h, rows_needed, s_uid);
@@ -1775,6 +1778,7 @@ int hero_create(const hero_options *opt, hero_ctx **out) {
 	if (opt->aligner != 0 && opt->aligner != 1) return HERO_ERR_ARG;
 	if (opt->records && opt->call_consensus) return HERO_ERR_ARG; // the reference writes no .aln under -c
 	if (opt->min_overlap < 0 || opt->min_overlap > 65535) return HERO_ERR_ARG;
+	if (opt->chunk_seeds_max < 0) return HERO_ERR_ARG;
 	if (opt->hash_factor < 1) return HERO_ERR_ARG;
 	if (opt->long_format && opt->call_consensus) return HERO_ERR_ARG; // -v is built for record mode (.aln)
 	if (opt->merge && (!opt->call_consensus || opt->mapping)) return HERO_ERR_ARG; // --merge acts on -c votes

@@ -27,7 +27,7 @@ class Options(ctypes.Structure):
         ("dp_waves", ctypes.c_int32), ("records", ctypes.c_int32), ("dp_per_lane", ctypes.c_int32),
         ("dp_eager", ctypes.c_int32), ("chunk_workers", ctypes.c_int32), ("dp_untagged", ctypes.c_int32),
         ("mapping", ctypes.c_int32), ("band_align_form", ctypes.c_int32), ("long_format", ctypes.c_int32),
-        ("merge", ctypes.c_int32), ("file_order", ctypes.c_int32),
+        ("merge", ctypes.c_int32), ("file_order", ctypes.c_int32), ("chunk_seeds_max", ctypes.c_int64),
     ]
 
 

@@ -75,6 +75,8 @@ typedef struct {
 	int32_t file_order;       /* 1: subjects run in file order and the tallies lie in read order; 0 (vote modes):
 	                             locality order, subjects run and tallies lie in the order of each read's
 	                             minimizer, so overlapping reads share cache lines; results are identical */
+	int64_t chunk_seeds_max;  /* seeds of one subject chunk before it runs as two halves (each counted again,
+	                             in subject order); 0 = 2^30, half of what the seed stage indexes */
 } hero_options;
 
 /* Counters and device timings (milliseconds, HIP events on the context's stream). */

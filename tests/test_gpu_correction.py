@@ -57,9 +57,9 @@ def test_plain_form_equals_batched_and_chunking_is_invisible():
     base = dict(call_consensus=1, k=31, aligner=0, mismatch_rate=0.04, indel_rate=0.03, min_overlap=40)
     outs, stats = [], []
     # one chunk; plain form; many chunks on two concurrent workers (the default); the same chunks
-    # one after the other
+    # one after the other; chunks split in halves, again and again, past 2,000 seeds
     for extra in (dict(dp_form=0), dict(dp_form=1), dict(dp_form=0, chunk_subjects=333),
-                  dict(dp_form=0, chunk_subjects=333, chunk_workers=1)):
+                  dict(dp_form=0, chunk_subjects=333, chunk_workers=1), dict(dp_form=0, chunk_seeds_max=2000)):
         with gpu.Context(**base, **extra) as ctx:
             ctx.set_queries(rs)
             ctx.build_index()
@@ -69,7 +69,8 @@ def test_plain_form_equals_batched_and_chunking_is_invisible():
     for t, c in outs[1:]:
         assert np.array_equal(t, outs[0][0]) and np.array_equal(c, outs[0][1])
     for key in ("seeds", "dp_tasks", "dp_calls_exec", "dp_cells_exec", "records_kept", "subjects"):
-        assert stats[2][key] == stats[3][key] == stats[0][key], key
+        assert stats[2][key] == stats[3][key] == stats[4][key] == stats[0][key], key
+    assert stats[4]["dp_graph_launches"] > 2 * stats[2]["dp_graph_launches"]  # the halves ran as chunks
 
 
 def test_separate_subject_set_matches_oracle():
