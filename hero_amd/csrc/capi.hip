@@ -577,14 +577,21 @@ template <class T> int staged_h2d(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, 
 	return HERO_OK;
 }
 
+// With `flags`, the reads' flags (read_flags) are computed in the packing pass into flags->good and
+// flags->active, and flags->good is the good array uploaded (`good` is not read).
+struct ReadFlagsOut {
+	int64_t min_overlap;
+	int k;
+	uint8_t *good, *active;
+};
 int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
-		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good);
+		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good, const ReadFlagsOut *flags = nullptr);
 int upload_reads(hero_ctx *c, ReadsDev &d, int64_t n, const char *seq, const int64_t *off, const int32_t *len,
-		const int64_t *uid, const std::vector<uint8_t> &good) {
-	return upload_reads_on(c, c->w.st, d, n, seq, off, len, uid, good);
+		const int64_t *uid, const std::vector<uint8_t> &good, const ReadFlagsOut *flags = nullptr) {
+	return upload_reads_on(c, c->w.st, d, n, seq, off, len, uid, good, flags);
 }
 int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const char *seq, const int64_t *off,
-		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good) {
+		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good, const ReadFlagsOut *flags) {
 	// everything goes through pinned staging, so the upload runs at the link's rate:
 	// [words | woff | boff | uid | len | good], the 2-bit words packed by every host thread
 	uint64_t nw = 0;
@@ -603,11 +610,14 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 		w_at += (uint64_t) (len[i] + 31) / 32, b_at += (uint64_t) len[i];
 	}
 	hwoff[n] = w_at, hboff[n] = b_at;
-	pack_reads_into(seq, off, len, n, hwoff, hw, words);
+	if (flags)
+		pack_reads_flags_into(seq, off, len, n, hwoff, hw, words, flags->min_overlap, flags->k, flags->good, flags->active);
+	else
+		pack_reads_into(seq, off, len, n, hwoff, hw, words);
 	if (n) {
 		std::memcpy(huid, uid, (size_t) n * 8);
 		std::memcpy(hlen, len, (size_t) n * 4);
-		std::memcpy(hgood, good.data(), (size_t) n);
+		std::memcpy(hgood, flags ? flags->good : good.data(), (size_t) n);
 	}
 	// grow-only buffers: a steady stream of batches allocates nothing after the largest one
 	HIPCHK(c, d.words.ensure(words));
@@ -1844,11 +1854,9 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 		expect = off[i] + len[i];
 		qb += len[i];
 	}
-	const int64_t min_ov = c->o.min_overlap;
-	const int k = c->o.k;
-#pragma omp parallel for schedule(static, 4096)
-	for (int64_t i = 0; i < n; i++)
-		read_flags(seq + off[i], len[i], min_ov, k, c->q_good[(size_t) i], c->q_active_as_subject[(size_t) i]);
+	// the reads' flags (read_flags) are computed while they are packed for the upload
+	const ReadFlagsOut flags{c->o.min_overlap, c->o.k, c->q_good.data(), c->q_active_as_subject.data()};
+	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good, &flags)) return r;
 	// reads that are not corrected go out as they came in: keep their bytes (all bytes when the
 	// layout has gaps, which the output keeps too)
 	c->q_seq.clear();
@@ -1868,7 +1876,6 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 				c->q_seq.insert(c->q_seq.end(), seq + off[i], seq + off[i] + len[i]);
 			}
 		}
-	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good)) return r;
 	HIPCHK(c, c->q_subject_good.ensure((size_t) n + 1));
 	if (n) HIPCHK(c, hipMemcpyAsync(c->q_subject_good.p, c->q_active_as_subject.data(), (size_t) n, hipMemcpyHostToDevice, c->w.st));
 	c->q_ordered = false;

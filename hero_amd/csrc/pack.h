@@ -87,6 +87,20 @@ inline void pack_reads(const char *seq, const int64_t *off, const int32_t *len, 
 	for (int64_t r = 0; r < n; r++) pack_read(seq + off[r], len[r], &words[(size_t) woff[(size_t) r]]);
 }
 
+// pack_reads_into and read_flags in one pass over each read (its bytes are still in the core's
+// cache for the second use): good[r] and active[r] as read_flags gives them.
+inline void pack_reads_flags_into(const char *seq, const int64_t *off, const int32_t *len, int64_t n,
+		const uint64_t *woff, uint64_t *out, size_t words, int64_t min_overlap, int k, uint8_t *good,
+		uint8_t *active) {
+	const size_t used = n ? (size_t) (woff[n - 1] + (uint64_t) ((len[n - 1] + 31) / 32)) : 0;
+	for (size_t i = used; i < words; i++) out[i] = 0;
+#pragma omp parallel for schedule(static, 4096)
+	for (int64_t r = 0; r < n; r++) {
+		read_flags(seq + off[r], len[r], min_overlap, k, good[r], active[r]);
+		pack_read(seq + off[r], len[r], out + woff[r]);
+	}
+}
+
 // pack_reads into a caller buffer of `words` u64 (woff from the caller; tail words cleared).
 inline void pack_reads_into(const char *seq, const int64_t *off, const int32_t *len, int64_t n, const uint64_t *woff,
 		uint64_t *out, size_t words) {
