@@ -577,6 +577,44 @@ template <class T> int staged_h2d(hero_ctx *c, hipStream_t st, HostBuf<T> &stg, 
 	return HERO_OK;
 }
 
+// Host passes over a read set's arrays, by every host thread: blocks of reads, each block's sums
+// first, then each block's own prefix sums from its offset.
+constexpr int64_t kHostBlock = 1 << 16;
+inline int64_t host_blocks(int64_t n) { return std::max<int64_t>(1, (n + kHostBlock - 1) / kHostBlock); }
+template <class T> void par_copy(T *dst, const T *src, int64_t n) {
+	const int64_t nblk = host_blocks(n);
+#pragma omp parallel for schedule(static)
+	for (int64_t b = 0; b < nblk; b++) {
+		const int64_t i0 = b * kHostBlock, i1 = std::min<int64_t>(n, i0 + kHostBlock);
+		if (i1 > i0) std::memcpy(dst + i0, src + i0, (size_t) (i1 - i0) * sizeof(T));
+	}
+}
+// woff / boff: each read's first word and first base (exclusive prefix sums of ceil(len / 32) and
+// len), entry n the totals; when the outputs are null only the totals are computed.
+void read_offsets(const int32_t *len, int64_t n, uint64_t *woff, uint64_t *boff, uint64_t &nw, uint64_t &nb) {
+	const int64_t nblk = host_blocks(n);
+	std::vector<uint64_t> bw((size_t) nblk + 1, 0), bb((size_t) nblk + 1, 0);
+#pragma omp parallel for schedule(static)
+	for (int64_t b = 0; b < nblk; b++) {
+		uint64_t sw = 0, sb = 0;
+		for (int64_t i = b * kHostBlock, e = std::min<int64_t>(n, (b + 1) * kHostBlock); i < e; i++)
+			sw += (uint64_t) (len[i] + 31) / 32, sb += (uint64_t) len[i];
+		bw[(size_t) b + 1] = sw, bb[(size_t) b + 1] = sb;
+	}
+	for (int64_t b = 0; b < nblk; b++) bw[(size_t) b + 1] += bw[(size_t) b], bb[(size_t) b + 1] += bb[(size_t) b];
+	nw = bw[(size_t) nblk], nb = bb[(size_t) nblk];
+	if (!woff) return;
+#pragma omp parallel for schedule(static)
+	for (int64_t b = 0; b < nblk; b++) {
+		uint64_t w = bw[(size_t) b], x = bb[(size_t) b];
+		for (int64_t i = b * kHostBlock, e = std::min<int64_t>(n, (b + 1) * kHostBlock); i < e; i++) {
+			woff[i] = w, boff[i] = x;
+			w += (uint64_t) (len[i] + 31) / 32, x += (uint64_t) len[i];
+		}
+	}
+	woff[n] = nw, boff[n] = nb;
+}
+
 // With `flags`, the reads' flags (read_flags) are computed in the packing pass into flags->good and
 // flags->active, and flags->good is the good array uploaded (`good` is not read).
 struct ReadFlagsOut {
@@ -594,8 +632,8 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 		const int32_t *len, const int64_t *uid, const std::vector<uint8_t> &good, const ReadFlagsOut *flags) {
 	// everything goes through pinned staging, so the upload runs at the link's rate:
 	// [words | woff | boff | uid | len | good], the 2-bit words packed by every host thread
-	uint64_t nw = 0;
-	for (int64_t i = 0; i < n; i++) nw += (uint64_t) (len[i] + 31) / 32;
+	uint64_t nw = 0, nb = 0;
+	read_offsets(len, n, nullptr, nullptr, nw, nb);
 	// DP tasks locate their segments by u32 word indices (DpTask::s_word, q_word)
 	if (nw + 2 > 0xFFFFFFFFull) return fail(c, HERO_ERR_LIMIT, "read set of %llu packed words exceeds 2^32 - 1", (unsigned long long) nw);
 	const size_t words = (size_t) nw + 2; // two zero words of tail padding
@@ -604,20 +642,15 @@ int upload_reads_on(hero_ctx *c, hipStream_t st, ReadsDev &d, int64_t n, const c
 	uint64_t *hw = c->stage.p, *hwoff = hw + words, *hboff = hwoff + n1, *huid = hboff + n1;
 	int32_t *hlen = (int32_t *) (huid + n1);
 	uint8_t *hgood = (uint8_t *) (huid + n1 + len_words);
-	uint64_t w_at = 0, b_at = 0;
-	for (int64_t i = 0; i < n; i++) {
-		hwoff[i] = w_at, hboff[i] = b_at;
-		w_at += (uint64_t) (len[i] + 31) / 32, b_at += (uint64_t) len[i];
-	}
-	hwoff[n] = w_at, hboff[n] = b_at;
+	read_offsets(len, n, hwoff, hboff, nw, nb);
 	if (flags)
 		pack_reads_flags_into(seq, off, len, n, hwoff, hw, words, flags->min_overlap, flags->k, flags->good, flags->active);
 	else
 		pack_reads_into(seq, off, len, n, hwoff, hw, words);
 	if (n) {
-		std::memcpy(huid, uid, (size_t) n * 8);
-		std::memcpy(hlen, len, (size_t) n * 4);
-		std::memcpy(hgood, flags ? flags->good : good.data(), (size_t) n);
+		par_copy(huid, (const uint64_t *) uid, n);
+		par_copy(hlen, len, n);
+		par_copy(hgood, flags ? flags->good : good.data(), n);
 	}
 	// grow-only buffers: a steady stream of batches allocates nothing after the largest one
 	HIPCHK(c, d.words.ensure(words));
@@ -665,8 +698,11 @@ int order_reads(hero_ctx *c, hipStream_t st, const ReadsDev &R, const uint8_t *a
 int check_reads(hero_ctx *c, int64_t n, const char *seq, const int64_t *off, const int32_t *len, const int64_t *uid) {
 	if (n < 0 || (n > 0 && (!seq || !off || !len || !uid))) return fail(c, HERO_ERR_ARG, "null read arrays");
 	if (n >= (int64_t) 1 << 32) return fail(c, HERO_ERR_LIMIT, "at most 2^32-1 reads per set");
+	int64_t bad = n; // the first read out of the length limits
+#pragma omp parallel for reduction(min : bad) schedule(static, 65536)
 	for (int64_t i = 0; i < n; i++)
-		if (len[i] < 0 || len[i] > kMaxReadLen) return fail(c, HERO_ERR_LIMIT, "read %lld has length %d (limit %d)", (long long) i, len[i], kMaxReadLen);
+		if (len[i] < 0 || len[i] > kMaxReadLen) bad = std::min<int64_t>(bad, i);
+	if (bad < n) return fail(c, HERO_ERR_LIMIT, "read %lld has length %d (limit %d)", (long long) bad, len[bad], kMaxReadLen);
 	return HERO_OK;
 }
 
@@ -1841,19 +1877,22 @@ int hero_set_queries(hero_ctx *c, int64_t n, const char *seq, const int64_t *off
 	if (!c) return HERO_ERR_ARG;
 	if (int r = check_reads(c, n, seq, off, len, uid)) return r;
 	HIPCHK(c, hipSetDevice(c->opt.device));
-	c->q_off.assign(off, off + n);
-	c->q_len.assign(len, len + n);
-	c->q_uid.assign(uid, uid + n);
+	c->q_off.resize((size_t) n), c->q_len.resize((size_t) n), c->q_uid.resize((size_t) n);
+	par_copy(c->q_off.data(), off, n);
+	par_copy(c->q_len.data(), len, n);
+	par_copy(c->q_uid.data(), uid, n);
 	c->q_good.resize((size_t) n);
 	c->q_active_as_subject.resize((size_t) n);
 	c->q_not_good.clear();
-	c->q_contiguous = true;
-	int64_t expect = 0, qb = 0;
+	// contiguous: each read starts where the one before it ends, the first at 0
+	int64_t qb = 0;
+	int contiguous = 1;
+#pragma omp parallel for reduction(+ : qb) reduction(&& : contiguous) schedule(static, 65536)
 	for (int64_t i = 0; i < n; i++) {
-		if (off[i] != expect) c->q_contiguous = false;
-		expect = off[i] + len[i];
 		qb += len[i];
+		contiguous = contiguous && off[i] == (i ? off[i - 1] + len[i - 1] : 0);
 	}
+	c->q_contiguous = contiguous != 0;
 	// the reads' flags (read_flags) are computed while they are packed for the upload
 	const ReadFlagsOut flags{c->o.min_overlap, c->o.k, c->q_good.data(), c->q_active_as_subject.data()};
 	if (int r = upload_reads(c, c->Qd, n, seq, off, len, uid, c->q_good, &flags)) return r;
