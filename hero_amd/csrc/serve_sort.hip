@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t *keys, uint32_
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t *kin, const uint64_t *vin, uint64_t *kout,
 		uint64_t *vout, uint32_t n, int shift, uint32_t mask, uint32_t ntiles, const uint32_t *offs) {
 	__shared__ uint32_t run[256];     // next output position of each digit in this tile
-	__shared__ uint32_t wcnt[4][256]; // this round: items of each digit per wavefront
+	__shared__ uint32_t wcnt[4][256]; // this digit pass: items of each digit per wavefront
 	const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	run[tid] = offs[(size_t) tid * ntiles + blockIdx.x];
